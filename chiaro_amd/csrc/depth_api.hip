@@ -1,0 +1,1071 @@
+// depth_api.hip - C ABI of the depth hot path (host side).
+//
+// chd_refine_multiview_depth mirrors refine_multiview_depth
+// (crates/chiaro-fusion/src/depth.rs:312-580): same inputs, same None cases
+// (invalid options, bad reference index, missing reference camera), same
+// outputs (dense depth map on the fine grid, replaced warps and reports for
+// every accepted calibrated target). A session keeps every input and
+// intermediate in HBM so the path can run repeatedly without host traffic.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/chiaro_depth.h"
+#include "depth_internal.hpp"
+
+using namespace chd;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &message) {
+    g_error = message;
+    return code;
+}
+
+#define CHD_HIP(expr)                                                                \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess)                                                        \
+            return fail(CHD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+size_t div_ceil(size_t a, size_t b) { return (a + b - 1) / b; }
+
+bool options_valid(const chd_options &o) {  // depth.rs:1879-1891
+    return o.enabled && o.grid_step > 0 && std::isfinite(o.near_depth) &&
+           std::isfinite(o.far_depth) && o.near_depth > 0.0 && o.far_depth > o.near_depth &&
+           o.planes >= 3 && o.patch_radius > 0 && o.minimum_support > 0 &&
+           o.best_view_count >= o.minimum_support && o.minimum_neighbour_support > 0;
+}
+
+void inverse_depth_samples(double near_depth, double far_depth, uint32_t count,
+                           double *out) {  // depth.rs:1834-1843
+    const double near_inverse = 1.0 / near_depth, far_inverse = 1.0 / far_depth;
+    for (uint32_t i = 0; i < count; i++) {
+        const double w = (double)i / (double)(count - 1);
+        out[i] = 1.0 / (near_inverse * (1.0 - w) + far_inverse * w);
+    }
+}
+
+// label 0 = global warp (None), labels 1.. = finite depths far -> near
+// (depth.rs:1196-1201)
+std::vector<double> label_depths(const chd_options &o) {
+    std::vector<double> finite(o.planes), labels(o.planes + 1);
+    inverse_depth_samples(o.near_depth, o.far_depth, o.planes, finite.data());
+    labels[0] = std::nan("");
+    for (uint32_t i = 0; i < o.planes; i++) labels[1 + i] = finite[o.planes - 1 - i];
+    return labels;
+}
+
+void grid_sizes(const chd_options &o, uint32_t width, uint32_t height, chd_grid_sizes &g) {
+    g.coarse_step = std::max(o.grid_step, 4u);
+    g.coarse_columns = (uint32_t)div_ceil(width, g.coarse_step) + 1;
+    g.coarse_rows = (uint32_t)div_ceil(height, g.coarse_step) + 1;
+    g.labels = o.planes + 1;
+    g.fine_step = std::max(g.coarse_step / 2, 4u);
+    g.fine_columns = (uint32_t)div_ceil(width, g.fine_step) + 1;
+    g.fine_rows = (uint32_t)div_ceil(height, g.fine_step) + 1;
+}
+
+int32_t tkey32_host(float f) {
+    int32_t k;
+    std::memcpy(&k, &f, 4);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+
+// MINIMUM_DIRECT_COMPONENT_NODES (depth.rs:37)
+constexpr uint32_t kMinimumComponentNodes = 24;
+
+struct TimedLaunch {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct chd_context {
+    int device;
+    hipStream_t stream;
+};
+
+struct chd_session {
+    chd_context *context = nullptr;
+    chd_options options{};
+    uint32_t count = 0, reference = 0;
+    bool none = false;
+    chd_grid_sizes grid{};
+    std::vector<void *> allocations;
+    std::vector<uint32_t> active;  // accepted, calibrated, non-reference views
+    ViewDev *views = nullptr;
+    uint32_t *vlist = nullptr;     // device copy of `active`
+    double *labels = nullptr;
+    float *scores = nullptr, *costs = nullptr, *guidance = nullptr;
+    uint8_t *tested = nullptr;
+    float *paths = nullptr, *aggregated = nullptr;
+    chd_node_depth *coarse = nullptr, *coarse_tmp = nullptr;
+    uint8_t *fillable = nullptr;
+    uint16_t *best_label = nullptr;
+    chd_node_depth *field[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *fine_guidance = nullptr;
+    uint8_t *fine_tested = nullptr;
+    WideEntry *wide = nullptr;
+    uint32_t *wide_count = nullptr;
+    uint32_t *parent = nullptr, *sizes = nullptr;
+    chd_node_warp *decisions[2] = {nullptr, nullptr};
+    std::vector<float *> target_points, target_confidence;
+    std::vector<uint8_t *> target_codes;
+    unsigned long long *target_counts = nullptr;  // 3 per view slot
+    unsigned long long *stat_counts = nullptr;    // tested, measured, regularized
+    unsigned long long *depth_keys = nullptr, *depth_keys_sorted = nullptr;
+    unsigned int *improvement_keys = nullptr, *improvement_keys_sorted = nullptr;
+    uint32_t *key_counts = nullptr;
+    void *sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    double *map_depth = nullptr;
+    float *map_confidence = nullptr;
+    uint8_t *map_provenance = nullptr;
+    uint32_t stages_done = 0;
+    bool timing = false;
+    std::vector<TimedLaunch> launches;
+    std::vector<chd_kernel_time> times;
+
+    ~chd_session() {
+        for (auto &t : launches) {
+            hipEventDestroy(t.start);
+            hipEventDestroy(t.stop);
+        }
+        for (void *p : allocations) hipFree(p);
+    }
+};
+
+namespace {
+
+template <class T>
+int dalloc(chd_session *s, T **out, size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(CHD_ERR_HIP, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                     "): " + hipGetErrorString(e));
+    s->allocations.push_back(p);
+    *out = (T *)p;
+    return CHD_OK;
+}
+
+#define CHD_CHECK(expr)        \
+    do {                       \
+        int _r = (expr);       \
+        if (_r != CHD_OK) return _r; \
+    } while (0)
+
+// Launch wrapper: optional HIP-event timing on the session's stream.
+template <class F>
+int timed(chd_session *s, const char *name, F &&launch) {
+    hipStream_t stream = s->context->stream;
+    TimedLaunch t{name, nullptr, nullptr};
+    if (s->timing) {
+        CHD_HIP(hipEventCreate(&t.start));
+        CHD_HIP(hipEventCreate(&t.stop));
+        CHD_HIP(hipEventRecord(t.start, stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) {
+        if (t.start) hipEventDestroy(t.start);
+        if (t.stop) hipEventDestroy(t.stop);
+        return fail(CHD_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
+    }
+    if (s->timing) {
+        CHD_HIP(hipEventRecord(t.stop, stream));
+        s->launches.push_back(t);
+    }
+    return CHD_OK;
+}
+
+int collect_times(chd_session *s) {
+    if (s->launches.empty()) return CHD_OK;
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    for (auto &t : s->launches) {
+        float ms = 0.0f;
+        CHD_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+        auto it = std::find_if(s->times.begin(), s->times.end(), [&](const chd_kernel_time &k) {
+            return std::strncmp(k.name, t.name, sizeof k.name) == 0;
+        });
+        if (it == s->times.end()) {
+            chd_kernel_time k{};
+            std::snprintf(k.name, sizeof k.name, "%s", t.name);
+            s->times.push_back(k);
+            it = s->times.end() - 1;
+        }
+        it->launches += 1;
+        it->total_ms += ms;
+        hipEventDestroy(t.start);
+        hipEventDestroy(t.stop);
+    }
+    s->launches.clear();
+    return CHD_OK;
+}
+
+PatchSpec patch_spec(const chd_session *s, uint32_t radius) {
+    PatchSpec p;
+    p.views = s->views;
+    p.vlist = s->vlist;
+    p.view_count = (uint32_t)s->active.size();
+    p.reference = s->reference;
+    p.radius = radius;
+    return p;
+}
+
+int run_cost(chd_session *s) {
+    const chd_grid_sizes &g = s->grid;
+    PatchSpec patch = patch_spec(s, s->options.patch_radius);
+    GridSpec grid{g.coarse_columns, g.coarse_rows, g.coarse_step};
+    return timed(s, "cost_volume", [&] {
+        return launch_cost_volume(s->context->stream, patch, grid, s->labels, g.labels,
+                                  s->options.minimum_support, s->options.best_view_count,
+                                  s->scores, s->costs, s->guidance, s->tested);
+    });
+}
+
+int run_sgm(chd_session *s, bool keep_aggregated) {
+    const chd_grid_sizes &g = s->grid;
+    hipStream_t st = s->context->stream;
+    CHD_CHECK(timed(s, "sgm_paths", [&] {
+        return launch_sgm(st, s->costs, s->guidance, g.coarse_columns, g.coarse_rows, g.labels,
+                          s->paths);
+    }));
+    CHD_CHECK(timed(s, "sgm_select", [&] {
+        return launch_sgm_select(st, s->paths, s->scores, s->tested, s->labels, g.coarse_columns,
+                                 g.coarse_rows, g.labels, s->options,
+                                 keep_aggregated ? s->aggregated : nullptr, s->coarse, s->fillable,
+                                 s->best_label);
+    }));
+    if (s->options.completion_iterations > 0) {
+        const size_t n = (size_t)g.coarse_columns * g.coarse_rows;
+        for (uint32_t it = 0; it < s->options.completion_iterations; it++) {
+            CHD_HIP(hipMemcpyAsync(s->coarse_tmp, s->coarse, n * sizeof(chd_node_depth),
+                                   hipMemcpyDeviceToDevice, st));
+            CHD_CHECK(timed(s, "complete_depth_field", [&] {
+                return launch_complete_depth_field(st, s->coarse_tmp, s->coarse, s->guidance,
+                                                   s->fillable, g.coarse_columns, g.coarse_rows,
+                                                   s->options.minimum_neighbour_support);
+            }));
+        }
+    }
+    return CHD_OK;
+}
+
+double inverse_step_of(const chd_options &o) {  // depth.rs:346-347
+    return (1.0 / o.near_depth - 1.0 / o.far_depth) / (double)(o.planes - 1);
+}
+
+int run_direct(chd_session *s) {
+    const chd_grid_sizes &g = s->grid;
+    hipStream_t st = s->context->stream;
+    const chd_options &o = s->options;
+    chd_options direct = o, wide = o;
+    direct.patch_radius = std::max(o.patch_radius, 3u);
+    wide.patch_radius = std::max(o.patch_radius * 2, 8u);
+    const double inverse_step = inverse_step_of(o);
+    GridSpec fine{g.fine_columns, g.fine_rows, g.fine_step};
+    GridSpec coarse{g.coarse_columns, g.coarse_rows, g.coarse_step};
+    const uint32_t fine_nodes = g.fine_columns * g.fine_rows;
+    CHD_HIP(launch_fill_u32(st, s->wide_count, 0u, 1));
+    CHD_CHECK(timed(s, "direct", [&] {
+        return launch_direct(st, patch_spec(s, direct.patch_radius), fine, coarse, s->coarse,
+                             inverse_step, direct, s->field[0], s->fine_guidance, s->fine_tested,
+                             s->wide, s->wide_count);
+    }));
+    CHD_CHECK(timed(s, "direct_wide", [&] {
+        return launch_direct_wide(st, patch_spec(s, wide.patch_radius), fine, wide, s->wide,
+                                  s->wide_count, fine_nodes, s->field[0]);
+    }));
+    CHD_CHECK(timed(s, "reject_isolated", [&] {
+        return launch_reject_isolated(st, s->field[0], s->field[1], s->fine_guidance,
+                                      g.fine_columns, g.fine_rows, inverse_step * 2.5);
+    }));
+    CHD_CHECK(timed(s, "reject_components", [&] {
+        return launch_reject_components(st, s->field[1], s->field[2], s->fine_guidance,
+                                        g.fine_columns, g.fine_rows, inverse_step * 2.5,
+                                        kMinimumComponentNodes, s->parent, s->sizes);
+    }));
+    CHD_CHECK(timed(s, "fit_planes", [&] {
+        return launch_fit_planes(st, s->field[2], s->field[3], s->fine_guidance, g.fine_columns,
+                                 g.fine_rows, inverse_step * 4.0);
+    }));
+    CHD_HIP(launch_fill_u64(st, s->stat_counts, 0ull, 3));
+    CHD_HIP(launch_fill_u32(st, s->key_counts, 0u, 2));
+    CHD_CHECK(timed(s, "field_stats", [&] {
+        return launch_field_stats(st, s->field[3], s->fine_tested, fine_nodes, s->stat_counts,
+                                  s->depth_keys, s->improvement_keys, s->key_counts);
+    }));
+    return timed(s, "final_map", [&] {
+        return launch_final_map(st, s->field[3], s->fine_tested, fine_nodes, s->map_depth,
+                                s->map_confidence, s->map_provenance);
+    });
+}
+
+int run_targets(chd_session *s) {
+    const chd_grid_sizes &g = s->grid;
+    hipStream_t st = s->context->stream;
+    const chd_options &o = s->options;
+    const double inverse_step = inverse_step_of(o);
+    GridSpec fine{g.fine_columns, g.fine_rows, g.fine_step};
+    const uint32_t fine_nodes = g.fine_columns * g.fine_rows;
+    CHD_HIP(launch_fill_u64(st, s->target_counts, 0ull, 3 * (size_t)s->count));
+    for (size_t k = 0; k < s->active.size(); k++) {
+        const uint32_t t = s->active[k];
+        PatchSpec patch = patch_spec(s, o.patch_radius);
+        patch.vlist = s->vlist + k;
+        patch.view_count = 1;
+        CHD_CHECK(timed(s, "targets", [&] {
+            return launch_targets(st, patch, fine, s->field[3], inverse_step, o, s->decisions[0]);
+        }));
+        CHD_CHECK(timed(s, "consensus", [&] {
+            return launch_consensus(st, s->decisions[0], s->decisions[1], g.fine_columns,
+                                    g.fine_rows);
+        }));
+        CHD_CHECK(timed(s, "boundaries", [&] {
+            return launch_boundaries(st, s->decisions[1], s->decisions[0], s->fine_guidance,
+                                     g.fine_columns, g.fine_rows);
+        }));
+        CHD_CHECK(timed(s, "target_outputs", [&] {
+            return launch_target_outputs(st, s->decisions[0], fine_nodes, s->target_points[t],
+                                         s->target_confidence[t], s->target_codes[t],
+                                         s->target_counts + 3 * (size_t)t);
+        }));
+    }
+    return CHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chd_last_error(void) { return g_error.c_str(); }
+int chd_abi_version(void) { return CHD_ABI_VERSION; }
+
+void chd_default_options(chd_options *o) {  // depth.rs:69-91
+    o->enabled = 1;
+    o->grid_step = 8;
+    o->near_depth = 500.0;
+    o->far_depth = 10000000.0;
+    o->planes = 96;
+    o->patch_radius = 4;
+    o->minimum_support = 2;
+    o->best_view_count = 3;
+    o->minimum_score = 0.45f;
+    o->minimum_margin = 0.01f;
+    o->minimum_improvement = 0.01f;
+    o->minimum_neighbour_support = 2;
+    o->completion_iterations = 0;
+}
+
+int chd_options_valid(const chd_options *o) { return o && options_valid(*o) ? 1 : 0; }
+
+int chd_grid_sizes_for(const chd_options *o, uint32_t width, uint32_t height,
+                       chd_grid_sizes *out) {
+    if (!o || !out || o->grid_step == 0) return fail(CHD_ERR_ARGUMENT, "grid sizes: bad argument");
+    grid_sizes(*o, width, height, *out);
+    return CHD_OK;
+}
+
+int chd_camera_canonical(const double k[9], const double rotation_wc[9],
+                         const double translation_wc[3], uint32_t width, uint32_t height,
+                         const chd_distortion *distortion, chd_camera *out) {
+    // ResolvedCamera::new, canonical pose (geometry.rs:64-133)
+    if (!k || !rotation_wc || !translation_wc || !out)
+        return fail(CHD_ERR_ARGUMENT, "camera: null argument");
+    std::memset(out, 0, sizeof *out);
+    out->width = width;
+    out->height = height;
+    std::memcpy(out->k, k, sizeof out->k);
+    const double *m = k;
+    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                       m[2] * (m[3] * m[7] - m[4] * m[6]);
+    if (std::fabs(det) < 1e-300) return fail(CHD_ERR_ARGUMENT, "singular intrinsic matrix");
+    const double inv = 1.0 / det;
+    auto c = [&](int r0, int c0, int r1, int c1) {
+        return m[r0 * 3 + c0] * m[r1 * 3 + c1] - m[r0 * 3 + c1] * m[r1 * 3 + c0];
+    };
+    const double ki[9] = {c(1, 1, 2, 2) * inv,  -c(0, 1, 2, 2) * inv, c(0, 1, 1, 2) * inv,
+                          -c(1, 0, 2, 2) * inv, c(0, 0, 2, 2) * inv,  -c(0, 0, 1, 2) * inv,
+                          c(1, 0, 2, 1) * inv,  -c(0, 0, 2, 1) * inv, c(0, 0, 1, 1) * inv};
+    std::memcpy(out->k_inverse, ki, sizeof ki);
+    if (distortion) {
+        out->has_distortion = 1;
+        out->distortion = *distortion;
+    }
+    out->flip_around_x = -1;
+    out->pose_kind = 0;
+    std::memcpy(out->rotation_wc, rotation_wc, sizeof out->rotation_wc);
+    const double *r = rotation_wc;
+    const double rcw[9] = {r[0], r[3], r[6], r[1], r[4], r[7], r[2], r[5], r[8]};
+    std::memcpy(out->rotation_cw, rcw, sizeof rcw);
+    std::memcpy(out->translation_wc, translation_wc, sizeof out->translation_wc);
+    const double *t = translation_wc;
+    for (int i = 0; i < 3; i++) {
+        const double v = rcw[i * 3] * t[0] + rcw[i * 3 + 1] * t[1] + rcw[i * 3 + 2] * t[2];
+        out->center[i] = v * -1.0;
+    }
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::memcpy(out->orientation_correction, I, sizeof I);
+    out->focal_px = k[0];
+    return CHD_OK;
+}
+
+int chd_context_create(int device, chd_context **out) {
+    if (!out) return fail(CHD_ERR_ARGUMENT, "context: null output");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(CHD_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(CHD_ERR_ARGUMENT, "device index out of range");
+    CHD_HIP(hipSetDevice(device));
+    auto *c = new chd_context;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CHD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return CHD_OK;
+}
+
+void chd_context_destroy(chd_context *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int chd_context_synchronize(chd_context *c) {
+    if (!c) return fail(CHD_ERR_ARGUMENT, "null context");
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
+}
+
+int chd_session_create(chd_context *context, const chd_view *views, uint32_t count,
+                       uint32_t reference_index, const chd_options *options,
+                       chd_session **out) {
+    if (!context || !views || !options || !out) return fail(CHD_ERR_ARGUMENT, "session: null argument");
+    *out = nullptr;
+    CHD_HIP(hipSetDevice(context->device));
+    auto *s = new chd_session;
+    s->context = context;
+    s->options = *options;
+    s->count = count;
+    s->reference = reference_index;
+    // the reference's None cases (depth.rs:318-324)
+    if (!options_valid(*options) || reference_index >= count ||
+        views[reference_index].camera == nullptr) {
+        s->none = true;
+        *out = s;
+        return CHD_OK;
+    }
+    auto bail = [&](int code) {
+        delete s;
+        return code;
+    };
+    if (count > CHD_MAX_VIEWS) return bail(fail(CHD_ERR_LIMIT, "more than CHD_MAX_VIEWS views"));
+    if (options->patch_radius > CHD_MAX_PATCH_RADIUS / 2)
+        return bail(fail(CHD_ERR_LIMIT, "patch_radius above CHD_MAX_PATCH_RADIUS / 2"));
+    if (options->planes + 1 > 512) return bail(fail(CHD_ERR_LIMIT, "more than 511 planes"));
+    for (uint32_t i = 0; i < count; i++) {
+        const chd_view &v = views[i];
+        if (!v.luminance || v.plane_width == 0 || v.plane_height == 0)
+            return bail(fail(CHD_ERR_ARGUMENT, "view " + std::to_string(i) + ": empty luminance plane"));
+        if (i != reference_index && v.accepted && v.camera &&
+            (!v.global_warp.points || v.global_warp.step == 0 || v.global_warp.columns == 0 ||
+             v.global_warp.rows == 0))
+            return bail(fail(CHD_ERR_ARGUMENT, "view " + std::to_string(i) + ": empty global warp"));
+    }
+    grid_sizes(*options, views[reference_index].width, views[reference_index].height, s->grid);
+    const chd_grid_sizes &g = s->grid;
+    const size_t coarse_nodes = (size_t)g.coarse_columns * g.coarse_rows;
+    const size_t fine_nodes = (size_t)g.fine_columns * g.fine_rows;
+    const size_t volume = coarse_nodes * g.labels;
+    if (volume * 8 > ((size_t)1 << 36))
+        return bail(fail(CHD_ERR_LIMIT, "cost volume above the session limit"));
+    for (uint32_t i = 0; i < count; i++)
+        if (i != reference_index && views[i].accepted && views[i].camera) s->active.push_back(i);
+
+    hipStream_t st = context->stream;
+    std::vector<ViewDev> host_views(count);
+    std::vector<chd_camera *> cams(count, nullptr);
+    int r = CHD_OK;
+#define CHD_ALLOC(ptr, n)                       \
+    do {                                        \
+        r = dalloc(s, &(ptr), (n));             \
+        if (r != CHD_OK) return bail(r);        \
+    } while (0)
+#define CHD_UPLOAD(dst, src, bytes)                                                          \
+    do {                                                                                     \
+        hipError_t _e = hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, st);    \
+        if (_e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("upload: ") + hipGetErrorString(_e))); \
+    } while (0)
+    for (uint32_t i = 0; i < count; i++) {
+        const chd_view &v = views[i];
+        ViewDev &d = host_views[i];
+        std::memset(&d, 0, sizeof d);
+        float *plane = nullptr;
+        const size_t plane_n = (size_t)v.plane_width * v.plane_height;
+        CHD_ALLOC(plane, plane_n);
+        CHD_UPLOAD(plane, v.luminance, plane_n * sizeof(float));
+        d.plane = plane;
+        d.plane_width = v.plane_width;
+        d.plane_height = v.plane_height;
+        if (v.global_warp.points && v.global_warp.columns && v.global_warp.rows) {
+            float2 *warp = nullptr;
+            const size_t wn = (size_t)v.global_warp.columns * v.global_warp.rows;
+            CHD_ALLOC(warp, wn);
+            CHD_UPLOAD(warp, v.global_warp.points, wn * sizeof(float2));
+            d.warp = warp;
+            d.warp_step = v.global_warp.step;
+            d.warp_columns = v.global_warp.columns;
+            d.warp_rows = v.global_warp.rows;
+        }
+        if (v.camera) {
+            chd_camera *cam = nullptr;
+            CHD_ALLOC(cam, 1);
+            CHD_UPLOAD(cam, v.camera, sizeof(chd_camera));
+            d.camera = cam;
+        }
+    }
+    CHD_ALLOC(s->views, count);
+    CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
+    CHD_ALLOC(s->vlist, std::max<size_t>(s->active.size(), 1));
+    if (!s->active.empty()) CHD_UPLOAD(s->vlist, s->active.data(), s->active.size() * sizeof(uint32_t));
+    std::vector<double> labels = label_depths(*options);
+    CHD_ALLOC(s->labels, labels.size());
+    CHD_UPLOAD(s->labels, labels.data(), labels.size() * sizeof(double));
+
+    CHD_ALLOC(s->scores, volume);
+    CHD_ALLOC(s->costs, volume);
+    CHD_ALLOC(s->guidance, coarse_nodes);
+    CHD_ALLOC(s->tested, coarse_nodes);
+    CHD_ALLOC(s->paths, volume * 8);
+    CHD_ALLOC(s->aggregated, volume);
+    CHD_ALLOC(s->coarse, coarse_nodes);
+    CHD_ALLOC(s->coarse_tmp, coarse_nodes);
+    CHD_ALLOC(s->fillable, coarse_nodes);
+    CHD_ALLOC(s->best_label, coarse_nodes);
+    for (int k = 0; k < 4; k++) CHD_ALLOC(s->field[k], fine_nodes);
+    CHD_ALLOC(s->fine_guidance, fine_nodes);
+    CHD_ALLOC(s->fine_tested, fine_nodes);
+    CHD_ALLOC(s->wide, fine_nodes);
+    CHD_ALLOC(s->wide_count, 1);
+    CHD_ALLOC(s->parent, fine_nodes);
+    CHD_ALLOC(s->sizes, fine_nodes);
+    CHD_ALLOC(s->decisions[0], fine_nodes);
+    CHD_ALLOC(s->decisions[1], fine_nodes);
+    s->target_points.assign(count, nullptr);
+    s->target_confidence.assign(count, nullptr);
+    s->target_codes.assign(count, nullptr);
+    for (uint32_t t : s->active) {
+        CHD_ALLOC(s->target_points[t], fine_nodes * 2);
+        CHD_ALLOC(s->target_confidence[t], fine_nodes);
+        CHD_ALLOC(s->target_codes[t], fine_nodes);
+    }
+    CHD_ALLOC(s->target_counts, 3 * (size_t)count);
+    CHD_ALLOC(s->stat_counts, 3);
+    CHD_ALLOC(s->depth_keys, fine_nodes);
+    CHD_ALLOC(s->depth_keys_sorted, fine_nodes);
+    CHD_ALLOC(s->improvement_keys, fine_nodes);
+    CHD_ALLOC(s->improvement_keys_sorted, fine_nodes);
+    CHD_ALLOC(s->key_counts, 2);
+    CHD_ALLOC(s->map_depth, fine_nodes);
+    CHD_ALLOC(s->map_confidence, fine_nodes);
+    CHD_ALLOC(s->map_provenance, fine_nodes);
+    {
+        size_t a = 0, b = 0;
+        (void)hipcub::DeviceRadixSort::SortKeys(nullptr, a, s->depth_keys, s->depth_keys_sorted,
+                                          (int)fine_nodes, 0, 64, st);
+        (void)hipcub::DeviceRadixSort::SortKeys(nullptr, b, s->improvement_keys,
+                                          s->improvement_keys_sorted, (int)fine_nodes, 0, 32, st);
+        s->sort_temp_bytes = std::max(a, b);
+        unsigned char *tmp = nullptr;
+        CHD_ALLOC(tmp, s->sort_temp_bytes);
+        s->sort_temp = tmp;
+    }
+#undef CHD_ALLOC
+#undef CHD_UPLOAD
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("upload sync: ") + hipGetErrorString(e)));
+    *out = s;
+    return CHD_OK;
+}
+
+void chd_session_destroy(chd_session *s) {
+    if (!s) return;
+    if (s->context) {
+        hipSetDevice(s->context->device);
+        hipStreamSynchronize(s->context->stream);
+    }
+    delete s;
+}
+
+int chd_session_grid(const chd_session *s, chd_grid_sizes *out) {
+    if (!s || !out) return fail(CHD_ERR_ARGUMENT, "null argument");
+    if (s->none) return CHD_NONE;
+    *out = s->grid;
+    return CHD_OK;
+}
+
+int chd_session_run(chd_session *s, uint32_t stages) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    CHD_HIP(hipSetDevice(s->context->device));
+    const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
+    if (stages & CHD_STAGE_COST) {
+        CHD_CHECK(run_cost(s));
+        s->stages_done |= CHD_STAGE_COST;
+    }
+    if (stages & CHD_STAGE_SGM) {
+        if (!(s->stages_done & CHD_STAGE_COST)) return fail(CHD_ERR_STATE, "SGM before cost volume");
+        CHD_CHECK(run_sgm(s, keep));
+        s->stages_done |= CHD_STAGE_SGM | (keep ? CHD_STAGE_KEEP_AGGREGATED : 0);
+    }
+    if (stages & CHD_STAGE_DIRECT) {
+        if (!(s->stages_done & CHD_STAGE_SGM)) return fail(CHD_ERR_STATE, "direct pass before SGM");
+        CHD_CHECK(run_direct(s));
+        s->stages_done |= CHD_STAGE_DIRECT;
+    }
+    if (stages & CHD_STAGE_TARGETS) {
+        if (!(s->stages_done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "targets before direct pass");
+        CHD_CHECK(run_targets(s));
+        s->stages_done |= CHD_STAGE_TARGETS;
+    }
+    return CHD_OK;
+}
+
+extern "C++" {
+namespace {
+
+int download(chd_session *s, void *dst, const void *src, size_t bytes) {
+    if (!dst) return CHD_OK;
+    CHD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->context->stream));
+    return CHD_OK;
+}
+
+// k-th smallest key of `n` device keys (radix sort), or false when n == 0.
+template <class K>
+int median_key(chd_session *s, K *keys, K *sorted, uint32_t n, int bits, K &out, bool &found) {
+    found = false;
+    if (n == 0) return CHD_OK;
+    hipStream_t st = s->context->stream;
+    size_t temp = s->sort_temp_bytes;
+    CHD_HIP(hipcub::DeviceRadixSort::SortKeys(s->sort_temp, temp, keys, sorted, (int)n, 0, bits, st));
+    CHD_HIP(hipMemcpyAsync(&out, sorted + n / 2, sizeof(K), hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    found = true;
+    return CHD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int chd_session_read_results(chd_session *s, chd_depth_map *map, chd_target_result *targets) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    if (!(s->stages_done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "results before the direct pass");
+    const chd_grid_sizes &g = s->grid;
+    const size_t fine = (size_t)g.fine_columns * g.fine_rows;
+    hipStream_t st = s->context->stream;
+    if (map) {
+        map->columns = g.fine_columns;
+        map->rows = g.fine_rows;
+        map->step = g.fine_step;
+        map->near_depth = s->options.near_depth;
+        map->far_depth = s->options.far_depth;
+        CHD_CHECK(download(s, map->depth, s->map_depth, fine * sizeof(double)));
+        CHD_CHECK(download(s, map->confidence, s->map_confidence, fine * sizeof(float)));
+        CHD_CHECK(download(s, map->provenance, s->map_provenance, fine));
+    }
+    if (targets) {
+        unsigned long long stats[3], counts[3 * CHD_MAX_VIEWS];
+        uint32_t key_counts[2];
+        CHD_HIP(hipMemcpyAsync(stats, s->stat_counts, sizeof stats, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipMemcpyAsync(counts, s->target_counts, 3 * s->count * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipMemcpyAsync(key_counts, s->key_counts, sizeof key_counts, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipStreamSynchronize(st));
+        unsigned long long dkey = 0;
+        unsigned int ikey = 0;
+        bool has_depth = false, has_improvement = false;
+        CHD_CHECK(median_key(s, s->depth_keys, s->depth_keys_sorted, key_counts[0], 64, dkey, has_depth));
+        CHD_CHECK(median_key(s, s->improvement_keys, s->improvement_keys_sorted, key_counts[1], 32,
+                             ikey, has_improvement));
+        double median_depth = 0.0;
+        float median_improvement = 0.0f;
+        if (has_depth) {
+            int64_t k = (int64_t)(dkey ^ 0x8000000000000000ull);
+            k ^= (int64_t)(((uint64_t)(k >> 63)) >> 1);
+            std::memcpy(&median_depth, &k, 8);
+        }
+        if (has_improvement) {
+            int32_t k = (int32_t)(ikey ^ 0x80000000u);
+            k ^= (int32_t)(((uint32_t)(k >> 31)) >> 1);
+            std::memcpy(&median_improvement, &k, 4);
+        }
+        const bool targets_ran = (s->stages_done & CHD_STAGE_TARGETS) != 0;
+        const unsigned long long tested = stats[0], measured = stats[1], regularized = stats[2];
+        const unsigned long long fallback =
+            tested > measured + regularized ? tested - (measured + regularized) : 0;
+        auto fraction = [](unsigned long long c, unsigned long long total) {
+            return total == 0 ? 0.0f : (float)c / (float)total;
+        };
+        for (uint32_t t = 0; t < s->count; t++) targets[t].replaced = 0;
+        for (uint32_t t : s->active) {
+            chd_target_result &out = targets[t];
+            if (!targets_ran) continue;
+            CHD_CHECK(download(s, out.points, s->target_points[t], fine * 2 * sizeof(float)));
+            CHD_CHECK(download(s, out.confidence, s->target_confidence[t], fine * sizeof(float)));
+            CHD_CHECK(download(s, out.decisions, s->target_codes[t], fine));
+            out.replaced = 1;
+            chd_depth_report &r = out.report;
+            std::memset(&r, 0, sizeof r);
+            r.tested_nodes = tested;
+            r.measured_nodes = measured;
+            r.regularized_nodes = regularized;
+            r.fallback_nodes = fallback;
+            r.refined_nodes = counts[3 * t + 0];
+            r.occluded_nodes = counts[3 * t + 1];
+            r.boundary_nodes = counts[3 * t + 2];
+            r.reconstructed_fraction = fraction(measured + regularized, tested);
+            r.refined_fraction = fraction(r.refined_nodes, tested);
+            r.occluded_fraction = fraction(r.occluded_nodes, tested);
+            r.has_median_depth = has_depth;
+            r.median_depth = median_depth;
+            r.has_median_score_improvement = has_improvement;
+            r.median_score_improvement = median_improvement;
+        }
+    }
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_session_read_cost_volume(chd_session *s, float *scores, float *costs, float *guidance,
+                                 uint8_t *tested) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    if (!(s->stages_done & CHD_STAGE_COST)) return fail(CHD_ERR_STATE, "cost volume not built");
+    const size_t nodes = (size_t)s->grid.coarse_columns * s->grid.coarse_rows;
+    const size_t volume = nodes * s->grid.labels;
+    CHD_CHECK(download(s, scores, s->scores, volume * sizeof(float)));
+    CHD_CHECK(download(s, costs, s->costs, volume * sizeof(float)));
+    CHD_CHECK(download(s, guidance, s->guidance, nodes * sizeof(float)));
+    CHD_CHECK(download(s, tested, s->tested, nodes));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_read_aggregated(chd_session *s, float *aggregated) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    if (!(s->stages_done & CHD_STAGE_KEEP_AGGREGATED))
+        return fail(CHD_ERR_STATE, "aggregated volume not kept (run with CHD_STAGE_KEEP_AGGREGATED)");
+    const size_t volume = (size_t)s->grid.coarse_columns * s->grid.coarse_rows * s->grid.labels;
+    CHD_CHECK(download(s, aggregated, s->aggregated, volume * sizeof(float)));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_read_coarse(chd_session *s, chd_node_depth *field, uint8_t *fillable,
+                            uint16_t *best_label) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    if (!(s->stages_done & CHD_STAGE_SGM)) return fail(CHD_ERR_STATE, "SGM not run");
+    const size_t nodes = (size_t)s->grid.coarse_columns * s->grid.coarse_rows;
+    CHD_CHECK(download(s, field, s->coarse, nodes * sizeof(chd_node_depth)));
+    CHD_CHECK(download(s, fillable, s->fillable, nodes));
+    CHD_CHECK(download(s, best_label, s->best_label, nodes * sizeof(uint16_t)));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_read_direct(chd_session *s, uint32_t stage, chd_node_depth *field,
+                            float *guidance, uint8_t *tested) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->none) return CHD_NONE;
+    if (stage > 3) return fail(CHD_ERR_ARGUMENT, "direct stage must be 0..3");
+    if (!(s->stages_done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "direct pass not run");
+    const size_t nodes = (size_t)s->grid.fine_columns * s->grid.fine_rows;
+    CHD_CHECK(download(s, field, s->field[stage], nodes * sizeof(chd_node_depth)));
+    CHD_CHECK(download(s, guidance, s->fine_guidance, nodes * sizeof(float)));
+    CHD_CHECK(download(s, tested, s->fine_tested, nodes));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_enable_timing(chd_session *s, int enable) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    s->timing = enable != 0;
+    return CHD_OK;
+}
+
+int chd_session_kernel_times(chd_session *s, chd_kernel_time *out, uint32_t capacity,
+                             uint32_t *count) {
+    if (!s || !count) return fail(CHD_ERR_ARGUMENT, "null argument");
+    CHD_CHECK(collect_times(s));
+    *count = (uint32_t)s->times.size();
+    for (uint32_t i = 0; i < capacity && i < s->times.size(); i++) out[i] = s->times[i];
+    return CHD_OK;
+}
+
+int chd_session_reset_kernel_times(chd_session *s) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    CHD_CHECK(collect_times(s));
+    s->times.clear();
+    return CHD_OK;
+}
+
+int chd_refine_multiview_depth(chd_context *context, const chd_view *views, uint32_t count,
+                               uint32_t reference_index, const chd_options *options,
+                               chd_depth_map *map, chd_target_result *targets) {
+    chd_session *s = nullptr;
+    int r = chd_session_create(context, views, count, reference_index, options, &s);
+    if (r != CHD_OK) return r;
+    if (s->none) {
+        chd_session_destroy(s);
+        return CHD_NONE;
+    }
+    r = chd_session_run(s, CHD_STAGE_ALL);
+    if (r == CHD_OK) r = chd_session_read_results(s, map, targets);
+    chd_session_destroy(s);
+    return r;
+}
+
+// ------------------------------------------------- stage entry points
+
+extern "C++" {
+namespace {
+
+struct Scratch {
+    std::vector<void *> p;
+    ~Scratch() {
+        for (void *q : p) hipFree(q);
+    }
+    template <class T>
+    int alloc(T **out, size_t n) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(CHD_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        p.push_back(q);
+        *out = (T *)q;
+        return CHD_OK;
+    }
+};
+
+}  // namespace
+}  // extern "C++"
+
+int chd_semi_global_costs(chd_context *c, const float *costs, const float *guidance,
+                          uint32_t columns, uint32_t rows, uint32_t labels, float *aggregated) {
+    if (!c || !costs || !guidance || !aggregated || columns == 0 || rows == 0 || labels == 0)
+        return fail(CHD_ERR_ARGUMENT, "semi_global_costs: bad argument");
+    if (labels > 512) return fail(CHD_ERR_LIMIT, "semi_global_costs: more than 512 labels");
+    CHD_HIP(hipSetDevice(c->device));
+    const size_t nodes = (size_t)columns * rows, volume = nodes * labels;
+    Scratch sc;
+    float *d_costs, *d_guidance, *d_paths, *d_out;
+    CHD_CHECK(sc.alloc(&d_costs, volume));
+    CHD_CHECK(sc.alloc(&d_guidance, nodes));
+    CHD_CHECK(sc.alloc(&d_paths, volume * 8));
+    CHD_CHECK(sc.alloc(&d_out, volume));
+    hipStream_t st = c->stream;
+    CHD_HIP(hipMemcpyAsync(d_costs, costs, volume * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_guidance, guidance, nodes * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_sgm(st, d_costs, d_guidance, columns, rows, labels, d_paths));
+    // the path sum with every node untested: select_depths writes None
+    double *d_labels;
+    uint8_t *d_tested, *d_fill;
+    chd_node_depth *d_field;
+    CHD_CHECK(sc.alloc(&d_labels, labels));
+    CHD_CHECK(sc.alloc(&d_tested, nodes));
+    CHD_CHECK(sc.alloc(&d_fill, nodes));
+    CHD_CHECK(sc.alloc(&d_field, nodes));
+    CHD_HIP(hipMemsetAsync(d_tested, 0, nodes, st));
+    chd_options o;
+    chd_default_options(&o);
+    CHD_HIP(launch_sgm_select(st, d_paths, d_costs, d_tested, d_labels, columns, rows, labels, o,
+                              d_out, d_field, d_fill, nullptr));
+    CHD_HIP(hipMemcpyAsync(aggregated, d_out, volume * 4, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_select_depths(chd_context *c, const double *label_depths, uint32_t labels,
+                      const float *scores, const uint8_t *tested, const float *regularised,
+                      uint32_t nodes, const chd_options *options, chd_node_depth *field,
+                      uint8_t *fillable, uint16_t *best_label) {
+    if (!c || !label_depths || !scores || !tested || !regularised || !options || !field || !fillable)
+        return fail(CHD_ERR_ARGUMENT, "select_depths: bad argument");
+    if (labels > 512) return fail(CHD_ERR_LIMIT, "select_depths: more than 512 labels");
+    CHD_HIP(hipSetDevice(c->device));
+    const size_t volume = (size_t)nodes * labels;
+    Scratch sc;
+    double *d_labels;
+    float *d_scores, *d_reg;
+    uint8_t *d_tested, *d_fill;
+    chd_node_depth *d_field;
+    uint16_t *d_best;
+    CHD_CHECK(sc.alloc(&d_labels, labels));
+    CHD_CHECK(sc.alloc(&d_scores, volume));
+    CHD_CHECK(sc.alloc(&d_reg, volume));
+    CHD_CHECK(sc.alloc(&d_tested, nodes));
+    CHD_CHECK(sc.alloc(&d_fill, nodes));
+    CHD_CHECK(sc.alloc(&d_field, nodes));
+    CHD_CHECK(sc.alloc(&d_best, nodes));
+    hipStream_t st = c->stream;
+    CHD_HIP(hipMemcpyAsync(d_labels, label_depths, labels * 8, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_scores, scores, volume * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_reg, regularised, volume * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_tested, tested, nodes, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_select_depths(st, d_reg, d_scores, d_tested, d_labels, nodes, labels, *options,
+                                 d_field, d_fill, d_best));
+    CHD_HIP(hipMemcpyAsync(field, d_field, nodes * sizeof(chd_node_depth), hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipMemcpyAsync(fillable, d_fill, nodes, hipMemcpyDeviceToHost, st));
+    if (best_label) CHD_HIP(hipMemcpyAsync(best_label, d_best, nodes * 2, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+extern "C++" {
+namespace {
+
+template <class Launch>
+int field_stage(chd_context *c, chd_node_depth *field, const float *guidance, uint32_t columns,
+                uint32_t rows, Launch &&launch) {
+    if (!c || !field || !guidance || columns == 0 || rows == 0)
+        return fail(CHD_ERR_ARGUMENT, "field stage: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)columns * rows;
+    Scratch sc;
+    chd_node_depth *src, *dst;
+    float *g;
+    uint32_t *parent, *sizes;
+    CHD_CHECK(sc.alloc(&src, n));
+    CHD_CHECK(sc.alloc(&dst, n));
+    CHD_CHECK(sc.alloc(&g, n));
+    CHD_CHECK(sc.alloc(&parent, n));
+    CHD_CHECK(sc.alloc(&sizes, n));
+    hipStream_t st = c->stream;
+    CHD_HIP(hipMemcpyAsync(src, field, n * sizeof(chd_node_depth), hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(g, guidance, n * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch(st, src, dst, g, parent, sizes));
+    CHD_HIP(hipMemcpyAsync(field, dst, n * sizeof(chd_node_depth), hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+template <class Launch>
+int decision_stage(chd_context *c, chd_node_warp *decisions, const float *guidance,
+                   uint32_t columns, uint32_t rows, Launch &&launch) {
+    if (!c || !decisions || columns == 0 || rows == 0)
+        return fail(CHD_ERR_ARGUMENT, "decision stage: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)columns * rows;
+    Scratch sc;
+    chd_node_warp *src, *dst;
+    float *g;
+    CHD_CHECK(sc.alloc(&src, n));
+    CHD_CHECK(sc.alloc(&dst, n));
+    CHD_CHECK(sc.alloc(&g, n));
+    hipStream_t st = c->stream;
+    CHD_HIP(hipMemcpyAsync(src, decisions, n * sizeof(chd_node_warp), hipMemcpyHostToDevice, st));
+    if (guidance) CHD_HIP(hipMemcpyAsync(g, guidance, n * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch(st, src, dst, g));
+    CHD_HIP(hipMemcpyAsync(decisions, dst, n * sizeof(chd_node_warp), hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int chd_reject_isolated_direct_depths(chd_context *c, chd_node_depth *field,
+                                      const float *guidance, uint32_t columns, uint32_t rows,
+                                      double tol) {
+    return field_stage(c, field, guidance, columns, rows,
+                       [&](hipStream_t st, chd_node_depth *src, chd_node_depth *dst, float *g,
+                           uint32_t *, uint32_t *) {
+                           return launch_reject_isolated(st, src, dst, g, columns, rows, tol);
+                       });
+}
+
+int chd_reject_small_direct_components(chd_context *c, chd_node_depth *field,
+                                       const float *guidance, uint32_t columns, uint32_t rows,
+                                       double tol, uint32_t minimum_nodes) {
+    return field_stage(c, field, guidance, columns, rows,
+                       [&](hipStream_t st, chd_node_depth *src, chd_node_depth *dst, float *g,
+                           uint32_t *parent, uint32_t *sizes) {
+                           return launch_reject_components(st, src, dst, g, columns, rows, tol,
+                                                           minimum_nodes, parent, sizes);
+                       });
+}
+
+int chd_fit_local_depth_planes(chd_context *c, chd_node_depth *field, const float *guidance,
+                               uint32_t columns, uint32_t rows, double tol) {
+    return field_stage(c, field, guidance, columns, rows,
+                       [&](hipStream_t st, chd_node_depth *src, chd_node_depth *dst, float *g,
+                           uint32_t *, uint32_t *) {
+                           return launch_fit_planes(st, src, dst, g, columns, rows, tol);
+                       });
+}
+
+int chd_enforce_warp_consensus(chd_context *c, chd_node_warp *decisions, uint32_t columns,
+                               uint32_t rows) {
+    return decision_stage(c, decisions, nullptr, columns, rows,
+                          [&](hipStream_t st, chd_node_warp *src, chd_node_warp *dst, float *) {
+                              return launch_consensus(st, src, dst, columns, rows);
+                          });
+}
+
+int chd_suppress_warp_boundaries(chd_context *c, chd_node_warp *decisions, const float *guidance,
+                                 uint32_t columns, uint32_t rows) {
+    if (!guidance) return fail(CHD_ERR_ARGUMENT, "suppress_warp_boundaries: null guidance");
+    return decision_stage(c, decisions, guidance, columns, rows,
+                          [&](hipStream_t st, chd_node_warp *src, chd_node_warp *dst, float *g) {
+                              return launch_boundaries(st, src, dst, g, columns, rows);
+                          });
+}
+
+int chd_device_expf(chd_context *c, const float *x, float *y, size_t n) {
+    if (!c || !x || !y) return fail(CHD_ERR_ARGUMENT, "expf: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    Scratch sc;
+    float *dx, *dy;
+    CHD_CHECK(sc.alloc(&dx, n));
+    CHD_CHECK(sc.alloc(&dy, n));
+    CHD_HIP(hipMemcpyAsync(dx, x, n * 4, hipMemcpyHostToDevice, c->stream));
+    CHD_HIP(launch_expf(c->stream, dx, dy, n));
+    CHD_HIP(hipMemcpyAsync(y, dy, n * 4, hipMemcpyDeviceToHost, c->stream));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
+}
+
+int chd_device_exp(chd_context *c, const double *x, double *y, size_t n) {
+    if (!c || !x || !y) return fail(CHD_ERR_ARGUMENT, "exp: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    Scratch sc;
+    double *dx, *dy;
+    CHD_CHECK(sc.alloc(&dx, n));
+    CHD_CHECK(sc.alloc(&dy, n));
+    CHD_HIP(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, c->stream));
+    CHD_HIP(launch_exp(c->stream, dx, dy, n));
+    CHD_HIP(hipMemcpyAsync(y, dy, n * 8, hipMemcpyDeviceToHost, c->stream));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
+}
+
+}  // extern "C"

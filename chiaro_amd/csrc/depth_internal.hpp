@@ -1,0 +1,133 @@
+// depth_internal.hpp - device data layout and kernel launchers of the depth
+// path, shared by the kernels and the C ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/chiaro_depth.h"
+
+namespace chd {
+
+// One module as the kernels see it. Pointers are device pointers.
+struct ViewDev {
+    const float *plane;   // plane_height x plane_width f32, row-major
+    uint32_t plane_width, plane_height;
+    const float2 *warp;   // global warp points (x, y), NaN = undefined
+    uint32_t warp_step, warp_columns, warp_rows;
+    const chd_camera *camera;  // nullptr = no calibration
+    uint32_t pad;
+};
+
+// ZNCC patch context shared by the cost, direct, wide and target kernels.
+struct PatchSpec {
+    const ViewDev *views;      // all views
+    const uint32_t *vlist;     // view indices scored by this launch
+    uint32_t view_count;       // entries of vlist
+    uint32_t reference;        // reference view index
+    uint32_t radius;           // patch radius (taps = (2r+1)^2)
+};
+
+// Grid of nodes at raster positions (column * step, row * step).
+struct GridSpec {
+    uint32_t columns, rows, step;
+};
+
+// Entry of the wide-fallback list of the direct pass (depth.rs:803-846).
+struct WideEntry {
+    uint32_t node;
+    uint32_t count;
+    uint32_t seeded;
+    uint32_t pad;
+    double depth[7];
+};
+
+// Kernel launchers (depth_kernels.hip). All enqueue on `stream` and return
+// the hipError_t of the launch.
+hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
+                              const double *labels, uint32_t label_count,
+                              uint32_t minimum_support, uint32_t best_view_count,
+                              float *scores, float *costs, float *guidance, uint8_t *tested);
+
+hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
+                      uint32_t columns, uint32_t rows, uint32_t labels, float *paths);
+
+hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float *scores,
+                             const uint8_t *tested, const double *labels, uint32_t columns,
+                             uint32_t rows, uint32_t label_count, const chd_options &options,
+                             float *aggregated, chd_node_depth *field, uint8_t *fillable,
+                             uint16_t *best_label);
+
+hipError_t launch_select_depths(hipStream_t stream, const float *regularised,
+                                const float *scores, const uint8_t *tested,
+                                const double *labels, uint32_t nodes, uint32_t label_count,
+                                const chd_options &options, chd_node_depth *field,
+                                uint8_t *fillable, uint16_t *best_label);
+
+hipError_t launch_complete_depth_field(hipStream_t stream, const chd_node_depth *previous,
+                                       chd_node_depth *field, const float *guidance,
+                                       const uint8_t *fillable, uint32_t columns,
+                                       uint32_t rows, uint32_t minimum_neighbour_support);
+
+hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                         const GridSpec &coarse, const chd_node_depth *coarse_field,
+                         double inverse_step, const chd_options &direct_options,
+                         chd_node_depth *field, float *guidance, uint8_t *tested,
+                         WideEntry *wide, uint32_t *wide_count);
+
+hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                              const chd_options &wide_options, const WideEntry *wide,
+                              const uint32_t *wide_count, uint32_t wide_capacity,
+                              chd_node_depth *field);
+
+hipError_t launch_reject_isolated(hipStream_t stream, const chd_node_depth *source,
+                                  chd_node_depth *field, const float *guidance,
+                                  uint32_t columns, uint32_t rows, double inverse_tolerance);
+
+hipError_t launch_reject_components(hipStream_t stream, const chd_node_depth *source,
+                                    chd_node_depth *field, const float *guidance,
+                                    uint32_t columns, uint32_t rows, double inverse_tolerance,
+                                    uint32_t minimum_nodes, uint32_t *parent, uint32_t *sizes);
+
+hipError_t launch_fit_planes(hipStream_t stream, const chd_node_depth *source,
+                             chd_node_depth *field, const float *guidance, uint32_t columns,
+                             uint32_t rows, double inverse_tolerance);
+
+hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                          const chd_node_depth *field, double inverse_step,
+                          const chd_options &options, chd_node_warp *decisions);
+
+hipError_t launch_consensus(hipStream_t stream, const chd_node_warp *source,
+                            chd_node_warp *decisions, uint32_t columns, uint32_t rows);
+
+hipError_t launch_boundaries(hipStream_t stream, const chd_node_warp *source,
+                             chd_node_warp *decisions, const float *guidance, uint32_t columns,
+                             uint32_t rows);
+
+// counts[0..3] += refined, occluded, boundary (device counters)
+hipError_t launch_target_outputs(hipStream_t stream, const chd_node_warp *decisions,
+                                 uint32_t nodes, float *points, float *confidence,
+                                 uint8_t *codes, unsigned long long *counts);
+
+// counts[0..2] = tested, measured, regularized; keys for the medians
+hipError_t launch_field_stats(hipStream_t stream, const chd_node_depth *field,
+                              const uint8_t *tested, uint32_t nodes,
+                              unsigned long long *counts, unsigned long long *depth_keys,
+                              unsigned int *improvement_keys, uint32_t *key_counts);
+
+hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
+                            const uint8_t *tested, uint32_t nodes, double *depth,
+                            float *confidence, uint8_t *provenance);
+
+hipError_t launch_fill_u32(hipStream_t stream, uint32_t *data, uint32_t value, size_t n);
+hipError_t launch_fill_u64(hipStream_t stream, unsigned long long *data,
+                           unsigned long long value, size_t n);
+
+hipError_t launch_expf(hipStream_t stream, const float *x, float *y, size_t n);
+hipError_t launch_exp(hipStream_t stream, const double *x, double *y, size_t n);
+
+// Dynamic LDS needed by the patch kernels, for a launch-size decision.
+size_t cost_lds_bytes(uint32_t nodes_per_block, uint32_t views, uint32_t taps,
+                      uint32_t label_count);
+
+}  // namespace chd

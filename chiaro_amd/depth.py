@@ -1,0 +1,282 @@
+"""Python face of the depth hot path: the reference's refine_multiview_depth
+(crates/chiaro-fusion/src/depth.rs:312-580) over the C ABI of
+libchiaro_depth.so, plus a device-resident Session for repeated runs.
+
+The arithmetic runs in the HIP kernels; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import _abi
+
+
+@dataclass
+class DepthOptions:
+    """DepthOptions and its Default (depth.rs:40-91)."""
+
+    enabled: bool = True
+    grid_step: int = 8
+    near_depth: float = 500.0
+    far_depth: float = 10_000_000.0
+    planes: int = 96
+    patch_radius: int = 4
+    minimum_support: int = 2
+    best_view_count: int = 3
+    minimum_score: float = 0.45
+    minimum_margin: float = 0.01
+    minimum_improvement: float = 0.01
+    minimum_neighbour_support: int = 2
+    completion_iterations: int = 0
+
+    def to_c(self) -> _abi.Options:
+        o = _abi.Options()
+        for f in fields(self):
+            setattr(o, f.name, int(getattr(self, f.name)) if f.name == "enabled" else getattr(self, f.name))
+        return o
+
+    def grid(self, width: int, height: int) -> _abi.GridSizes:
+        g = _abi.GridSizes()
+        _abi.check(_abi.depth_lib().chd_grid_sizes_for(C.byref(self.to_c()), width, height,
+                                                        C.byref(g)), "grid sizes")
+        return g
+
+
+@dataclass
+class DenseDepthMap:
+    """DenseDepthMap (depth.rs:133-159): NaN depth is None."""
+
+    columns: int
+    rows: int
+    step: int
+    near_depth: float
+    far_depth: float
+    depth: np.ndarray        # (rows, columns) float64
+    confidence: np.ndarray   # (rows, columns) float32
+    provenance: np.ndarray   # (rows, columns) uint8
+
+    def sample_nearest(self, x: float, y: float):
+        if x < 0 or y < 0 or self.step == 0:
+            return None
+        column = int(np.round(np.float32(x) / np.float32(self.step)))
+        row = int(np.round(np.float32(y) / np.float32(self.step)))
+        if column >= self.columns or row >= self.rows:
+            return None
+        return (self.depth[row, column], self.confidence[row, column], self.provenance[row, column])
+
+
+@dataclass
+class TargetResult:
+    points: np.ndarray       # (rows, columns, 2) float32, NaN = undefined
+    confidence: np.ndarray   # (rows, columns) float32
+    decisions: np.ndarray    # (rows, columns) uint8 NodeWarp codes
+    report: dict
+
+
+class Context:
+    """A HIP stream on one device (one per thread of use)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.depth_lib()
+        handle = C.c_void_p()
+        _abi.check(self.lib.chd_context_create(device, C.byref(handle)), "context create")
+        self.handle = handle
+
+    def synchronize(self):
+        _abi.check(self.lib.chd_context_synchronize(self.handle), "synchronize")
+
+    def close(self):
+        if self.handle:
+            self.lib.chd_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ViewArrays:
+    """Keeps the numpy buffers behind a chd_view array alive."""
+
+    def __init__(self, modules):
+        self.keep = []
+        self.cams = []
+        n = len(modules)
+        self.views = (_abi.View * n)()
+        for i, m in enumerate(modules):
+            plane = np.ascontiguousarray(m.luminance, dtype=np.float32)
+            warp = np.ascontiguousarray(m.warp_points, dtype=np.float32)
+            conf = np.where(np.isnan(warp[..., 0]), 0.0, 1.0).astype(np.float32)
+            self.keep += [plane, warp, conf]
+            v = self.views[i]
+            v.luminance = _abi.fptr(plane)
+            v.plane_height, v.plane_width = plane.shape
+            v.width, v.height = m.width, m.height
+            if m.camera is not None:
+                cam = _abi.Camera()
+                C.pointer(cam)[0] = m.camera
+                self.cams.append(cam)
+                v.camera = C.pointer(cam)
+            v.global_warp.step = m.warp_step
+            v.global_warp.rows, v.global_warp.columns = warp.shape[:2]
+            v.global_warp.points = _abi.fptr(warp)
+            v.global_warp.confidence = _abi.fptr(conf)
+            v.accepted = 1 if m.accepted else 0
+
+
+def _report_dict(r: _abi.DepthReport) -> dict:
+    out = {name: getattr(r, name) for name, _ in _abi.DepthReport._fields_}
+    out["median_depth"] = r.median_depth if r.has_median_depth else None
+    out["median_score_improvement"] = (
+        r.median_score_improvement if r.has_median_score_improvement else None)
+    return out
+
+
+class Session:
+    """Device-resident inputs and intermediates of one call (chd_session)."""
+
+    def __init__(self, context: Context, modules, reference_index: int, options: DepthOptions):
+        self.context = context
+        self.lib = context.lib
+        self.modules = modules
+        self.options = options
+        self._views = _ViewArrays(modules)
+        handle = C.c_void_p()
+        _abi.check(self.lib.chd_session_create(context.handle, self._views.views, len(modules),
+                                               reference_index, C.byref(options.to_c()),
+                                               C.byref(handle)), "session create")
+        self.handle = handle
+        g = _abi.GridSizes()
+        self.none = self.lib.chd_session_grid(handle, C.byref(g)) == _abi.CHD_NONE
+        self.grid = g
+
+    def close(self):
+        if self.handle:
+            self.lib.chd_session_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, stages: int = _abi.CHD_STAGE_ALL) -> int:
+        return _abi.check(self.lib.chd_session_run(self.handle, stages), "session run")
+
+    def enable_timing(self, on: bool = True):
+        _abi.check(self.lib.chd_session_enable_timing(self.handle, int(on)), "timing")
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        arr = (_abi.KernelTime * cap)()
+        count = C.c_uint32()
+        _abi.check(self.lib.chd_session_kernel_times(self.handle, arr, cap, C.byref(count)), "times")
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(count.value)}
+
+    def reset_kernel_times(self):
+        _abi.check(self.lib.chd_session_reset_kernel_times(self.handle), "reset times")
+
+    # --- intermediate results (parity checks)
+    def cost_volume(self):
+        g = self.grid
+        nodes = g.coarse_columns * g.coarse_rows
+        scores = np.empty((nodes, g.labels), np.float32)
+        costs = np.empty((nodes, g.labels), np.float32)
+        guidance = np.empty(nodes, np.float32)
+        tested = np.empty(nodes, np.uint8)
+        _abi.check(self.lib.chd_session_read_cost_volume(
+            self.handle, _abi.ptr(scores), _abi.ptr(costs), _abi.ptr(guidance), _abi.ptr(tested)),
+            "read cost volume")
+        return scores, costs, guidance, tested
+
+    def aggregated(self):
+        g = self.grid
+        out = np.empty((g.coarse_columns * g.coarse_rows, g.labels), np.float32)
+        _abi.check(self.lib.chd_session_read_aggregated(self.handle, _abi.ptr(out)), "read aggregated")
+        return out
+
+    def coarse(self):
+        g = self.grid
+        nodes = g.coarse_columns * g.coarse_rows
+        field = np.zeros(nodes, _abi.NODE_DTYPE)
+        fillable = np.empty(nodes, np.uint8)
+        best = np.empty(nodes, np.uint16)
+        _abi.check(self.lib.chd_session_read_coarse(self.handle, _abi.ptr(field), _abi.ptr(fillable),
+                                                    _abi.ptr(best)), "read coarse")
+        return field, fillable, best
+
+    def direct(self, stage: int):
+        g = self.grid
+        nodes = g.fine_columns * g.fine_rows
+        field = np.zeros(nodes, _abi.NODE_DTYPE)
+        guidance = np.empty(nodes, np.float32)
+        tested = np.empty(nodes, np.uint8)
+        _abi.check(self.lib.chd_session_read_direct(self.handle, stage, _abi.ptr(field),
+                                                    _abi.ptr(guidance), _abi.ptr(tested)),
+                   "read direct")
+        return field, guidance, tested
+
+    def results(self):
+        """(DenseDepthMap, {view index: TargetResult}) or (None, {})."""
+        if self.none:
+            return None, {}
+        g = self.grid
+        shape = (g.fine_rows, g.fine_columns)
+        depth = np.empty(shape, np.float64)
+        conf = np.empty(shape, np.float32)
+        prov = np.empty(shape, np.uint8)
+        m = _abi.DepthMap()
+        m.depth = depth.ctypes.data_as(C.POINTER(C.c_double))
+        m.confidence = _abi.fptr(conf)
+        m.provenance = prov.ctypes.data_as(C.POINTER(C.c_uint8))
+        n = len(self.modules)
+        targets = (_abi.TargetResult * n)()
+        keep = []
+        for t in targets:
+            pts = np.empty(shape + (2,), np.float32)
+            cf = np.empty(shape, np.float32)
+            dec = np.empty(shape, np.uint8)
+            keep.append((pts, cf, dec))
+            t.points = _abi.fptr(pts)
+            t.confidence = _abi.fptr(cf)
+            t.decisions = dec.ctypes.data_as(C.POINTER(C.c_uint8))
+        rc = _abi.check(self.lib.chd_session_read_results(self.handle, C.byref(m), targets),
+                        "read results")
+        if rc == _abi.CHD_NONE:
+            return None, {}
+        dmap = DenseDepthMap(m.columns, m.rows, m.step, m.near_depth, m.far_depth, depth, conf, prov)
+        out = {}
+        for i, t in enumerate(targets):
+            if t.replaced:
+                pts, cf, dec = keep[i]
+                out[i] = TargetResult(pts, cf, dec, _report_dict(t.report))
+        return dmap, out
+
+
+def refine_multiview_depth(modules, reference_index: int, options: DepthOptions,
+                           context: Context | None = None):
+    """refine_multiview_depth (depth.rs:312-317) on the GPU.
+
+    Returns (DenseDepthMap | None, {target index: TargetResult}); a target
+    result replaces that module's warp and report as the reference does
+    (depth.rs:526-546)."""
+    own = context is None
+    ctx = Context() if own else context
+    try:
+        session = Session(ctx, modules, reference_index, options)
+        try:
+            if session.none:
+                return None, {}
+            session.run(_abi.CHD_STAGE_ALL)
+            return session.results()
+        finally:
+            session.close()
+    finally:
+        if own:
+            ctx.close()

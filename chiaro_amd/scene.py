@@ -1,0 +1,189 @@
+"""Seeded synthetic captures for the depth path (the reference has no scene
+generator; see chiaro_amd/csrc/scene.c for the model).
+
+A Capture holds, per module, exactly what refine_multiview_depth reads
+(crates/chiaro-fusion/src/depth.rs:312-317): the half-resolution log
+luminance plane, raster size, calibrated camera, the global warp of
+align_module(refine = false) and the accepted flag.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .depth import DepthOptions
+
+
+@dataclass
+class ModuleInput:
+    name: str
+    luminance: np.ndarray          # (plane_height, plane_width) float32
+    width: int                     # raster size
+    height: int
+    camera: _abi.Camera | None
+    warp_points: np.ndarray        # (rows, columns, 2) float32, NaN = undefined
+    warp_step: int
+    accepted: bool = True
+
+
+@dataclass
+class Capture:
+    modules: list[ModuleInput]
+    reference_index: int
+    options: DepthOptions
+    description: dict = field(default_factory=dict)
+    truth_step: int = 8
+    truth_inverse_distance: np.ndarray | None = None  # 1/mm on the coarse grid
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _module(focal_px, width, height, center_mm, rotation=None) -> _abi.Module:
+    m = _abi.Module()
+    m.focal_px = focal_px
+    m.width = width
+    m.height = height
+    for i in range(3):
+        m.center[i] = center_mm[i]
+    r = np.eye(3) if rotation is None else np.asarray(rotation, dtype=np.float64)
+    for i in range(9):
+        m.rotation_wc[i] = r.reshape(-1)[i]
+    return m
+
+
+def camera_for(module: _abi.Module) -> _abi.Camera:
+    cam = _abi.Camera()
+    if _abi.scene_lib().chs_camera(C.byref(module), C.byref(cam)) != 0:
+        raise RuntimeError("singular intrinsics")
+    return cam
+
+
+def identity_warp(width: int, height: int, step: int = 32) -> np.ndarray:
+    """Warp::from_fn(width, height, step, Some) (align.rs:299-300)."""
+    columns = -(-width // step) + 1
+    rows = -(-height // step) + 1
+    ys, xs = np.meshgrid(np.arange(rows) * step, np.arange(columns) * step, indexing="ij")
+    return np.stack([xs, ys], axis=-1).astype(np.float32)
+
+
+def global_warp(reference: _abi.Camera, target: _abi.Camera, width: int, height: int,
+                step: int = 32) -> np.ndarray:
+    columns = -(-width // step) + 1
+    rows = -(-height // step) + 1
+    points = np.empty((rows, columns, 2), dtype=np.float32)
+    confidence = np.empty((rows, columns), dtype=np.float32)
+    _abi.scene_lib().chs_global_warp(C.byref(reference), C.byref(target), width, height, step,
+                                     _abi.ptr(points), _abi.ptr(confidence))
+    return points
+
+
+def _render(desc: _abi.SceneDesc, modules: list[_abi.Module], reference: int, noise: float,
+            bayer: _abi.Bayer | None):
+    n = len(modules)
+    arr = (_abi.Module * n)(*modules)
+    if bayer is None:
+        images = [np.empty((m.height, m.width), dtype=np.float32) for m in modules]
+        ptrs = (C.POINTER(C.c_float) * n)(*[_abi.fptr(a) for a in images])
+        rc = _abi.scene_lib().chs_render(C.byref(desc), arr, n, reference, noise, None, ptrs, None,
+                                         _threads())
+    else:
+        images = [np.empty((m.height, m.width), dtype=np.uint16) for m in modules]
+        ptrs = (C.POINTER(C.c_uint16) * n)(*[a.ctypes.data_as(C.POINTER(C.c_uint16)) for a in images])
+        rc = _abi.scene_lib().chs_render(C.byref(desc), arr, n, reference, 0.0, C.byref(bayer),
+                                         None, ptrs, _threads())
+    if rc != 0:
+        raise RuntimeError("scene rendering failed")
+    return images
+
+
+def luminance_half_f32(image: np.ndarray, black: float = 0.0, white: float = 1.0) -> np.ndarray:
+    h, w = image.shape
+    out = np.empty((h // 2, w // 2), dtype=np.float32)
+    _abi.scene_lib().chs_luminance_half_f32(_abi.ptr(np.ascontiguousarray(image)), w, h, black,
+                                            white, _abi.ptr(out))
+    return out
+
+
+def luminance_half_u16(raw: np.ndarray, shift: int, black: float, white: float) -> np.ndarray:
+    h, w = raw.shape
+    out = np.empty((h // 2, w // 2), dtype=np.float32)
+    _abi.scene_lib().chs_luminance_half_u16(_abi.ptr(np.ascontiguousarray(raw)), w, h, shift,
+                                            black, white, _abi.ptr(out))
+    return out
+
+
+def _assemble(modules, planes, reference, options, description, desc, names=None) -> Capture:
+    cams = [camera_for(m) for m in modules]
+    ref = modules[reference]
+    inputs = []
+    for i, (m, plane, cam) in enumerate(zip(modules, planes, cams)):
+        if i == reference:
+            warp = identity_warp(ref.width, ref.height)
+        else:
+            warp = global_warp(cams[reference], cam, ref.width, ref.height)
+        inputs.append(ModuleInput(name=(names[i] if names else f"M{i}"), luminance=plane,
+                                  width=m.width, height=m.height, camera=cam, warp_points=warp,
+                                  warp_step=32))
+    step = max(options.grid_step, 4)
+    columns = -(-ref.width // step) + 1
+    rows = -(-ref.height // step) + 1
+    truth = np.empty((rows, columns), dtype=np.float32)
+    _abi.scene_lib().chs_truth_inverse_distance(C.byref(desc), C.byref(ref), step, _abi.ptr(truth))
+    return Capture(modules=inputs, reference_index=reference, options=options,
+                   description=description, truth_step=step, truth_inverse_distance=truth)
+
+
+def config0(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float = 1600.0,
+            planes: int = 64, plane_count: int = 8) -> Capture:
+    """Three views: reference plus neighbours at 20 mm / 0 deg and 35 mm / 60
+    deg; pinhole, principal point at the image centre; single-channel f32
+    images in [0, 1] with Gaussian noise 0.01; Voronoi scene of fronto-parallel
+    planes with inverse depth U[0.05, 1.25] 1/m; 64 hypotheses, near 800 mm,
+    far 20000 mm. The image is taken as the raster with black 0 and white 1."""
+    modules = [
+        _module(focal_px, width, height, (0.0, 0.0, 0.0)),
+        _module(focal_px, width, height, (20.0, 0.0, 0.0)),
+        _module(focal_px, width, height, (35.0 * math.cos(math.radians(60)),
+                                          35.0 * math.sin(math.radians(60)), 0.0)),
+    ]
+    desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.05e-3,
+                          inverse_depth_max=1.25e-3, texture_cell_px=32)
+    images = _render(desc, modules, 0, 0.01, None)
+    planes_ = [luminance_half_f32(img) for img in images]
+    options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
+    return _assemble(modules, planes_, 0, options,
+                     {"workload": "config0", "seed": seed, "width": width, "height": height,
+                      "views": 3, "planes": planes}, desc)
+
+
+BAYER_12BIT = dict(black_level=64.0, white_level=4095.0, exposure=0.85, electrons_per_dn=0.5,
+                   read_noise_dn=2.0)
+
+
+def config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: float = 3300.0,
+            planes: int = 128, plane_count: int = 16, ring_mm: float = 30.0) -> Capture:
+    """Five modules: the reference at the centre and four on a 30 mm ring at
+    0/90/180/270 deg; f = 3300 px; 12-bit RGGB Bayer codes (black 64, Poisson
+    shot noise at 0.5 e-/DN, read noise 2 DN) converted by luminance_half with
+    codes shifted left by 4 (black 64*16, white 4095*16); 16 planes; 128
+    hypotheses, near 800 mm, far 20000 mm."""
+    modules = [_module(focal_px, width, height, (0.0, 0.0, 0.0))]
+    for k in range(4):
+        a = math.radians(90.0 * k)
+        modules.append(_module(focal_px, width, height, (ring_mm * math.cos(a), ring_mm * math.sin(a), 0.0)))
+    desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.05e-3,
+                          inverse_depth_max=1.25e-3, texture_cell_px=32)
+    bayer = _abi.Bayer(**BAYER_12BIT)
+    raws = _render(desc, modules, 0, 0.0, bayer)
+    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
+    options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
+    return _assemble(modules, planes_, 0, options,
+                     {"workload": "config1", "seed": seed, "width": width, "height": height,
+                      "views": 5, "planes": planes}, desc)

@@ -1,0 +1,341 @@
+/*
+ * chiaro_depth.h - C ABI of the MI355X (gfx950) depth hot path.
+ *
+ * The path is chiaro-fusion's calibrated multi-view depth refinement:
+ * plane-sweep ZNCC cost field -> 8-path semi-global matching -> winner-take-all
+ * with sub-label parabola -> direct remeasurement on the fine grid ->
+ * consistency filters -> per-target cross-camera decisions.
+ *
+ * Every entry point is plain C: pointers, sizes and POD structs, no C++ or
+ * torch types. Errors never cross the boundary as exceptions: functions return
+ * CHD_OK (0), CHD_NONE (1, the reference's `None`), or a negative CHD_ERR_*
+ * code, and chd_last_error() holds a message for the calling thread.
+ *
+ * Reference interface (paths relative to the chiaro repository):
+ *   refine_multiview_depth  crates/chiaro-fusion/src/depth.rs:312-317
+ *   DepthOptions            crates/chiaro-fusion/src/depth.rs:40-91
+ *   DepthAlignmentReport    crates/chiaro-fusion/src/depth.rs:93-113
+ *   DenseDepthMap / Node    crates/chiaro-fusion/src/depth.rs:115-141
+ *   AlignInput              crates/chiaro-fusion/src/align.rs:247-257
+ *   Warp                    crates/chiaro-fusion/src/align.rs:35-45
+ *   ResolvedCamera          crates/chiaro-fusion/src/geometry.rs:31-62
+ */
+#ifndef CHIARO_DEPTH_H
+#define CHIARO_DEPTH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHD_ABI_VERSION 1
+
+#define CHD_OK 0
+#define CHD_NONE 1
+#define CHD_ERR_ARGUMENT (-1)
+#define CHD_ERR_HIP (-2)
+#define CHD_ERR_NO_DEVICE (-3)
+#define CHD_ERR_LIMIT (-4)
+#define CHD_ERR_STATE (-5)
+
+/* Largest supported patch radius of the ZNCC kernels (the direct pass uses
+ * max(2 r, 8), so r <= 8 keeps every pass within this bound). */
+#define CHD_MAX_PATCH_RADIUS 16
+/* Largest number of views (reference included) in one call. */
+#define CHD_MAX_VIEWS 32
+
+/* DepthOptions (depth.rs:40-67); defaults from chd_default_options
+ * (depth.rs:69-91). */
+typedef struct chd_options {
+    int32_t enabled;
+    uint32_t grid_step;
+    double near_depth;
+    double far_depth;
+    uint32_t planes;
+    uint32_t patch_radius;
+    uint32_t minimum_support;
+    uint32_t best_view_count;
+    float minimum_score;
+    float minimum_margin;
+    float minimum_improvement;
+    uint32_t minimum_neighbour_support;
+    uint32_t completion_iterations;
+} chd_options;
+
+/* Brown distortion k1, k2, p1, p2, k3 (calibration.rs PolynomialDistortion;
+ * geometry.rs:303-370). Missing coefficients count as 0. */
+typedef struct chd_distortion {
+    double center[2];
+    double normalization[2];
+    double coeffs[5];
+    uint32_t coeff_count;
+} chd_distortion;
+
+/* Plain-data form of ResolvedCamera (geometry.rs:31-62). Matrices are
+ * row-major m[row * 3 + column]. pose_kind 0 = Canonical
+ * (rotation_wc/rotation_cw/translation_wc/center), 1 = Mirror
+ * (real_cw/reflect/virtual_center). flip_around_x: -1 = None, 0 = Some(false),
+ * 1 = Some(true). */
+typedef struct chd_camera {
+    uint32_t width;
+    uint32_t height;
+    double k[9];
+    double k_inverse[9];
+    int32_t has_distortion;
+    chd_distortion distortion;
+    int32_t flip_around_x;
+    int32_t pose_kind;
+    double rotation_wc[9];
+    double rotation_cw[9];
+    double translation_wc[3];
+    double center[3];
+    double real_cw[9];
+    double reflect[9];
+    double virtual_center[3];
+    double orientation_correction[9];
+    double focal_px;
+} chd_camera;
+
+/* Warp (align.rs:35-45): columns*rows target points, (x, y) interleaved,
+ * NaN where the mapping is undefined, and one confidence per node. */
+typedef struct chd_warp {
+    uint32_t step;
+    uint32_t columns;
+    uint32_t rows;
+    float *points;
+    float *confidence;
+} chd_warp;
+
+/* One module: AlignInput (align.rs:247-257) plus the fields of its
+ * ModuleAlignment that the path reads (warp, report.accepted). */
+typedef struct chd_view {
+    const float *luminance; /* plane_height rows of plane_width f32 */
+    uint32_t plane_width;
+    uint32_t plane_height;
+    uint32_t width; /* raster size */
+    uint32_t height;
+    const chd_camera *camera; /* NULL = no calibration (None) */
+    chd_warp global_warp;     /* alignments[i].warp on entry */
+    int32_t accepted;         /* alignments[i].report.accepted */
+} chd_view;
+
+/* DepthAlignmentReport (depth.rs:93-113). */
+typedef struct chd_depth_report {
+    uint64_t tested_nodes;
+    uint64_t measured_nodes;
+    uint64_t regularized_nodes;
+    uint64_t fallback_nodes;
+    uint64_t refined_nodes;
+    uint64_t occluded_nodes;
+    uint64_t boundary_nodes;
+    float reconstructed_fraction;
+    float refined_fraction;
+    float occluded_fraction;
+    int32_t has_median_depth;
+    double median_depth;
+    int32_t has_median_score_improvement;
+    float median_score_improvement;
+} chd_depth_report;
+
+/* NodeWarp codes (depth.rs:256-271). */
+#define CHD_WARP_UNDEFINED 0
+#define CHD_WARP_GLOBAL 1
+#define CHD_WARP_REFINED 2
+#define CHD_WARP_OCCLUDED 3
+#define CHD_WARP_BOUNDARY 4
+
+/* Per-target result: the refined warp that replaces alignments[i].warp
+ * (depth.rs:526-546). Buffers are caller-allocated on the fine grid. */
+typedef struct chd_target_result {
+    int32_t replaced;   /* out: 1 when the warp and report were produced */
+    float *points;      /* fine columns*rows*2 */
+    float *confidence;  /* fine columns*rows */
+    uint8_t *decisions; /* optional: NodeWarp code per fine node */
+    chd_depth_report report;
+} chd_target_result;
+
+/* DepthProvenance (depth.rs:115-121). */
+#define CHD_PROVENANCE_UNSUPPORTED 0
+#define CHD_PROVENANCE_GLOBAL 1
+#define CHD_PROVENANCE_MEASURED 2
+#define CHD_PROVENANCE_REGULARIZED 3
+
+/* DenseDepthMap (depth.rs:133-141) with caller-allocated node arrays. */
+typedef struct chd_depth_map {
+    uint32_t columns;
+    uint32_t rows;
+    uint32_t step;
+    double near_depth;
+    double far_depth;
+    double *depth;        /* NaN = None */
+    float *confidence;
+    uint8_t *provenance;  /* CHD_PROVENANCE_* */
+} chd_depth_map;
+
+/* Option<NodeDepth> (depth.rs:248-254); valid = 0 is None. */
+typedef struct chd_node_depth {
+    double depth;
+    float confidence;
+    float improvement;
+    uint8_t valid;
+    uint8_t regularized;
+    uint8_t pad[6];
+} chd_node_depth;
+
+/* Grid shapes of one call (depth.rs:326-328, 726-728, 1196-1202). */
+typedef struct chd_grid_sizes {
+    uint32_t coarse_step;
+    uint32_t coarse_columns;
+    uint32_t coarse_rows;
+    uint32_t labels;
+    uint32_t fine_step;
+    uint32_t fine_columns;
+    uint32_t fine_rows;
+} chd_grid_sizes;
+
+/* ---------------------------------------------------------------- host */
+
+const char *chd_last_error(void);
+int chd_abi_version(void);
+
+/* DepthOptions::default (depth.rs:69-91). */
+void chd_default_options(chd_options *out);
+
+/* valid_options (depth.rs:1879-1891): 1 when valid. */
+int chd_options_valid(const chd_options *options);
+
+/* Grid shapes for a reference raster of width x height. */
+int chd_grid_sizes_for(const chd_options *options, uint32_t width, uint32_t height,
+                       chd_grid_sizes *out);
+
+/* ResolvedCamera::new for a canonical pose and a single intrinsics bundle
+ * (geometry.rs:64-133, calibration.rs:186-198): k_inverse = inverse(k),
+ * rotation_cw = transpose(rotation_wc), center = -(rotation_cw t).
+ * distortion may be NULL. */
+int chd_camera_canonical(const double k[9], const double rotation_wc[9],
+                         const double translation_wc[3], uint32_t width,
+                         uint32_t height, const chd_distortion *distortion,
+                         chd_camera *out);
+
+/* ------------------------------------------------------------- device */
+
+typedef struct chd_context chd_context;
+typedef struct chd_session chd_session;
+
+/* One context per thread of use; it owns a HIP stream on `device`. */
+int chd_context_create(int device, chd_context **out);
+void chd_context_destroy(chd_context *context);
+int chd_context_synchronize(chd_context *context);
+
+/* Drop-in for refine_multiview_depth (depth.rs:312-580). Inputs are host
+ * buffers; outputs are written to caller-allocated host buffers sized by
+ * chd_grid_sizes_for. `targets` has `count` entries (entry i belongs to
+ * view i). Returns CHD_NONE where the reference returns None. */
+int chd_refine_multiview_depth(chd_context *context, const chd_view *views,
+                               uint32_t count, uint32_t reference_index,
+                               const chd_options *options, chd_depth_map *map,
+                               chd_target_result *targets);
+
+/* Device-resident form of the same call: create uploads every input to HBM
+ * once; run enqueues the whole path on the context's stream (asynchronous);
+ * read_* copy results back after a synchronize. */
+int chd_session_create(chd_context *context, const chd_view *views, uint32_t count,
+                       uint32_t reference_index, const chd_options *options,
+                       chd_session **out);
+void chd_session_destroy(chd_session *session);
+
+#define CHD_STAGE_COST 1u     /* build_cost_volume, depth.rs:1187-1277 */
+#define CHD_STAGE_SGM 2u      /* semi_global_costs + select_depths, :1279-1467 */
+#define CHD_STAGE_DIRECT 4u   /* measure_direct_depths + filters, :715-1060 */
+#define CHD_STAGE_TARGETS 8u  /* per-target decisions, :420-547 */
+#define CHD_STAGE_ALL 15u
+/* Keep the aggregated SGM volume (debug/parity only; costs HBM traffic). */
+#define CHD_STAGE_KEEP_AGGREGATED 256u
+
+int chd_session_run(chd_session *session, uint32_t stages);
+int chd_session_grid(const chd_session *session, chd_grid_sizes *out);
+int chd_session_read_results(chd_session *session, chd_depth_map *map,
+                             chd_target_result *targets);
+/* Cost volume, label-contiguous [node][label] (depth.rs:1270-1276). NaN
+ * scores are the reference's f32::NAN for unsupported labels. */
+int chd_session_read_cost_volume(chd_session *session, float *scores, float *costs,
+                                 float *guidance, uint8_t *tested);
+int chd_session_read_aggregated(chd_session *session, float *aggregated);
+/* select_depths output (depth.rs:1385-1467) plus the WTA label index (0 when
+ * the node was not tested). */
+int chd_session_read_coarse(chd_session *session, chd_node_depth *field,
+                            uint8_t *fillable, uint16_t *best_label);
+/* Fine-grid field after measure_direct_depths (stage 0), after
+ * reject_isolated_direct_depths (1), after reject_small_direct_components (2)
+ * and after fit_local_depth_planes (3); guidance and tested of the fine grid. */
+int chd_session_read_direct(chd_session *session, uint32_t stage, chd_node_depth *field,
+                            float *guidance, uint8_t *tested);
+
+typedef struct chd_kernel_time {
+    char name[48];
+    uint32_t launches;
+    double total_ms;
+} chd_kernel_time;
+/* Enable per-kernel HIP-event timing on the session's stream. */
+int chd_session_enable_timing(chd_session *session, int enable);
+int chd_session_kernel_times(chd_session *session, chd_kernel_time *out,
+                             uint32_t capacity, uint32_t *count);
+int chd_session_reset_kernel_times(chd_session *session);
+
+/* Stage entry points on host arrays, for parity checks of one stage. */
+
+/* semi_global_costs (depth.rs:1279-1311). */
+int chd_semi_global_costs(chd_context *context, const float *costs,
+                          const float *guidance, uint32_t columns, uint32_t rows,
+                          uint32_t labels, float *aggregated);
+
+/* select_depths (depth.rs:1385-1467). label_depths[0] is ignored (label 0
+ * is the global warp); scores/regularised are [node][label]. */
+int chd_select_depths(chd_context *context, const double *label_depths,
+                      uint32_t labels, const float *scores, const uint8_t *tested,
+                      const float *regularised, uint32_t nodes,
+                      const chd_options *options, chd_node_depth *field,
+                      uint8_t *fillable, uint16_t *best_label);
+
+/* reject_isolated_direct_depths (depth.rs:881-924),
+ * reject_small_direct_components (:933-989), fit_local_depth_planes
+ * (:995-1060), each in place on a columns x rows field. */
+int chd_reject_isolated_direct_depths(chd_context *context, chd_node_depth *field,
+                                      const float *guidance, uint32_t columns,
+                                      uint32_t rows, double inverse_tolerance);
+int chd_reject_small_direct_components(chd_context *context, chd_node_depth *field,
+                                       const float *guidance, uint32_t columns,
+                                       uint32_t rows, double inverse_tolerance,
+                                       uint32_t minimum_nodes);
+int chd_fit_local_depth_planes(chd_context *context, chd_node_depth *field,
+                               const float *guidance, uint32_t columns, uint32_t rows,
+                               double inverse_tolerance);
+
+/* NodeWarp decision record used by the consensus and boundary stages. */
+typedef struct chd_node_warp {
+    uint32_t code; /* CHD_WARP_* */
+    float global[2];
+    float point[2];
+    float confidence;
+    uint32_t measured;
+} chd_node_warp;
+
+/* enforce_warp_consensus (depth.rs:582-618) and suppress_warp_boundaries
+ * (:620-672), in place. */
+int chd_enforce_warp_consensus(chd_context *context, chd_node_warp *decisions,
+                               uint32_t columns, uint32_t rows);
+int chd_suppress_warp_boundaries(chd_context *context, chd_node_warp *decisions,
+                                 const float *guidance, uint32_t columns, uint32_t rows);
+
+/* Device f32 exp / f64 exp used by the kernels, evaluated on the GPU for
+ * n inputs (their parity with the platform libm is checked by the tests). */
+int chd_device_expf(chd_context *context, const float *x, float *y, size_t n);
+int chd_device_exp(chd_context *context, const double *x, double *y, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHIARO_DEPTH_H */

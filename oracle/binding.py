@@ -1,0 +1,172 @@
+"""ctypes binding of the CPU oracle (liboracle_depth.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, as the checker and as the CPU baseline. The
+struct types are the C ABI's (include/chiaro_depth.h), shared with the
+product's bindings.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from chiaro_amd import _abi
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_depth.so"
+P = C.POINTER
+VP = C.c_void_p
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+class Debug(C.Structure):
+    _fields_ = [
+        ("scores", VP), ("costs", VP), ("guidance", VP), ("tested", VP), ("aggregated", VP),
+        ("coarse", VP), ("fillable", VP), ("best_label", VP), ("direct", VP * 4),
+        ("fine_guidance", VP), ("fine_tested", VP), ("seconds", C.c_double * 5),
+    ]
+
+
+SIGNATURES = {
+    "or_expf": (C.c_float, C.c_float),
+    "or_exp": (C.c_double, C.c_double),
+    "or_libm_expf": (C.c_float, C.c_float),
+    "or_libm_exp": (C.c_double, C.c_double),
+    "or_set_libm_mode": (None, C.c_int),
+    "or_expf_mismatches": (C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32),
+    "or_inverse3": (C.c_int, VP, VP),
+    "or_determinant3": (C.c_double, VP),
+    "or_camera_canonical": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, P(_abi.Distortion),
+                            P(_abi.Camera)),
+    "or_distort": (None, P(_abi.Distortion), VP, VP),
+    "or_undistort": (None, P(_abi.Distortion), VP, VP),
+    "or_pixel_to_ray": (None, P(_abi.Camera), VP, VP, VP),
+    "or_project": (C.c_int, P(_abi.Camera), VP, C.c_int, VP),
+    "or_map_from": (C.c_int, P(_abi.Camera), P(_abi.Camera), VP, C.c_double, VP),
+    "or_plane_sample": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_float, P(C.c_float)),
+    "or_warp_map": (C.c_int, P(_abi.Warp), C.c_float, C.c_float, VP),
+    "or_global_warp": (C.c_int, P(_abi.Camera), P(_abi.Camera), C.c_uint32, C.c_uint32,
+                       C.c_uint32, P(_abi.Warp)),
+    "or_options_valid": (C.c_int, P(_abi.Options)),
+    "or_default_options": (None, P(_abi.Options)),
+    "or_inverse_depth_samples": (None, C.c_double, C.c_double, C.c_uint32, VP),
+    "or_sublabel_depth": (C.c_double, VP, C.c_uint32, VP, C.c_uint32),
+    "or_aggregate": (C.c_int, VP, C.c_uint32, P(_abi.Options), P(C.c_float)),
+    "or_warped_patch_zncc": (C.c_int, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_uint32,
+                             P(_abi.Warp), C.c_double, C.c_double, C.c_double, C.c_double,
+                             C.c_uint32, P(C.c_float)),
+    "or_semi_global_costs": (None, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),
+    "or_diagonal_paths": (C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, VP, VP),
+    "or_select_depths": (None, VP, C.c_uint32, VP, VP, VP, C.c_uint32, P(_abi.Options), VP, VP, VP),
+    "or_complete_depth_field": (None, VP, VP, VP, C.c_uint32, C.c_uint32, P(_abi.Options)),
+    "or_direct_depth_candidates": (C.c_uint32, C.c_int, C.c_double, C.c_double, P(_abi.Options), VP),
+    "or_select_direct_depth": (C.c_int, VP, VP, VP, C.c_uint32, C.c_int, C.c_float, C.c_int,
+                               P(_abi.Options), P(_abi.NodeDepth)),
+    "or_best_depth_candidate": (C.c_int, VP, VP, VP, C.c_uint32, P(_abi.Options)),
+    "or_reject_isolated_direct_depths": (None, VP, VP, C.c_uint32, C.c_uint32, C.c_double),
+    "or_reject_small_direct_components": (None, VP, VP, C.c_uint32, C.c_uint32, C.c_double,
+                                          C.c_uint32),
+    "or_fit_local_depth_planes": (None, VP, VP, C.c_uint32, C.c_uint32, C.c_double),
+    "or_warp_decisions_agree": (C.c_int, C.c_uint32, VP, VP),
+    "or_enforce_warp_consensus": (None, VP, C.c_uint32, C.c_uint32),
+    "or_suppress_warp_boundaries": (None, VP, VP, C.c_uint32, C.c_uint32),
+    "or_diagnostic_samples": (None, P(_abi.DepthMap), VP, VP),
+    "or_visualization_samples": (None, P(_abi.DepthMap), VP),
+    "or_refine_multiview_depth": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options),
+                                  P(_abi.DepthMap), P(_abi.TargetResult), P(Debug), C.c_uint32,
+                                  C.c_int),
+    "or_cost_volume_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options),
+                            C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP, VP),
+}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        handle = C.CDLL(str(LIB_PATH))
+        for name, sig in SIGNATURES.items():
+            f = getattr(handle, name)
+            f.restype = sig[0]
+            f.argtypes = list(sig[1:])
+        _lib = handle
+    return _lib
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def refine(capture, threads: int = 0, stop_after_sgm: bool = False, libm: bool = False):
+    """or_refine_multiview_depth on a chiaro_amd.scene.Capture.
+
+    Returns (map dict | None, {target: dict}, debug dict)."""
+    from chiaro_amd.depth import _ViewArrays
+
+    L = lib()
+    L.or_set_libm_mode(1 if libm else 0)
+    mods = capture.modules
+    ref = mods[capture.reference_index]
+    grid = capture.options.grid(ref.width, ref.height)
+    coarse_nodes = grid.coarse_columns * grid.coarse_rows
+    fine_nodes = grid.fine_columns * grid.fine_rows
+    vol = (coarse_nodes, grid.labels)
+    dbg = {
+        "scores": np.empty(vol, np.float32), "costs": np.empty(vol, np.float32),
+        "guidance": np.empty(coarse_nodes, np.float32), "tested": np.empty(coarse_nodes, np.uint8),
+        "aggregated": np.empty(vol, np.float32), "coarse": np.zeros(coarse_nodes, _abi.NODE_DTYPE),
+        "fillable": np.empty(coarse_nodes, np.uint8), "best_label": np.empty(coarse_nodes, np.uint16),
+        "direct": [np.zeros(fine_nodes, _abi.NODE_DTYPE) for _ in range(4)],
+        "fine_guidance": np.empty(fine_nodes, np.float32), "fine_tested": np.empty(fine_nodes, np.uint8),
+    }
+    d = Debug()
+    for k in ("scores", "costs", "guidance", "tested", "aggregated", "coarse", "fillable",
+              "best_label", "fine_guidance", "fine_tested"):
+        setattr(d, k, dbg[k].ctypes.data)
+    for i in range(4):
+        d.direct[i] = dbg["direct"][i].ctypes.data
+    shape = (grid.fine_rows, grid.fine_columns)
+    depth = np.empty(shape, np.float64)
+    conf = np.empty(shape, np.float32)
+    prov = np.empty(shape, np.uint8)
+    m = _abi.DepthMap()
+    m.depth = depth.ctypes.data_as(P(C.c_double))
+    m.confidence = conf.ctypes.data_as(P(C.c_float))
+    m.provenance = prov.ctypes.data_as(P(C.c_uint8))
+    n = len(mods)
+    targets = (_abi.TargetResult * n)()
+    keep = []
+    for t in targets:
+        pts = np.empty(shape + (2,), np.float32)
+        cf = np.empty(shape, np.float32)
+        dec = np.empty(shape, np.uint8)
+        keep.append((pts, cf, dec))
+        t.points = pts.ctypes.data_as(P(C.c_float))
+        t.confidence = cf.ctypes.data_as(P(C.c_float))
+        t.decisions = dec.ctypes.data_as(P(C.c_uint8))
+    views = _ViewArrays(mods)
+    rc = L.or_refine_multiview_depth(views.views, n, capture.reference_index,
+                                     C.byref(capture.options.to_c()), C.byref(m), targets,
+                                     C.byref(d), threads, 1 if stop_after_sgm else 0)
+    dbg["seconds"] = list(d.seconds)
+    if rc == _abi.CHD_NONE:
+        return None, {}, dbg
+    out = {}
+    for i, t in enumerate(targets):
+        if t.replaced:
+            pts, cf, dec = keep[i]
+            rep = {name: getattr(t.report, name) for name, _ in _abi.DepthReport._fields_}
+            out[i] = {"points": pts, "confidence": cf, "decisions": dec, "report": rep}
+    dmap = {"depth": depth, "confidence": conf, "provenance": prov, "columns": m.columns,
+            "rows": m.rows, "step": m.step}
+    return dmap, out, dbg

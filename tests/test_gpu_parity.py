@@ -1,0 +1,139 @@
+"""GPU path against the CPU oracle on the same seeded captures: every
+integer output (labels, flags, provenance, decision codes) and every float
+output (scores, costs, aggregated SGM volume, depths, confidences, warp
+points) must agree bit for bit."""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from chiaro_amd import _abi, scene
+from chiaro_amd.depth import DepthOptions, Session
+from oracle import binding
+
+from parity import assert_bitwise, assert_nodes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _small_config0(seed, scale=0.25, **kw):
+    w, h = int(1040 * scale) // 8 * 8, int(780 * scale) // 8 * 8
+    return scene.config0(seed=seed, width=w, height=h, focal_px=1600.0 * scale, **kw)
+
+
+def _full_parity(ctx, cap, label):
+    _say(f"parity case {label}: {cap.description}")
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    try:
+        sess.run(_abi.CHD_STAGE_ALL | _abi.CHD_STAGE_KEEP_AGGREGATED)
+        ctx.synchronize()
+        dmap, targets = sess.results()
+        scores, costs, guidance, tested = sess.cost_volume()
+        aggregated = sess.aggregated()
+        coarse, fillable, best = sess.coarse()
+        direct = [sess.direct(k) for k in range(4)]
+    finally:
+        sess.close()
+    omap, otargets, dbg = binding.refine(cap, threads=8)
+    assert_bitwise("scores", scores, dbg["scores"])
+    assert_bitwise("costs", costs, dbg["costs"])
+    assert_bitwise("guidance", guidance, dbg["guidance"])
+    assert_bitwise("tested", tested, dbg["tested"])
+    assert_bitwise("aggregated", aggregated, dbg["aggregated"])
+    assert_nodes("coarse", coarse, dbg["coarse"])
+    assert_bitwise("fillable", fillable, dbg["fillable"])
+    assert_bitwise("best_label", best, dbg["best_label"])
+    for k in range(4):
+        assert_nodes(f"direct[{k}]", direct[k][0], dbg["direct"][k])
+    assert_bitwise("fine_guidance", direct[0][1], dbg["fine_guidance"])
+    assert_bitwise("fine_tested", direct[0][2], dbg["fine_tested"])
+    assert_bitwise("map.depth", dmap.depth, omap["depth"])
+    assert_bitwise("map.confidence", dmap.confidence, omap["confidence"])
+    assert_bitwise("map.provenance", dmap.provenance, omap["provenance"])
+    assert sorted(targets) == sorted(otargets)
+    for t in targets:
+        assert_bitwise(f"target{t}.decisions", targets[t].decisions, otargets[t]["decisions"])
+        assert_bitwise(f"target{t}.points", targets[t].points, otargets[t]["points"])
+        assert_bitwise(f"target{t}.confidence", targets[t].confidence, otargets[t]["confidence"])
+        for key, want in otargets[t]["report"].items():
+            got = targets[t].report[key]
+            if key in ("median_depth", "median_score_improvement"):
+                if not otargets[t]["report"]["has_" + key]:
+                    assert got is None
+                    continue
+            assert (got == want) or (isinstance(want, float) and math.isnan(want) and math.isnan(got)), \
+                f"target {t} report {key}: {got} != {want}"
+    return dbg
+
+
+def test_device_expf_matches_oracle_and_libm(gpu_context, oracle_lib):
+    _say("expf parity: 2^22 strided inputs")
+    bits_ = np.arange(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
+    x = bits_.view(np.float32).copy()
+    y = np.empty_like(x)
+    _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(x), _abi.ptr(y), x.size), "expf")
+    want = np.array([oracle_lib.or_libm_expf(float(v)) for v in x[::97]], dtype=np.float32)
+    assert_bitwise("expf vs libm", y[::97], want)
+    dom = np.linspace(-12.0, 0.0, 200001, dtype=np.float32)
+    yd = np.empty_like(dom)
+    _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(dom), _abi.ptr(yd), dom.size), "expf")
+    want = np.array([oracle_lib.or_libm_expf(float(v)) for v in dom], dtype=np.float32)
+    assert_bitwise("expf domain", yd, want)
+
+
+def test_device_exp_matches_libm(gpu_context, oracle_lib):
+    _say("exp parity on [-1.5, 0]")
+    rng = np.random.default_rng(3)
+    x = -1.5 * rng.random(100000)
+    x = np.concatenate([x, -2.5 * rng.random(20000).astype(np.float32).astype(np.float64),
+                        -np.array([0, 1, 2, 4, 5, 8], dtype=np.float64) / 6.0])
+    y = np.empty_like(x)
+    _abi.check(gpu_context.lib.chd_device_exp(gpu_context.handle, _abi.ptr(x), _abi.ptr(y), x.size), "exp")
+    want = np.array([oracle_lib.or_libm_exp(float(v)) for v in x])
+    assert_bitwise("exp", y, want)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_config0_quarter_bit_exact(gpu_context, seed):
+    cap = _small_config0(seed)
+    _full_parity(gpu_context, cap, f"config0 quarter seed {seed}")
+
+
+def test_config0_half_bit_exact(gpu_context):
+    cap = _small_config0(7, scale=0.5)
+    _full_parity(gpu_context, cap, "config0 half seed 7")
+
+
+def test_config1_small_bit_exact(gpu_context):
+    cap = scene.config1(seed=2, width=416, height=312, focal_px=330.0, planes=32)
+    _full_parity(gpu_context, cap, "config1 416x312 32 planes")
+
+
+def test_completion_and_options_bit_exact(gpu_context):
+    cap = _small_config0(4)
+    cap.options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=40, patch_radius=3,
+                               completion_iterations=3, best_view_count=2, grid_step=12)
+    _full_parity(gpu_context, cap, "config0 options variant")
+
+
+def test_sgm_stage_on_oracle_costs(gpu_context, oracle_lib):
+    _say("sgm stage on random costs")
+    rng = np.random.default_rng(11)
+    for columns, rows, labels in [(5, 1, 4), (17, 9, 65), (31, 23, 129), (7, 40, 257)]:
+        costs = rng.random((rows * columns, labels), dtype=np.float32) * 2.25
+        guidance = rng.random(rows * columns, dtype=np.float32) * 3
+        got = np.empty_like(costs)
+        _abi.check(gpu_context.lib.chd_semi_global_costs(gpu_context.handle, _abi.ptr(costs),
+                                                         _abi.ptr(guidance), columns, rows, labels,
+                                                         _abi.ptr(got)), "sgm")
+        want = np.empty_like(costs)
+        oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), columns, rows, labels,
+                                        _abi.ptr(want))
+        assert_bitwise(f"sgm {columns}x{rows}x{labels}", got, want)
