@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the depth hot path on one MI355X.
+
+Workload (default): 5 modules of 4160x3120 at f = 3300 px, the reference at
+the centre of a 30 mm ring of four, 12-bit RGGB Bayer captures of a seeded
+16-plane scene converted by luminance_half, 128 inverse-depth hypotheses
+(near 800 mm, far 20000 mm), 8-path SGM, default options otherwise.
+
+A step is one cost-field + SGM + winner-take-all pass over the capture with
+every input already resident in HBM (the metric is "Mpix*hyp/s cost+SGM").
+value = reference raster pixels x planes / step time / 1e6. The full path
+(direct pass, filters and per-target decisions added) is timed as well and
+reported beside it.
+
+Prints one JSON line. --gpus above 1 is rejected: this build measures one
+GPU (the path shards by capture).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("Mpix·hyp/s cost+SGM on 1 GPU; learned-band speedup at ≥98% "
+          "±1-hyp agreement")
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Algorithmic work per (node, label, valid view) ZNCC of the cost volume:
+# 81 taps x (3 bilinear samples' worth of f32 arithmetic + 2 weights) in
+# f32 and the tap coordinates in f64 (the per-unit figures of the
+# measurement plan: 1,620 FP32 + 540 FP64 flops).
+FP32_PER_ZNCC = 1620
+FP64_PER_ZNCC = 540
+# HBM bytes per (node, label) of cost + SGM + WTA: 8 B cost-volume write,
+# 8 paths x 12 B, 4 B WTA read.
+BYTES_PER_UNIT = 108
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(cap, rows: int, threads: int) -> dict:
+    """The oracle's cost volume over a band of coarse rows with the
+    reference's row-band threading, plus its single-threaded SGM over the
+    same band, scaled to the whole capture."""
+    import ctypes as C
+
+    import numpy as np
+
+    from chiaro_amd.depth import _ViewArrays
+    from oracle import binding
+
+    lib = binding.lib()
+    lib.or_set_libm_mode(1)  # the platform libm, as the reference calls it
+    ref = cap.modules[cap.reference_index]
+    g = cap.options.grid(ref.width, ref.height)
+    rows = min(rows, g.coarse_rows)
+    first = (g.coarse_rows - rows) // 2
+    nodes = rows * g.coarse_columns
+    scores = np.empty((nodes, g.labels), np.float32)
+    costs = np.empty((nodes, g.labels), np.float32)
+    guidance = np.empty(nodes, np.float32)
+    tested = np.empty(nodes, np.uint8)
+    views = _ViewArrays(cap.modules)
+    t0 = time.perf_counter()
+    lib.or_cost_volume_rows(views.views, len(cap.modules), cap.reference_index,
+                            C.byref(cap.options.to_c()), first, first + rows, threads,
+                            scores.ctypes.data, costs.ctypes.data, guidance.ctypes.data,
+                            tested.ctypes.data)
+    t1 = time.perf_counter()
+    out = np.empty_like(costs)
+    lib.or_semi_global_costs(costs.ctypes.data, guidance.ctypes.data, g.coarse_columns, rows,
+                             g.labels, out.ctypes.data)
+    t2 = time.perf_counter()
+    scale = g.coarse_rows / rows
+    total = (t2 - t0) * scale
+    pixels = ref.width * ref.height
+    return {
+        "value": pixels * cap.options.planes / total / 1e6,
+        "unit": "Mpix·hyp/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle cost volume over coarse rows {first}..{first + rows} of {g.coarse_rows} "
+                   f"({threads} threads, row bands) + its serial SGM over that band "
+                   f"({t1 - t0:.2f} s + {t2 - t1:.2f} s), scaled by {scale:.2f} to the capture; "
+                   f"host CPU: {cpu_model()}"),
+        "seconds_sample": t2 - t0,
+        "seconds_cost_sample": t1 - t0,
+        "seconds_sgm_sample": t2 - t1,
+        "seconds_estimated_full": total,
+    }
+
+
+def load_traffic(kernel: str, workload: str):
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        data = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    entry = data.get(workload, {}).get(kernel)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["config1", "config0"], default="config1")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--full-steps", type=int, default=2,
+                    help="steps of the whole path (direct pass and targets) timed beside")
+    ap.add_argument("--cpu-rows", type=int, default=64,
+                    help="coarse rows of the CPU-baseline sample (0 skips it)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB of device memory once (calibrates rocprofv3 byte counters)")
+    args = ap.parse_args()
+
+    if args.copy_probe > 0:
+        import torch
+        n = int(args.copy_probe * (1 << 30)) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": n * 4}))
+        return 0
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(captures are independent, so the path shards by capture across GPUs)",
+              file=sys.stderr)
+        return 2
+
+    from chiaro_amd import _abi, scene
+    from chiaro_amd.depth import Context, Session
+
+    t = time.perf_counter()
+    cap = scene.config1(seed=args.seed) if args.workload == "config1" else scene.config0(seed=args.seed)
+    log(f"[bench] {args.workload} capture generated in {time.perf_counter() - t:.1f} s")
+    ref = cap.modules[cap.reference_index]
+    ctx = Context(0)
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    g = sess.grid
+    units = g.coarse_columns * g.coarse_rows * g.labels
+    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+
+    for _ in range(args.warmup):
+        sess.run(stages)
+    ctx.synchronize()
+    import torch  # plumbing only: device-wide synchronisation around the timed region
+    torch.cuda.synchronize()
+    sess.reset_kernel_times()
+    sess.enable_timing(True)
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sess.run(stages)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sess.enable_timing(False)
+    kernels = sess.kernel_times()
+    work = sess.work()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    pixels = ref.width * ref.height
+    value = pixels * cap.options.planes / (ms_per_step / 1000.0) / 1e6
+
+    # the whole path, timed beside the headline step
+    full_ms = None
+    full_kernels = {}
+    if args.full_steps > 0:
+        sess.run(_abi.CHD_STAGE_ALL)
+        ctx.synchronize()
+        sess.reset_kernel_times()
+        sess.enable_timing(True)
+        f0 = time.perf_counter()
+        for _ in range(args.full_steps):
+            sess.run(_abi.CHD_STAGE_ALL)
+        ctx.synchronize()
+        full_ms = (time.perf_counter() - f0) * 1000.0 / args.full_steps
+        sess.enable_timing(False)
+        full_kernels = {k: round(v[1] / args.full_steps, 3) for k, v in sess.kernel_times().items()}
+        work_full = sess.work()
+    else:
+        work_full = work
+
+    launches, total_ms = kernels["cost_volume"]
+    cost_ms = total_ms / launches
+    achieved = work["cost_zncc"] * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "traffic": load_traffic("cost_volume", args.workload),
+        "kernel": "k_cost_volume",
+        "compute": "vector FP32 + FP64 ALU (ZNCC gathers have no matrix form)",
+        "avg_launch_ms": round(cost_ms, 4),
+        "algorithmic_fp32_per_zncc": FP32_PER_ZNCC,
+        "algorithmic_fp64_per_zncc": FP64_PER_ZNCC,
+        "zncc_per_launch": work["cost_zncc"],
+        "fp64_achieved_tflops": round(work["cost_zncc"] * FP64_PER_ZNCC / (cost_ms / 1000.0) / 1e12, 3),
+    }
+    sgm_ms = sum(v[1] for k, v in kernels.items() if k.startswith("sgm")) / args.steps
+    hbm = {
+        "kernels": "sgm_paths + sgm_select",
+        "ms_per_step": round(sgm_ms, 4),
+        "algorithmic_bytes": units * (BYTES_PER_UNIT - 8),
+        "achieved_GBs": round(units * (BYTES_PER_UNIT - 8) / (sgm_ms / 1000.0) / 1e9, 1),
+        "peak_GBs": PEAK_HBM_GBS,
+    }
+    hbm["frac"] = round(hbm["achieved_GBs"] / PEAK_HBM_GBS, 4)
+
+    cpu = None
+    if args.cpu_rows > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline: {args.cpu_rows} coarse rows on {threads} threads")
+        cpu = cpu_baseline(cap, args.cpu_rows, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix·hyp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 coordinates)",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.workload}: {len(cap.modules)} modules {ref.width}x{ref.height}, "
+                         f"{cap.options.planes} hypotheses, 8-path SGM, seed {args.seed}"),
+            "reference_raster": [ref.width, ref.height],
+            "coarse_grid": [g.coarse_columns, g.coarse_rows, g.coarse_step],
+            "labels": g.labels,
+            "views": len(cap.modules),
+            "parallelism": "1 GPU",
+        },
+        "units_per_s": round(units / (ms_per_step / 1000.0), 1),
+        "kernels_ms": {k: round(v[1] / args.steps, 4) for k, v in kernels.items()},
+        "roofline": roofline,
+        "sgm_hbm": hbm,
+        "full_path_ms": None if full_ms is None else round(full_ms, 2),
+        "full_path_kernels_ms": full_kernels,
+        "work": work_full,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    sess.close()
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

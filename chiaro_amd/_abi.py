@@ -198,6 +198,11 @@ WARP_DTYPE = np.dtype(
 assert WARP_DTYPE.itemsize == C.sizeof(NodeWarp) == 28
 
 
+class Work(C.Structure):
+    _fields_ = [("cost_zncc", C.c_uint64), ("direct_zncc", C.c_uint64), ("wide_zncc", C.c_uint64),
+                ("target_zncc", C.c_uint64), ("wide_nodes", C.c_uint64)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint32), ("total_ms", C.c_double)]
 
@@ -268,6 +273,7 @@ DEPTH_SIGNATURES = {
     "chd_session_read_aggregated": (C.c_int, VP, VP),
     "chd_session_read_coarse": (C.c_int, VP, VP, VP, VP),
     "chd_session_read_direct": (C.c_int, VP, C.c_uint32, VP, VP, VP),
+    "chd_session_work": (C.c_int, VP, P(Work)),
     "chd_session_enable_timing": (C.c_int, VP, C.c_int),
     "chd_session_kernel_times": (C.c_int, VP, P(KernelTime), C.c_uint32, P(C.c_uint32)),
     "chd_session_reset_kernel_times": (C.c_int, VP),

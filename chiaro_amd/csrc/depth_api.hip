@@ -132,6 +132,7 @@ struct chd_session {
     double *map_depth = nullptr;
     float *map_confidence = nullptr;
     uint8_t *map_provenance = nullptr;
+    unsigned long long *work = nullptr;  // cost, direct, wide, target ZNCCs
     uint32_t stages_done = 0;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -228,10 +229,11 @@ int run_cost(chd_session *s) {
     const chd_grid_sizes &g = s->grid;
     PatchSpec patch = patch_spec(s, s->options.patch_radius);
     GridSpec grid{g.coarse_columns, g.coarse_rows, g.coarse_step};
+    CHD_HIP(launch_fill_u64(s->context->stream, s->work, 0ull, 1));
     return timed(s, "cost_volume", [&] {
         return launch_cost_volume(s->context->stream, patch, grid, s->labels, g.labels,
                                   s->options.minimum_support, s->options.best_view_count,
-                                  s->scores, s->costs, s->guidance, s->tested);
+                                  s->scores, s->costs, s->guidance, s->tested, s->work);
     });
 }
 
@@ -279,14 +281,15 @@ int run_direct(chd_session *s) {
     GridSpec coarse{g.coarse_columns, g.coarse_rows, g.coarse_step};
     const uint32_t fine_nodes = g.fine_columns * g.fine_rows;
     CHD_HIP(launch_fill_u32(st, s->wide_count, 0u, 1));
+    CHD_HIP(launch_fill_u64(st, s->work + 1, 0ull, 2));
     CHD_CHECK(timed(s, "direct", [&] {
         return launch_direct(st, patch_spec(s, direct.patch_radius), fine, coarse, s->coarse,
                              inverse_step, direct, s->field[0], s->fine_guidance, s->fine_tested,
-                             s->wide, s->wide_count);
+                             s->wide, s->wide_count, s->work + 1);
     }));
     CHD_CHECK(timed(s, "direct_wide", [&] {
         return launch_direct_wide(st, patch_spec(s, wide.patch_radius), fine, wide, s->wide,
-                                  s->wide_count, fine_nodes, s->field[0]);
+                                  s->wide_count, fine_nodes, s->field[0], s->work + 2);
     }));
     CHD_CHECK(timed(s, "reject_isolated", [&] {
         return launch_reject_isolated(st, s->field[0], s->field[1], s->fine_guidance,
@@ -321,13 +324,15 @@ int run_targets(chd_session *s) {
     GridSpec fine{g.fine_columns, g.fine_rows, g.fine_step};
     const uint32_t fine_nodes = g.fine_columns * g.fine_rows;
     CHD_HIP(launch_fill_u64(st, s->target_counts, 0ull, 3 * (size_t)s->count));
+    CHD_HIP(launch_fill_u64(st, s->work + 3, 0ull, 1));
     for (size_t k = 0; k < s->active.size(); k++) {
         const uint32_t t = s->active[k];
         PatchSpec patch = patch_spec(s, o.patch_radius);
         patch.vlist = s->vlist + k;
         patch.view_count = 1;
         CHD_CHECK(timed(s, "targets", [&] {
-            return launch_targets(st, patch, fine, s->field[3], inverse_step, o, s->decisions[0]);
+            return launch_targets(st, patch, fine, s->field[3], inverse_step, o, s->decisions[0],
+                                  s->work + 3);
         }));
         CHD_CHECK(timed(s, "consensus", [&] {
             return launch_consensus(st, s->decisions[0], s->decisions[1], g.fine_columns,
@@ -583,6 +588,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     CHD_ALLOC(s->improvement_keys, fine_nodes);
     CHD_ALLOC(s->improvement_keys_sorted, fine_nodes);
     CHD_ALLOC(s->key_counts, 2);
+    CHD_ALLOC(s->work, 4);
     CHD_ALLOC(s->map_depth, fine_nodes);
     CHD_ALLOC(s->map_confidence, fine_nodes);
     CHD_ALLOC(s->map_provenance, fine_nodes);
@@ -804,6 +810,23 @@ int chd_session_read_direct(chd_session *s, uint32_t stage, chd_node_depth *fiel
     CHD_CHECK(download(s, guidance, s->fine_guidance, nodes * sizeof(float)));
     CHD_CHECK(download(s, tested, s->fine_tested, nodes));
     CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_work(chd_session *s, chd_work *out) {
+    if (!s || !out) return fail(CHD_ERR_ARGUMENT, "null argument");
+    if (s->none) return CHD_NONE;
+    unsigned long long w[4];
+    uint32_t wide = 0;
+    hipStream_t st = s->context->stream;
+    CHD_HIP(hipMemcpyAsync(w, s->work, sizeof w, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipMemcpyAsync(&wide, s->wide_count, sizeof wide, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    out->cost_zncc = w[0];
+    out->direct_zncc = w[1];
+    out->wide_zncc = w[2];
+    out->target_zncc = w[3];
+    out->wide_nodes = wide;
     return CHD_OK;
 }
 

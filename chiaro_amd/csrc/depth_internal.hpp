@@ -47,7 +47,8 @@ struct WideEntry {
 hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
                               const double *labels, uint32_t label_count,
                               uint32_t minimum_support, uint32_t best_view_count,
-                              float *scores, float *costs, float *guidance, uint8_t *tested);
+                              float *scores, float *costs, float *guidance, uint8_t *tested,
+                              unsigned long long *work);
 
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
                       uint32_t columns, uint32_t rows, uint32_t labels, float *paths);
@@ -73,12 +74,12 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
                          const GridSpec &coarse, const chd_node_depth *coarse_field,
                          double inverse_step, const chd_options &direct_options,
                          chd_node_depth *field, float *guidance, uint8_t *tested,
-                         WideEntry *wide, uint32_t *wide_count);
+                         WideEntry *wide, uint32_t *wide_count, unsigned long long *work);
 
 hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                               const chd_options &wide_options, const WideEntry *wide,
                               const uint32_t *wide_count, uint32_t wide_capacity,
-                              chd_node_depth *field);
+                              chd_node_depth *field, unsigned long long *work);
 
 hipError_t launch_reject_isolated(hipStream_t stream, const chd_node_depth *source,
                                   chd_node_depth *field, const float *guidance,
@@ -95,7 +96,8 @@ hipError_t launch_fit_planes(hipStream_t stream, const chd_node_depth *source,
 
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                           const chd_node_depth *field, double inverse_step,
-                          const chd_options &options, chd_node_warp *decisions);
+                          const chd_options &options, chd_node_warp *decisions,
+                          unsigned long long *work);
 
 hipError_t launch_consensus(hipStream_t stream, const chd_node_warp *source,
                             chd_node_warp *decisions, uint32_t columns, uint32_t rows);

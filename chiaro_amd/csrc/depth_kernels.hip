@@ -246,13 +246,14 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
 // raster displacement (dx, dy).
 __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t a, uint32_t V,
                                            uint32_t T, const DevPlane &tp, double dx, double dy,
-                                           float &score) {
+                                           float &score, unsigned int *evaluations) {
     const float4 e = P.ext[n * V + a];
     const float xmin = to_plane((double)e.x + dx), xmax = to_plane((double)e.y + dx);
     const float ymin = to_plane((double)e.z + dy), ymax = to_plane((double)e.w + dy);
     if (!(xmin >= 0.0f && ymin >= 0.0f && xmax <= (float)(tp.width - 1) &&
           ymax <= (float)(tp.height - 1)))
         return false;
+    atomicAdd(evaluations, 1u);
     const float2 *w = P.wwr + (size_t)n * T;
     const float2 *g = P.gq + (size_t)(n * V + a) * T;
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
@@ -334,8 +335,10 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
                                                      uint32_t minimum_support,
                                                      uint32_t best_view_count, float *scores,
                                                      float *costs, float *guidance,
-                                                     uint8_t *tested) {
+                                                     uint8_t *tested, unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    unsigned int *evaluations = &evaluation_count;
     const uint32_t V = S.view_count;
     const PatchLayout lay = patch_layout(nb, V, T);
     const PatchLds P = bind_patch(lds, lay);
@@ -353,6 +356,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
         P.node_ok[n] = g < nodes ? 1 : 0;
         node_tested[n] = 0;
     }
+    if (tid == 0) evaluation_count = 0;
     __syncthreads();
     patch_prologue(S, P, nb, T, centre_value);
 
@@ -366,7 +370,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
             bool ok = true;
             if (l > 0) ok = delta_at_depth(P, n, a, V, *view.camera, labels[l], dx, dy);
             float sc;
-            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc)) s = sc;
+            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc, evaluations)) s = sc;
         }
         res[i] = s;
     }
@@ -398,6 +402,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
         guidance[g] = centre_value[n];  // reference_guidance (depth.rs:1824-1832)
         tested[g] = (uint8_t)node_tested[n];
     }
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
 }
 
 size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t label_count) {
@@ -408,7 +413,8 @@ size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t label
 hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
                               const double *labels, uint32_t label_count,
                               uint32_t minimum_support, uint32_t best_view_count,
-                              float *scores, float *costs, float *guidance, uint8_t *tested) {
+                              float *scores, float *costs, float *guidance, uint8_t *tested,
+                              unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count > 0 ? patch.view_count : 1;
     uint32_t nb = 1024 / label_count;
@@ -422,7 +428,8 @@ hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const 
     PatchSpec s = patch;
     s.view_count = patch.view_count;
     hipLaunchKernelGGL(k_cost_volume, blocks, dim3(256), lds, stream, s, grid, labels, label_count,
-                       nb, T, minimum_support, best_view_count, scores, costs, guidance, tested);
+                       nb, T, minimum_support, best_view_count, scores, costs, guidance, tested,
+                       work);
     return hipGetLastError();
 }
 
@@ -955,7 +962,7 @@ __device__ __forceinline__ void clear_node(chd_node_depth &n) {
 // depth cand[node][k - 1]; results land in res[node][view][slot].
 __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             const double *cand, const uint32_t *ncand, const uint32_t *base,
-                            uint32_t total, float *res) {
+                            uint32_t total, float *res, unsigned int *evaluations) {
     const uint32_t V = S.view_count;
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
         uint32_t n = 0;
@@ -972,7 +979,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
                 ok = delta_at_depth(P, n, a, V, *view.camera, cand[n * kMaxCandidates + slot - 1],
                                     dx, dy);
             float sc;
-            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc)) s = sc;
+            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc, evaluations)) s = sc;
         }
         res[(size_t)n * V * (kMaxCandidates + 1) + (size_t)a * (kMaxCandidates + 1) + slot] = s;
     }
@@ -983,8 +990,10 @@ __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpe
                                                 double inverse_step, chd_options o, uint32_t T,
                                                 chd_node_depth *field, float *guidance,
                                                 uint8_t *tested, WideEntry *wide,
-                                                uint32_t *wide_count) {
+                                                uint32_t *wide_count, unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    unsigned int *evaluations = &evaluation_count;
     const uint32_t V = S.view_count, nb = kDirectNodes;
     const PatchLayout lay = patch_layout(nb, V, T);
     const PatchLds P = bind_patch(lds, lay);
@@ -1036,6 +1045,7 @@ __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpe
         }
         ncand[n] = count;
     }
+    if (tid == 0) evaluation_count = 0;
     __syncthreads();
     if (tid == 0) {
         uint32_t b = 0;
@@ -1046,7 +1056,7 @@ __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpe
         base[nb] = b;
     }
     patch_prologue(S, P, nb, T, centre_value);
-    score_slots(S, P, nb, T, cand, ncand, base, base[nb], res);
+    score_slots(S, P, nb, T, cand, ncand, base, base[nb], res, evaluations);
     __syncthreads();
     // aggregate each (node, slot) over the views
     for (uint32_t i = tid; i < nb * (kMaxCandidates + 1); i += blockDim.x) {
@@ -1091,13 +1101,14 @@ __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpe
         }
         field[g] = selected;
     }
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
 }
 
 hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                          const GridSpec &coarse, const chd_node_depth *coarse_field,
                          double inverse_step, const chd_options &direct_options,
                          chd_node_depth *field, float *guidance, uint8_t *tested,
-                         WideEntry *wide, uint32_t *wide_count) {
+                         WideEntry *wide, uint32_t *wide_count, unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count > 0 ? patch.view_count : 1;
     const uint32_t nb = kDirectNodes;
@@ -1108,7 +1119,7 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
     const uint32_t nodes = fine.columns * fine.rows;
     hipLaunchKernelGGL(k_direct, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch, fine,
                        coarse, coarse_field, inverse_step, direct_options, T, field, guidance,
-                       tested, wide, wide_count);
+                       tested, wide, wide_count, work);
     return hipGetLastError();
 }
 
@@ -1120,8 +1131,12 @@ constexpr uint32_t kWideNodes = 4;
 __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, chd_options o,
                                                      uint32_t T, const WideEntry *wide,
                                                      const uint32_t *wide_count,
-                                                     chd_node_depth *field) {
+                                                     chd_node_depth *field,
+                                                     unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    unsigned int *evaluations = &evaluation_count;
+    if (threadIdx.x == 0) evaluation_count = 0;
     const uint32_t V = S.view_count, nb = kWideNodes;
     const PatchLayout lay = patch_layout(nb, V, T);
     const PatchLds P = bind_patch(lds, lay);
@@ -1160,7 +1175,7 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
             base[nb] = b;
         }
         patch_prologue(S, P, nb, T, centre_value);
-        score_slots(S, P, nb, T, cand, ncand, base, base[nb], res);
+        score_slots(S, P, nb, T, cand, ncand, base, base[nb], res, evaluations);
         __syncthreads();
         for (uint32_t i = tid; i < nb * 8; i += blockDim.x) {
             const uint32_t n = i / 8, slot = i % 8;
@@ -1184,12 +1199,14 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
             field[w.node] = selected;
         }
     }
+    __syncthreads();
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
 }
 
 hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                               const chd_options &wide_options, const WideEntry *wide,
                               const uint32_t *wide_count, uint32_t wide_capacity,
-                              chd_node_depth *field) {
+                              chd_node_depth *field, unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count > 0 ? patch.view_count : 1;
     const uint32_t nb = kWideNodes;
@@ -1201,7 +1218,7 @@ hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const 
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_direct_wide, dim3(blocks), dim3(256), lds, stream, patch, fine,
-                       wide_options, T, wide, wide_count, field);
+                       wide_options, T, wide, wide_count, field, work);
     return hipGetLastError();
 }
 
@@ -1403,8 +1420,10 @@ constexpr uint32_t kTargetNodes = 16;
 __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
                                                  const chd_node_depth *__restrict__ field,
                                                  double inverse_step, chd_options o, uint32_t T,
-                                                 chd_node_warp *decisions) {
+                                                 chd_node_warp *decisions,
+                                                 unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluations;
     const uint32_t nb = kTargetNodes;
     const PatchLayout lay = patch_layout(nb, 1, T);
     const PatchLds P = bind_patch(lds, lay);
@@ -1447,6 +1466,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             decisions[g] = d;
         }
     }
+    if (tid == 0) evaluations = 0;
     __syncthreads();
     patch_prologue(S, P, nb, T, centre_value);
     // the five depth-offset points (map_at_depth); any failure ends the
@@ -1478,7 +1498,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         if (k == 5) {
             // score_one_view(.., None, ..): delta 0
             float sc;
-            if (P.node_ok[n] && P.nv_ok[n] && patch_zncc(P, n, 0, 1, T, tp, 0.0, 0.0, sc)) s = sc;
+            if (P.node_ok[n] && P.nv_ok[n] && patch_zncc(P, n, 0, 1, T, tp, 0.0, 0.0, sc, &evaluations)) s = sc;
             baseline[n] = s;
         } else if (state[n] == 1) {
             // score_one_view_at_point (depth.rs:1710-1736)
@@ -1486,7 +1506,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             const double dx = (double)(q.x - gp[n].x), dy = (double)(q.y - gp[n].y);
             float sc;
             if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                patch_zncc(P, n, 0, 1, T, tp, dx, dy, sc))
+                patch_zncc(P, n, 0, 1, T, tp, dx, dy, sc, &evaluations))
                 s = sc;
             phase[n * 8 + k] = s;
         }
@@ -1531,7 +1551,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         const double dx = (double)(point.x - gp[n].x), dy = (double)(point.y - gp[n].y);
         float s = __int_as_float(0x7fc00000), sc;
         if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-            patch_zncc(P, n, 0, 1, T, tp, dx, dy, sc))
+            patch_zncc(P, n, 0, 1, T, tp, dx, dy, sc, &evaluations))
             s = sc;
         phase[n * 8 + k] = s;
     }
@@ -1588,11 +1608,13 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         }
         decisions[g] = d;
     }
+    if (tid == 0 && work && evaluations) atomicAdd(work, (unsigned long long)evaluations);
 }
 
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                           const chd_node_depth *field, double inverse_step,
-                          const chd_options &options, chd_node_warp *decisions) {
+                          const chd_options &options, chd_node_warp *decisions,
+                          unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t nb = kTargetNodes;
     PatchLayout l = patch_layout(nb, 1, T);
@@ -1601,7 +1623,7 @@ hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const Grid
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
     hipLaunchKernelGGL(k_targets, dim3((nodes + nb - 1) / nb), dim3(128), lds, stream, patch,
-                       fine, field, inverse_step, options, T, decisions);
+                       fine, field, inverse_step, options, T, decisions, work);
     return hipGetLastError();
 }
 

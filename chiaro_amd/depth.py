@@ -179,6 +179,11 @@ class Session:
         _abi.check(self.lib.chd_session_kernel_times(self.handle, arr, cap, C.byref(count)), "times")
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(count.value)}
 
+    def work(self) -> dict:
+        w = _abi.Work()
+        _abi.check(self.lib.chd_session_work(self.handle, C.byref(w)), "work")
+        return {name: getattr(w, name) for name, _ in _abi.Work._fields_}
+
     def reset_kernel_times(self):
         _abi.check(self.lib.chd_session_reset_kernel_times(self.handle), "reset times")
 

@@ -273,6 +273,18 @@ int chd_session_read_coarse(chd_session *session, chd_node_depth *field,
 int chd_session_read_direct(chd_session *session, uint32_t stage, chd_node_depth *field,
                             float *guidance, uint8_t *tested);
 
+/* Work the last run of each stage performed, counted by the kernels:
+ * ZNCC evaluations that ran the full tap loop (after the parallax gate and
+ * the bounds test), and nodes sent to the direct pass's wide fallback. */
+typedef struct chd_work {
+    uint64_t cost_zncc;   /* (node, label, view), build_cost_volume */
+    uint64_t direct_zncc; /* (node, candidate, view), measure_direct_depths */
+    uint64_t wide_zncc;   /* (node, candidate, view), its r = 8 fallback */
+    uint64_t target_zncc; /* per-target refinement, all targets */
+    uint64_t wide_nodes;
+} chd_work;
+int chd_session_work(chd_session *session, chd_work *out);
+
 typedef struct chd_kernel_time {
     char name[48];
     uint32_t launches;

@@ -63,6 +63,7 @@ LAYOUT = {
     "chd_grid_sizes": (_abi.GridSizes, ["labels", "fine_rows"]),
     "chd_node_warp": (_abi.NodeWarp, ["point", "measured"]),
     "chd_kernel_time": (_abi.KernelTime, ["launches", "total_ms"]),
+    "chd_work": (_abi.Work, ["direct_zncc", "wide_nodes"]),
 }
 
 
