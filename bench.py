@@ -165,11 +165,13 @@ def main() -> int:
     units = g.coarse_columns * g.coarse_rows * g.labels
     stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
 
+    # Every kernel of the path runs on the context's stream, so a stream
+    # synchronise on both sides brackets the timed region. (torch is not
+    # used here: it carries its own HIP runtime, which cannot share the
+    # process with the system runtime this library links.)
     for _ in range(args.warmup):
         sess.run(stages)
     ctx.synchronize()
-    import torch  # plumbing only: device-wide synchronisation around the timed region
-    torch.cuda.synchronize()
     sess.reset_kernel_times()
     sess.enable_timing(True)
     log(f"[bench] timing {args.steps} steps")
@@ -177,7 +179,6 @@ def main() -> int:
     for _ in range(args.steps):
         sess.run(stages)
     ctx.synchronize()
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
     sess.enable_timing(False)
     kernels = sess.kernel_times()

@@ -522,13 +522,23 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         const chd_view &v = views[i];
         ViewDev &d = host_views[i];
         std::memset(&d, 0, sizeof d);
+        // planes are stored with one replicated extra column and row
         float *plane = nullptr;
-        const size_t plane_n = (size_t)v.plane_width * v.plane_height;
-        CHD_ALLOC(plane, plane_n);
-        CHD_UPLOAD(plane, v.luminance, plane_n * sizeof(float));
+        const size_t pitch = (size_t)v.plane_width + 1;
+        CHD_ALLOC(plane, pitch * ((size_t)v.plane_height + 1));
+        {
+            hipError_t e2 = hipMemcpy2DAsync(plane, pitch * sizeof(float), v.luminance,
+                                             (size_t)v.plane_width * sizeof(float),
+                                             (size_t)v.plane_width * sizeof(float),
+                                             v.plane_height, hipMemcpyHostToDevice, st);
+            if (e2 == hipSuccess) e2 = launch_pad_plane(st, plane, v.plane_width, v.plane_height);
+            if (e2 != hipSuccess)
+                return bail(fail(CHD_ERR_HIP, std::string("plane upload: ") + hipGetErrorString(e2)));
+        }
         d.plane = plane;
         d.plane_width = v.plane_width;
         d.plane_height = v.plane_height;
+        d.plane_pitch = (uint32_t)pitch;
         if (v.global_warp.points && v.global_warp.columns && v.global_warp.rows) {
             float2 *warp = nullptr;
             const size_t wn = (size_t)v.global_warp.columns * v.global_warp.rows;

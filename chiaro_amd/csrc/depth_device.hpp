@@ -252,9 +252,14 @@ __device__ inline void undistort(const chd_distortion &d, double px, double py, 
     oy = ey * d.normalization[1] + d.center[1];
 }
 
-__device__ __forceinline__ void reflect_axis(const chd_camera &c, double &x, double &y) {
-    if (c.flip_around_x == 1) y = -y;  // geometry.rs:143-150
-    else if (c.flip_around_x == 0) x = -x;
+// reflect_image_axis (geometry.rs:143-150), written as two value selects
+// (a branch that negates one of two referenced locals makes the compiler
+// address them through a stack slot).
+__device__ __forceinline__ void reflect_axis(int flip, double &x, double &y) {
+    const double rx = flip == 0 ? -x : x;
+    const double ry = flip == 1 ? -y : y;
+    x = rx;
+    y = ry;
 }
 
 // pixel_to_ray (geometry.rs:152-181)
@@ -265,7 +270,7 @@ __device__ inline void pixel_to_ray(const chd_camera &c, double px, double py, d
     d3 d = normalize3(mul_vec3(c.k_inverse, d3{ix, iy, 1.0}));
     if (c.flip_around_x >= 0) {
         double x = d.x, y = d.y;
-        reflect_axis(c, x, y);
+        reflect_axis(c.flip_around_x, x, y);
         d = normalize3(d3{x, y, d.z});
     }
     d3 world;
@@ -303,7 +308,7 @@ __device__ inline bool project(const chd_camera &c, d3 point, double &ox, double
     }
     if (cam.z <= 0.0) return false;
     double nx = cam.x / cam.z, ny = cam.y / cam.z;
-    reflect_axis(c, nx, ny);
+    reflect_axis(c.flip_around_x, nx, ny);
     const double *k = c.k;
     double ix = k[0] * nx + k[1] * ny + k[2];
     double iy = k[4] * ny + k[5];
@@ -321,9 +326,13 @@ __device__ inline bool project(const chd_camera &c, d3 point, double &ox, double
 
 // ----------------------------------------------------- image.rs / align.rs
 
+// A luminance plane stored with one replicated extra column and row
+// (pitch = width + 1, height + 1 rows): Plane::sample reads
+// at(min(x0 + 1, w - 1)), and at x0 = w - 1 the padding holds that same
+// value, so the pair (x0, x0 + 1) can always be loaded together.
 struct DevPlane {
     const float *data;
-    uint32_t width, height;
+    uint32_t width, height, pitch;
 };
 
 struct DevWarp {
@@ -331,33 +340,48 @@ struct DevWarp {
     uint32_t step, columns, rows;
 };
 
-// Plane::sample (image.rs:451-466)
+// Plane::sample (image.rs:451-466) on a padded plane.
 __device__ __forceinline__ bool plane_sample(const DevPlane &p, float x, float y, float &out) {
     if (!(x >= 0.0f && y >= 0.0f && x <= (float)(p.width - 1) && y <= (float)(p.height - 1)))
         return false;
     float fx = floorf(x), fy = floorf(y);
     uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
-    uint32_t x1 = min(x0 + 1, p.width - 1), y1 = min(y0 + 1, p.height - 1);
     float tx = x - fx, ty = y - fy;
-    const float *r0 = p.data + (size_t)y0 * p.width;
-    const float *r1 = p.data + (size_t)y1 * p.width;
-    float top = r0[x0] * (1.0f - tx) + r0[x1] * tx;
-    float bottom = r1[x0] * (1.0f - tx) + r1[x1] * tx;
+    const float *r0 = p.data + (size_t)y0 * p.pitch + x0;
+    const float *r1 = r0 + p.pitch;
+    float top = r0[0] * (1.0f - tx) + r0[1] * tx;
+    float bottom = r1[0] * (1.0f - tx) + r1[1] * tx;
     out = top * (1.0f - ty) + bottom * ty;
     return true;
 }
 
-// Plane::sample without the bounds test, for callers that proved the point
-// lies inside [0, w-1] x [0, h-1].
-__device__ __forceinline__ float plane_sample_inside(const DevPlane &p, float x, float y) {
-    float fx = floorf(x), fy = floorf(y);
-    uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
-    uint32_t x1 = min(x0 + 1, p.width - 1), y1 = min(y0 + 1, p.height - 1);
-    float tx = x - fx, ty = y - fy;
-    const float *r0 = p.data + (size_t)y0 * p.width;
-    const float *r1 = p.data + (size_t)y1 * p.width;
-    float top = r0[x0] * (1.0f - tx) + r0[x1] * tx;
-    float bottom = r1[x0] * (1.0f - tx) + r1[x1] * tx;
+// Buffer resource over a padded plane (gfx9-family raw buffer, 32-bit
+// data format); reads past the plane return 0 instead of faulting. Callers
+// pass a plane that is uniform across the wavefront; reading the address
+// through readfirstlane keeps the resource in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const DevPlane &p) {
+    const uint64_t a = (uint64_t)p.data;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    const uint32_t bytes = __builtin_amdgcn_readfirstlane(p.pitch * (p.height + 1) * 4u);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0,
+                                             (int)bytes, 0x00020000);
+}
+
+// Plane::sample without the bounds test, for points proven inside
+// [0, w-1] x [0, h-1]: two 8-byte loads fetch (x0, x0 + 1) of rows y0 and
+// y0 + 1.
+__device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc, uint32_t pitch,
+                                                     float x, float y) {
+    const float fx = floorf(x), fy = floorf(y);
+    const uint32_t offset = ((uint32_t)fy * pitch + (uint32_t)fx) * 4u;
+    const float2 top2 =
+        __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
+    const float2 bot2 = __builtin_bit_cast(
+        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset + pitch * 4u, 0, 0));
+    const float tx = x - fx, ty = y - fy;
+    const float top = top2.x * (1.0f - tx) + top2.y * tx;
+    const float bottom = bot2.x * (1.0f - tx) + bot2.y * tx;
     return top * (1.0f - ty) + bottom * ty;
 }
 
@@ -385,7 +409,11 @@ __device__ __forceinline__ bool warp_map(const DevWarp &w, float x, float y, flo
 }
 
 // plane coordinate of a raster position: ((p - 0.5) * 0.5) as f32
-// (depth.rs:1774-1800)
-__device__ __forceinline__ float to_plane(double p) { return (float)((p - 0.5) * 0.5); }
+// (depth.rs:1774-1800). Halving is exact and commutes with rounding, so
+// round(p - 0.5) * 0.5 == round(p * 0.5 - 0.25), which one fused
+// multiply-add with the exact product p * 0.5 computes.
+__device__ __forceinline__ float to_plane(double p) {
+    return (float)__builtin_fma(p, 0.5, -0.25);
+}
 
 }  // namespace chd

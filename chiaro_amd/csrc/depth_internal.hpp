@@ -11,12 +11,12 @@ namespace chd {
 
 // One module as the kernels see it. Pointers are device pointers.
 struct ViewDev {
-    const float *plane;   // plane_height x plane_width f32, row-major
-    uint32_t plane_width, plane_height;
+    const float *plane;   // (plane_height + 1) rows of plane_pitch f32, the last
+                          // column and row replicated (see DevPlane)
+    uint32_t plane_width, plane_height, plane_pitch;
     const float2 *warp;   // global warp points (x, y), NaN = undefined
     uint32_t warp_step, warp_columns, warp_rows;
     const chd_camera *camera;  // nullptr = no calibration
-    uint32_t pad;
 };
 
 // ZNCC patch context shared by the cost, direct, wide and target kernels.
@@ -120,6 +120,10 @@ hipError_t launch_field_stats(hipStream_t stream, const chd_node_depth *field,
 hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
                             const uint8_t *tested, uint32_t nodes, double *depth,
                             float *confidence, uint8_t *provenance);
+
+// Replicates the last column and row of a plane stored with pitch width + 1
+// and height + 1 rows.
+hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height);
 
 hipError_t launch_fill_u32(hipStream_t stream, uint32_t *data, uint32_t value, size_t n);
 hipError_t launch_fill_u64(hipStream_t stream, unsigned long long *data,

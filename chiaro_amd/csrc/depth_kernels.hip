@@ -33,7 +33,7 @@ namespace chd {
 // ---------------------------------------------------------- patch context
 
 struct PatchLayout {
-    uint32_t pos, ray, ext, gc, gq, wwr, rv, centre, sums, spatial, node_ok, nv_ok, ray_ok,
+    uint32_t cams, pos, ray, ext, gc, tap, rv, centre, sums, spatial, node_ok, nv_ok, ray_ok,
         total;
 };
 
@@ -42,12 +42,12 @@ __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15
 __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t) {
     PatchLayout l;
     uint32_t o = 0;
+    l.cams = o; o = align16(o + v * (uint32_t)sizeof(chd_camera));
     l.pos = o; o = align16(o + nb * 16u);
     l.ray = o; o = align16(o + nb * v * 6u * 8u);
     l.ext = o; o = align16(o + nb * v * 16u);
     l.gc = o; o = align16(o + nb * v * 8u);
-    l.gq = o; o = align16(o + nb * v * t * 8u);
-    l.wwr = o; o = align16(o + nb * t * 8u);
+    l.tap = o; o = align16(o + nb * v * t * 16u);
     l.rv = o; o = align16(o + nb * t * 4u);
     l.centre = o; o = align16(o + nb * 4u);
     l.sums = o; o = align16(o + nb * 3u * 4u);
@@ -60,12 +60,12 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
 }
 
 struct PatchLds {
+    chd_camera *cams;  // the scored views' cameras, in view-slot order
     double2 *pos;
     double *ray;
     float4 *ext;
     float2 *gc;
-    float2 *gq;
-    float2 *wwr;
+    float4 *tap;  // [node][view][tap]: weight, weight * reference, warp x, warp y
     float *rv;
     float *centre;
     float *sums;
@@ -77,12 +77,12 @@ struct PatchLds {
 
 __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l) {
     PatchLds p;
+    p.cams = (chd_camera *)(base + l.cams);
     p.pos = (double2 *)(base + l.pos);
     p.ray = (double *)(base + l.ray);
     p.ext = (float4 *)(base + l.ext);
     p.gc = (float2 *)(base + l.gc);
-    p.gq = (float2 *)(base + l.gq);
-    p.wwr = (float2 *)(base + l.wwr);
+    p.tap = (float4 *)(base + l.tap);
     p.rv = (float *)(base + l.rv);
     p.centre = (float *)(base + l.centre);
     p.sums = (float *)(base + l.sums);
@@ -98,6 +98,7 @@ __device__ __forceinline__ DevPlane view_plane(const ViewDev &v) {
     p.data = v.plane;
     p.width = v.plane_width;
     p.height = v.plane_height;
+    p.pitch = v.plane_pitch;
     return p;
 }
 __device__ __forceinline__ DevWarp view_warp(const ViewDev &v) {
@@ -132,6 +133,13 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
         P.ray_ok[i] = 1;
     }
     for (uint32_t n = tid; n < nb; n += nth) centre_value[n] = 0.0f;
+    {
+        constexpr uint32_t words = sizeof(chd_camera) / 4;
+        for (uint32_t i = tid; i < V * words; i += nth) {
+            const uint32_t *src = (const uint32_t *)S.views[S.vlist[i / words]].camera;
+            ((uint32_t *)P.cams)[i] = src[i % words];
+        }
+    }
     __syncthreads();
 
     // phase 1: spatial weights, reference samples, warp taps, corrected rays
@@ -172,7 +180,8 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
             P.nv_ok[n * V + a] = 0;
             q = make_float2(0.0f, 0.0f);
         }
-        P.gq[i] = q;
+        P.tap[i].z = q.x;
+        P.tap[i].w = q.y;
     }
     for (uint32_t i = tid; i < nb * V; i += nth) {
         uint32_t n = i / V, a = i % V;
@@ -188,7 +197,7 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
         // corrected_reference = reference.map_from(target, gc, 1e8)
         // (depth.rs:1752-1756), then the reference ray through it.
         d3 o, d;
-        pixel_to_ray(*tv.camera, (double)gc.x, (double)gc.y, o, d);
+        pixel_to_ray(P.cams[a], (double)gc.x, (double)gc.y, o, d);
         d3 far{o.x + d.x * kInfinityDepth, o.y + d.y * kInfinityDepth, o.z + d.z * kInfinityDepth};
         double cx, cy;
         if (!project(*R.camera, far, cx, cy)) {
@@ -202,24 +211,30 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
     }
     __syncthreads();
 
-    // phase 2: bilateral weights, tap extremes
+    // phase 2: bilateral weights (copied into every view's tap record), tap
+    // extremes
     for (uint32_t i = tid; i < nb * T; i += nth) {
         uint32_t n = i / T, t = i % T;
         if (!P.node_ok[n]) continue;
         float rv = P.rv[i];
         float range = libm_expf(-1.2f * fabsf(rv - P.centre[n]));
         float w = P.spatial[t] * range;
-        P.wwr[i] = make_float2(w, w * rv);
+        float wr = w * rv;
+        for (uint32_t a = 0; a < V; a++) {
+            float4 *e = P.tap + ((size_t)n * V + a) * T + t;
+            e->x = w;
+            e->y = wr;
+        }
     }
     for (uint32_t i = tid; i < nb * V; i += nth) {
         if (!P.nv_ok[i]) continue;
-        const float2 *g = P.gq + (size_t)i * T;
-        float x0 = g[0].x, x1 = g[0].x, y0 = g[0].y, y1 = g[0].y;
+        const float4 *g = P.tap + (size_t)i * T;
+        float x0 = g[0].z, x1 = g[0].z, y0 = g[0].w, y1 = g[0].w;
         for (uint32_t t = 1; t < T; t++) {
-            x0 = fminf(x0, g[t].x);
-            x1 = fmaxf(x1, g[t].x);
-            y0 = fminf(y0, g[t].y);
-            y1 = fmaxf(y1, g[t].y);
+            x0 = fminf(x0, g[t].z);
+            x1 = fmaxf(x1, g[t].z);
+            y0 = fminf(y0, g[t].w);
+            y1 = fmaxf(y1, g[t].w);
         }
         P.ext[i] = make_float4(x0, x1, y0, y1);
     }
@@ -227,10 +242,10 @@ __device__ void patch_prologue(const PatchSpec &S, const PatchLds &P, uint32_t n
 
     // phase 3: label- and view-independent sums, in tap order
     for (uint32_t n = tid; n < nb; n += nth) {
-        if (!P.node_ok[n]) continue;
+        if (!P.node_ok[n] || V == 0) continue;
         float count = 0.0f, sum_r = 0.0f, sum_rr = 0.0f;
         for (uint32_t t = 0; t < T; t++) {
-            float2 w = P.wwr[n * T + t];
+            const float4 w = P.tap[(size_t)n * V * T + t];
             count += w.x;
             sum_r += w.y;
             sum_rr += w.y * P.rv[n * T + t];
@@ -254,19 +269,19 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
           ymax <= (float)(tp.height - 1)))
         return false;
     atomicAdd(evaluations, 1u);
-    const float2 *w = P.wwr + (size_t)n * T;
-    const float2 *g = P.gq + (size_t)(n * V + a) * T;
+    const float4 *taps = P.tap + (size_t)(n * V + a) * T;
+    const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
+#pragma unroll 9
     for (uint32_t t = 0; t < T; t++) {
-        const float2 wt = w[t];
-        const float2 q = g[t];
-        const float x = to_plane((double)q.x + dx);
-        const float y = to_plane((double)q.y + dy);
-        const float tv = plane_sample_inside(tp, x, y);
-        const float wv = wt.x * tv;
+        const float4 e = taps[t];
+        const float x = to_plane((double)e.z + dx);
+        const float y = to_plane((double)e.w + dy);
+        const float tv = plane_sample_inside(rsrc, tp.pitch, x, y);
+        const float wv = e.x * tv;
         st += wv;
         stt += wv * tv;
-        sp += wt.y * tv;
+        sp += e.y * tv;
     }
     const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
     const float covariance = sp - sr * st / count;
@@ -360,19 +375,23 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
     __syncthreads();
     patch_prologue(S, P, nb, T, centre_value);
 
-    const uint32_t items = nb * V * L;
-    for (uint32_t i = tid; i < items; i += nth) {
-        const uint32_t n = i / (V * L), a = (i / L) % V, l = i % L;
-        float s = __int_as_float(0x7fc00000);
-        if (P.node_ok[n] && P.nv_ok[n * V + a]) {
-            const ViewDev &view = S.views[S.vlist[a]];
-            double dx = 0.0, dy = 0.0;
-            bool ok = true;
-            if (l > 0) ok = delta_at_depth(P, n, a, V, *view.camera, labels[l], dx, dy);
-            float sc;
-            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc, evaluations)) s = sc;
+    // The view loop is outermost so that the target plane (and its buffer
+    // resource) and camera are uniform across the block.
+    for (uint32_t a = 0; a < V; a++) {
+        const ViewDev &view = S.views[S.vlist[a]];
+        const DevPlane tp = view_plane(view);
+        for (uint32_t i = tid; i < nb * L; i += nth) {
+            const uint32_t n = i / L, l = i % L;
+            float s = __int_as_float(0x7fc00000);
+            if (P.node_ok[n] && P.nv_ok[n * V + a]) {
+                double dx = 0.0, dy = 0.0;
+                bool ok = true;
+                if (l > 0) ok = delta_at_depth(P, n, a, V, P.cams[a], labels[l], dx, dy);
+                float sc;
+                if (ok && patch_zncc(P, n, a, V, T, tp, dx, dy, sc, evaluations)) s = sc;
+            }
+            res[((size_t)n * V + a) * L + l] = s;
         }
-        res[i] = s;
     }
     __syncthreads();
 
@@ -420,7 +439,8 @@ hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const 
     uint32_t nb = 1024 / label_count;
     if (nb < 1) nb = 1;
     if (nb > 8) nb = 8;
-    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 64 * 1024) nb--;
+    // 32 KB of LDS per 256-thread block keeps 5 blocks (20 waves) per CU
+    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 32 * 1024) nb--;
     size_t lds = cost_lds_bytes(nb, V, T, label_count);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = grid.columns * grid.rows;
@@ -870,7 +890,7 @@ hipError_t launch_complete_depth_field(hipStream_t stream, const chd_node_depth 
 
 // ------------------------------------------------------------ direct pass
 
-constexpr uint32_t kDirectNodes = 8;     // fine nodes per block
+constexpr uint32_t kDirectNodes = 8;     // fine nodes per block (at most)
 constexpr uint32_t kMaxCandidates = 32;  // direct_depth_candidates bound
 
 // select_direct_depth (depth.rs:1109-1165) over `count` candidates whose
@@ -957,44 +977,46 @@ __device__ __forceinline__ void clear_node(chd_node_depth &n) {
     for (int k = 0; k < 6; k++) n.pad[k] = 0;
 }
 
-// Candidate scoring shared by the direct and wide kernels: items are
-// (node, view, slot) with slot 0 the global-warp baseline and slot k the
-// depth cand[node][k - 1]; results land in res[node][view][slot].
+// Candidate scoring shared by the direct and wide kernels: for each view,
+// items are (node, slot) with slot 0 the global-warp baseline and slot k the
+// depth cand[node][k - 1] (base[] is the per-view prefix of 1 + ncand);
+// results land in res[node][view][slot].
 __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             const double *cand, const uint32_t *ncand, const uint32_t *base,
                             uint32_t total, float *res, unsigned int *evaluations) {
     const uint32_t V = S.view_count;
-    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        uint32_t n = 0;
-        while (n + 1 < nb && base[n + 1] <= i) n++;
-        const uint32_t slots = 1 + ncand[n];
-        const uint32_t local = i - base[n];
-        const uint32_t a = local / slots, slot = local % slots;
-        float s = __int_as_float(0x7fc00000);
-        if (P.node_ok[n] && P.nv_ok[n * V + a]) {
-            const ViewDev &view = S.views[S.vlist[a]];
-            double dx = 0.0, dy = 0.0;
-            bool ok = true;
-            if (slot > 0)
-                ok = delta_at_depth(P, n, a, V, *view.camera, cand[n * kMaxCandidates + slot - 1],
-                                    dx, dy);
-            float sc;
-            if (ok && patch_zncc(P, n, a, V, T, view_plane(view), dx, dy, sc, evaluations)) s = sc;
+    for (uint32_t a = 0; a < V; a++) {
+        const ViewDev &view = S.views[S.vlist[a]];
+        const DevPlane tp = view_plane(view);
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            uint32_t n = 0;
+            while (n + 1 < nb && base[n + 1] <= i) n++;
+            const uint32_t slot = i - base[n];
+            float s = __int_as_float(0x7fc00000);
+            if (P.node_ok[n] && P.nv_ok[n * V + a]) {
+                double dx = 0.0, dy = 0.0;
+                bool ok = true;
+                if (slot > 0)
+                    ok = delta_at_depth(P, n, a, V, P.cams[a], cand[n * kMaxCandidates + slot - 1],
+                                        dx, dy);
+                float sc;
+                if (ok && patch_zncc(P, n, a, V, T, tp, dx, dy, sc, evaluations)) s = sc;
+            }
+            res[(size_t)n * V * (kMaxCandidates + 1) + (size_t)a * (kMaxCandidates + 1) + slot] = s;
         }
-        res[(size_t)n * V * (kMaxCandidates + 1) + (size_t)a * (kMaxCandidates + 1) + slot] = s;
     }
 }
 
 __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpec Cg,
                                                 const chd_node_depth *__restrict__ coarse,
                                                 double inverse_step, chd_options o, uint32_t T,
-                                                chd_node_depth *field, float *guidance,
+                                                uint32_t nb, chd_node_depth *field, float *guidance,
                                                 uint8_t *tested, WideEntry *wide,
                                                 uint32_t *wide_count, unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     unsigned int *evaluations = &evaluation_count;
-    const uint32_t V = S.view_count, nb = kDirectNodes;
+    const uint32_t V = S.view_count;
     const PatchLayout lay = patch_layout(nb, V, T);
     const PatchLds P = bind_patch(lds, lay);
     double *cand = (double *)(lds + lay.total);                 // [nb][32]
@@ -1051,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_direct(PatchSpec S, GridSpec F, GridSpe
         uint32_t b = 0;
         for (uint32_t n = 0; n < nb; n++) {
             base[n] = b;
-            b += (node0 + n < nodes) ? (1 + ncand[n]) * V : 0;
+            b += (node0 + n < nodes) ? 1 + ncand[n] : 0;
         }
         base[nb] = b;
     }
@@ -1111,25 +1133,31 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
                          WideEntry *wide, uint32_t *wide_count, unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count > 0 ? patch.view_count : 1;
-    const uint32_t nb = kDirectNodes;
-    PatchLayout l = patch_layout(nb, V, T);
-    size_t lds = (size_t)l.total + nb * kMaxCandidates * 8 + (size_t)nb * V * (kMaxCandidates + 1) * 4 +
-                 nb * (kMaxCandidates + 1) * 4 + nb * 4 + nb * 4 + (nb + 1) * 4 + nb * 4;
+    auto bytes = [&](uint32_t nb) {
+        PatchLayout l = patch_layout(nb, V, T);
+        return (size_t)l.total + nb * kMaxCandidates * 8 +
+               (size_t)nb * V * (kMaxCandidates + 1) * 4 + nb * (kMaxCandidates + 1) * 4 +
+               nb * 4 + nb * 4 + (nb + 1) * 4 + nb * 4;
+    };
+    uint32_t nb = kDirectNodes;
+    while (nb > 1 && bytes(nb) > 64 * 1024) nb--;
+    const size_t lds = bytes(nb);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
     hipLaunchKernelGGL(k_direct, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch, fine,
-                       coarse, coarse_field, inverse_step, direct_options, T, field, guidance,
+                       coarse, coarse_field, inverse_step, direct_options, T, nb, field, guidance,
                        tested, wide, wide_count, work);
     return hipGetLastError();
 }
 
-constexpr uint32_t kWideNodes = 4;
+constexpr uint32_t kWideNodes = 4;  // wide-fallback nodes per block (at most)
 
 // The r = max(2r, 8) re-measurement of nodes whose first pass produced no
 // supported depth but a candidate scoring at least 0.30 (depth.rs:803-846).
 // A fixed grid walks the entry list.
 __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, chd_options o,
-                                                     uint32_t T, const WideEntry *wide,
+                                                     uint32_t T, uint32_t nb,
+                                                     const WideEntry *wide,
                                                      const uint32_t *wide_count,
                                                      chd_node_depth *field,
                                                      unsigned long long *work) {
@@ -1137,7 +1165,7 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
     __shared__ unsigned int evaluation_count;
     unsigned int *evaluations = &evaluation_count;
     if (threadIdx.x == 0) evaluation_count = 0;
-    const uint32_t V = S.view_count, nb = kWideNodes;
+    const uint32_t V = S.view_count;
     const PatchLayout lay = patch_layout(nb, V, T);
     const PatchLds P = bind_patch(lds, lay);
     double *cand = (double *)(lds + lay.total);                 // [nb][32]
@@ -1170,7 +1198,7 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
             uint32_t b = 0;
             for (uint32_t n = 0; n < nb; n++) {
                 base[n] = b;
-                b += (first + n < count) ? (1 + ncand[n]) * V : 0;
+                b += (first + n < count) ? 1 + ncand[n] : 0;
             }
             base[nb] = b;
         }
@@ -1209,16 +1237,21 @@ hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const 
                               chd_node_depth *field, unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count > 0 ? patch.view_count : 1;
-    const uint32_t nb = kWideNodes;
-    PatchLayout l = patch_layout(nb, V, T);
-    size_t lds = (size_t)l.total + nb * kMaxCandidates * 8 + (size_t)nb * V * (kMaxCandidates + 1) * 4 +
-                 nb * (kMaxCandidates + 1) * 4 + nb * 4 + nb * 4 + (nb + 1) * 4;
+    auto bytes = [&](uint32_t nb) {
+        PatchLayout l = patch_layout(nb, V, T);
+        return (size_t)l.total + nb * kMaxCandidates * 8 +
+               (size_t)nb * V * (kMaxCandidates + 1) * 4 + nb * (kMaxCandidates + 1) * 4 +
+               nb * 4 + nb * 4 + (nb + 1) * 4;
+    };
+    uint32_t nb = kWideNodes;
+    while (nb > 1 && bytes(nb) > 96 * 1024) nb--;
+    const size_t lds = bytes(nb);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     uint32_t blocks = (wide_capacity + nb - 1) / nb;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_direct_wide, dim3(blocks), dim3(256), lds, stream, patch, fine,
-                       wide_options, T, wide, wide_count, field, work);
+                       wide_options, T, nb, wide, wide_count, field, work);
     return hipGetLastError();
 }
 
@@ -1262,13 +1295,22 @@ hipError_t launch_reject_isolated(hipStream_t stream, const chd_node_depth *sour
 // Connected components of the symmetric relation of
 // reject_small_direct_components by concurrent union-find: roots link
 // larger index under smaller with compare-and-swap.
+// Parents always point to a smaller index of the same component, so a walk
+// ends at the root; while walking, each visited node is re-pointed to its
+// grandparent (a shorter path to the same root; concurrent writers only
+// ever store ancestors).
 __device__ __forceinline__ uint32_t cc_find(uint32_t *parent, uint32_t x) {
-    uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (p != x) {
-        x = p;
-        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t curr = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (curr != x) {
+        uint32_t prev = x, next;
+        while (curr > (next = __hip_atomic_load(&parent[curr], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(&parent[prev], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            prev = curr;
+            curr = next;
+        }
     }
-    return x;
+    return curr;
 }
 
 __device__ __forceinline__ bool cc_edge(const chd_node_depth *source, const float *guidance,
@@ -1278,12 +1320,28 @@ __device__ __forceinline__ bool cc_edge(const chd_node_depth *source, const floa
     return image_edge <= 0.6f && inverse_difference <= tol;
 }
 
-__global__ void k_cc_init(const chd_node_depth *source, uint32_t *parent, uint32_t *sizes,
-                          uint32_t n) {
+// Each node starts under its smallest-index neighbour it is related to (or
+// itself), which already links most of a surface into short chains.
+__global__ void k_cc_init(const chd_node_depth *source, const float *guidance, uint32_t C,
+                          uint32_t R, double tol, uint32_t *parent, uint32_t *sizes) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    parent[i] = i;
+    if (i >= C * R) return;
     sizes[i] = 0;
+    uint32_t p = i;
+    if (source[i].valid) {
+        const int row = (int)(i / C), column = (int)(i % C);
+        const int nx[4] = {column - 1, column, column + 1, column - 1};
+        const int ny[4] = {row - 1, row - 1, row - 1, row};
+        for (int k = 0; k < 4; k++) {  // neighbours of smaller index, ascending
+            if (nx[k] < 0 || ny[k] < 0 || nx[k] >= (int)C) continue;
+            const uint32_t j = (uint32_t)ny[k] * C + (uint32_t)nx[k];
+            if (source[j].valid && cc_edge(source, guidance, i, j, tol)) {
+                p = j;
+                break;
+            }
+        }
+    }
+    parent[i] = p;
 }
 
 __global__ void k_cc_union(const chd_node_depth *source, const float *guidance, uint32_t C,
@@ -1316,12 +1374,23 @@ __global__ void k_cc_union(const chd_node_depth *source, const float *guidance, 
     }
 }
 
+// Component sizes: neighbouring lanes mostly share a root, so each run of
+// equal roots in a wavefront adds its length with one atomic.
 __global__ void k_cc_count(const chd_node_depth *source, uint32_t *parent, uint32_t *sizes,
                            uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !source[i].valid) return;
-    const uint32_t root = cc_find(parent, i);
-    atomicAdd(&sizes[root], 1u);
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool valid = i < n && source[i].valid;
+    const uint32_t root = valid ? cc_find(parent, i) : 0xffffffffu;
+    const uint32_t before = __shfl_up(root, 1, 64);
+    const bool head = valid && (lane == 0 || before != root);
+    const bool boundary = !valid || lane == 0 || before != root;
+    const unsigned long long ends = __ballot(boundary);
+    if (head) {
+        const unsigned long long later = lane == 63 ? 0ull : ends >> (lane + 1);
+        const uint32_t next = later ? lane + 1 + (uint32_t)__builtin_ctzll(later) : 64u;
+        atomicAdd(&sizes[root], next - lane);
+    }
 }
 
 __global__ void k_cc_apply(const chd_node_depth *source, chd_node_depth *field, uint32_t *parent,
@@ -1344,7 +1413,8 @@ hipError_t launch_reject_components(hipStream_t stream, const chd_node_depth *so
                                       hipMemcpyDeviceToDevice, stream);
         return e;
     }
-    hipLaunchKernelGGL(k_cc_init, grid, block, 0, stream, source, parent, sizes, n);
+    hipLaunchKernelGGL(k_cc_init, grid, block, 0, stream, source, guidance, columns, rows,
+                       inverse_tolerance, parent, sizes);
     hipLaunchKernelGGL(k_cc_union, grid, block, 0, stream, source, guidance, columns, rows,
                        inverse_tolerance, parent);
     hipLaunchKernelGGL(k_cc_count, grid, block, 0, stream, source, parent, sizes, n);
@@ -1484,7 +1554,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         if (ok) {
             const double *r = P.ray + (size_t)n * 6;
             d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
-            ok = project(*view.camera, point, qx, qy) && finite_d(qx) && finite_d(qy);
+            ok = project(P.cams[0], point, qx, qy) && finite_d(qx) && finite_d(qy);
         }
         points[i] = make_float2((float)qx, (float)qy);
         if (!ok) state[n] = -1;
@@ -1790,21 +1860,38 @@ hipError_t launch_target_outputs(hipStream_t stream, const chd_node_warp *decisi
 
 // Report inputs (depth.rs:394-418): counts and total_cmp sort keys of the
 // selected depths and of the measured nodes' improvements.
+// Wavefront-aggregated: one atomic per wave for each count and key list.
+__device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
+    const unsigned long long mask = __ballot(take);
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __builtin_ctzll(mask | (1ull << 63));
+    uint32_t base = 0;
+    if (mask && (int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
 __global__ void k_field_stats(const chd_node_depth *field, const uint8_t *tested, uint32_t n,
                               unsigned long long *counts, unsigned long long *depth_keys,
                               unsigned int *improvement_keys, uint32_t *key_counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (tested[i]) atomicAdd(&counts[0], 1ull);
-    const chd_node_depth d = field[i];
-    if (!d.valid) return;
-    atomicAdd(&counts[d.regularized ? 2 : 1], 1ull);
-    const uint32_t slot = atomicAdd(&key_counts[0], 1u);
-    depth_keys[slot] = (unsigned long long)tkey64(d.depth) ^ 0x8000000000000000ull;
-    if (!d.regularized) {
-        const uint32_t s2 = atomicAdd(&key_counts[1], 1u);
-        improvement_keys[s2] = (unsigned int)tkey32(d.improvement) ^ 0x80000000u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool in = i < n;
+    chd_node_depth d;
+    clear_node(d);
+    if (in) d = field[i];
+    const bool t = in && tested[i];
+    const bool measured = d.valid && !d.regularized, regularized = d.valid && d.regularized;
+    const unsigned long long mt = __ballot(t), mm = __ballot(measured), mr = __ballot(regularized);
+    if (lane == 0) {
+        if (mt) atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
+        if (mm) atomicAdd(&counts[1], (unsigned long long)__popcll(mm));
+        if (mr) atomicAdd(&counts[2], (unsigned long long)__popcll(mr));
     }
+    const uint32_t slot = wave_append(d.valid, &key_counts[0]);
+    if (d.valid) depth_keys[slot] = (unsigned long long)tkey64(d.depth) ^ 0x8000000000000000ull;
+    const uint32_t s2 = wave_append(measured, &key_counts[1]);
+    if (measured) improvement_keys[s2] = (unsigned int)tkey32(d.improvement) ^ 0x80000000u;
 }
 
 hipError_t launch_field_stats(hipStream_t stream, const chd_node_depth *field,
@@ -1841,6 +1928,23 @@ hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
 }
 
 // ----------------------------------------------------------------- small
+
+__global__ void k_pad_plane(float *plane, uint32_t width, uint32_t height) {
+    const uint32_t pitch = width + 1;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < height) plane[(size_t)i * pitch + width] = plane[(size_t)i * pitch + width - 1];
+    if (i < pitch) {
+        const uint32_t x = i < width ? i : width - 1;
+        plane[(size_t)height * pitch + i] = plane[(size_t)(height - 1) * pitch + x];
+    }
+}
+
+hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height) {
+    const uint32_t n = (width + 1 > height ? width + 1 : height);
+    hipLaunchKernelGGL(k_pad_plane, dim3((n + 255) / 256), dim3(256), 0, stream, plane, width,
+                       height);
+    return hipGetLastError();
+}
 
 __global__ void k_fill_u32(uint32_t *data, uint32_t value, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -137,3 +137,127 @@ def test_sgm_stage_on_oracle_costs(gpu_context, oracle_lib):
         oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), columns, rows, labels,
                                         _abi.ptr(want))
         assert_bitwise(f"sgm {columns}x{rows}x{labels}", got, want)
+
+
+def _camera_variants(oracle_lib, cap):
+    """Replace the target cameras by a distorted canonical camera, a mirror
+    camera with a flipped axis and a camera with an orientation correction
+    (geometry.rs:64-133, 143-243), with global warps from the oracle's
+    align_module(refine = false)."""
+    ref = cap.modules[cap.reference_index]
+    w, h = ref.width, ref.height
+    f = ref.camera.k[0]
+    k = np.array([f, 0, (w - 1) / 2, 0, f, (h - 1) / 2, 0, 0, 1], np.float64)
+    cams = []
+    # distorted canonical camera, 20 mm to the right
+    d = _abi.Distortion()
+    d.center[:] = [(w - 1) / 2 + 3.0, (h - 1) / 2 - 2.0]
+    d.normalization[:] = [f, f]
+    d.coeffs[:] = [0.02, -0.05, 0.0005, -0.0004, 0.01]
+    d.coeff_count = 5
+    c = _abi.Camera()
+    t = np.array([-20.0, 0.0, 0.0])
+    assert oracle_lib.or_camera_canonical(k.ctypes.data, np.eye(3).ctypes.data, t.ctypes.data, w, h,
+                                          C.byref(d), C.byref(c))
+    cams.append(c)
+    # mirror camera: real module looking along -x, folded by a 45 deg mirror
+    m = _abi.Camera()
+    m.width, m.height = w, h
+    for i in range(9):
+        m.k[i] = k[i]
+    kinv = np.empty(9)
+    oracle_lib.or_inverse3(k.ctypes.data, kinv.ctypes.data)
+    for i in range(9):
+        m.k_inverse[i] = kinv[i]
+    m.flip_around_x = 1
+    m.pose_kind = 1
+    real_cw = np.array([[0, 0, -1], [0, 1, 0], [1, 0, 0]], np.float64)  # camera z -> world -x
+    n = np.array([1.0, 0.0, 1.0]) / math.sqrt(2.0)
+    reflect = np.eye(3) - 2.0 * np.outer(n, n)
+    for i in range(9):
+        m.real_cw[i] = real_cw.reshape(-1)[i]
+        m.reflect[i] = reflect.reshape(-1)[i]
+    m.virtual_center[:] = [17.5, 30.311, 0.0]
+    for i in range(9):
+        m.orientation_correction[i] = np.eye(3).reshape(-1)[i]
+    m.focal_px = f
+    cams.append(m)
+    # canonical camera with an orientation correction of ~0.2 deg
+    o = _abi.Camera()
+    t = np.array([0.0, -15.0, 0.0])
+    assert oracle_lib.or_camera_canonical(k.ctypes.data, np.eye(3).ctypes.data, t.ctypes.data, w, h,
+                                          None, C.byref(o))
+    a = math.radians(0.2)
+    rot = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    for i in range(9):
+        o.orientation_correction[i] = rot.reshape(-1)[i]
+    cams.append(o)
+    return cams
+
+
+def _oracle_warp(oracle_lib, ref_cam, cam, w, h):
+    columns, rows = -(-w // 32) + 1, -(-h // 32) + 1
+    pts = np.empty((rows, columns, 2), np.float32)
+    conf = np.empty((rows, columns), np.float32)
+    wp = _abi.Warp()
+    wp.points = pts.ctypes.data_as(C.POINTER(C.c_float))
+    wp.confidence = conf.ctypes.data_as(C.POINTER(C.c_float))
+    oracle_lib.or_global_warp(C.byref(ref_cam), C.byref(cam), w, h, 32, C.byref(wp))
+    return pts
+
+
+def test_camera_model_branches_bit_exact(gpu_context, oracle_lib):
+    cap = _small_config0(5)
+    ref = cap.modules[0]
+    cams = _camera_variants(oracle_lib, cap)
+    base = cap.modules[1]
+    mods = [ref]
+    for i, cam in enumerate(cams):
+        src = cap.modules[1 + (i % 2)]
+        mods.append(scene.ModuleInput(name=f"V{i}", luminance=src.luminance, width=ref.width,
+                                      height=ref.height, camera=cam,
+                                      warp_points=_oracle_warp(oracle_lib, ref.camera, cam, ref.width, ref.height),
+                                      warp_step=32))
+    cap.modules = mods
+    _full_parity(gpu_context, cap, "distortion / mirror / orientation cameras")
+
+
+def test_skipped_views_and_undefined_warps_bit_exact(gpu_context):
+    """A rejected alignment, an uncalibrated module and NaN (undefined) warp
+    regions are skipped exactly as the reference skips them."""
+    cap = scene.config1(seed=8, width=416, height=312, focal_px=330.0, planes=24)
+    cap.modules[1].accepted = False
+    cap.modules[2].camera = None
+    w = cap.modules[3].warp_points.copy()
+    w[: w.shape[0] // 3, : w.shape[1] // 2] = np.nan
+    cap.modules[3].warp_points = w
+    _full_parity(gpu_context, cap, "skipped views and undefined warps")
+
+
+def test_none_cases(gpu_context):
+    from chiaro_amd.depth import refine_multiview_depth
+    cap = _small_config0(2, scale=0.125)
+    for bad in [dict(planes=2), dict(far_depth=100.0), dict(best_view_count=1), dict(enabled=False)]:
+        o = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=16)
+        for key, value in bad.items():
+            setattr(o, key, value)
+        dmap, targets = refine_multiview_depth(cap.modules, 0, o, gpu_context)
+        assert dmap is None and targets == {}
+    cap.modules[0].camera = None
+    dmap, targets = refine_multiview_depth(cap.modules, 0, cap.options, gpu_context)
+    assert dmap is None
+    dmap, targets = refine_multiview_depth(cap.modules, 5, cap.options, gpu_context)
+    assert dmap is None
+
+
+def test_drop_in_matches_session(gpu_context):
+    """chd_refine_multiview_depth (host buffers in and out) equals the
+    session form and the oracle."""
+    from chiaro_amd.depth import refine_multiview_depth
+    cap = _small_config0(9)
+    dmap, targets = refine_multiview_depth(cap.modules, 0, cap.options, gpu_context)
+    omap, otargets, _ = binding.refine(cap)
+    assert_bitwise("drop-in depth", dmap.depth, omap["depth"])
+    assert_bitwise("drop-in provenance", dmap.provenance, omap["provenance"])
+    for t in otargets:
+        assert_bitwise(f"drop-in target {t}", targets[t].points, otargets[t]["points"])
