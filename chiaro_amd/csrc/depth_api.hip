@@ -118,8 +118,9 @@ struct chd_session {
     uint8_t *fine_tested = nullptr;
     WideEntry *wide = nullptr;
     uint32_t *wide_count = nullptr;
+    float *direct_scores = nullptr;  // [fine node][kDirectSlots]
     uint32_t *parent = nullptr, *sizes = nullptr;
-    chd_node_warp *decisions[2] = {nullptr, nullptr};
+    chd_node_warp *decisions[2] = {nullptr, nullptr};  // [0]: [target slot][node]
     std::vector<float *> target_points, target_confidence;
     std::vector<uint8_t *> target_codes;
     unsigned long long *target_counts = nullptr;  // 3 per view slot
@@ -285,7 +286,7 @@ int run_direct(chd_session *s) {
     CHD_CHECK(timed(s, "direct", [&] {
         return launch_direct(st, patch_spec(s, direct.patch_radius), fine, coarse, s->coarse,
                              inverse_step, direct, s->field[0], s->fine_guidance, s->fine_tested,
-                             s->wide, s->wide_count, s->work + 1);
+                             s->wide, s->wide_count, s->direct_scores, s->work + 1);
     }));
     CHD_CHECK(timed(s, "direct_wide", [&] {
         return launch_direct_wide(st, patch_spec(s, wide.patch_radius), fine, wide, s->wide,
@@ -325,25 +326,23 @@ int run_targets(chd_session *s) {
     const uint32_t fine_nodes = g.fine_columns * g.fine_rows;
     CHD_HIP(launch_fill_u64(st, s->target_counts, 0ull, 3 * (size_t)s->count));
     CHD_HIP(launch_fill_u64(st, s->work + 3, 0ull, 1));
+    if (s->active.empty()) return CHD_OK;
+    CHD_CHECK(timed(s, "targets", [&] {
+        return launch_targets(st, patch_spec(s, o.patch_radius), fine, s->field[3], inverse_step,
+                              o, s->decisions[0], s->work + 3);
+    }));
     for (size_t k = 0; k < s->active.size(); k++) {
         const uint32_t t = s->active[k];
-        PatchSpec patch = patch_spec(s, o.patch_radius);
-        patch.vlist = s->vlist + k;
-        patch.view_count = 1;
-        CHD_CHECK(timed(s, "targets", [&] {
-            return launch_targets(st, patch, fine, s->field[3], inverse_step, o, s->decisions[0],
-                                  s->work + 3);
-        }));
+        chd_node_warp *decisions = s->decisions[0] + k * fine_nodes;
         CHD_CHECK(timed(s, "consensus", [&] {
-            return launch_consensus(st, s->decisions[0], s->decisions[1], g.fine_columns,
-                                    g.fine_rows);
+            return launch_consensus(st, decisions, s->decisions[1], g.fine_columns, g.fine_rows);
         }));
         CHD_CHECK(timed(s, "boundaries", [&] {
-            return launch_boundaries(st, s->decisions[1], s->decisions[0], s->fine_guidance,
+            return launch_boundaries(st, s->decisions[1], decisions, s->fine_guidance,
                                      g.fine_columns, g.fine_rows);
         }));
         CHD_CHECK(timed(s, "target_outputs", [&] {
-            return launch_target_outputs(st, s->decisions[0], fine_nodes, s->target_points[t],
+            return launch_target_outputs(st, decisions, fine_nodes, s->target_points[t],
                                          s->target_confidence[t], s->target_codes[t],
                                          s->target_counts + 3 * (size_t)t);
         }));
@@ -579,9 +578,10 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     CHD_ALLOC(s->fine_tested, fine_nodes);
     CHD_ALLOC(s->wide, fine_nodes);
     CHD_ALLOC(s->wide_count, 1);
+    CHD_ALLOC(s->direct_scores, (size_t)fine_nodes * kDirectSlots);
     CHD_ALLOC(s->parent, fine_nodes);
     CHD_ALLOC(s->sizes, fine_nodes);
-    CHD_ALLOC(s->decisions[0], fine_nodes);
+    CHD_ALLOC(s->decisions[0], std::max<size_t>(s->active.size(), 1) * fine_nodes);
     CHD_ALLOC(s->decisions[1], fine_nodes);
     s->target_points.assign(count, nullptr);
     s->target_confidence.assign(count, nullptr);

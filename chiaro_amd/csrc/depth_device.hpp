@@ -166,6 +166,16 @@ __device__ __forceinline__ double libm_exp(double x) {
     return __builtin_fma(scale, tmp, scale);
 }
 
+// Option<NodeDepth>::None
+__device__ __forceinline__ void clear_node(chd_node_depth &n) {
+    n.depth = 0.0;
+    n.confidence = 0.0f;
+    n.improvement = 0.0f;
+    n.valid = 0;
+    n.regularized = 0;
+    for (int k = 0; k < 6; k++) n.pad[k] = 0;
+}
+
 // ------------------------------------------------------------- math.rs
 
 struct d3 {
@@ -370,16 +380,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const DevPlane &p) 
 
 // Plane::sample without the bounds test, for points proven inside
 // [0, w-1] x [0, h-1]: two 8-byte loads fetch (x0, x0 + 1) of rows y0 and
-// y0 + 1.
+// y0 + 1 (the second through the scalar offset `row_bytes` = pitch * 4).
+// For x >= 0, truncation is floor and x - floor(x) is exact, which is what
+// the hardware fract computes.
 __device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc, uint32_t pitch,
-                                                     float x, float y) {
-    const float fx = floorf(x), fy = floorf(y);
-    const uint32_t offset = ((uint32_t)fy * pitch + (uint32_t)fx) * 4u;
+                                                     uint32_t row_bytes, float x, float y) {
+    const uint32_t offset = ((uint32_t)y * pitch + (uint32_t)x) * 4u;
     const float2 top2 =
         __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
     const float2 bot2 = __builtin_bit_cast(
-        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset + pitch * 4u, 0, 0));
-    const float tx = x - fx, ty = y - fy;
+        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, row_bytes, 0));
+    const float tx = __builtin_amdgcn_fractf(x), ty = __builtin_amdgcn_fractf(y);
     const float top = top2.x * (1.0f - tx) + top2.y * tx;
     const float bottom = bot2.x * (1.0f - tx) + bot2.y * tx;
     return top * (1.0f - ty) + bottom * ty;

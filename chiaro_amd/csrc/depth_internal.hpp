@@ -33,6 +33,10 @@ struct GridSpec {
     uint32_t columns, rows, step;
 };
 
+// Aggregated scores per fine node in the direct pass: the global-warp
+// baseline and up to 32 candidate depths (depth.rs:1078-1107).
+constexpr uint32_t kDirectSlots = 33;
+
 // Entry of the wide-fallback list of the direct pass (depth.rs:803-846).
 struct WideEntry {
     uint32_t node;
@@ -42,8 +46,9 @@ struct WideEntry {
     double depth[7];
 };
 
-// Kernel launchers (depth_kernels.hip). All enqueue on `stream` and return
-// the hipError_t of the launch.
+// Kernel launchers (the ZNCC stages in depth_zncc.hip, the rest in
+// depth_kernels.hip). All enqueue on `stream` and return the hipError_t of
+// the launch.
 hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
                               const double *labels, uint32_t label_count,
                               uint32_t minimum_support, uint32_t best_view_count,
@@ -74,7 +79,8 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
                          const GridSpec &coarse, const chd_node_depth *coarse_field,
                          double inverse_step, const chd_options &direct_options,
                          chd_node_depth *field, float *guidance, uint8_t *tested,
-                         WideEntry *wide, uint32_t *wide_count, unsigned long long *work);
+                         WideEntry *wide, uint32_t *wide_count, float *agg,
+                         unsigned long long *work);
 
 hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                               const chd_options &wide_options, const WideEntry *wide,
@@ -94,6 +100,7 @@ hipError_t launch_fit_planes(hipStream_t stream, const chd_node_depth *source,
                              chd_node_depth *field, const float *guidance, uint32_t columns,
                              uint32_t rows, double inverse_tolerance);
 
+// Every view of patch.vlist is a target; decisions is [view slot][node].
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                           const chd_node_depth *field, double inverse_step,
                           const chd_options &options, chd_node_warp *decisions,

@@ -1,0 +1,1069 @@
+// depth_zncc.hip - the ZNCC kernels of the depth hot path (gfx950).
+//
+// Stage map (reference lines in crates/chiaro-fusion/src/depth.rs):
+//   k_cost_volume      build_cost_volume              :1187-1277
+//   k_direct_score     measure_direct_depths, scoring :752-802
+//   k_direct_select    select_direct_depth + fallback trigger :796-810, 1109-1185
+//   k_direct_wide      the r = 8 fallback             :803-846
+//   k_targets          per-target refinement and decision :420-487, 1634-1736
+//
+// All of them evaluate warped_patch_zncc (depth.rs:1772-1822) through one
+// block-level patch context in LDS, built in two parts:
+//  * per node (patch_nodes): the reference-tap samples, bilateral weights w
+//    and w * r, and the label- and view-independent sums sum(w), sum(w r),
+//    sum(w r^2) accumulated in tap order (the reference keeps each of its
+//    five sums in its own accumulator, so hoisting these three is exact);
+//  * per view, rebuilt inside the view loop (patch_view): the global-warp
+//    tap positions, the centre's global point, the label-independent half of
+//    map_at_depth (the corrected reference ray) and the min/max warp tap,
+//    which decides Plane::sample's bounds test for all taps at once because
+//    ((f64(g) + delta) - 0.5) * 0.5 -> f32 is monotone in g.
+// A work item is one (node, hypothesis) ZNCC for the current view, looping
+// the taps in the reference's dy-major order. With the view loop outermost
+// the target plane, its buffer resource and the camera are uniform across
+// the block.
+#include <hip/hip_runtime.h>
+
+#include "depth_device.hpp"
+#include "depth_internal.hpp"
+
+namespace chd {
+
+constexpr uint32_t kMaxCandidates = 32;  // direct_depth_candidates bound (depth.rs:1101-1105)
+
+// ---------------------------------------------------------- patch context
+
+struct PatchLayout {
+    uint32_t cams, pos, ray, ext, gc, gq, wr, sums, centre, spatial, node_ok, nv_ok, gc_ok,
+        ray_ok, total;
+};
+
+__host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
+
+__host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t) {
+    PatchLayout l;
+    uint32_t o = 0;
+    l.cams = o; o = align16(o + v * (uint32_t)sizeof(chd_camera));
+    l.pos = o; o = align16(o + nb * 16u);
+    l.ray = o; o = align16(o + nb * 6u * 8u);
+    l.ext = o; o = align16(o + nb * 16u);
+    l.gc = o; o = align16(o + nb * 8u);
+    l.gq = o; o = align16(o + nb * t * 8u);  // also holds the reference samples early on
+    l.wr = o; o = align16(o + nb * t * 8u);
+    l.sums = o; o = align16(o + nb * 3u * 4u);
+    l.centre = o; o = align16(o + nb * 4u);
+    l.spatial = o; o = align16(o + t * 4u);
+    l.node_ok = o; o = align16(o + nb * 4u);
+    l.nv_ok = o; o = align16(o + nb * 4u);
+    l.gc_ok = o; o = align16(o + nb * 4u);
+    l.ray_ok = o; o = align16(o + nb * 4u);
+    l.total = o;
+    return l;
+}
+
+struct PatchLds {
+    chd_camera *cams;  // cameras of the scored views, in view-slot order
+    double2 *pos;      // raster position of each node
+    double *ray;       // [node][6]: corrected reference ray (current view)
+    float4 *ext;       // [node]: min x, max x, min y, max y warp tap (current view)
+    float2 *gc;        // [node]: global point of the centre (current view)
+    float2 *gq;        // [node][tap]: global-warp tap positions (current view)
+    float *rv;         // [node][tap]: reference samples (aliases gq until the views)
+    float2 *wr;        // [node][tap]: w, w * r
+    float *sums;       // [node][3]: sum w, sum w r, sum w r^2
+    float *centre;
+    float *spatial;    // [tap]
+    int *node_ok;      // every reference sample of the patch exists
+    int *nv_ok;        // current view: global warp defined at the centre and every tap
+    int *gc_ok;        // current view: global warp defined at the centre
+    int *ray_ok;       // current view: corrected reference ray exists
+};
+
+__device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l) {
+    PatchLds p;
+    p.cams = (chd_camera *)(base + l.cams);
+    p.pos = (double2 *)(base + l.pos);
+    p.ray = (double *)(base + l.ray);
+    p.ext = (float4 *)(base + l.ext);
+    p.gc = (float2 *)(base + l.gc);
+    p.gq = (float2 *)(base + l.gq);
+    p.rv = (float *)(base + l.gq);
+    p.wr = (float2 *)(base + l.wr);
+    p.sums = (float *)(base + l.sums);
+    p.centre = (float *)(base + l.centre);
+    p.spatial = (float *)(base + l.spatial);
+    p.node_ok = (int *)(base + l.node_ok);
+    p.nv_ok = (int *)(base + l.nv_ok);
+    p.gc_ok = (int *)(base + l.gc_ok);
+    p.ray_ok = (int *)(base + l.ray_ok);
+    return p;
+}
+
+__device__ __forceinline__ DevPlane view_plane(const ViewDev &v) {
+    DevPlane p;
+    p.data = v.plane;
+    p.width = v.plane_width;
+    p.height = v.plane_height;
+    p.pitch = v.plane_pitch;
+    return p;
+}
+
+__device__ __forceinline__ DevWarp view_warp(const ViewDev &v) {
+    DevWarp w;
+    w.points = v.warp;
+    w.step = v.warp_step;
+    w.columns = v.warp_columns;
+    w.rows = v.warp_rows;
+    return w;
+}
+
+// Node half of the patch context. On entry P.pos[n] and P.node_ok[n] (node
+// exists) are set and visible. On exit node_ok[n] is 0 when the centre or any
+// tap of the reference patch falls outside the reference plane (every ZNCC of
+// the node is then None), centre_value[n] holds reference_guidance
+// (depth.rs:1824-1832), and the cameras of all scored views are in LDS.
+__device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
+                            float *centre_value) {
+    const uint32_t V = S.view_count, tid = threadIdx.x, nth = blockDim.x;
+    const int r = (int)S.radius;
+    const uint32_t width = 2u * S.radius + 1u;
+    const DevPlane rp = view_plane(S.views[S.reference]);
+    const float sigma = rmaxf((float)S.radius * 0.75f, 1.0f);
+    constexpr uint32_t words = sizeof(chd_camera) / 4;
+    for (uint32_t i = tid; i < V * words; i += nth) {
+        const uint32_t *src = (const uint32_t *)S.views[S.vlist[i / words]].camera;
+        ((uint32_t *)P.cams)[i] = src[i % words];
+    }
+    for (uint32_t n = tid; n < nb; n += nth) centre_value[n] = 0.0f;
+    __syncthreads();
+    for (uint32_t t = tid; t < T; t += nth) {
+        const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+        const float distance_sq = (float)(dx * dx + dy * dy);
+        P.spatial[t] = libm_expf(-distance_sq / (2.0f * sigma * sigma));
+    }
+    for (uint32_t n = tid; n < nb; n += nth) {
+        const double2 p = P.pos[n];
+        float c;
+        if (plane_sample(rp, to_plane(p.x), to_plane(p.y), c)) {
+            P.centre[n] = c;
+            centre_value[n] = c;
+        } else {
+            P.node_ok[n] = 0;
+        }
+    }
+    for (uint32_t i = tid; i < nb * T; i += nth) {
+        const uint32_t n = i / T, t = i % T;
+        const double2 p = P.pos[n];
+        const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+        float v;
+        if (!plane_sample(rp, to_plane(p.x + (double)dx * 2.0), to_plane(p.y + (double)dy * 2.0),
+                          v)) {
+            P.node_ok[n] = 0;
+            v = 0.0f;
+        }
+        P.rv[i] = v;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nb * T; i += nth) {
+        const uint32_t n = i / T, t = i % T;
+        if (!P.node_ok[n]) continue;
+        const float rv = P.rv[i];
+        const float range = libm_expf(-1.2f * fabsf(rv - P.centre[n]));
+        const float w = P.spatial[t] * range;
+        P.wr[i] = make_float2(w, w * rv);
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < nb; n += nth) {
+        if (!P.node_ok[n]) continue;
+        float count = 0.0f, sum_r = 0.0f, sum_rr = 0.0f;
+        for (uint32_t t = 0; t < T; t++) {
+            const float2 w = P.wr[n * T + t];
+            count += w.x;
+            sum_r += w.y;
+            sum_rr += w.y * P.rv[n * T + t];
+        }
+        P.sums[n * 3 + 0] = count;
+        P.sums[n * 3 + 1] = sum_r;
+        P.sums[n * 3 + 2] = sum_rr;
+    }
+    __syncthreads();
+}
+
+// View half of the patch context for view slot `a`. Must be entered by the
+// whole block after every read of the previous view's context has finished.
+__device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
+                           uint32_t a) {
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const int r = (int)S.radius;
+    const uint32_t width = 2u * S.radius + 1u;
+    const ViewDev &view = S.views[S.vlist[a]];
+    const DevWarp gw = view_warp(view);
+    const chd_camera &reference = *S.views[S.reference].camera;
+    for (uint32_t n = tid; n < nb; n += nth) {
+        P.nv_ok[n] = 1;
+        P.gc_ok[n] = 1;
+        P.ray_ok[n] = 1;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nb * T; i += nth) {
+        const uint32_t n = i / T, t = i % T;
+        const double2 p = P.pos[n];
+        const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+        float2 q;
+        if (!warp_map(gw, (float)(p.x + (double)dx * 2.0), (float)(p.y + (double)dy * 2.0), q)) {
+            P.nv_ok[n] = 0;
+            q = make_float2(0.0f, 0.0f);
+        }
+        P.gq[i] = q;
+    }
+    for (uint32_t n = tid; n < nb; n += nth) {
+        const double2 p = P.pos[n];
+        float2 gc;
+        if (!warp_map(gw, (float)p.x, (float)p.y, gc)) {
+            P.nv_ok[n] = 0;
+            P.gc_ok[n] = 0;
+            P.ray_ok[n] = 0;
+            continue;
+        }
+        P.gc[n] = gc;
+        // corrected_reference = reference.map_from(target, gc, 1e8)
+        // (depth.rs:1752-1756), then the reference ray through it
+        d3 o, d;
+        pixel_to_ray(P.cams[a], (double)gc.x, (double)gc.y, o, d);
+        const d3 far{o.x + d.x * kInfinityDepth, o.y + d.y * kInfinityDepth,
+                     o.z + d.z * kInfinityDepth};
+        double cx, cy;
+        if (!project(reference, far, cx, cy)) {
+            P.ray_ok[n] = 0;
+            continue;
+        }
+        pixel_to_ray(reference, cx, cy, o, d);
+        double *ray = P.ray + (size_t)n * 6;
+        ray[0] = o.x;
+        ray[1] = o.y;
+        ray[2] = o.z;
+        ray[3] = d.x;
+        ray[4] = d.y;
+        ray[5] = d.z;
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < nb; n += nth) {
+        if (!P.nv_ok[n]) continue;
+        const float2 *g = P.gq + (size_t)n * T;
+        float x0 = g[0].x, x1 = g[0].x, y0 = g[0].y, y1 = g[0].y;
+        for (uint32_t t = 1; t < T; t++) {
+            x0 = fminf(x0, g[t].x);
+            x1 = fmaxf(x1, g[t].x);
+            y0 = fminf(y0, g[t].y);
+            y1 = fmaxf(y1, g[t].y);
+        }
+        P.ext[n] = make_float4(x0, x1, y0, y1);
+    }
+    __syncthreads();
+}
+
+// warped_patch_zncc (depth.rs:1772-1822) of node n against the current view
+// displaced by the raster delta (dx, dy); false is the reference's None.
+__device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
+                                           const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
+                                           uint32_t row_bytes, double dx, double dy, float &score,
+                                           unsigned int *evaluations) {
+    const float4 e = P.ext[n];
+    const float xmin = to_plane((double)e.x + dx), xmax = to_plane((double)e.y + dx);
+    const float ymin = to_plane((double)e.z + dy), ymax = to_plane((double)e.w + dy);
+    if (!(xmin >= 0.0f && ymin >= 0.0f && xmax <= (float)(tp.width - 1) &&
+          ymax <= (float)(tp.height - 1)))
+        return false;
+    atomicAdd(evaluations, 1u);
+    const float2 *wr = P.wr + (size_t)n * T;
+    const float2 *gq = P.gq + (size_t)n * T;
+    float st = 0.0f, stt = 0.0f, sp = 0.0f;
+#pragma unroll 9
+    for (uint32_t t = 0; t < T; t++) {
+        const float2 w = wr[t];
+        const float2 q = gq[t];
+        const float x = to_plane((double)q.x + dx);
+        const float y = to_plane((double)q.y + dy);
+        const float tv = plane_sample_inside(rsrc, tp.pitch, row_bytes, x, y);
+        const float wv = w.x * tv;
+        st += wv;
+        stt += wv * tv;
+        sp += w.y * tv;
+    }
+    const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
+    const float covariance = sp - sr * st / count;
+    const float reference_energy = srr - sr * sr / count;
+    const float target_energy = stt - st * st / count;
+    const float denominator = sqrtf(rmaxf(reference_energy, 0.0f) * rmaxf(target_energy, 0.0f));
+    if (!(denominator > 1.0e-6f)) return false;
+    score = rclampf(covariance / denominator, -1.0f, 1.0f);
+    return true;
+}
+
+// map_at_depth (depth.rs:1738-1759) from the cached corrected ray of node n,
+// then score_one_view's 128 px parallax gate (depth.rs:1604-1614).
+__device__ __forceinline__ bool delta_at_depth(const PatchLds &P, uint32_t n,
+                                               const chd_camera &target, double depth, double &dx,
+                                               double &dy) {
+    if (!P.ray_ok[n]) return false;
+    const double *r = P.ray + (size_t)n * 6;
+    const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
+    double qx, qy;
+    if (!project(target, point, qx, qy)) return false;
+    if (!(finite_d(qx) && finite_d(qy))) return false;
+    const float2 gc = P.gc[n];
+    dx = qx - (double)gc.x;
+    dy = qy - (double)gc.y;
+    return !(fabs(dx) > 128.0 || fabs(dy) > 128.0);
+}
+
+// aggregate (depth.rs:1761-1770) of V view scores at stride `stride` (NaN =
+// the view returned None): mean of the best best_view_count in descending
+// total_cmp order, summed from -0.0.
+__device__ __forceinline__ bool aggregate_scores(const float *res, uint32_t stride, uint32_t V,
+                                                 uint32_t minimum_support,
+                                                 uint32_t best_view_count, float &out) {
+    uint32_t count = 0;
+    for (uint32_t a = 0; a < V; a++) count += res[a * stride] == res[a * stride] ? 1u : 0u;
+    if (count < minimum_support) return false;
+    const uint32_t k = best_view_count < count ? best_view_count : count;
+    if (k < minimum_support) return false;
+    uint32_t used = 0;
+    float sum = -0.0f;
+    for (uint32_t i = 0; i < k; i++) {
+        int best = -1;
+        int32_t best_key = 0;
+        for (uint32_t a = 0; a < V; a++) {
+            const float s = res[a * stride];
+            if ((used >> a) & 1u || s != s) continue;
+            const int32_t key = tkey32(s);
+            if (best < 0 || key > best_key) {
+                best = (int)a;
+                best_key = key;
+            }
+        }
+        used |= 1u << best;
+        sum += res[best * stride];
+    }
+    out = sum / (float)k;
+    return true;
+}
+
+__device__ __forceinline__ float nan_f() { return __int_as_float(0x7fc00000); }
+
+__device__ __forceinline__ double2 node_position(const GridSpec &G, uint32_t g) {
+    const uint32_t row = g / G.columns, column = g % G.columns;
+    return make_double2((double)((size_t)column * G.step), (double)((size_t)row * G.step));
+}
+
+// ------------------------------------------------------------ cost volume
+
+__global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, const double *labels,
+                                                     uint32_t L, uint32_t nb, uint32_t T,
+                                                     uint32_t minimum_support,
+                                                     uint32_t best_view_count, float *scores,
+                                                     float *costs, float *guidance,
+                                                     uint8_t *tested, unsigned long long *work) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    const uint32_t V = S.view_count;
+    const PatchLayout lay = patch_layout(nb, V, T);
+    const PatchLds P = bind_patch(lds, lay);
+    float *res = (float *)(lds + lay.total);          // [nb][V][L]
+    float *centre_value = res + (size_t)nb * V * L;   // [nb]
+    int *node_tested = (int *)(centre_value + nb);    // [nb]
+    const uint32_t nodes = G.columns * G.rows, node0 = blockIdx.x * nb;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+
+    for (uint32_t n = tid; n < nb; n += nth) {
+        P.pos[n] = node_position(G, node0 + n);
+        P.node_ok[n] = node0 + n < nodes ? 1 : 0;
+        node_tested[n] = 0;
+    }
+    if (tid == 0) evaluation_count = 0;
+    __syncthreads();
+    patch_nodes(S, P, nb, T, centre_value);
+
+    for (uint32_t a = 0; a < V; a++) {
+        patch_view(S, P, nb, T, a);
+        const DevPlane tp = view_plane(S.views[S.vlist[a]]);
+        const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
+        const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+        for (uint32_t i = tid; i < nb * L; i += nth) {
+            const uint32_t n = i / L, l = i % L;
+            float s = nan_f();
+            if (P.node_ok[n] && P.nv_ok[n]) {
+                double dx = 0.0, dy = 0.0;
+                bool ok = true;
+                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], labels[l], dx, dy);
+                float sc;
+                if (ok && patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                    s = sc;
+            }
+            res[((size_t)n * V + a) * L + l] = s;
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t i = tid; i < nb * L; i += nth) {
+        const uint32_t n = i / L, l = i % L, g = node0 + n;
+        if (g >= nodes) continue;
+        float score;
+        if (!aggregate_scores(res + (size_t)n * V * L + l, L, V, minimum_support, best_view_count,
+                              score))
+            score = nan_f();
+        float cost;
+        if (finite_f(score)) {
+            node_tested[n] = 1;
+            const float near_prior =
+                l == 0 ? 0.0f : kNearDepthPrior * (float)(l - 1) / (float)(L - 2);
+            cost = rclampf(1.0f - score, 0.0f, 2.0f) + near_prior;
+        } else {
+            cost = kMissingCost;
+        }
+        scores[(size_t)g * L + l] = score;
+        costs[(size_t)g * L + l] = cost;
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < nb; n += nth) {
+        const uint32_t g = node0 + n;
+        if (g >= nodes) continue;
+        guidance[g] = centre_value[n];
+        tested[g] = (uint8_t)node_tested[n];
+    }
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+}
+
+size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t label_count) {
+    const PatchLayout l = patch_layout(nb, views, taps);
+    return (size_t)l.total + (size_t)nb * views * label_count * 4 + (size_t)nb * 8;
+}
+
+hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
+                              const double *labels, uint32_t label_count,
+                              uint32_t minimum_support, uint32_t best_view_count,
+                              float *scores, float *costs, float *guidance, uint8_t *tested,
+                              unsigned long long *work) {
+    const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
+    const uint32_t V = patch.view_count;
+    // about 1024 (node, label) units per block, within 32 KB of LDS (5
+    // blocks of 256 threads per CU)
+    uint32_t nb = 1024 / label_count;
+    if (nb < 1) nb = 1;
+    if (nb > 16) nb = 16;
+    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 32 * 1024) nb--;
+    const size_t lds = cost_lds_bytes(nb, V, T, label_count);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const uint32_t nodes = grid.columns * grid.rows;
+    hipLaunchKernelGGL(k_cost_volume, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
+                       grid, labels, label_count, nb, T, minimum_support, best_view_count, scores,
+                       costs, guidance, tested, work);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ direct pass
+
+// nearest_coarse_depth (depth.rs:1062-1076) and direct_depth_candidates
+// (depth.rs:1078-1107); returns the candidate count.
+__device__ uint32_t direct_candidates(const GridSpec &Cg, const chd_node_depth *coarse, double px,
+                                      double py, double inverse_step, const chd_options &o,
+                                      double *c, bool &seeded) {
+    const double far_inverse = 1.0 / o.far_depth, near_inverse = 1.0 / o.near_depth;
+    const uint32_t cc = sat_u32d(rclamp(round(px / (double)Cg.step), 0.0, (double)(Cg.columns - 1)));
+    const uint32_t cr = sat_u32d(rclamp(round(py / (double)Cg.step), 0.0, (double)(Cg.rows - 1)));
+    const chd_node_depth seed = coarse[(size_t)cr * Cg.columns + cc];
+    seeded = seed.valid != 0;
+    uint32_t count = 0;
+    if (seeded) {
+        for (int offset = -6; offset <= 6; offset++) {
+            const double inverse = rclamp(1.0 / seed.depth + (double)offset * inverse_step * 0.5,
+                                          far_inverse, near_inverse);
+            const double depth = 1.0 / inverse;
+            if (count == 0 || fabs(1.0 / c[count - 1] - 1.0 / depth) > inverse_step * 0.1)
+                c[count++] = depth;
+        }
+    } else {
+        // inverse_depth_samples(near, far, 32), reversed (depth.rs:1834-1843)
+        for (uint32_t k = 0; k < 32; k++) {
+            const double w = (double)(31 - k) / 31.0;
+            c[k] = 1.0 / (near_inverse * (1.0 - w) + far_inverse * w);
+        }
+        count = 32;
+    }
+    return count;
+}
+
+// select_direct_depth (depth.rs:1109-1165) over `count` candidates whose
+// aggregated scores are score[k] (NaN = None).
+__device__ bool select_direct_depth(const double *depths, const float *score, uint32_t count,
+                                    bool has_baseline, float baseline, bool seeded,
+                                    const chd_options &o, chd_node_depth &out) {
+    const double far_inverse = 1.0 / o.far_depth;
+    const double inverse_range = 1.0 / o.near_depth - far_inverse;
+    uint32_t idx[kMaxCandidates];
+    float obj[kMaxCandidates];
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (score[i] != score[i]) continue;
+        const double near_fraction = rclamp((1.0 / depths[i] - far_inverse) / inverse_range, 0.0, 1.0);
+        const float objective = score[i] - kNearDepthPrior * (float)near_fraction;
+        uint32_t j = n;  // stable, descending by total_cmp
+        while (j > 0 && tkey32(obj[j - 1]) < tkey32(objective)) {
+            obj[j] = obj[j - 1];
+            idx[j] = idx[j - 1];
+            j--;
+        }
+        obj[j] = objective;
+        idx[j] = i;
+        n++;
+    }
+    if (n == 0) return false;
+    const uint32_t best_index = idx[0];
+    const float best_score = score[best_index];
+    const float best_objective = obj[0];
+    float competing = best_objective;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t d = idx[k] > best_index ? idx[k] - best_index : best_index - idx[k];
+        if (d > 1) {
+            competing = obj[k];
+            break;
+        }
+    }
+    const float margin = rmaxf(best_objective - competing, 0.0f);
+    const float improvement = has_baseline ? best_score - baseline : 0.0f;
+    const float score_floor = rmaxf(o.minimum_score - 0.03f, kMinimumRegularizedScore);
+    const float required_margin = seeded ? o.minimum_margin * 0.6f : o.minimum_margin;
+    const bool improves_global = !has_baseline || improvement >= o.minimum_improvement * 0.5f;
+    const bool supported = best_score >= score_floor && improves_global &&
+                           (margin >= required_margin || improvement >= o.minimum_improvement);
+    if (!supported) return false;
+    const float score_confidence =
+        rclampf((best_score - score_floor) / rmaxf(1.0f - score_floor, 1.0e-3f), 0.0f, 1.0f);
+    const float margin_confidence = rclampf(margin / rmaxf(required_margin, 1.0e-3f), 0.0f, 1.0f);
+    out.valid = 1;
+    out.depth = depths[best_index];
+    out.confidence =
+        rclampf(0.30f + 0.45f * score_confidence + 0.25f * margin_confidence, 0.0f, 1.0f);
+    out.improvement = improvement;
+    out.regularized = 0;
+    return true;
+}
+
+// best_depth_candidate (depth.rs:1167-1185): the last maximum.
+__device__ int best_depth_candidate(const double *depths, const float *score, uint32_t count,
+                                    const chd_options &o) {
+    const double far_inverse = 1.0 / o.far_depth;
+    const double inverse_range = 1.0 / o.near_depth - far_inverse;
+    int best = -1;
+    int32_t best_key = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (score[i] != score[i]) continue;
+        const double near_fraction = rclamp((1.0 / depths[i] - far_inverse) / inverse_range, 0.0, 1.0);
+        const int32_t key = tkey32(score[i] - kNearDepthPrior * (float)near_fraction);
+        if (best < 0 || key >= best_key) {
+            best = (int)i;
+            best_key = key;
+        }
+    }
+    return best;
+}
+
+// Candidate scoring shared by the direct and wide kernels for view slot a:
+// items are (node, slot), slot 0 the global-warp baseline and slot k the
+// depth cand[node * stride + k - 1]; base[] is the prefix of 1 + ncand over
+// the block's nodes. Results land in res[(node * V + view) * (stride + 1) +
+// slot].
+__device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
+                            uint32_t a, const double *cand, const uint32_t *base, uint32_t stride,
+                            float *res, unsigned int *evaluations) {
+    const uint32_t V = S.view_count, total = base[nb];
+    const DevPlane tp = view_plane(S.views[S.vlist[a]]);
+    const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
+    const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        uint32_t n = 0;
+        while (n + 1 < nb && base[n + 1] <= i) n++;
+        const uint32_t slot = i - base[n];
+        float s = nan_f();
+        if (P.node_ok[n] && P.nv_ok[n]) {
+            double dx = 0.0, dy = 0.0;
+            bool ok = true;
+            if (slot > 0)
+                ok = delta_at_depth(P, n, P.cams[a], cand[n * stride + slot - 1], dx, dy);
+            float sc;
+            if (ok && patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, evaluations)) s = sc;
+        }
+        res[((size_t)n * V + a) * (stride + 1) + slot] = s;
+    }
+}
+
+// The scoring half of measure_direct_depths (depth.rs:752-795): for every
+// fine node the global baseline and each candidate depth, aggregated over
+// the views, go to agg[node][slot] (NaN = None); also guidance and tested.
+__global__ void __launch_bounds__(256) k_direct_score(PatchSpec S, GridSpec F, GridSpec Cg,
+                                                      const chd_node_depth *__restrict__ coarse,
+                                                      double inverse_step, chd_options o,
+                                                      uint32_t T, uint32_t nb, float *agg,
+                                                      float *guidance, uint8_t *tested,
+                                                      unsigned long long *work) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    const uint32_t V = S.view_count;
+    const PatchLayout lay = patch_layout(nb, V, T);
+    const PatchLds P = bind_patch(lds, lay);
+    double *cand = (double *)(lds + lay.total);                 // [nb][32]
+    float *res = (float *)(cand + nb * kMaxCandidates);         // [nb][V][33]
+    float *centre_value = res + (size_t)nb * V * kDirectSlots;  // [nb]
+    uint32_t *ncand = (uint32_t *)(centre_value + nb);          // [nb]
+    uint32_t *node_tested = ncand + nb;                         // [nb]
+    uint32_t *base = node_tested + nb;                          // [nb + 1]
+    const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
+
+    for (uint32_t n = tid; n < nb; n += blockDim.x) {
+        const uint32_t g = node0 + n;
+        const double2 p = node_position(F, g);
+        P.pos[n] = p;
+        P.node_ok[n] = g < nodes ? 1 : 0;
+        node_tested[n] = 0;
+        bool seeded;
+        ncand[n] = g < nodes ? direct_candidates(Cg, coarse, p.x, p.y, inverse_step, o,
+                                                 cand + n * kMaxCandidates, seeded)
+                             : 0;
+    }
+    if (tid == 0) evaluation_count = 0;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t b = 0;
+        for (uint32_t n = 0; n < nb; n++) {
+            base[n] = b;
+            b += node0 + n < nodes ? 1 + ncand[n] : 0;
+        }
+        base[nb] = b;
+    }
+    patch_nodes(S, P, nb, T, centre_value);
+    for (uint32_t a = 0; a < V; a++) {
+        patch_view(S, P, nb, T, a);
+        score_slots(S, P, nb, T, a, cand, base, kMaxCandidates, res, &evaluation_count);
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x) {
+        const uint32_t n = i / kDirectSlots, slot = i % kDirectSlots;
+        const uint32_t g = node0 + n;
+        if (g >= nodes || slot > ncand[n]) continue;
+        float a;
+        if (!aggregate_scores(res + (size_t)n * V * kDirectSlots + slot, kDirectSlots, V,
+                              o.minimum_support, o.best_view_count, a))
+            a = nan_f();
+        else
+            node_tested[n] = 1;  // baseline or any candidate measured (depth.rs:794-795)
+        agg[(size_t)g * kDirectSlots + slot] = a;
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < nb; n += blockDim.x) {
+        const uint32_t g = node0 + n;
+        if (g >= nodes) continue;
+        guidance[g] = centre_value[n];
+        tested[g] = (uint8_t)node_tested[n];
+    }
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+}
+
+// The selection half (depth.rs:796-810): select_direct_depth, or an entry
+// of the wide-fallback list when the best candidate scores at least 0.30.
+// One thread per fine node; the candidates are recomputed exactly as the
+// scoring kernel made them.
+__global__ void __launch_bounds__(256) k_direct_select(GridSpec F, GridSpec Cg,
+                                                       const chd_node_depth *__restrict__ coarse,
+                                                       double inverse_step, chd_options o,
+                                                       const float *__restrict__ agg,
+                                                       chd_node_depth *field, WideEntry *wide,
+                                                       uint32_t *wide_count) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= F.columns * F.rows) return;
+    const double2 p = node_position(F, g);
+    double c[kMaxCandidates];
+    bool seeded;
+    const uint32_t count = direct_candidates(Cg, coarse, p.x, p.y, inverse_step, o, c, seeded);
+    const float *sc = agg + (size_t)g * kDirectSlots;
+    const float baseline = sc[0];
+    chd_node_depth selected;
+    clear_node(selected);
+    if (!select_direct_depth(c, sc + 1, count, baseline == baseline, baseline, seeded, o,
+                             selected)) {
+        const int best = best_depth_candidate(c, sc + 1, count, o);
+        if (best >= 0 && sc[1 + best] >= kMinimumRegularizedScore) {
+            const uint32_t first = best >= 3 ? (uint32_t)best - 3 : 0;
+            const uint32_t last = (uint32_t)best + 3 < count - 1 ? (uint32_t)best + 3 : count - 1;
+            WideEntry e;
+            e.node = g;
+            e.count = last - first + 1;
+            e.seeded = seeded ? 1u : 0u;
+            e.pad = 0;
+            for (uint32_t k = 0; k < 7; k++) e.depth[k] = k < e.count ? c[first + k] : 0.0;
+            wide[atomicAdd(wide_count, 1u)] = e;
+        }
+    }
+    field[g] = selected;
+}
+
+size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
+    const PatchLayout l = patch_layout(nb, V, T);
+    return (size_t)l.total + (size_t)nb * kMaxCandidates * 8 +
+           (size_t)nb * V * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4;
+}
+
+hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                         const GridSpec &coarse, const chd_node_depth *coarse_field,
+                         double inverse_step, const chd_options &direct_options,
+                         chd_node_depth *field, float *guidance, uint8_t *tested,
+                         WideEntry *wide, uint32_t *wide_count, float *agg,
+                         unsigned long long *work) {
+    const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
+    const uint32_t V = patch.view_count;
+    // 16 nodes give 224 (seeded) to 528 (unseeded) items per view for 256
+    // threads; 40 KB of LDS keeps 4 blocks per CU
+    uint32_t nb = 16;
+    while (nb > 1 && direct_lds_bytes(nb, V, T) > 40 * 1024) nb--;
+    const size_t lds = direct_lds_bytes(nb, V, T);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const uint32_t nodes = fine.columns * fine.rows;
+    hipLaunchKernelGGL(k_direct_score, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
+                       fine, coarse, coarse_field, inverse_step, direct_options, T, nb, agg,
+                       guidance, tested, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_direct_select, dim3((nodes + 255) / 256), dim3(256), 0, stream, fine,
+                       coarse, coarse_field, inverse_step, direct_options, agg, field, wide,
+                       wide_count);
+    return hipGetLastError();
+}
+
+constexpr uint32_t kWideNodes = 8;  // wide-fallback nodes per block (at most)
+constexpr uint32_t kWideSlots = 8;  // baseline + at most 7 candidates
+
+size_t wide_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
+    const PatchLayout l = patch_layout(nb, V, T);
+    return (size_t)l.total + (size_t)nb * kWideSlots * 8 + (size_t)nb * V * kWideSlots * 4 +
+           (size_t)nb * kWideSlots * 4 + (size_t)nb * 4 * 2 + (nb + 1) * 4;
+}
+
+// The max(2r, 8) re-measurement of nodes whose first pass produced no
+// supported depth but a candidate scoring at least 0.30 (depth.rs:803-846).
+// A fixed grid walks the entry list.
+__global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, chd_options o,
+                                                     uint32_t T, uint32_t nb,
+                                                     const WideEntry *wide,
+                                                     const uint32_t *wide_count,
+                                                     chd_node_depth *field,
+                                                     unsigned long long *work) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    const uint32_t V = S.view_count;
+    const PatchLayout lay = patch_layout(nb, V, T);
+    const PatchLds P = bind_patch(lds, lay);
+    double *cand = (double *)(lds + lay.total);                   // [nb][7]
+    float *res = (float *)(cand + nb * kWideSlots);               // [nb][V][8]
+    float *agg = res + (size_t)nb * V * kWideSlots;               // [nb][8]
+    float *centre_value = agg + nb * kWideSlots;                  // [nb]
+    uint32_t *ncand = (uint32_t *)(centre_value + nb);            // [nb]
+    uint32_t *base = ncand + nb;                                  // [nb + 1]
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) evaluation_count = 0;
+    const uint32_t count = *wide_count;
+    for (uint32_t first = blockIdx.x * nb; first < count; first += gridDim.x * nb) {
+        __syncthreads();
+        for (uint32_t n = tid; n < nb; n += blockDim.x) {
+            const uint32_t e = first + n;
+            if (e < count) {
+                const WideEntry w = wide[e];
+                P.pos[n] = node_position(F, w.node);
+                P.node_ok[n] = 1;
+                ncand[n] = w.count;
+                for (uint32_t k = 0; k < 7; k++) cand[n * (kWideSlots - 1) + k] = w.depth[k];
+            } else {
+                P.pos[n] = make_double2(0.0, 0.0);
+                P.node_ok[n] = 0;
+                ncand[n] = 0;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t b = 0;
+            for (uint32_t n = 0; n < nb; n++) {
+                base[n] = b;
+                b += first + n < count ? 1 + ncand[n] : 0;
+            }
+            base[nb] = b;
+        }
+        patch_nodes(S, P, nb, T, centre_value);
+        for (uint32_t a = 0; a < V; a++) {
+            patch_view(S, P, nb, T, a);
+            score_slots(S, P, nb, T, a, cand, base, kWideSlots - 1, res, &evaluation_count);
+            __syncthreads();
+        }
+        for (uint32_t i = tid; i < nb * kWideSlots; i += blockDim.x) {
+            const uint32_t n = i / kWideSlots, slot = i % kWideSlots;
+            float a = nan_f();
+            if (first + n < count && slot <= ncand[n] &&
+                !aggregate_scores(res + (size_t)n * V * kWideSlots + slot, kWideSlots, V,
+                                  o.minimum_support, o.best_view_count, a))
+                a = nan_f();
+            agg[i] = a;
+        }
+        __syncthreads();
+        for (uint32_t n = tid; n < nb; n += blockDim.x) {
+            const uint32_t e = first + n;
+            if (e >= count) continue;
+            const float *sc = agg + n * kWideSlots;
+            chd_node_depth selected;
+            clear_node(selected);
+            select_direct_depth(cand + n * (kWideSlots - 1), sc + 1, ncand[n], sc[0] == sc[0], sc[0],
+                                wide[e].seeded != 0, o, selected);
+            field[wide[e].node] = selected;
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+}
+
+hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                              const chd_options &wide_options, const WideEntry *wide,
+                              const uint32_t *wide_count, uint32_t wide_capacity,
+                              chd_node_depth *field, unsigned long long *work) {
+    const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
+    const uint32_t V = patch.view_count;
+    uint32_t nb = kWideNodes;
+    while (nb > 1 && wide_lds_bytes(nb, V, T) > 64 * 1024) nb--;
+    const size_t lds = wide_lds_bytes(nb, V, T);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    uint32_t blocks = (wide_capacity + nb - 1) / nb;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_direct_wide, dim3(blocks), dim3(256), lds, stream, patch, fine,
+                       wide_options, T, nb, wide, wide_count, field, work);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ per-target stage
+
+constexpr uint32_t kTargetNodes = 16;
+
+// refine_one_view (depth.rs:1634-1708) and the decision block
+// (depth.rs:429-486) for every target (S.vlist) over a tile of fine nodes,
+// writing decisions[target slot][node]. Per target: phase 0 maps the five
+// shared-depth offsets, phase 1 scores them and the global baseline, phase 2
+// the eight +-1.5 px residuals around the best depth. Any None inside
+// refine_one_view makes the whole refinement None (its `?`s).
+__global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
+                                                 const chd_node_depth *__restrict__ field,
+                                                 double inverse_step, chd_options o, uint32_t T,
+                                                 uint32_t nb, chd_node_warp *decisions,
+                                                 unsigned long long *work) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    const uint32_t V = S.view_count;
+    const PatchLayout lay = patch_layout(nb, V, T);
+    const PatchLds P = bind_patch(lds, lay);
+    float2 *points = (float2 *)(lds + lay.total);  // [nb][5]
+    float2 *best_point = points + nb * 5;          // [nb]
+    float *phase = (float *)(best_point + nb);     // [nb][8]
+    float *centre_value = phase + nb * 8;          // [nb]
+    float *best_score = centre_value + nb;         // [nb]
+    float *baseline = best_score + nb;             // [nb]
+    int *state = (int *)(baseline + nb);           // [nb] 0 skip, 1 refine, -1 None
+    const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
+    const double min_inverse = 1.0 / o.far_depth, max_inverse = 1.0 / o.near_depth;
+    const float steps[3] = {-1.5f, 0.0f, 1.5f};
+
+    for (uint32_t n = tid; n < nb; n += blockDim.x) {
+        P.pos[n] = node_position(F, node0 + n);
+        P.node_ok[n] = node0 + n < nodes ? 1 : 0;
+    }
+    if (tid == 0) evaluation_count = 0;
+    __syncthreads();
+    patch_nodes(S, P, nb, T, centre_value);
+
+    for (uint32_t a = 0; a < V; a++) {
+        patch_view(S, P, nb, T, a);
+        const DevPlane tp = view_plane(S.views[S.vlist[a]]);
+        const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
+        const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+        chd_node_warp *out = decisions + (size_t)a * nodes;
+        // decision skeleton: Undefined / Global / to refine (depth.rs:433-440)
+        for (uint32_t n = tid; n < nb; n += blockDim.x) {
+            const uint32_t g = node0 + n;
+            state[n] = 0;
+            if (g >= nodes) continue;
+            chd_node_warp d;
+            d.code = CHD_WARP_UNDEFINED;
+            d.global[0] = d.global[1] = 0.0f;
+            d.point[0] = d.point[1] = 0.0f;
+            d.confidence = 0.0f;
+            d.measured = 0;
+            if (P.gc_ok[n]) {
+                d.code = CHD_WARP_GLOBAL;
+                d.global[0] = P.gc[n].x;
+                d.global[1] = P.gc[n].y;
+                if (field[g].valid) state[n] = 1;
+            }
+            out[g] = d;
+        }
+        __syncthreads();
+        // phase 0: map_at_depth at the five offsets
+        for (uint32_t i = tid; i < nb * 5; i += blockDim.x) {
+            const uint32_t n = i / 5, k = i % 5;
+            if (state[n] != 1) continue;
+            const double offsets[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
+            const double shared_inverse = 1.0 / field[node0 + n].depth;
+            const double inverse =
+                rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
+            const double depth = 1.0 / inverse;
+            bool ok = P.ray_ok[n] != 0;
+            double qx = 0.0, qy = 0.0;
+            if (ok) {
+                const double *r = P.ray + (size_t)n * 6;
+                const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
+                ok = project(P.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
+            }
+            points[i] = make_float2((float)qx, (float)qy);
+            if (!ok) state[n] = -1;
+        }
+        __syncthreads();
+        // phase 1: the five offsets (score_one_view_at_point) and the baseline
+        // (score_one_view with depth None: delta 0)
+        for (uint32_t i = tid; i < nb * 6; i += blockDim.x) {
+            const uint32_t n = i / 6, k = i % 6;
+            if (state[n] == 0) continue;
+            float s = nan_f(), sc;
+            if (k == 5) {
+                if (P.node_ok[n] && P.nv_ok[n] &&
+                    patch_zncc(P, n, T, tp, rsrc, row_bytes, 0.0, 0.0, sc, &evaluation_count))
+                    s = sc;
+                baseline[n] = s;
+            } else if (state[n] == 1) {
+                const float2 q = points[n * 5 + k];
+                const double dx = (double)(q.x - P.gc[n].x), dy = (double)(q.y - P.gc[n].y);
+                if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
+                    patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                    s = sc;
+                phase[n * 8 + k] = s;
+            }
+        }
+        __syncthreads();
+        for (uint32_t n = tid; n < nb; n += blockDim.x) {
+            if (state[n] != 1) continue;
+            const float magnitude[5] = {1.0f, 0.5f, 0.0f, 0.5f, 1.0f};
+            bool have = false, failed = false;
+            float bo = 0.0f, bs = 0.0f;
+            float2 bp = make_float2(0.0f, 0.0f);
+            for (uint32_t k = 0; k < 5; k++) {
+                const float s = phase[n * 8 + k];
+                if (s != s) {
+                    failed = true;
+                    break;
+                }
+                const float objective = s - magnitude[k] * 0.002f;
+                if (!have || objective > bo) {
+                    have = true;
+                    bo = objective;
+                    bs = s;
+                    bp = points[n * 5 + k];
+                }
+            }
+            if (failed) {
+                state[n] = -1;
+                continue;
+            }
+            best_score[n] = bs;
+            best_point[n] = bp;
+        }
+        __syncthreads();
+        // phase 2: residual offsets, dy-major
+        for (uint32_t i = tid; i < nb * 8; i += blockDim.x) {
+            const uint32_t n = i / 8, k = i % 8;
+            if (state[n] != 1) continue;
+            const uint32_t cell = k < 4 ? k : k + 1;
+            const float2 point = make_float2(best_point[n].x + steps[cell % 3],
+                                             best_point[n].y + steps[cell / 3]);
+            const double dx = (double)(point.x - P.gc[n].x), dy = (double)(point.y - P.gc[n].y);
+            float s = nan_f(), sc;
+            if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
+                patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                s = sc;
+            phase[n * 8 + k] = s;
+        }
+        __syncthreads();
+        for (uint32_t n = tid; n < nb; n += blockDim.x) {
+            const uint32_t g = node0 + n;
+            if (state[n] == 0 || g >= nodes) continue;
+            bool has_selected = false;
+            float selected_score = 0.0f;
+            float2 selected_point = make_float2(0.0f, 0.0f);
+            if (state[n] == 1) {
+                float b_obj = best_score[n], b_score = best_score[n];
+                float2 b_point = best_point[n];
+                bool failed = false;
+                for (uint32_t k = 0; k < 8; k++) {
+                    const float s = phase[n * 8 + k];
+                    if (s != s) {
+                        failed = true;
+                        break;
+                    }
+                    const uint32_t cell = k < 4 ? k : k + 1;
+                    const float dyo = steps[cell / 3], dxo = steps[cell % 3];
+                    const float residual_sq = dxo * dxo + dyo * dyo;
+                    const float objective = s - residual_sq * 0.002f;
+                    if (objective > b_obj) {
+                        b_obj = objective;
+                        b_score = s;
+                        b_point = make_float2(best_point[n].x + dxo, best_point[n].y + dyo);
+                    }
+                }
+                if (!failed) {
+                    has_selected = true;
+                    selected_score = b_score;
+                    selected_point = b_point;
+                }
+            }
+            const chd_node_depth node = field[g];
+            const float base = baseline[n];
+            const bool has_base = base == base;
+            const bool supported = has_selected && selected_score >= o.minimum_score &&
+                                   (!has_base || selected_score + 0.01f >= base);
+            const bool contradicted = has_selected && has_base && selected_score + 0.12f < base;
+            chd_node_warp d = out[g];
+            if (contradicted) {
+                d.code = CHD_WARP_OCCLUDED;
+                d.measured = node.regularized ? 0u : 1u;
+            } else if (supported) {
+                d.code = CHD_WARP_REFINED;
+                d.point[0] = selected_point.x;
+                d.point[1] = selected_point.y;
+                d.confidence =
+                    node.regularized ? node.confidence * 0.6f : 0.5f + 0.5f * node.confidence;
+                d.measured = node.regularized ? 0u : 1u;
+            }
+            out[g] = d;
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+}
+
+hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
+                          const chd_node_depth *field, double inverse_step,
+                          const chd_options &options, chd_node_warp *decisions,
+                          unsigned long long *work) {
+    const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
+    const uint32_t V = patch.view_count;
+    const uint32_t nb = kTargetNodes;
+    const PatchLayout l = patch_layout(nb, V, T);
+    const size_t lds = (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 4;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const uint32_t nodes = fine.columns * fine.rows;
+    hipLaunchKernelGGL(k_targets, dim3((nodes + nb - 1) / nb), dim3(128), lds, stream, patch,
+                       fine, field, inverse_step, options, T, nb, decisions, work);
+    return hipGetLastError();
+}
+
+}  // namespace chd
