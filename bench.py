@@ -128,7 +128,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--full-steps", type=int, default=2,
                     help="steps of the whole path (direct pass and targets) timed beside")
-    ap.add_argument("--cpu-rows", type=int, default=64,
+    ap.add_argument("--cpu-rows", type=int, default=256,
                     help="coarse rows of the CPU-baseline sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
