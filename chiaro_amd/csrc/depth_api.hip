@@ -488,6 +488,11 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         const chd_view &v = views[i];
         if (!v.luminance || v.plane_width == 0 || v.plane_height == 0)
             return bail(fail(CHD_ERR_ARGUMENT, "view " + std::to_string(i) + ": empty luminance plane"));
+        // the kernels address a plane (one padding column and row added)
+        // through a 32-bit byte range and 24-bit row products
+        if (((size_t)v.plane_width + 1) * ((size_t)v.plane_height + 1) * 4 > 0xffffffffull ||
+            v.plane_width >= (1u << 24) - 1 || v.plane_height >= (1u << 24) - 1)
+            return bail(fail(CHD_ERR_LIMIT, "view " + std::to_string(i) + ": luminance plane above 4 GiB"));
         if (i != reference_index && v.accepted && v.camera &&
             (!v.global_warp.points || v.global_warp.step == 0 || v.global_warp.columns == 0 ||
              v.global_warp.rows == 0))
@@ -538,6 +543,8 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         d.plane_width = v.plane_width;
         d.plane_height = v.plane_height;
         d.plane_pitch = (uint32_t)pitch;
+        d.plane_lines = v.plane_height + 1;
+        d.plane_transposed = 0;
         if (v.global_warp.points && v.global_warp.columns && v.global_warp.rows) {
             float2 *warp = nullptr;
             const size_t wn = (size_t)v.global_warp.columns * v.global_warp.rows;
@@ -559,6 +566,48 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
     CHD_ALLOC(s->vlist, std::max<size_t>(s->active.size(), 1));
     if (!s->active.empty()) CHD_UPLOAD(s->vlist, s->active.data(), s->active.size() * sizeof(uint32_t));
+    if (!s->active.empty()) {
+        // Target planes whose epipolar lines run mostly vertically are stored
+        // column by column, so that the samples of neighbouring hypotheses
+        // (one per lane) share cache lines. The samples are the same.
+        uint32_t *vertical_d = nullptr;
+        CHD_ALLOC(vertical_d, s->active.size());
+        std::vector<uint32_t> vertical(s->active.size(), 0);
+        hipError_t e = launch_epipolar_axes(st, s->views, s->vlist, (uint32_t)s->active.size(),
+                                            reference_index, options->near_depth,
+                                            options->far_depth, vertical_d);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(vertical.data(), vertical_d, vertical.size() * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            return bail(fail(CHD_ERR_HIP, std::string("epipolar axes: ") + hipGetErrorString(e)));
+        std::vector<void *> row_major;
+        for (size_t k = 0; k < s->active.size(); k++) {
+            if (!vertical[k]) continue;
+            ViewDev &d = host_views[s->active[k]];
+            float *columns = nullptr;
+            CHD_ALLOC(columns, ((size_t)d.plane_width + 1) * ((size_t)d.plane_height + 1));
+            e = launch_transpose_plane(st, d.plane, columns, d.plane_width, d.plane_height);
+            if (e != hipSuccess)
+                return bail(fail(CHD_ERR_HIP, std::string("plane transpose: ") + hipGetErrorString(e)));
+            row_major.push_back((void *)d.plane);
+            d.plane = columns;
+            d.plane_pitch = d.plane_height + 1;
+            d.plane_lines = d.plane_width + 1;
+            d.plane_transposed = 1;
+        }
+        if (!row_major.empty()) {
+            CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
+            e = hipStreamSynchronize(st);
+            if (e != hipSuccess)
+                return bail(fail(CHD_ERR_HIP, std::string("plane transpose: ") + hipGetErrorString(e)));
+            for (void *p : row_major) {
+                s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), p));
+                hipFree(p);
+            }
+        }
+    }
     std::vector<double> labels = label_depths(*options);
     CHD_ALLOC(s->labels, labels.size());
     CHD_UPLOAD(s->labels, labels.data(), labels.size() * sizeof(double));

@@ -340,9 +340,17 @@ __device__ inline bool project(const chd_camera &c, d3 point, double &ox, double
 // (pitch = width + 1, height + 1 rows): Plane::sample reads
 // at(min(x0 + 1, w - 1)), and at x0 = w - 1 the padding holds that same
 // value, so the pair (x0, x0 + 1) can always be loaded together.
+// A transposed plane holds the same padded samples column by column (pitch
+// = height + 1, width + 1 stored columns), so that the pair (y0, y0 + 1) is
+// adjacent instead; the target planes whose epipolar lines run mostly
+// vertically are stored that way, which keeps the samples of neighbouring
+// depth hypotheses in few cache lines.
 struct DevPlane {
     const float *data;
-    uint32_t width, height, pitch;
+    uint32_t width, height;  // logical size of the plane
+    uint32_t pitch;          // f32 between stored rows (columns when transposed)
+    uint32_t lines;          // stored rows (columns when transposed), padding included
+    uint32_t transposed;
 };
 
 struct DevWarp {
@@ -350,7 +358,7 @@ struct DevWarp {
     uint32_t step, columns, rows;
 };
 
-// Plane::sample (image.rs:451-466) on a padded plane.
+// Plane::sample (image.rs:451-466) on a padded row-major plane.
 __device__ __forceinline__ bool plane_sample(const DevPlane &p, float x, float y, float &out) {
     if (!(x >= 0.0f && y >= 0.0f && x <= (float)(p.width - 1) && y <= (float)(p.height - 1)))
         return false;
@@ -373,26 +381,36 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const DevPlane &p) 
     const uint64_t a = (uint64_t)p.data;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    const uint32_t bytes = __builtin_amdgcn_readfirstlane(p.pitch * (p.height + 1) * 4u);
+    const uint32_t bytes = __builtin_amdgcn_readfirstlane(p.pitch * p.lines * 4u);
     return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0,
                                              (int)bytes, 0x00020000);
 }
 
 // Plane::sample without the bounds test, for points proven inside
 // [0, w-1] x [0, h-1]: two 8-byte loads fetch (x0, x0 + 1) of rows y0 and
-// y0 + 1 (the second through the scalar offset `row_bytes` = pitch * 4).
-// For x >= 0, truncation is floor and x - floor(x) is exact, which is what
-// the hardware fract computes.
+// y0 + 1 (the second through the scalar offset `line_bytes` = pitch * 4),
+// or, on a transposed plane, (y0, y0 + 1) of columns x0 and x0 + 1. The
+// interpolation is the same either way. For x >= 0, truncation is floor and
+// x - floor(x) is exact, which is what the hardware fract computes.
+template <bool Transposed>
 __device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc, uint32_t pitch,
-                                                     uint32_t row_bytes, float x, float y) {
-    const uint32_t offset = ((uint32_t)y * pitch + (uint32_t)x) * 4u;
-    const float2 top2 =
+                                                     uint32_t line_bytes, float x, float y) {
+    // stored lines and pitches stay below 2^24, so the line product fits the
+    // full-rate 24-bit multiply
+    const uint32_t line = Transposed ? (uint32_t)x : (uint32_t)y;
+    const uint32_t along = Transposed ? (uint32_t)y : (uint32_t)x;
+    const uint32_t offset = (__umul24(line, pitch) + along) * 4u;
+    const float2 a =
         __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
-    const float2 bot2 = __builtin_bit_cast(
-        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, row_bytes, 0));
+    const float2 b = __builtin_bit_cast(
+        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, line_bytes, 0));
+    // a = (x0, y0) and its neighbour along the stored line; b = the same pair
+    // on the next line
+    const float p00 = a.x, p10 = Transposed ? b.x : a.y;
+    const float p01 = Transposed ? a.y : b.x, p11 = b.y;
     const float tx = __builtin_amdgcn_fractf(x), ty = __builtin_amdgcn_fractf(y);
-    const float top = top2.x * (1.0f - tx) + top2.y * tx;
-    const float bottom = bot2.x * (1.0f - tx) + bot2.y * tx;
+    const float top = p00 * (1.0f - tx) + p10 * tx;
+    const float bottom = p01 * (1.0f - tx) + p11 * tx;
     return top * (1.0f - ty) + bottom * ty;
 }
 

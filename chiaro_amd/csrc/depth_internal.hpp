@@ -11,9 +11,10 @@ namespace chd {
 
 // One module as the kernels see it. Pointers are device pointers.
 struct ViewDev {
-    const float *plane;   // (plane_height + 1) rows of plane_pitch f32, the last
-                          // column and row replicated (see DevPlane)
-    uint32_t plane_width, plane_height, plane_pitch;
+    const float *plane;   // plane_lines lines of plane_pitch f32, the last
+                          // column and row replicated; rows, or columns when
+                          // plane_transposed (see DevPlane)
+    uint32_t plane_width, plane_height, plane_pitch, plane_lines, plane_transposed;
     const float2 *warp;   // global warp points (x, y), NaN = undefined
     uint32_t warp_step, warp_columns, warp_rows;
     const chd_camera *camera;  // nullptr = no calibration
@@ -131,6 +132,17 @@ hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
 // Replicates the last column and row of a plane stored with pitch width + 1
 // and height + 1 rows.
 hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height);
+
+// Copies a padded row-major plane (width + 1 by height + 1) into column-major
+// order (pitch height + 1).
+hipError_t launch_transpose_plane(hipStream_t stream, const float *rows, float *columns,
+                                  uint32_t width, uint32_t height);
+
+// vertical[k] = 1 when, at the reference centre, the displacement between the
+// near and far depth in view vlist[k] runs more vertically than horizontally.
+hipError_t launch_epipolar_axes(hipStream_t stream, const ViewDev *views, const uint32_t *vlist,
+                                uint32_t count, uint32_t reference, double near_depth,
+                                double far_depth, uint32_t *vertical);
 
 hipError_t launch_fill_u32(hipStream_t stream, uint32_t *data, uint32_t value, size_t n);
 hipError_t launch_fill_u64(hipStream_t stream, unsigned long long *data,

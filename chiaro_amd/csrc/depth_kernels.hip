@@ -925,6 +925,57 @@ hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, ui
     return hipGetLastError();
 }
 
+// 32 x 32 tiles through LDS; both sides coalesced.
+__global__ void __launch_bounds__(256) k_transpose_plane(const float *rows, float *columns,
+                                                         uint32_t width, uint32_t height) {
+    __shared__ float tile[32][33];
+    const uint32_t w = width + 1, h = height + 1;  // padded extents
+    const uint32_t x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {
+        const uint32_t x = x0 + threadIdx.x, y = y0 + k;
+        if (x < w && y < h) tile[k][threadIdx.x] = rows[(size_t)y * w + x];
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.y; k < 32; k += 8) {
+        const uint32_t y = y0 + threadIdx.x, x = x0 + k;
+        if (x < w && y < h) columns[(size_t)x * h + y] = tile[threadIdx.x][k];
+    }
+}
+
+hipError_t launch_transpose_plane(hipStream_t stream, const float *rows, float *columns,
+                                  uint32_t width, uint32_t height) {
+    const dim3 grid((width + 1 + 31) / 32, (height + 1 + 31) / 32);
+    hipLaunchKernelGGL(k_transpose_plane, grid, dim3(32, 8), 0, stream, rows, columns, width,
+                       height);
+    return hipGetLastError();
+}
+
+__global__ void k_epipolar_axes(const ViewDev *views, const uint32_t *vlist, uint32_t count,
+                                uint32_t reference, double near_depth, double far_depth,
+                                uint32_t *vertical) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const chd_camera &r = *views[reference].camera;
+    const chd_camera &t = *views[vlist[k]].camera;
+    d3 o, d;
+    pixel_to_ray(r, (double)(r.width / 2), (double)(r.height / 2), o, d);
+    const d3 pn{o.x + d.x * near_depth, o.y + d.y * near_depth, o.z + d.z * near_depth};
+    const d3 pf{o.x + d.x * far_depth, o.y + d.y * far_depth, o.z + d.z * far_depth};
+    double xn, yn, xf, yf;
+    uint32_t v = 0;
+    if (project(t, pn, xn, yn) && project(t, pf, xf, yf)) v = fabs(yn - yf) > fabs(xn - xf);
+    vertical[k] = v;
+}
+
+hipError_t launch_epipolar_axes(hipStream_t stream, const ViewDev *views, const uint32_t *vlist,
+                                uint32_t count, uint32_t reference, double near_depth,
+                                double far_depth, uint32_t *vertical) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_epipolar_axes, dim3((count + 63) / 64), dim3(64), 0, stream, views,
+                       vlist, count, reference, near_depth, far_depth, vertical);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t *data, uint32_t value, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) data[i] = value;

@@ -48,7 +48,7 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
     l.ray = o; o = align16(o + nb * 6u * 8u);
     l.ext = o; o = align16(o + nb * 16u);
     l.gc = o; o = align16(o + nb * 8u);
-    l.gq = o; o = align16(o + nb * t * 8u);  // also holds the reference samples early on
+    l.gq = o; o = align16(o + nb * t * 16u);  // also holds the reference samples early on
     l.wr = o; o = align16(o + nb * t * 8u);
     l.sums = o; o = align16(o + nb * 3u * 4u);
     l.centre = o; o = align16(o + nb * 4u);
@@ -67,7 +67,8 @@ struct PatchLds {
     double *ray;       // [node][6]: corrected reference ray (current view)
     float4 *ext;       // [node]: min x, max x, min y, max y warp tap (current view)
     float2 *gc;        // [node]: global point of the centre (current view)
-    float2 *gq;        // [node][tap]: global-warp tap positions (current view)
+    double2 *gq;       // [node][tap]: global-warp tap positions (current view), f32
+                       // values held as f64 for the tap loop's f64 coordinate sums
     float *rv;         // [node][tap]: reference samples (aliases gq until the views)
     float2 *wr;        // [node][tap]: w, w * r
     float *sums;       // [node][3]: sum w, sum w r, sum w r^2
@@ -86,7 +87,7 @@ __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l)
     p.ray = (double *)(base + l.ray);
     p.ext = (float4 *)(base + l.ext);
     p.gc = (float2 *)(base + l.gc);
-    p.gq = (float2 *)(base + l.gq);
+    p.gq = (double2 *)(base + l.gq);
     p.rv = (float *)(base + l.gq);
     p.wr = (float2 *)(base + l.wr);
     p.sums = (float *)(base + l.sums);
@@ -105,6 +106,8 @@ __device__ __forceinline__ DevPlane view_plane(const ViewDev &v) {
     p.width = v.plane_width;
     p.height = v.plane_height;
     p.pitch = v.plane_pitch;
+    p.lines = v.plane_lines;
+    p.transposed = v.plane_transposed;
     return p;
 }
 
@@ -214,7 +217,7 @@ __device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
             P.nv_ok[n] = 0;
             q = make_float2(0.0f, 0.0f);
         }
-        P.gq[i] = q;
+        P.gq[i] = make_double2((double)q.x, (double)q.y);
     }
     for (uint32_t n = tid; n < nb; n += nth) {
         const double2 p = P.pos[n];
@@ -249,24 +252,46 @@ __device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
     __syncthreads();
     for (uint32_t n = tid; n < nb; n += nth) {
         if (!P.nv_ok[n]) continue;
-        const float2 *g = P.gq + (size_t)n * T;
-        float x0 = g[0].x, x1 = g[0].x, y0 = g[0].y, y1 = g[0].y;
+        const double2 *g = P.gq + (size_t)n * T;
+        double x0 = g[0].x, x1 = g[0].x, y0 = g[0].y, y1 = g[0].y;
         for (uint32_t t = 1; t < T; t++) {
-            x0 = fminf(x0, g[t].x);
-            x1 = fmaxf(x1, g[t].x);
-            y0 = fminf(y0, g[t].y);
-            y1 = fmaxf(y1, g[t].y);
+            x0 = fmin(x0, g[t].x);
+            x1 = fmax(x1, g[t].x);
+            y0 = fmin(y0, g[t].y);
+            y1 = fmax(y1, g[t].y);
         }
-        P.ext[n] = make_float4(x0, x1, y0, y1);
+        P.ext[n] = make_float4((float)x0, (float)x1, (float)y0, (float)y1);
     }
     __syncthreads();
+}
+
+// The three target sums of warped_patch_zncc over the taps in the
+// reference's dy-major order, each in its own f32 accumulator.
+template <bool Transposed>
+__device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
+                                         __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
+                                         double dx, double dy, float &st, float &stt, float &sp) {
+    const float2 *wr = P.wr + (size_t)n * T;
+    const double2 *gq = P.gq + (size_t)n * T;
+#pragma unroll 9
+    for (uint32_t t = 0; t < T; t++) {
+        const float2 w = wr[t];
+        const double2 q = gq[t];
+        const float x = to_plane(q.x + dx);
+        const float y = to_plane(q.y + dy);
+        const float tv = plane_sample_inside<Transposed>(rsrc, pitch, line_bytes, x, y);
+        const float wv = w.x * tv;
+        st += wv;
+        stt += wv * tv;
+        sp += w.y * tv;
+    }
 }
 
 // warped_patch_zncc (depth.rs:1772-1822) of node n against the current view
 // displaced by the raster delta (dx, dy); false is the reference's None.
 __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
                                            const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
-                                           uint32_t row_bytes, double dx, double dy, float &score,
+                                           uint32_t line_bytes, double dx, double dy, float &score,
                                            unsigned int *evaluations) {
     const float4 e = P.ext[n];
     const float xmin = to_plane((double)e.x + dx), xmax = to_plane((double)e.y + dx);
@@ -275,21 +300,11 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
           ymax <= (float)(tp.height - 1)))
         return false;
     atomicAdd(evaluations, 1u);
-    const float2 *wr = P.wr + (size_t)n * T;
-    const float2 *gq = P.gq + (size_t)n * T;
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
-#pragma unroll 9
-    for (uint32_t t = 0; t < T; t++) {
-        const float2 w = wr[t];
-        const float2 q = gq[t];
-        const float x = to_plane((double)q.x + dx);
-        const float y = to_plane((double)q.y + dy);
-        const float tv = plane_sample_inside(rsrc, tp.pitch, row_bytes, x, y);
-        const float wv = w.x * tv;
-        st += wv;
-        stt += wv * tv;
-        sp += w.y * tv;
-    }
+    if (tp.transposed)
+        tap_sums<true>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+    else
+        tap_sums<false>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
     const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
     const float covariance = sp - sr * st / count;
     const float reference_energy = srr - sr * sr / count;
@@ -388,7 +403,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
         patch_view(S, P, nb, T, a);
         const DevPlane tp = view_plane(S.views[S.vlist[a]]);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-        const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         for (uint32_t i = tid; i < nb * L; i += nth) {
             const uint32_t n = i / L, l = i % L;
             float s = nan_f();
@@ -397,7 +412,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
                 bool ok = true;
                 if (l > 0) ok = delta_at_depth(P, n, P.cams[a], labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                if (ok && patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                     s = sc;
             }
             res[((size_t)n * V + a) * L + l] = s;
@@ -446,12 +461,13 @@ hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const 
                               unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    // about 1024 (node, label) units per block, within 32 KB of LDS (5
-    // blocks of 256 threads per CU)
+    // about 1024 (node, label) units per block, within 40 KB of LDS (4
+    // blocks of 256 threads per CU, the occupancy the tap loop's registers
+    // allow)
     uint32_t nb = 1024 / label_count;
     if (nb < 1) nb = 1;
     if (nb > 16) nb = 16;
-    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 32 * 1024) nb--;
+    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 40 * 1024) nb--;
     const size_t lds = cost_lds_bytes(nb, V, T, label_count);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = grid.columns * grid.rows;
@@ -579,7 +595,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
     const uint32_t V = S.view_count, total = base[nb];
     const DevPlane tp = view_plane(S.views[S.vlist[a]]);
     const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-    const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+    const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
         uint32_t n = 0;
         while (n + 1 < nb && base[n + 1] <= i) n++;
@@ -591,7 +607,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             if (slot > 0)
                 ok = delta_at_depth(P, n, P.cams[a], cand[n * stride + slot - 1], dx, dy);
             float sc;
-            if (ok && patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, evaluations)) s = sc;
+            if (ok && patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, evaluations)) s = sc;
         }
         res[((size_t)n * V + a) * (stride + 1) + slot] = s;
     }
@@ -887,7 +903,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         patch_view(S, P, nb, T, a);
         const DevPlane tp = view_plane(S.views[S.vlist[a]]);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-        const uint32_t row_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         chd_node_warp *out = decisions + (size_t)a * nodes;
         // decision skeleton: Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
@@ -937,14 +953,14 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             float s = nan_f(), sc;
             if (k == 5) {
                 if (P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc(P, n, T, tp, rsrc, row_bytes, 0.0, 0.0, sc, &evaluation_count))
+                    patch_zncc(P, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc, &evaluation_count))
                     s = sc;
                 baseline[n] = s;
             } else if (state[n] == 1) {
                 const float2 q = points[n * 5 + k];
                 const double dx = (double)(q.x - P.gc[n].x), dy = (double)(q.y - P.gc[n].y);
                 if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                    patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                     s = sc;
                 phase[n * 8 + k] = s;
             }
@@ -988,7 +1004,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             const double dx = (double)(point.x - P.gc[n].x), dy = (double)(point.y - P.gc[n].y);
             float s = nan_f(), sc;
             if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                patch_zncc(P, n, T, tp, rsrc, row_bytes, dx, dy, sc, &evaluation_count))
+                patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                 s = sc;
             phase[n * 8 + k] = s;
         }

@@ -43,7 +43,9 @@ extern "C" {
 /* Largest supported patch radius of the ZNCC kernels (the direct pass uses
  * max(2 r, 8), so r <= 8 keeps every pass within this bound). */
 #define CHD_MAX_PATCH_RADIUS 16
-/* Largest number of views (reference included) in one call. */
+/* Largest number of views (reference included) in one call. Each
+ * luminance plane must also stay below 4 GiB as (width + 1) x (height + 1)
+ * f32 (CHD_ERR_LIMIT otherwise). */
 #define CHD_MAX_VIEWS 32
 
 /* DepthOptions (depth.rs:40-67); defaults from chd_default_options
