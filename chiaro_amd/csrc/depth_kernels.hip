@@ -21,6 +21,31 @@ namespace chd {
 
 constexpr uint32_t kSgmMaxChunks = 8;  // labels <= 512
 
+// DPP controls of the gfx9 family (whole-wave shifts and row broadcasts).
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138,
+              kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ float dpp(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), Ctrl,
+                                                      RowMask, 0xf, false));
+}
+
+// Minimum over the wavefront, returned in a scalar register. min is exact
+// and the path values are never -0.0 or NaN, so the reduction order does not
+// matter.
+__device__ __forceinline__ float wave_min_uniform(float v) {
+    const float inf = __int_as_float(0x7f800000);
+    v = fminf(v, dpp<kDppRowShr1>(inf, v));
+    v = fminf(v, dpp<kDppRowShr2>(inf, v));
+    v = fminf(v, dpp<kDppRowShr4>(inf, v));
+    v = fminf(v, dpp<kDppRowShr8>(inf, v));
+    v = fminf(v, dpp<kDppRowBcast15, 0xa>(inf, v));
+    v = fminf(v, dpp<kDppRowBcast31, 0xc>(inf, v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
@@ -30,11 +55,18 @@ __device__ __forceinline__ float wave_min(float v) {
 // One wavefront walks one path of one of the 8 families, lanes over labels
 // (label = lane + 64 j), writing that family's L_r volume. Family order is
 // H->, H<-, V-down, V-up, DR->, DR<-, DL->, DL<- (depth.rs:1287-1309).
+// The path geometry is wave-uniform (scalar registers); the costs of the
+// next kSgmPrefetch nodes are loaded ahead so that the recurrence waits on
+// its own arithmetic, not on memory. Lanes past the last label hold +inf,
+// which the shifts turn into the reference's missing neighbours.
+constexpr uint32_t kSgmPrefetch = 8;
+
+template <uint32_t J>
 __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ costs,
                                                    const float *__restrict__ guidance, uint32_t C,
-                                                   uint32_t R, uint32_t L, uint32_t J,
+                                                   uint32_t R, uint32_t L,
                                                    float *__restrict__ paths) {
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nH = R, nV = C, nD = C + R - 1;
     uint32_t d, p;
@@ -85,67 +117,110 @@ __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ cos
     float *out = paths + (size_t)d * volume;
     const float inf = __int_as_float(0x7f800000);
 
-    float prev[kSgmMaxChunks];
-    int64_t node = start;
+    float prev[J];
 #pragma unroll
-    for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
+    for (uint32_t j = 0; j < J; j++) {
+        const uint32_t l = lane + 64 * j;
         prev[j] = inf;
-        if (j < J) {
-            uint32_t l = lane + 64 * j;
-            if (l < L) {
-                prev[j] = costs[(size_t)node * L + l];
-                out[(size_t)node * L + l] = prev[j];
-            }
+        if (l < L) {
+            prev[j] = costs[(size_t)start * L + l];
+            out[(size_t)start * L + l] = prev[j];
         }
     }
-    for (uint32_t i = 1; i < len; i++) {
-        const int64_t previous_node = node;
-        node += stride;
-        const float edge = fabsf(guidance[node] - guidance[previous_node]);
-        const float edge_scale = 1.0f / (1.0f + 4.0f * edge);
-        const float p1 = kSgmSmallPenalty * rmaxf(edge_scale, 0.35f);
-        const float p2 = kSgmLargePenalty * rmaxf(edge_scale, 0.12f);
-        float m = inf;
+    float g_prev = guidance[start];
+    // ring of prefetched costs and guidance for steps i .. i + kSgmPrefetch - 1
+    float ahead[kSgmPrefetch][J];
+    float g_ahead[kSgmPrefetch];
 #pragma unroll
-        for (uint32_t j = 0; j < kSgmMaxChunks; j++)
-            if (j < J) m = fminf(m, prev[j]);
-        m = wave_min(m);
-        const float jump = m + p2;
-        float cur[kSgmMaxChunks];
+    for (uint32_t k = 0; k < kSgmPrefetch; k++) {
+        const uint32_t i = 1 + k;
+        const int64_t node = start + (int64_t)i * stride;
+        g_ahead[k] = i < len ? guidance[node] : 0.0f;
 #pragma unroll
-        for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
-            cur[j] = inf;
-            if (j < J) {
-                float up = __shfl_up(prev[j], 1, 64);
-                float dn = __shfl_down(prev[j], 1, 64);
-                float carry_up = j > 0 ? __shfl(prev[j > 0 ? j - 1 : 0], 63, 64) : inf;
-                float carry_dn = j + 1 < J ? __shfl(prev[j + 1 < kSgmMaxChunks ? j + 1 : j], 0, 64) : inf;
-                if (lane == 0) up = carry_up;
-                if (lane == 63) dn = carry_dn;
-                const uint32_t l = lane + 64 * j;
-                if (l < L) {
-                    const float same = prev[j];
-                    const float lower = l > 0 ? up + p1 : inf;
-                    const float higher = l + 1 < L ? dn + p1 : inf;
-                    const float mn = fminf(fminf(fminf(same, lower), higher), jump);
-                    cur[j] = costs[(size_t)node * L + l] + mn - m;
-                    out[(size_t)node * L + l] = cur[j];
+        for (uint32_t j = 0; j < J; j++) {
+            const uint32_t l = lane + 64 * j;
+            ahead[k][j] = i < len && l < L ? costs[(size_t)node * L + l] : inf;
+        }
+    }
+    for (uint32_t base = 1; base < len; base += kSgmPrefetch) {
+#pragma unroll
+        for (uint32_t k = 0; k < kSgmPrefetch; k++) {
+            const uint32_t i = base + k;
+            if (i >= len) break;
+            const int64_t node = start + (int64_t)i * stride;
+            float cost[J];
+#pragma unroll
+            for (uint32_t j = 0; j < J; j++) cost[j] = ahead[k][j];
+            const float g = g_ahead[k];
+            {
+                const uint32_t ip = i + kSgmPrefetch;
+                const int64_t pnode = start + (int64_t)ip * stride;
+                if (ip < len) {
+                    g_ahead[k] = guidance[pnode];
+#pragma unroll
+                    for (uint32_t j = 0; j < J; j++) {
+                        const uint32_t l = lane + 64 * j;
+                        ahead[k][j] = l < L ? costs[(size_t)pnode * L + l] : inf;
+                    }
                 }
             }
-        }
+            const float edge = fabsf(g - g_prev);
+            g_prev = g;
+            const float edge_scale = 1.0f / (1.0f + 4.0f * edge);
+            const float p1 = kSgmSmallPenalty * rmaxf(edge_scale, 0.35f);
+            const float p2 = kSgmLargePenalty * rmaxf(edge_scale, 0.12f);
+            float m = prev[0];
 #pragma unroll
-        for (uint32_t j = 0; j < kSgmMaxChunks; j++) prev[j] = cur[j];
+            for (uint32_t j = 1; j < J; j++) m = fminf(m, prev[j]);
+            m = wave_min_uniform(m);
+            const float jump = m + p2;
+            float cur[J];
+#pragma unroll
+            for (uint32_t j = 0; j < J; j++) {
+                // neighbours across chunk edges come from lane 63 of the
+                // previous chunk and lane 0 of the next
+                const float carry_up =
+                    j > 0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(prev[j > 0 ? j - 1 : 0]), 63)) : inf;
+                const float carry_dn =
+                    j + 1 < J ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(prev[j + 1 < J ? j + 1 : j]), 0)) : inf;
+                const float up = dpp<kDppWaveShr1>(carry_up, prev[j]);
+                const float dn = dpp<kDppWaveShl1>(carry_dn, prev[j]);
+                const float mn = fminf(fminf(fminf(prev[j], up + p1), dn + p1), jump);
+                cur[j] = cost[j] + mn - m;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < J; j++) {
+                const uint32_t l = lane + 64 * j;
+                if (l < L) out[(size_t)node * L + l] = cur[j];
+                prev[j] = l < L ? cur[j] : inf;
+            }
+        }
     }
 }
 
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
                       uint32_t columns, uint32_t rows, uint32_t labels, float *paths) {
     const uint32_t J = (labels + 63) / 64;
-    if (J > kSgmMaxChunks) return hipErrorInvalidValue;
     const uint64_t waves = 2ull * rows + 2ull * columns + 4ull * (columns + rows - 1);
-    dim3 blocks((uint32_t)((waves + 3) / 4));
-    hipLaunchKernelGGL(k_sgm_paths, blocks, dim3(256), 0, stream, costs, guidance, columns, rows,
-                       labels, J, paths);
+    const dim3 blocks((uint32_t)((waves + 3) / 4)), threads(256);
+    switch (J) {
+#define CHD_SGM_CASE(n)                                                                        \
+    case n:                                                                                    \
+        hipLaunchKernelGGL(k_sgm_paths<n>, blocks, threads, 0, stream, costs, guidance,        \
+                           columns, rows, labels, paths);                                      \
+        break;
+        CHD_SGM_CASE(1)
+        CHD_SGM_CASE(2)
+        CHD_SGM_CASE(3)
+        CHD_SGM_CASE(4)
+        CHD_SGM_CASE(5)
+        CHD_SGM_CASE(6)
+        CHD_SGM_CASE(7)
+        CHD_SGM_CASE(8)
+#undef CHD_SGM_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
