@@ -40,7 +40,10 @@ struct PatchLayout {
 
 __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
-__host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t) {
+// f64_taps: the per-view tap positions are kept as f64 (16 B per tap, two
+// conversions fewer per tap in the loop) rather than f32 (8 B, half the LDS).
+__host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t,
+                                                    bool f64_taps) {
     PatchLayout l;
     uint32_t o = 0;
     l.cams = o; o = align16(o + v * (uint32_t)sizeof(chd_camera));
@@ -48,7 +51,7 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
     l.ray = o; o = align16(o + nb * 6u * 8u);
     l.ext = o; o = align16(o + nb * 16u);
     l.gc = o; o = align16(o + nb * 8u);
-    l.gq = o; o = align16(o + nb * t * 16u);  // also holds the reference samples early on
+    l.gq = o; o = align16(o + nb * t * (f64_taps ? 16u : 8u));  // first holds the reference samples
     l.wr = o; o = align16(o + nb * t * 8u);
     l.sums = o; o = align16(o + nb * 3u * 4u);
     l.centre = o; o = align16(o + nb * 4u);
@@ -68,8 +71,10 @@ struct PatchLds {
     float4 *ext;       // [node]: min x, max x, min y, max y warp tap (current view)
     float2 *gc;        // [node]: global point of the centre (current view)
     double2 *gq;       // [node][tap]: global-warp tap positions (current view), f32
-                       // values held as f64 for the tap loop's f64 coordinate sums
+                       // values held as f64 (f64_taps layouts)
+    float2 *gqf;       // the same positions as f32 (other layouts; aliases gq)
     float *rv;         // [node][tap]: reference samples (aliases gq until the views)
+    bool f64_taps;
     float2 *wr;        // [node][tap]: w, w * r
     float *sums;       // [node][3]: sum w, sum w r, sum w r^2
     float *centre;
@@ -80,8 +85,10 @@ struct PatchLds {
     int *ray_ok;       // current view: corrected reference ray exists
 };
 
-__device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l) {
+__device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l, bool f64_taps) {
     PatchLds p;
+    p.f64_taps = f64_taps;
+    p.gqf = (float2 *)(base + l.gq);
     p.cams = (chd_camera *)(base + l.cams);
     p.pos = (double2 *)(base + l.pos);
     p.ray = (double *)(base + l.ray);
@@ -217,7 +224,10 @@ __device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
             P.nv_ok[n] = 0;
             q = make_float2(0.0f, 0.0f);
         }
-        P.gq[i] = make_double2((double)q.x, (double)q.y);
+        if (P.f64_taps)
+            P.gq[i] = make_double2((double)q.x, (double)q.y);
+        else
+            P.gqf[i] = q;
     }
     for (uint32_t n = tid; n < nb; n += nth) {
         const double2 p = P.pos[n];
@@ -252,33 +262,46 @@ __device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
     __syncthreads();
     for (uint32_t n = tid; n < nb; n += nth) {
         if (!P.nv_ok[n]) continue;
-        const double2 *g = P.gq + (size_t)n * T;
-        double x0 = g[0].x, x1 = g[0].x, y0 = g[0].y, y1 = g[0].y;
-        for (uint32_t t = 1; t < T; t++) {
-            x0 = fmin(x0, g[t].x);
-            x1 = fmax(x1, g[t].x);
-            y0 = fmin(y0, g[t].y);
-            y1 = fmax(y1, g[t].y);
+        float x0 = 0.0f, x1 = 0.0f, y0 = 0.0f, y1 = 0.0f;
+        for (uint32_t t = 0; t < T; t++) {
+            // f64-held positions are exact f32 values
+            const size_t i = (size_t)n * T + t;
+            const float gx = P.f64_taps ? (float)P.gq[i].x : P.gqf[i].x;
+            const float gy = P.f64_taps ? (float)P.gq[i].y : P.gqf[i].y;
+            x0 = t == 0 ? gx : fminf(x0, gx);
+            x1 = t == 0 ? gx : fmaxf(x1, gx);
+            y0 = t == 0 ? gy : fminf(y0, gy);
+            y1 = t == 0 ? gy : fmaxf(y1, gy);
         }
-        P.ext[n] = make_float4((float)x0, (float)x1, (float)y0, (float)y1);
+        P.ext[n] = make_float4(x0, x1, y0, y1);
     }
     __syncthreads();
 }
 
 // The three target sums of warped_patch_zncc over the taps in the
 // reference's dy-major order, each in its own f32 accumulator.
-template <bool Transposed>
+template <bool Transposed, bool F64Taps>
 __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
                                          double dx, double dy, float &st, float &stt, float &sp) {
     const float2 *wr = P.wr + (size_t)n * T;
     const double2 *gq = P.gq + (size_t)n * T;
+    const float2 *gqf = P.gqf + (size_t)n * T;
 #pragma unroll 9
     for (uint32_t t = 0; t < T; t++) {
         const float2 w = wr[t];
-        const double2 q = gq[t];
-        const float x = to_plane(q.x + dx);
-        const float y = to_plane(q.y + dy);
+        double qx, qy;
+        if constexpr (F64Taps) {
+            const double2 q = gq[t];
+            qx = q.x;
+            qy = q.y;
+        } else {
+            const float2 q = gqf[t];
+            qx = (double)q.x;
+            qy = (double)q.y;
+        }
+        const float x = to_plane(qx + dx);
+        const float y = to_plane(qy + dy);
         const float tv = plane_sample_inside<Transposed>(rsrc, pitch, line_bytes, x, y);
         const float wv = w.x * tv;
         st += wv;
@@ -289,6 +312,8 @@ __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t
 
 // warped_patch_zncc (depth.rs:1772-1822) of node n against the current view
 // displaced by the raster delta (dx, dy); false is the reference's None.
+// F64Taps must match the layout's f64_taps.
+template <bool F64Taps>
 __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
                                            const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
                                            uint32_t line_bytes, double dx, double dy, float &score,
@@ -302,9 +327,9 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
     atomicAdd(evaluations, 1u);
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
     if (tp.transposed)
-        tap_sums<true>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<true, F64Taps>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
     else
-        tap_sums<false>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<false, F64Taps>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
     const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
     const float covariance = sp - sr * st / count;
     const float reference_energy = srr - sr * sr / count;
@@ -382,8 +407,8 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T);
-    const PatchLds P = bind_patch(lds, lay);
+    const PatchLayout lay = patch_layout(nb, V, T, true);
+    const PatchLds P = bind_patch(lds, lay, true);
     float *res = (float *)(lds + lay.total);          // [nb][V][L]
     float *centre_value = res + (size_t)nb * V * L;   // [nb]
     int *node_tested = (int *)(centre_value + nb);    // [nb]
@@ -412,7 +437,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
                 bool ok = true;
                 if (l > 0) ok = delta_at_depth(P, n, P.cams[a], labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
+                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                     s = sc;
             }
             res[((size_t)n * V + a) * L + l] = s;
@@ -450,7 +475,7 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
 }
 
 size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t label_count) {
-    const PatchLayout l = patch_layout(nb, views, taps);
+    const PatchLayout l = patch_layout(nb, views, taps, true);
     return (size_t)l.total + (size_t)nb * views * label_count * 4 + (size_t)nb * 8;
 }
 
@@ -587,18 +612,18 @@ __device__ int best_depth_candidate(const double *depths, const float *score, ui
 // Candidate scoring shared by the direct and wide kernels for view slot a:
 // items are (node, slot), slot 0 the global-warp baseline and slot k the
 // depth cand[node * stride + k - 1]; base[] is the prefix of 1 + ncand over
-// the block's nodes. Results land in res[(node * V + view) * (stride + 1) +
-// slot].
+// the block's nodes and item_node[] maps an item to its node. Results land
+// in res[(node * V + view) * (stride + 1) + slot].
 __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
-                            uint32_t a, const double *cand, const uint32_t *base, uint32_t stride,
-                            float *res, unsigned int *evaluations) {
+                            uint32_t a, const double *cand, const uint32_t *base,
+                            const uint8_t *item_node, uint32_t stride, float *res,
+                            unsigned int *evaluations) {
     const uint32_t V = S.view_count, total = base[nb];
     const DevPlane tp = view_plane(S.views[S.vlist[a]]);
     const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
     const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        uint32_t n = 0;
-        while (n + 1 < nb && base[n + 1] <= i) n++;
+        const uint32_t n = item_node[i];
         const uint32_t slot = i - base[n];
         float s = nan_f();
         if (P.node_ok[n] && P.nv_ok[n]) {
@@ -607,7 +632,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             if (slot > 0)
                 ok = delta_at_depth(P, n, P.cams[a], cand[n * stride + slot - 1], dx, dy);
             float sc;
-            if (ok && patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, evaluations)) s = sc;
+            if (ok && patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, evaluations)) s = sc;
         }
         res[((size_t)n * V + a) * (stride + 1) + slot] = s;
     }
@@ -616,7 +641,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
 // The scoring half of measure_direct_depths (depth.rs:752-795): for every
 // fine node the global baseline and each candidate depth, aggregated over
 // the views, go to agg[node][slot] (NaN = None); also guidance and tested.
-__global__ void __launch_bounds__(256) k_direct_score(PatchSpec S, GridSpec F, GridSpec Cg,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_direct_score(PatchSpec S, GridSpec F, GridSpec Cg,
                                                       const chd_node_depth *__restrict__ coarse,
                                                       double inverse_step, chd_options o,
                                                       uint32_t T, uint32_t nb, float *agg,
@@ -625,14 +650,15 @@ __global__ void __launch_bounds__(256) k_direct_score(PatchSpec S, GridSpec F, G
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T);
-    const PatchLds P = bind_patch(lds, lay);
+    const PatchLayout lay = patch_layout(nb, V, T, false);
+    const PatchLds P = bind_patch(lds, lay, false);
     double *cand = (double *)(lds + lay.total);                 // [nb][32]
     float *res = (float *)(cand + nb * kMaxCandidates);         // [nb][V][33]
     float *centre_value = res + (size_t)nb * V * kDirectSlots;  // [nb]
     uint32_t *ncand = (uint32_t *)(centre_value + nb);          // [nb]
     uint32_t *node_tested = ncand + nb;                         // [nb]
     uint32_t *base = node_tested + nb;                          // [nb + 1]
+    uint8_t *item_node = (uint8_t *)(base + nb + 1);            // [nb * 33]
     const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
 
     for (uint32_t n = tid; n < nb; n += blockDim.x) {
@@ -657,9 +683,13 @@ __global__ void __launch_bounds__(256) k_direct_score(PatchSpec S, GridSpec F, G
         base[nb] = b;
     }
     patch_nodes(S, P, nb, T, centre_value);
+    for (uint32_t n = tid; n < nb; n += blockDim.x)
+        for (uint32_t k = 0; node0 + n < nodes && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
+    __syncthreads();
     for (uint32_t a = 0; a < V; a++) {
         patch_view(S, P, nb, T, a);
-        score_slots(S, P, nb, T, a, cand, base, kMaxCandidates, res, &evaluation_count);
+        score_slots(S, P, nb, T, a, cand, base, item_node, kMaxCandidates, res,
+                    &evaluation_count);
         __syncthreads();
     }
     for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x) {
@@ -723,9 +753,10 @@ __global__ void __launch_bounds__(256) k_direct_select(GridSpec F, GridSpec Cg,
 }
 
 size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
-    const PatchLayout l = patch_layout(nb, V, T);
+    const PatchLayout l = patch_layout(nb, V, T, false);
     return (size_t)l.total + (size_t)nb * kMaxCandidates * 8 +
-           (size_t)nb * V * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4;
+           (size_t)nb * V * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4 +
+           (size_t)nb * kDirectSlots;
 }
 
 hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
@@ -758,9 +789,9 @@ constexpr uint32_t kWideNodes = 8;  // wide-fallback nodes per block (at most)
 constexpr uint32_t kWideSlots = 8;  // baseline + at most 7 candidates
 
 size_t wide_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
-    const PatchLayout l = patch_layout(nb, V, T);
+    const PatchLayout l = patch_layout(nb, V, T, false);
     return (size_t)l.total + (size_t)nb * kWideSlots * 8 + (size_t)nb * V * kWideSlots * 4 +
-           (size_t)nb * kWideSlots * 4 + (size_t)nb * 4 * 2 + (nb + 1) * 4;
+           (size_t)nb * kWideSlots * 4 + (size_t)nb * 4 * 2 + (nb + 1) * 4 + (size_t)nb * kWideSlots;
 }
 
 // The max(2r, 8) re-measurement of nodes whose first pass produced no
@@ -775,14 +806,15 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T);
-    const PatchLds P = bind_patch(lds, lay);
+    const PatchLayout lay = patch_layout(nb, V, T, false);
+    const PatchLds P = bind_patch(lds, lay, false);
     double *cand = (double *)(lds + lay.total);                   // [nb][7]
     float *res = (float *)(cand + nb * kWideSlots);               // [nb][V][8]
     float *agg = res + (size_t)nb * V * kWideSlots;               // [nb][8]
     float *centre_value = agg + nb * kWideSlots;                  // [nb]
     uint32_t *ncand = (uint32_t *)(centre_value + nb);            // [nb]
     uint32_t *base = ncand + nb;                                  // [nb + 1]
+    uint8_t *item_node = (uint8_t *)(base + nb + 1);              // [nb * 8]
     const uint32_t tid = threadIdx.x;
     if (tid == 0) evaluation_count = 0;
     const uint32_t count = *wide_count;
@@ -812,9 +844,14 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
             base[nb] = b;
         }
         patch_nodes(S, P, nb, T, centre_value);
+        for (uint32_t n = tid; n < nb; n += blockDim.x)
+            for (uint32_t k = 0; first + n < count && k <= ncand[n]; k++)
+                item_node[base[n] + k] = (uint8_t)n;
+        __syncthreads();
         for (uint32_t a = 0; a < V; a++) {
             patch_view(S, P, nb, T, a);
-            score_slots(S, P, nb, T, a, cand, base, kWideSlots - 1, res, &evaluation_count);
+            score_slots(S, P, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
+                        &evaluation_count);
             __syncthreads();
         }
         for (uint32_t i = tid; i < nb * kWideSlots; i += blockDim.x) {
@@ -862,15 +899,16 @@ hipError_t launch_direct_wide(hipStream_t stream, const PatchSpec &patch, const 
 
 // ------------------------------------------------------ per-target stage
 
-constexpr uint32_t kTargetNodes = 16;
+constexpr uint32_t kTargetNodes = 32;
 
 // refine_one_view (depth.rs:1634-1708) and the decision block
 // (depth.rs:429-486) for every target (S.vlist) over a tile of fine nodes,
-// writing decisions[target slot][node]. Per target: phase 0 maps the five
-// shared-depth offsets, phase 1 scores them and the global baseline, phase 2
-// the eight +-1.5 px residuals around the best depth. Any None inside
-// refine_one_view makes the whole refinement None (its `?`s).
-__global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
+// writing decisions[target slot][node]. Per target, phase A runs one item
+// per (node, shared-depth offset): map_at_depth and its ZNCC (any None among
+// the five makes the refinement None, the reference's `?`s; running all five
+// first changes nothing since they have no side effects), plus the global
+// baseline; phase B the eight +-1.5 px residuals around the best depth.
+__global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
                                                  const chd_node_depth *__restrict__ field,
                                                  double inverse_step, chd_options o, uint32_t T,
                                                  uint32_t nb, chd_node_warp *decisions,
@@ -878,8 +916,8 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T);
-    const PatchLds P = bind_patch(lds, lay);
+    const PatchLayout lay = patch_layout(nb, V, T, false);
+    const PatchLds P = bind_patch(lds, lay, false);
     float2 *points = (float2 *)(lds + lay.total);  // [nb][5]
     float2 *best_point = points + nb * 5;          // [nb]
     float *phase = (float *)(best_point + nb);     // [nb][8]
@@ -889,7 +927,6 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
     int *state = (int *)(baseline + nb);           // [nb] 0 skip, 1 refine, -1 None
     const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
     const double min_inverse = 1.0 / o.far_depth, max_inverse = 1.0 / o.near_depth;
-    const float steps[3] = {-1.5f, 0.0f, 1.5f};
 
     for (uint32_t n = tid; n < nb; n += blockDim.x) {
         P.pos[n] = node_position(F, node0 + n);
@@ -905,71 +942,55 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         chd_node_warp *out = decisions + (size_t)a * nodes;
-        // decision skeleton: Undefined / Global / to refine (depth.rs:433-440)
+        // Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
-            state[n] = 0;
-            if (g >= nodes) continue;
-            chd_node_warp d;
-            d.code = CHD_WARP_UNDEFINED;
-            d.global[0] = d.global[1] = 0.0f;
-            d.point[0] = d.point[1] = 0.0f;
-            d.confidence = 0.0f;
-            d.measured = 0;
-            if (P.gc_ok[n]) {
-                d.code = CHD_WARP_GLOBAL;
-                d.global[0] = P.gc[n].x;
-                d.global[1] = P.gc[n].y;
-                if (field[g].valid) state[n] = 1;
-            }
-            out[g] = d;
+            state[n] = g < nodes && P.gc_ok[n] && field[g].valid ? 1 : 0;
         }
         __syncthreads();
-        // phase 0: map_at_depth at the five offsets
-        for (uint32_t i = tid; i < nb * 5; i += blockDim.x) {
-            const uint32_t n = i / 5, k = i % 5;
-            if (state[n] != 1) continue;
-            const double offsets[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
-            const double shared_inverse = 1.0 / field[node0 + n].depth;
-            const double inverse =
-                rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
-            const double depth = 1.0 / inverse;
-            bool ok = P.ray_ok[n] != 0;
-            double qx = 0.0, qy = 0.0;
-            if (ok) {
-                const double *r = P.ray + (size_t)n * 6;
-                const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
-                ok = project(P.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
-            }
-            points[i] = make_float2((float)qx, (float)qy);
-            if (!ok) state[n] = -1;
-        }
-        __syncthreads();
-        // phase 1: the five offsets (score_one_view_at_point) and the baseline
-        // (score_one_view with depth None: delta 0)
+        // phase A: offsets k < 5 (score_one_view_at_point, depth.rs:1710-1736)
+        // and the baseline k = 5 (score_one_view with no depth: delta 0)
         for (uint32_t i = tid; i < nb * 6; i += blockDim.x) {
             const uint32_t n = i / 6, k = i % 6;
             if (state[n] == 0) continue;
             float s = nan_f(), sc;
             if (k == 5) {
                 if (P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc(P, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc, &evaluation_count))
+                    patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc,
+                                      &evaluation_count))
                     s = sc;
                 baseline[n] = s;
-            } else if (state[n] == 1) {
-                const float2 q = points[n * 5 + k];
+                continue;
+            }
+            const double offsets[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
+            const double shared_inverse = 1.0 / field[node0 + n].depth;
+            const double inverse =
+                rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
+            const double depth = 1.0 / inverse;
+            double qx = 0.0, qy = 0.0;
+            bool ok = P.ray_ok[n] != 0;
+            if (ok) {
+                const double *r = P.ray + (size_t)n * 6;
+                const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
+                ok = project(P.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
+            }
+            const float2 q = make_float2((float)qx, (float)qy);
+            points[n * 5 + k] = q;
+            if (ok) {
                 const double dx = (double)(q.x - P.gc[n].x), dy = (double)(q.y - P.gc[n].y);
                 if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
+                    patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                                      &evaluation_count))
                     s = sc;
-                phase[n * 8 + k] = s;
             }
+            phase[n * 8 + k] = s;
         }
         __syncthreads();
+        // the best shared-depth offset: first maximum of score - |offset| * 0.002
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             if (state[n] != 1) continue;
             const float magnitude[5] = {1.0f, 0.5f, 0.0f, 0.5f, 1.0f};
-            bool have = false, failed = false;
+            bool failed = false;
             float bo = 0.0f, bs = 0.0f;
             float2 bp = make_float2(0.0f, 0.0f);
             for (uint32_t k = 0; k < 5; k++) {
@@ -979,8 +1000,7 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
                     break;
                 }
                 const float objective = s - magnitude[k] * 0.002f;
-                if (!have || objective > bo) {
-                    have = true;
+                if (k == 0 || objective > bo) {
                     bo = objective;
                     bs = s;
                     bp = points[n * 5 + k];
@@ -994,7 +1014,8 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             best_point[n] = bp;
         }
         __syncthreads();
-        // phase 2: residual offsets, dy-major
+        // phase B: residual offsets, dy-major
+        const float steps[3] = {-1.5f, 0.0f, 1.5f};
         for (uint32_t i = tid; i < nb * 8; i += blockDim.x) {
             const uint32_t n = i / 8, k = i % 8;
             if (state[n] != 1) continue;
@@ -1004,60 +1025,72 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
             const double dx = (double)(point.x - P.gc[n].x), dy = (double)(point.y - P.gc[n].y);
             float s = nan_f(), sc;
             if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                patch_zncc(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
+                patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                 s = sc;
             phase[n * 8 + k] = s;
         }
         __syncthreads();
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
-            if (state[n] == 0 || g >= nodes) continue;
-            bool has_selected = false;
-            float selected_score = 0.0f;
-            float2 selected_point = make_float2(0.0f, 0.0f);
-            if (state[n] == 1) {
-                float b_obj = best_score[n], b_score = best_score[n];
-                float2 b_point = best_point[n];
-                bool failed = false;
-                for (uint32_t k = 0; k < 8; k++) {
-                    const float s = phase[n * 8 + k];
-                    if (s != s) {
-                        failed = true;
-                        break;
-                    }
-                    const uint32_t cell = k < 4 ? k : k + 1;
-                    const float dyo = steps[cell / 3], dxo = steps[cell % 3];
-                    const float residual_sq = dxo * dxo + dyo * dyo;
-                    const float objective = s - residual_sq * 0.002f;
-                    if (objective > b_obj) {
-                        b_obj = objective;
-                        b_score = s;
-                        b_point = make_float2(best_point[n].x + dxo, best_point[n].y + dyo);
-                    }
-                }
-                if (!failed) {
-                    has_selected = true;
-                    selected_score = b_score;
-                    selected_point = b_point;
-                }
+            if (g >= nodes) continue;
+            chd_node_warp d;
+            d.code = CHD_WARP_UNDEFINED;
+            d.global[0] = d.global[1] = 0.0f;
+            d.point[0] = d.point[1] = 0.0f;
+            d.confidence = 0.0f;
+            d.measured = 0;
+            if (P.gc_ok[n]) {
+                d.code = CHD_WARP_GLOBAL;
+                d.global[0] = P.gc[n].x;
+                d.global[1] = P.gc[n].y;
             }
-            const chd_node_depth node = field[g];
-            const float base = baseline[n];
-            const bool has_base = base == base;
-            const bool supported = has_selected && selected_score >= o.minimum_score &&
-                                   (!has_base || selected_score + 0.01f >= base);
-            const bool contradicted = has_selected && has_base && selected_score + 0.12f < base;
-            chd_node_warp d = out[g];
-            if (contradicted) {
-                d.code = CHD_WARP_OCCLUDED;
-                d.measured = node.regularized ? 0u : 1u;
-            } else if (supported) {
-                d.code = CHD_WARP_REFINED;
-                d.point[0] = selected_point.x;
-                d.point[1] = selected_point.y;
-                d.confidence =
-                    node.regularized ? node.confidence * 0.6f : 0.5f + 0.5f * node.confidence;
-                d.measured = node.regularized ? 0u : 1u;
+            if (state[n] != 0) {
+                bool has_selected = false;
+                float selected_score = 0.0f;
+                float2 selected_point = make_float2(0.0f, 0.0f);
+                if (state[n] == 1) {
+                    float b_obj = best_score[n], b_score = best_score[n];
+                    float2 b_point = best_point[n];
+                    bool failed = false;
+                    for (uint32_t k = 0; k < 8; k++) {
+                        const float s = phase[n * 8 + k];
+                        if (s != s) {
+                            failed = true;
+                            break;
+                        }
+                        const uint32_t cell = k < 4 ? k : k + 1;
+                        const float dyo = steps[cell / 3], dxo = steps[cell % 3];
+                        const float residual_sq = dxo * dxo + dyo * dyo;
+                        const float objective = s - residual_sq * 0.002f;
+                        if (objective > b_obj) {
+                            b_obj = objective;
+                            b_score = s;
+                            b_point = make_float2(best_point[n].x + dxo, best_point[n].y + dyo);
+                        }
+                    }
+                    if (!failed) {
+                        has_selected = true;
+                        selected_score = b_score;
+                        selected_point = b_point;
+                    }
+                }
+                const chd_node_depth node = field[g];
+                const float base = baseline[n];
+                const bool has_base = base == base;
+                const bool supported = has_selected && selected_score >= o.minimum_score &&
+                                       (!has_base || selected_score + 0.01f >= base);
+                const bool contradicted = has_selected && has_base && selected_score + 0.12f < base;
+                if (contradicted) {
+                    d.code = CHD_WARP_OCCLUDED;
+                    d.measured = node.regularized ? 0u : 1u;
+                } else if (supported) {
+                    d.code = CHD_WARP_REFINED;
+                    d.point[0] = selected_point.x;
+                    d.point[1] = selected_point.y;
+                    d.confidence =
+                        node.regularized ? node.confidence * 0.6f : 0.5f + 0.5f * node.confidence;
+                    d.measured = node.regularized ? 0u : 1u;
+                }
             }
             out[g] = d;
         }
@@ -1066,18 +1099,24 @@ __global__ void __launch_bounds__(128) k_targets(PatchSpec S, GridSpec F,
     if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
 }
 
+size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
+    const PatchLayout l = patch_layout(nb, V, T, false);
+    return (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 4;
+}
+
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                           const chd_node_depth *field, double inverse_step,
                           const chd_options &options, chd_node_warp *decisions,
                           unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    const uint32_t nb = kTargetNodes;
-    const PatchLayout l = patch_layout(nb, V, T);
-    const size_t lds = (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 4;
+    // 32 nodes: 192 phase-A and 256 phase-B items for 256 threads
+    uint32_t nb = kTargetNodes;
+    while (nb > 1 && targets_lds_bytes(nb, V, T) > 52 * 1024) nb--;
+    const size_t lds = targets_lds_bytes(nb, V, T);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
-    hipLaunchKernelGGL(k_targets, dim3((nodes + nb - 1) / nb), dim3(128), lds, stream, patch,
+    hipLaunchKernelGGL(k_targets, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
                        fine, field, inverse_step, options, T, nb, decisions, work);
     return hipGetLastError();
 }
