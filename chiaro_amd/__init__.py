@@ -2,7 +2,8 @@
 plane-sweep ZNCC cost field, 8-path SGM, WTA with sub-label fit, direct
 fine-grid remeasurement, consistency filters and per-target cross-camera
 decisions, behind the C ABI in include/chiaro_depth.h."""
-from .depth import Context, DenseDepthMap, DepthOptions, Session, TargetResult, refine_multiview_depth
+from .depth import (Context, DenseDepthMap, DepthOptions, Session, TargetResult, luminance_half,
+                    refine_multiview_depth)
 
-__all__ = ["Context", "DenseDepthMap", "DepthOptions", "Session", "TargetResult",
+__all__ = ["Context", "DenseDepthMap", "DepthOptions", "Session", "TargetResult", "luminance_half",
            "refine_multiview_depth"]

@@ -287,6 +287,10 @@ DEPTH_SIGNATURES = {
     "chd_fit_local_depth_planes": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_double),
     "chd_enforce_warp_consensus": (C.c_int, VP, VP, C.c_uint32, C.c_uint32),
     "chd_suppress_warp_boundaries": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32),
+    "chd_luminance_half": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                           C.c_int, VP),
+    "chd_luminance_half_device": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, C.c_float,
+                                  C.c_float, C.c_int, VP),
     "chd_device_expf": (C.c_int, VP, VP, VP, C.c_size_t),
     "chd_device_exp": (C.c_int, VP, VP, VP, C.c_size_t),
 }

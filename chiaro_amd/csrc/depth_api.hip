@@ -94,6 +94,9 @@ struct TimedLaunch {
 struct chd_context {
     int device;
     hipStream_t stream;
+    // luminance_half table for the last (black, white) pair
+    float *luminance_table = nullptr;
+    float luminance_black = 0.0f, luminance_white = 0.0f;
 };
 
 struct chd_session {
@@ -448,6 +451,7 @@ void chd_context_destroy(chd_context *c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    if (c->luminance_table) hipFree(c->luminance_table);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -543,8 +547,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         d.plane_width = v.plane_width;
         d.plane_height = v.plane_height;
         d.plane_pitch = (uint32_t)pitch;
-        d.plane_lines = v.plane_height + 1;
-        d.plane_transposed = 0;
+        d.columns = nullptr;
         if (v.global_warp.points && v.global_warp.columns && v.global_warp.rows) {
             float2 *warp = nullptr;
             const size_t wn = (size_t)v.global_warp.columns * v.global_warp.rows;
@@ -567,9 +570,10 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     CHD_ALLOC(s->vlist, std::max<size_t>(s->active.size(), 1));
     if (!s->active.empty()) CHD_UPLOAD(s->vlist, s->active.data(), s->active.size() * sizeof(uint32_t));
     if (!s->active.empty()) {
-        // Target planes whose epipolar lines run mostly vertically are stored
-        // column by column, so that the samples of neighbouring hypotheses
-        // (one per lane) share cache lines. The samples are the same.
+        // Target planes whose epipolar lines run mostly vertically get a
+        // column-major copy, so that the cost volume's samples of
+        // neighbouring hypotheses (one per lane) share cache lines. The
+        // samples are the same.
         uint32_t *vertical_d = nullptr;
         CHD_ALLOC(vertical_d, s->active.size());
         std::vector<uint32_t> vertical(s->active.size(), 0);
@@ -582,7 +586,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess)
             return bail(fail(CHD_ERR_HIP, std::string("epipolar axes: ") + hipGetErrorString(e)));
-        std::vector<void *> row_major;
+        bool any = false;
         for (size_t k = 0; k < s->active.size(); k++) {
             if (!vertical[k]) continue;
             ViewDev &d = host_views[s->active[k]];
@@ -591,22 +595,10 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
             e = launch_transpose_plane(st, d.plane, columns, d.plane_width, d.plane_height);
             if (e != hipSuccess)
                 return bail(fail(CHD_ERR_HIP, std::string("plane transpose: ") + hipGetErrorString(e)));
-            row_major.push_back((void *)d.plane);
-            d.plane = columns;
-            d.plane_pitch = d.plane_height + 1;
-            d.plane_lines = d.plane_width + 1;
-            d.plane_transposed = 1;
+            d.columns = columns;
+            any = true;
         }
-        if (!row_major.empty()) {
-            CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
-            e = hipStreamSynchronize(st);
-            if (e != hipSuccess)
-                return bail(fail(CHD_ERR_HIP, std::string("plane transpose: ") + hipGetErrorString(e)));
-            for (void *p : row_major) {
-                s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), p));
-                hipFree(p);
-            }
-        }
+        if (any) CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
     }
     std::vector<double> labels = label_depths(*options);
     CHD_ALLOC(s->labels, labels.size());
@@ -932,6 +924,41 @@ int chd_refine_multiview_depth(chd_context *context, const chd_view *views, uint
 extern "C++" {
 namespace {
 
+// The luminance_half table (image.rs:401-426 with a unit flat field, after
+// reserve_highlight_headroom's halving of black and white when `headroom`):
+// entry s is the plane value of a cell whose samples sum to s, computed in
+// f32 with the platform libm's logf as the reference computes it. Cached per
+// context for the last (black, white) pair.
+int luminance_table(chd_context *c, float black, float white, bool headroom) {
+    if (headroom) {
+        black *= 0.5f;
+        white *= 0.5f;
+    }
+    if (c->luminance_table && std::memcmp(&black, &c->luminance_black, 4) == 0 &&
+        std::memcmp(&white, &c->luminance_white, 4) == 0)
+        return CHD_OK;
+    std::vector<float> table(kLuminanceTableSize);
+    const float range = std::fmax(white - black, 1.0f);  // f32::max: NaN gives 1
+    for (uint32_t s = 0; s < kLuminanceTableSize; s++) {
+        const float linear = std::fmax(((float)s * 0.25f - black) / range, 0.0f) * 1.0f;
+        table[s] = logf(1.0f + 1000.0f * linear);
+    }
+    CHD_HIP(hipStreamSynchronize(c->stream));  // the previous table may be in use
+    if (!c->luminance_table) {
+        hipError_t e = hipMalloc(&c->luminance_table, kLuminanceTableSize * sizeof(float));
+        if (e != hipSuccess) {
+            c->luminance_table = nullptr;
+            return fail(CHD_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+    }
+    CHD_HIP(hipMemcpyAsync(c->luminance_table, table.data(), table.size() * sizeof(float),
+                           hipMemcpyHostToDevice, c->stream));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    c->luminance_black = black;
+    c->luminance_white = white;
+    return CHD_OK;
+}
+
 struct Scratch {
     std::vector<void *> p;
     ~Scratch() {
@@ -1120,6 +1147,39 @@ int chd_suppress_warp_boundaries(chd_context *c, chd_node_warp *decisions, const
                           [&](hipStream_t st, chd_node_warp *src, chd_node_warp *dst, float *g) {
                               return launch_boundaries(st, src, dst, g, columns, rows);
                           });
+}
+
+int chd_luminance_half_device(chd_context *c, const uint16_t *samples, uint32_t width,
+                              uint32_t height, float black_q6, float white_q6,
+                              int reserve_headroom, float *plane) {
+    if (!c || !samples || !plane || width < 2 || height < 2)
+        return fail(CHD_ERR_ARGUMENT, "luminance_half: bad argument");
+    if (((uintptr_t)samples & 3u) != 0)
+        return fail(CHD_ERR_ARGUMENT, "luminance_half: samples must be 4-byte aligned");
+    CHD_HIP(hipSetDevice(c->device));
+    CHD_CHECK(luminance_table(c, black_q6, white_q6, reserve_headroom != 0));
+    CHD_HIP(launch_luminance_half(c->stream, samples, width, height, c->luminance_table,
+                                  reserve_headroom != 0, plane));
+    return CHD_OK;
+}
+
+int chd_luminance_half(chd_context *c, const uint16_t *samples, uint32_t width, uint32_t height,
+                       float black_q6, float white_q6, int reserve_headroom, float *plane) {
+    if (!c || !samples || !plane || width < 2 || height < 2)
+        return fail(CHD_ERR_ARGUMENT, "luminance_half: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)width * height, m = (size_t)(width / 2) * (height / 2);
+    Scratch sc;
+    uint16_t *d_samples;
+    float *d_plane;
+    CHD_CHECK(sc.alloc(&d_samples, n));
+    CHD_CHECK(sc.alloc(&d_plane, m));
+    CHD_HIP(hipMemcpyAsync(d_samples, samples, n * 2, hipMemcpyHostToDevice, c->stream));
+    CHD_CHECK(chd_luminance_half_device(c, d_samples, width, height, black_q6, white_q6,
+                                        reserve_headroom, d_plane));
+    CHD_HIP(hipMemcpyAsync(plane, d_plane, m * 4, hipMemcpyDeviceToHost, c->stream));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
 }
 
 int chd_device_expf(chd_context *c, const float *x, float *y, size_t n) {

@@ -342,9 +342,9 @@ __device__ inline bool project(const chd_camera &c, d3 point, double &ox, double
 // value, so the pair (x0, x0 + 1) can always be loaded together.
 // A transposed plane holds the same padded samples column by column (pitch
 // = height + 1, width + 1 stored columns), so that the pair (y0, y0 + 1) is
-// adjacent instead; the target planes whose epipolar lines run mostly
-// vertically are stored that way, which keeps the samples of neighbouring
-// depth hypotheses in few cache lines.
+// adjacent instead; target planes whose epipolar lines run mostly vertically
+// get such a copy, which keeps the samples of neighbouring depth hypotheses
+// in few cache lines.
 struct DevPlane {
     const float *data;
     uint32_t width, height;  // logical size of the plane

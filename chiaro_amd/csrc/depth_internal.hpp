@@ -11,10 +11,11 @@ namespace chd {
 
 // One module as the kernels see it. Pointers are device pointers.
 struct ViewDev {
-    const float *plane;   // plane_lines lines of plane_pitch f32, the last
-                          // column and row replicated; rows, or columns when
-                          // plane_transposed (see DevPlane)
-    uint32_t plane_width, plane_height, plane_pitch, plane_lines, plane_transposed;
+    const float *plane;    // (plane_height + 1) rows of plane_pitch f32, the last
+                           // column and row replicated (see DevPlane)
+    const float *columns;  // the same padded plane column by column (pitch
+                           // plane_height + 1), or nullptr (see DevPlane)
+    uint32_t plane_width, plane_height, plane_pitch;
     const float2 *warp;   // global warp points (x, y), NaN = undefined
     uint32_t warp_step, warp_columns, warp_rows;
     const chd_camera *camera;  // nullptr = no calibration
@@ -132,6 +133,15 @@ hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
 // Replicates the last column and row of a plane stored with pitch width + 1
 // and height + 1 rows.
 hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height);
+
+// Sums of a 2x2 cell of u16 samples index the luminance table.
+constexpr uint32_t kLuminanceTableSize = 4u * 65535u + 1u;
+
+// luminance_half of a u16 mosaic (width x height) into a (width / 2) x
+// (height / 2) f32 plane through the table of kLuminanceTableSize entries.
+hipError_t launch_luminance_half(hipStream_t stream, const uint16_t *samples, uint32_t width,
+                                 uint32_t height, const float *table, int headroom,
+                                 float *plane);
 
 // Copies a padded row-major plane (width + 1 by height + 1) into column-major
 // order (pitch height + 1).

@@ -1000,6 +1000,55 @@ hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, ui
     return hipGetLastError();
 }
 
+// Mosaic::luminance_half (image.rs:401-426) with a unit flat field: the
+// f32 sum of a 2x2 CFA cell is an exact integer below 2^18, so the rest of
+// the per-pixel formula, (sum * 0.25 - black) / range clamped at 0 and then
+// ln(1 + 1000 x), is a function of that integer, tabulated by the host with
+// the platform libm. With `headroom`, reserve_highlight_headroom
+// (image.rs:69-75) first maps each u16 sample s to (s + 1) / 2 in wrapping
+// u16 arithmetic. One thread per plane pixel, one 4-byte load per cell row
+// when the width is even (cells then start 4-byte aligned in a 4-byte
+// aligned mosaic), 2-byte loads otherwise.
+__global__ void __launch_bounds__(256) k_luminance_half(const uint16_t *__restrict__ samples,
+                                                        uint32_t width, uint32_t plane_width,
+                                                        uint32_t plane_height,
+                                                        const float *__restrict__ table,
+                                                        int headroom, float *__restrict__ plane) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= plane_width || j >= plane_height) return;
+    const uint16_t *r0 = samples + (size_t)(2 * j) * width + 2 * i;
+    uint32_t a, b, c, d;
+    if ((width & 1u) == 0) {
+        const uint32_t top = *(const uint32_t *)r0, bottom = *(const uint32_t *)(r0 + width);
+        a = top & 0xffffu;
+        b = top >> 16;
+        c = bottom & 0xffffu;
+        d = bottom >> 16;
+    } else {
+        a = r0[0];
+        b = r0[1];
+        c = r0[width];
+        d = r0[width + 1];
+    }
+    if (headroom) {
+        a = ((a + 1u) & 0xffffu) >> 1;
+        b = ((b + 1u) & 0xffffu) >> 1;
+        c = ((c + 1u) & 0xffffu) >> 1;
+        d = ((d + 1u) & 0xffffu) >> 1;
+    }
+    plane[(size_t)j * plane_width + i] = table[a + b + c + d];
+}
+
+hipError_t launch_luminance_half(hipStream_t stream, const uint16_t *samples, uint32_t width,
+                                 uint32_t height, const float *table, int headroom,
+                                 float *plane) {
+    const uint32_t pw = width / 2, ph = height / 2;
+    if (pw == 0 || ph == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_luminance_half, dim3((pw + 255) / 256, ph), dim3(256), 0, stream,
+                       samples, width, pw, ph, table, headroom, plane);
+    return hipGetLastError();
+}
+
 // 32 x 32 tiles through LDS; both sides coalesced.
 __global__ void __launch_bounds__(256) k_transpose_plane(const float *rows, float *columns,
                                                          uint32_t width, uint32_t height) {

@@ -285,3 +285,20 @@ def refine_multiview_depth(modules, reference_index: int, options: DepthOptions,
     finally:
         if own:
             ctx.close()
+
+
+def luminance_half(context: Context, samples: np.ndarray, black_q6: float, white_q6: float,
+                   reserve_headroom: bool = False) -> np.ndarray:
+    """Mosaic::luminance_half (image.rs:401-426) of a u16 mosaic on the GPU,
+    with a unit flat field; reserve_headroom first applies
+    reserve_highlight_headroom (image.rs:69-75). Returns the (h/2, w/2) f32
+    plane."""
+    raw = np.ascontiguousarray(samples, dtype=np.uint16)
+    if raw.ndim != 2:
+        raise ValueError("samples must be a 2-D array")
+    height, width = raw.shape
+    plane = np.empty((height // 2, width // 2), np.float32)
+    _abi.check(context.lib.chd_luminance_half(
+        context.handle, raw.ctypes.data, width, height, float(black_q6), float(white_q6),
+        int(bool(reserve_headroom)), plane.ctypes.data), "luminance_half")
+    return plane

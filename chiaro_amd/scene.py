@@ -39,6 +39,7 @@ class Capture:
     description: dict = field(default_factory=dict)
     truth_step: int = 8
     truth_inverse_distance: np.ndarray | None = None  # 1/mm on the coarse grid
+    raws: list | None = None  # sensor codes (u16 Bayer) when the capture has them
 
 
 def _threads() -> int:
@@ -184,6 +185,8 @@ def config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: floa
     raws = _render(desc, modules, 0, 0.0, bayer)
     planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
     options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
-    return _assemble(modules, planes_, 0, options,
-                     {"workload": "config1", "seed": seed, "width": width, "height": height,
-                      "views": 5, "planes": planes}, desc)
+    cap = _assemble(modules, planes_, 0, options,
+                    {"workload": "config1", "seed": seed, "width": width, "height": height,
+                     "views": 5, "planes": planes}, desc)
+    cap.raws = raws
+    return cap

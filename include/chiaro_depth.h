@@ -343,6 +343,23 @@ int chd_enforce_warp_consensus(chd_context *context, chd_node_warp *decisions,
 int chd_suppress_warp_boundaries(chd_context *context, chd_node_warp *decisions,
                                  const float *guidance, uint32_t columns, uint32_t rows);
 
+/* Input preparation: Mosaic::luminance_half (crates/chiaro-fusion/src/
+ * image.rs:401-426, called per module at pipeline.rs:1106-1109) of a u16 CFA
+ * mosaic in calibration-raster order (width x height, row-major, the
+ * pipeline's Q6 samples), with a unit flat field (no vignetting mesh). With
+ * reserve_headroom != 0 the samples, black_q6 and white_q6 are first halved
+ * as reserve_highlight_headroom does (image.rs:69-75). Writes the
+ * (width / 2) x (height / 2) f32 plane that chd_view.luminance takes.
+ * chd_luminance_half copies host buffers in and out; the _device form takes
+ * device buffers (samples 4-byte aligned) and only enqueues on the context's
+ * stream. */
+int chd_luminance_half(chd_context *context, const uint16_t *samples, uint32_t width,
+                       uint32_t height, float black_q6, float white_q6, int reserve_headroom,
+                       float *plane);
+int chd_luminance_half_device(chd_context *context, const uint16_t *samples, uint32_t width,
+                              uint32_t height, float black_q6, float white_q6,
+                              int reserve_headroom, float *plane);
+
 /* Device f32 exp / f64 exp used by the kernels, evaluated on the GPU for
  * n inputs (their parity with the platform libm is checked by the tests). */
 int chd_device_expf(chd_context *context, const float *x, float *y, size_t n);

@@ -81,6 +81,8 @@ SIGNATURES = {
     "or_suppress_warp_boundaries": (None, VP, VP, C.c_uint32, C.c_uint32),
     "or_diagnostic_samples": (None, P(_abi.DepthMap), VP, VP),
     "or_visualization_samples": (None, P(_abi.DepthMap), VP),
+    "or_luminance_half_u16": (None, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_int,
+                              VP),
     "or_refine_multiview_depth": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options),
                                   P(_abi.DepthMap), P(_abi.TargetResult), P(Debug), C.c_uint32,
                                   C.c_int),

@@ -1611,6 +1611,28 @@ static void depth_color(double normalized, uint16_t out[3]) { /* depth.rs:1861-1
         out[c] = f64_u16(round((stops[lower][c] * (1.0 - fraction) + stops[upper][c] * fraction) * 65535.0));
 }
 
+void or_luminance_half_u16(const uint16_t *samples, uint32_t width, uint32_t height,
+                           float black_q6, float white_q6, int reserve_headroom, float *plane) {
+    if (reserve_headroom) { /* image.rs:69-75; u16 `+ 1` wraps in a release build */
+        black_q6 *= 0.5f;
+        white_q6 *= 0.5f;
+    }
+    uint32_t w = width / 2, h = height / 2; /* image.rs:401-426 */
+    float range = rmaxf(white_q6 - black_q6, 1.0f);
+    for (uint32_t j = 0; j < h; j++) {
+        const uint16_t *row0 = samples + (size_t)(2 * j) * width;
+        const uint16_t *row1 = samples + (size_t)(2 * j + 1) * width;
+        for (uint32_t i = 0; i < w; i++) {
+            uint16_t v[4] = {row0[2 * i], row0[2 * i + 1], row1[2 * i], row1[2 * i + 1]};
+            if (reserve_headroom)
+                for (int k = 0; k < 4; k++) v[k] = (uint16_t)((uint16_t)(v[k] + 1) / 2);
+            float sum = (float)v[0] + (float)v[1] + (float)v[2] + (float)v[3];
+            float linear = rmaxf((sum * 0.25f - black_q6) / range, 0.0f) * 1.0f;
+            plane[(size_t)j * w + i] = logf(1.0f + 1000.0f * linear);
+        }
+    }
+}
+
 void or_visualization_samples(const chd_depth_map *map, uint16_t *rgb) { /* depth.rs:224-245 */
     double log_range = log(map->far_depth / map->near_depth);
     size_t n = (size_t)map->columns * map->rows;

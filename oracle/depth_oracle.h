@@ -115,6 +115,12 @@ void or_diagnostic_samples(const chd_depth_map *map, uint16_t *inverse_depth,
                            uint16_t *provenance_rgb);
 void or_visualization_samples(const chd_depth_map *map, uint16_t *rgb);
 
+/* Mosaic::luminance_half (image.rs:401-426) with a unit flat field, after
+ * reserve_highlight_headroom (image.rs:69-75) when reserve_headroom != 0;
+ * always with the platform logf. */
+void or_luminance_half_u16(const uint16_t *samples, uint32_t width, uint32_t height,
+                           float black_q6, float white_q6, int reserve_headroom, float *plane);
+
 /* Intermediate results of the full path (any pointer may be NULL). */
 typedef struct or_debug {
     float *scores;       /* coarse nodes x labels */

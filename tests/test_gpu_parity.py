@@ -261,3 +261,33 @@ def test_drop_in_matches_session(gpu_context):
     assert_bitwise("drop-in provenance", dmap.provenance, omap["provenance"])
     for t in otargets:
         assert_bitwise(f"drop-in target {t}", targets[t].points, otargets[t]["points"])
+
+
+@pytest.mark.parametrize("shape,black,white,headroom", [
+    ((3120, 4160), 64.0 * 16, 4095.0 * 16, False),  # config 1 module size, 12-bit codes << 4
+    ((781, 1043), 1024.0, 65520.0, False),          # odd sizes
+    ((64, 96), 1024.0, 65520.0, True),              # headroom halving with wrapping
+    ((16, 16), 5000.0, 100.0, False),               # white below black
+])
+def test_luminance_half_bit_exact(gpu_context, oracle_lib, shape, black, white, headroom):
+    from chiaro_amd import luminance_half
+    _say(f"luminance_half {shape} black {black} white {white} headroom {headroom}")
+    rng = np.random.default_rng(11)
+    samples = rng.integers(0, 65536, size=shape, dtype=np.uint16)
+    samples[0, :8] = [65535, 0, 65535, 65535, 0, 0, 1, 65534]
+    h, w = shape
+    got = luminance_half(gpu_context, samples, black, white, headroom)
+    want = np.empty((h // 2, w // 2), np.float32)
+    oracle_lib.or_luminance_half_u16(samples.ctypes.data, w, h, black, white, int(headroom),
+                                     want.ctypes.data)
+    assert_bitwise("luminance_half", got, want)
+
+
+def test_luminance_half_feeds_the_path(gpu_context):
+    """The GPU planes of a config 1 crop equal the host-prepared ones the
+    parity captures use."""
+    from chiaro_amd import luminance_half
+    cap = scene.config1(seed=2, width=416, height=312, focal_px=330.0)
+    for k, raw in enumerate(cap.raws):
+        got = luminance_half(gpu_context, raw << 4, 64.0 * 16, 4095.0 * 16)
+        assert_bitwise(f"module {k} plane", got, cap.modules[k].luminance)

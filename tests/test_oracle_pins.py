@@ -381,3 +381,63 @@ def test_components_follow_image_edges(oracle_lib):
     oracle_lib.or_reject_small_direct_components(f.ctypes.data, g.ctypes.data, 5, 5, 1e-4, 12)
     v = f["valid"].reshape(5, 5)
     assert v[:, :3].all() and not v[:, 3:].any()
+
+
+# ------------------------------------------------------------ input prep
+
+def _libm_logf():
+    libm = C.CDLL("libm.so.6")
+    f = libm.logf
+    f.restype = C.c_float
+    f.argtypes = [C.c_float]
+    return f
+
+
+def _luminance_numpy(samples, black, white, headroom):
+    """image.rs:401-426 (and :69-75) in numpy f32 arithmetic, with the same
+    platform logf called per value: an independent restatement."""
+    logf = _libm_logf()
+    s = samples.astype(np.uint32)
+    black, white = np.float32(black), np.float32(white)
+    if headroom:
+        s = ((s + 1) & 0xFFFF) // 2
+        black, white = black * np.float32(0.5), white * np.float32(0.5)
+    h, w = s.shape[0] // 2, s.shape[1] // 2
+    cells = s[:2 * h, :2 * w].reshape(h, 2, w, 2).astype(np.float32)
+    total = ((cells[:, 0, :, 0] + cells[:, 0, :, 1]) + cells[:, 1, :, 0]) + cells[:, 1, :, 1]
+    rng = np.float32(max(white - black, np.float32(1.0)))
+    linear = np.maximum((total * np.float32(0.25) - black) / rng, np.float32(0.0)) * np.float32(1.0)
+    arg = np.float32(1.0) + np.float32(1000.0) * linear
+    return np.array([logf(float(v)) for v in arg.ravel()], np.float32).reshape(h, w)
+
+
+@pytest.mark.parametrize("shape,black,white,headroom", [
+    ((6, 8), 64.0 * 64, 4095.0 * 64, False),
+    ((7, 9), 1024.0, 65520.0, False),     # odd sizes drop the last row and column
+    ((6, 8), 1024.0, 65520.0, True),      # headroom: 65535 wraps to 0
+    ((4, 4), 5000.0, 100.0, False),       # white below black: range clamps to 1
+])
+def test_luminance_half_matches_independent_restatement(oracle_lib, shape, black, white,
+                                                        headroom):
+    rng = np.random.default_rng(7)
+    samples = rng.integers(0, 65536, size=shape, dtype=np.uint16)
+    samples.flat[0] = 65535
+    samples.flat[1] = 0
+    h, w = shape
+    got = np.empty((h // 2, w // 2), np.float32)
+    oracle_lib.or_luminance_half_u16(samples.ctypes.data, w, h, black, white, int(headroom),
+                                     got.ctypes.data)
+    want = _luminance_numpy(samples, black, white, headroom)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_luminance_half_closed_forms(oracle_lib):
+    """A flat cell at the black level gives ln(1) = 0, one at white gives
+    ln(1001), and values below black clamp to 0."""
+    black, white = 64.0 * 16, 4095.0 * 16
+    samples = np.array([[1024, 1024, 65520, 65520, 0, 0],
+                        [1024, 1024, 65520, 65520, 0, 0]], np.uint16)
+    got = np.empty((1, 3), np.float32)
+    oracle_lib.or_luminance_half_u16(samples.ctypes.data, 6, 2, black, white, 0, got.ctypes.data)
+    assert got[0, 0] == 0.0 and got[0, 2] == 0.0
+    assert abs(float(got[0, 1]) - math.log(1001.0)) < 1e-6
