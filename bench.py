@@ -206,6 +206,18 @@ def main() -> int:
     else:
         work_full = work
 
+    # input preparation (luminance_half of the five u16 mosaics), host
+    # buffers in and out: the kernel's own time is in the rocprof trace
+    prep_ms = None
+    if cap.raws is not None:
+        from chiaro_amd import luminance_half
+        shifted = [r << 4 for r in cap.raws]
+        luminance_half(ctx, shifted[0], 64.0 * 16, 4095.0 * 16)
+        p0 = time.perf_counter()
+        for r in shifted:
+            luminance_half(ctx, r, 64.0 * 16, 4095.0 * 16)
+        prep_ms = (time.perf_counter() - p0) * 1000.0
+
     launches, total_ms = kernels["cost_volume"]
     cost_ms = total_ms / launches
     achieved = work["cost_zncc"] * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
@@ -267,6 +279,7 @@ def main() -> int:
         "roofline": roofline,
         "sgm_hbm": hbm,
         "full_path_ms": None if full_ms is None else round(full_ms, 2),
+        "input_prep_ms_incl_pcie": None if prep_ms is None else round(prep_ms, 2),
         "full_path_kernels_ms": full_kernels,
         "work": work_full,
         "cpu_baseline": cpu,

@@ -32,6 +32,7 @@ CHD_STAGE_DIRECT = 4
 CHD_STAGE_TARGETS = 8
 CHD_STAGE_ALL = 15
 CHD_STAGE_KEEP_AGGREGATED = 256
+CHD_STAGE_KEEP_VIEW_SCORES = 512
 
 WARP_UNDEFINED, WARP_GLOBAL, WARP_REFINED, WARP_OCCLUDED, WARP_BOUNDARY = range(5)
 PROVENANCE_UNSUPPORTED, PROVENANCE_GLOBAL, PROVENANCE_MEASURED, PROVENANCE_REGULARIZED = range(4)
@@ -207,6 +208,11 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint32), ("total_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("cost_nodes_per_block", C.c_uint32), ("cost_label_slab", C.c_uint32),
+                ("sgm_paths_per_block", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 # ------------------------------------------------------------- loaders
 
 _depth = None
@@ -277,6 +283,13 @@ DEPTH_SIGNATURES = {
     "chd_session_enable_timing": (C.c_int, VP, C.c_int),
     "chd_session_kernel_times": (C.c_int, VP, P(KernelTime), C.c_uint32, P(C.c_uint32)),
     "chd_session_reset_kernel_times": (C.c_int, VP),
+    "chd_session_read_view_scores": (C.c_int, VP, VP),
+    "chd_session_set_tuning": (C.c_int, VP, P(Tuning)),
+    "chd_session_get_tuning": (C.c_int, VP, P(Tuning)),
+    "chd_autotune": (C.c_int, VP, C.c_char_p, P(Tuning)),
+    "chd_batch_create": (C.c_int, VP, P(VP), C.c_uint32, P(VP)),
+    "chd_batch_run": (C.c_int, VP, C.c_uint32),
+    "chd_batch_destroy": (None, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),
     "chd_select_depths": (C.c_int, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32, P(Options), VP, VP,
                           VP),

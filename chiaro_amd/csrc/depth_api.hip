@@ -107,6 +107,7 @@ struct chd_session {
     chd_grid_sizes grid{};
     std::vector<void *> allocations;
     std::vector<uint32_t> active;  // accepted, calibrated, non-reference views
+    std::vector<std::pair<uint32_t, uint32_t>> plane_sizes;  // per view
     ViewDev *views = nullptr;
     uint32_t *vlist = nullptr;     // device copy of `active`
     double *labels = nullptr;
@@ -137,6 +138,8 @@ struct chd_session {
     float *map_confidence = nullptr;
     uint8_t *map_provenance = nullptr;
     unsigned long long *work = nullptr;  // cost, direct, wide, target ZNCCs
+    float *view_scores = nullptr;        // [view slot][node][label], on request
+    chd_tuning tuning{};
     uint32_t stages_done = 0;
     bool timing = false;
     std::vector<TimedLaunch> launches;
@@ -229,25 +232,67 @@ PatchSpec patch_spec(const chd_session *s, uint32_t radius) {
     return p;
 }
 
-int run_cost(chd_session *s) {
+// The session's cost-volume job in its current launch shape.
+CostJob cost_job(const chd_session *s, float *view_scores) {
     const chd_grid_sizes &g = s->grid;
-    PatchSpec patch = patch_spec(s, s->options.patch_radius);
-    GridSpec grid{g.coarse_columns, g.coarse_rows, g.coarse_step};
-    CHD_HIP(launch_fill_u64(s->context->stream, s->work, 0ull, 1));
-    return timed(s, "cost_volume", [&] {
-        return launch_cost_volume(s->context->stream, patch, grid, s->labels, g.labels,
-                                  s->options.minimum_support, s->options.best_view_count,
-                                  s->scores, s->costs, s->guidance, s->tested, s->work);
+    CostJob j{};
+    j.S = patch_spec(s, s->options.patch_radius);
+    j.G = GridSpec{g.coarse_columns, g.coarse_rows, g.coarse_step};
+    j.labels = s->labels;
+    j.L = g.labels;
+    j.minimum_support = s->options.minimum_support;
+    j.best_view_count = s->options.best_view_count;
+    j.scores = s->scores;
+    j.costs = s->costs;
+    j.guidance = s->guidance;
+    j.tested = s->tested;
+    j.view_scores = view_scores;
+    j.work = s->work;
+    cost_job_shape(j, s->tuning.cost_nodes_per_block, s->tuning.cost_label_slab);
+    return j;
+}
+
+// View-score slices are allocated on first request.
+int view_score_buffer(chd_session *s, bool keep, float **out) {
+    *out = nullptr;
+    if (!keep) return CHD_OK;
+    if (!s->view_scores) {
+        const size_t n = std::max<size_t>(s->active.size(), 1) * (size_t)s->grid.coarse_columns *
+                         s->grid.coarse_rows * s->grid.labels;
+        CHD_CHECK(dalloc(s, &s->view_scores, n));
+    }
+    *out = s->view_scores;
+    return CHD_OK;
+}
+
+// The clears a cost-volume launch needs first.
+int prepare_cost(chd_session *s) {
+    hipStream_t st = s->context->stream;
+    CHD_HIP(launch_fill_u64(st, s->work, 0ull, 1));
+    CHD_HIP(launch_fill_u8(st, s->tested, 0, (size_t)s->grid.coarse_columns * s->grid.coarse_rows));
+    return CHD_OK;
+}
+
+int run_cost(chd_session *s, bool keep_view_scores) {
+    float *view_scores = nullptr;
+    CHD_CHECK(view_score_buffer(s, keep_view_scores, &view_scores));
+    const CostJob job = cost_job(s, view_scores);
+    CHD_CHECK(prepare_cost(s));
+    return timed(s, "cost_volume", [&] { return launch_cost_volume(s->context->stream, job); });
+}
+
+int run_sgm_paths(chd_session *s) {
+    const chd_grid_sizes &g = s->grid;
+    return timed(s, "sgm_paths", [&] {
+        return launch_sgm(s->context->stream, s->costs, s->guidance, g.coarse_columns,
+                          g.coarse_rows, g.labels, s->paths, s->tuning.sgm_paths_per_block);
     });
 }
 
-int run_sgm(chd_session *s, bool keep_aggregated) {
+// select_depths and the optional completion after the path sweeps.
+int run_sgm_rest(chd_session *s, bool keep_aggregated) {
     const chd_grid_sizes &g = s->grid;
     hipStream_t st = s->context->stream;
-    CHD_CHECK(timed(s, "sgm_paths", [&] {
-        return launch_sgm(st, s->costs, s->guidance, g.coarse_columns, g.coarse_rows, g.labels,
-                          s->paths);
-    }));
     CHD_CHECK(timed(s, "sgm_select", [&] {
         return launch_sgm_select(st, s->paths, s->scores, s->tested, s->labels, g.coarse_columns,
                                  g.coarse_rows, g.labels, s->options,
@@ -543,6 +588,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
             if (e2 != hipSuccess)
                 return bail(fail(CHD_ERR_HIP, std::string("plane upload: ") + hipGetErrorString(e2)));
         }
+        s->plane_sizes.emplace_back(v.plane_width, v.plane_height);
         d.plane = plane;
         d.plane_width = v.plane_width;
         d.plane_height = v.plane_height;
@@ -683,14 +729,18 @@ int chd_session_run(chd_session *s, uint32_t stages) {
     if (s->none) return CHD_NONE;
     CHD_HIP(hipSetDevice(s->context->device));
     const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
+    const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
     if (stages & CHD_STAGE_COST) {
-        CHD_CHECK(run_cost(s));
-        s->stages_done |= CHD_STAGE_COST;
+        CHD_CHECK(run_cost(s, keep_views));
+        s->stages_done = (s->stages_done & ~CHD_STAGE_KEEP_VIEW_SCORES) | CHD_STAGE_COST |
+                         (keep_views ? CHD_STAGE_KEEP_VIEW_SCORES : 0);
     }
     if (stages & CHD_STAGE_SGM) {
         if (!(s->stages_done & CHD_STAGE_COST)) return fail(CHD_ERR_STATE, "SGM before cost volume");
-        CHD_CHECK(run_sgm(s, keep));
-        s->stages_done |= CHD_STAGE_SGM | (keep ? CHD_STAGE_KEEP_AGGREGATED : 0);
+        CHD_CHECK(run_sgm_paths(s));
+        CHD_CHECK(run_sgm_rest(s, keep));
+        s->stages_done = (s->stages_done & ~CHD_STAGE_KEEP_AGGREGATED) | CHD_STAGE_SGM |
+                         (keep ? CHD_STAGE_KEEP_AGGREGATED : 0);
     }
     if (stages & CHD_STAGE_DIRECT) {
         if (!(s->stages_done & CHD_STAGE_SGM)) return fail(CHD_ERR_STATE, "direct pass before SGM");
@@ -834,6 +884,43 @@ int chd_session_read_aggregated(chd_session *s, float *aggregated) {
     const size_t volume = (size_t)s->grid.coarse_columns * s->grid.coarse_rows * s->grid.labels;
     CHD_CHECK(download(s, aggregated, s->aggregated, volume * sizeof(float)));
     CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_read_view_scores(chd_session *s, float *out) {
+    if (!s || !out) return fail(CHD_ERR_ARGUMENT, "read_view_scores: bad argument");
+    if (s->none) return CHD_NONE;
+    if (!(s->stages_done & CHD_STAGE_KEEP_VIEW_SCORES))
+        return fail(CHD_ERR_STATE, "view scores not kept (run with CHD_STAGE_KEEP_VIEW_SCORES)");
+    const size_t n = s->active.size() * (size_t)s->grid.coarse_columns * s->grid.coarse_rows *
+                     s->grid.labels;
+    CHD_CHECK(download(s, out, s->view_scores, n * sizeof(float)));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
+    return CHD_OK;
+}
+
+int chd_session_set_tuning(chd_session *s, const chd_tuning *t) {
+    if (!s || !t) return fail(CHD_ERR_ARGUMENT, "set_tuning: bad argument");
+    if (t->sgm_paths_per_block != 0 && t->sgm_paths_per_block != 1 &&
+        t->sgm_paths_per_block != 2 && t->sgm_paths_per_block != 4)
+        return fail(CHD_ERR_ARGUMENT, "set_tuning: sgm_paths_per_block must be 1, 2 or 4");
+    if (t->cost_nodes_per_block > 64)
+        return fail(CHD_ERR_LIMIT, "set_tuning: more than 64 nodes per block");
+    if (!s->none) {
+        chd_tuning old = s->tuning;
+        s->tuning = *t;
+        if (cost_job_lds(cost_job(s, nullptr)) > 160 * 1024) {
+            s->tuning = old;
+            return fail(CHD_ERR_LIMIT, "set_tuning: cost work item above 160 KB of LDS");
+        }
+    }
+    s->tuning = *t;
+    return CHD_OK;
+}
+
+int chd_session_get_tuning(const chd_session *s, chd_tuning *out) {
+    if (!s || !out) return fail(CHD_ERR_ARGUMENT, "get_tuning: bad argument");
+    *out = s->tuning;
     return CHD_OK;
 }
 
@@ -994,7 +1081,7 @@ int chd_semi_global_costs(chd_context *c, const float *costs, const float *guida
     hipStream_t st = c->stream;
     CHD_HIP(hipMemcpyAsync(d_costs, costs, volume * 4, hipMemcpyHostToDevice, st));
     CHD_HIP(hipMemcpyAsync(d_guidance, guidance, nodes * 4, hipMemcpyHostToDevice, st));
-    CHD_HIP(launch_sgm(st, d_costs, d_guidance, columns, rows, labels, d_paths));
+    CHD_HIP(launch_sgm(st, d_costs, d_guidance, columns, rows, labels, d_paths, 4));
     // the path sum with every node untested: select_depths writes None
     double *d_labels;
     uint8_t *d_tested, *d_fill;
@@ -1208,6 +1295,339 @@ int chd_device_exp(chd_context *c, const double *x, double *y, size_t n) {
     CHD_HIP(hipMemcpyAsync(y, dy, n * 8, hipMemcpyDeviceToHost, c->stream));
     CHD_HIP(hipStreamSynchronize(c->stream));
     return CHD_OK;
+}
+
+// ------------------------------------------------------------ autotuning
+
+extern "C++" {
+namespace {
+
+// Key of the work's shape for the tuning cache.
+std::string shape_key(const chd_session *s) {
+    const chd_grid_sizes &g = s->grid;
+    const uint32_t r = s->options.patch_radius;
+    std::string key = "chd-tune-1 views=" + std::to_string(s->active.size()) +
+                      " taps=" + std::to_string((2 * r + 1) * (2 * r + 1)) +
+                      " labels=" + std::to_string(g.labels) + " grid=" +
+                      std::to_string(g.coarse_columns) + "x" + std::to_string(g.coarse_rows);
+    for (uint32_t v : s->active)
+        key += " " + std::to_string(s->plane_sizes[v].first) + "x" +
+               std::to_string(s->plane_sizes[v].second);
+    return key;
+}
+
+bool read_tuning_cache(const char *path, const std::string &key, chd_tuning &out) {
+    if (!path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[4096];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        char *tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        chd_tuning t{};
+        if (key == line && std::sscanf(tab + 1, "%u %u %u", &t.cost_nodes_per_block,
+                                       &t.cost_label_slab, &t.sgm_paths_per_block) == 3) {
+            out = t;  // the last entry for a key wins
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void append_tuning_cache(const char *path, const std::string &key, const chd_tuning &t) {
+    if (!path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s\t%u %u %u\n", key.c_str(), t.cost_nodes_per_block, t.cost_label_slab,
+                 t.sgm_paths_per_block);
+    std::fclose(f);
+}
+
+// Mean milliseconds of `reps` runs of `launch` on the session's stream.
+template <class F>
+int time_launches(chd_session *s, int reps, float &ms, F &&launch) {
+    hipStream_t st = s->context->stream;
+    hipEvent_t a, b;
+    CHD_HIP(hipEventCreate(&a));
+    CHD_HIP(hipEventCreate(&b));
+    int r = launch();  // warm-up
+    if (r == CHD_OK) r = hipEventRecord(a, st) == hipSuccess ? CHD_OK : CHD_ERR_HIP;
+    for (int i = 0; r == CHD_OK && i < reps; i++) r = launch();
+    if (r == CHD_OK) r = hipEventRecord(b, st) == hipSuccess ? CHD_OK : CHD_ERR_HIP;
+    if (r == CHD_OK) r = hipEventSynchronize(b) == hipSuccess ? CHD_OK : CHD_ERR_HIP;
+    if (r == CHD_OK) r = hipEventElapsedTime(&ms, a, b) == hipSuccess ? CHD_OK : CHD_ERR_HIP;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (r != CHD_OK) return r == CHD_ERR_HIP ? fail(CHD_ERR_HIP, "autotune timing") : r;
+    ms /= (float)reps;
+    return CHD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int chd_autotune(chd_session *s, const char *cache_path, chd_tuning *out) {
+    if (!s || !out) return fail(CHD_ERR_ARGUMENT, "autotune: bad argument");
+    if (s->none) return CHD_NONE;
+    CHD_HIP(hipSetDevice(s->context->device));
+    const std::string key = shape_key(s);
+    chd_tuning best{};
+    if (read_tuning_cache(cache_path, key, best)) {
+        CHD_CHECK(chd_session_set_tuning(s, &best));
+        *out = best;
+        return CHD_OK;
+    }
+    const bool timing = s->timing;
+    s->timing = false;
+    const uint32_t L = s->grid.labels;
+    float best_ms = 0.0f;
+    bool have = false;
+    int r = CHD_OK;
+    const uint32_t slabs[2] = {L, (L + 1) / 2};
+    const uint32_t nodes[7] = {2, 4, 6, 8, 10, 12, 16};
+    for (uint32_t si = 0; r == CHD_OK && si < 2; si++) {
+        for (uint32_t ni = 0; r == CHD_OK && ni < 7; ni++) {
+            chd_tuning t = s->tuning;
+            t.cost_nodes_per_block = nodes[ni];
+            t.cost_label_slab = slabs[si];
+            s->tuning = t;
+            if (cost_job_lds(cost_job(s, nullptr)) > 64 * 1024) continue;
+            float ms = 0.0f;
+            r = time_launches(s, 2, ms, [&] { return run_cost(s, false); });
+            if (r == CHD_OK && (!have || ms < best_ms)) {
+                best = t;
+                best_ms = ms;
+                have = true;
+            }
+        }
+    }
+    if (r == CHD_OK && !have) r = fail(CHD_ERR_LIMIT, "autotune: no cost shape fits");
+    if (r == CHD_OK) {
+        s->tuning = best;
+        const uint32_t wpb[3] = {1, 2, 4};
+        float sgm_best = 0.0f;
+        for (int k = 0; r == CHD_OK && k < 3; k++) {
+            chd_tuning t = best;
+            t.sgm_paths_per_block = wpb[k];
+            s->tuning = t;
+            float ms = 0.0f;
+            r = time_launches(s, 3, ms, [&] { return run_sgm_paths(s); });
+            if (r == CHD_OK && (k == 0 || ms < sgm_best)) {
+                sgm_best = ms;
+                best.sgm_paths_per_block = wpb[k];
+            }
+        }
+    }
+    s->timing = timing;
+    if (r != CHD_OK) return r;
+    s->tuning = best;
+    append_tuning_cache(cache_path, key, best);
+    *out = best;
+    return CHD_OK;
+}
+
+// ------------------------------------------------------------ batches
+
+struct chd_batch {
+    chd_context *context = nullptr;
+    std::vector<chd_session *> sessions;
+    std::vector<void *> allocations;
+    CostJob *jobs[2] = {nullptr, nullptr};  // without / with per-view score slices
+    bool views_ready = false;
+    uint32_t *first = nullptr, *counter = nullptr;
+    size_t lds = 0;
+    uint32_t blocks = 0;
+    SgmJob *sgm_jobs = nullptr;
+    uint32_t *sgm_first = nullptr;
+    uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4;
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
+    std::vector<hipGraph_t> templates;
+
+    ~chd_batch() {
+        for (auto &g : graphs) hipGraphExecDestroy(g.second);
+        for (hipGraph_t g : templates) hipGraphDestroy(g);
+        for (void *p : allocations) hipFree(p);
+    }
+};
+
+extern "C++" {
+namespace {
+
+template <class T>
+int balloc(chd_batch *b, T **out, size_t count) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(CHD_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b->allocations.push_back(p);
+    *out = (T *)p;
+    return CHD_OK;
+}
+
+// Enqueues the stages of every session; no host synchronisation, so that
+// the sequence can be captured into a graph.
+int batch_enqueue(chd_batch *b, uint32_t stages) {
+    hipStream_t st = b->context->stream;
+    const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
+    const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
+    const uint32_t n = (uint32_t)b->sessions.size();
+    if (stages & CHD_STAGE_COST) {
+        for (chd_session *s : b->sessions) CHD_CHECK(prepare_cost(s));
+        CHD_HIP(launch_cost_volume_queue(st, b->jobs[keep_views ? 1 : 0], b->first, n, b->lds,
+                                         b->blocks, b->counter));
+    }
+    if (stages & CHD_STAGE_SGM) {
+        CHD_HIP(launch_sgm_batch(st, b->sgm_jobs, b->sgm_first, n, b->sgm_total, b->labels,
+                                 b->sgm_paths_per_block));
+        for (chd_session *s : b->sessions) CHD_CHECK(run_sgm_rest(s, keep));
+    }
+    if (stages & CHD_STAGE_DIRECT)
+        for (chd_session *s : b->sessions) CHD_CHECK(run_direct(s));
+    if (stages & CHD_STAGE_TARGETS)
+        for (chd_session *s : b->sessions) CHD_CHECK(run_targets(s));
+    return CHD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t count,
+                     chd_batch **out) {
+    if (!c || !sessions || !out || count == 0) return fail(CHD_ERR_ARGUMENT, "batch: bad argument");
+    *out = nullptr;
+    CHD_HIP(hipSetDevice(c->device));
+    auto *b = new chd_batch;
+    b->context = c;
+    auto bail = [&](int code) {
+        delete b;
+        return code;
+    };
+    for (uint32_t k = 0; k < count; k++) {
+        chd_session *s = sessions[k];
+        if (!s || s->context != c)
+            return bail(fail(CHD_ERR_ARGUMENT, "batch: session missing or on another context"));
+        if (s->none) return bail(fail(CHD_ERR_ARGUMENT, "batch: a session is the reference's None"));
+        if (k > 0 && s->grid.labels != b->labels)
+            return bail(fail(CHD_ERR_ARGUMENT, "batch: sessions differ in label count"));
+        b->labels = s->grid.labels;
+        b->sessions.push_back(s);
+    }
+    // cost jobs, the item prefix, and the largest work item
+    std::vector<CostJob> plain(count);
+    std::vector<uint32_t> first(count + 1, 0);
+    for (uint32_t k = 0; k < count; k++) {
+        plain[k] = cost_job(b->sessions[k], nullptr);
+        first[k + 1] = first[k] + plain[k].tiles * plain[k].slabs;
+        b->lds = std::max(b->lds, cost_job_lds(plain[k]));
+    }
+    if (b->lds > 160 * 1024) return bail(fail(CHD_ERR_LIMIT, "batch: cost work item above 160 KB of LDS"));
+    int cus = 0;
+    CHD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(4, (uint32_t)(160 * 1024 / std::max<size_t>(b->lds, 1))));
+    b->blocks = (uint32_t)std::max(cus, 1) * per_cu;
+    // SGM jobs
+    std::vector<SgmJob> sgm(count);
+    std::vector<uint32_t> sgm_first(count + 1, 0);
+    uint32_t wpb = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        chd_session *s = b->sessions[k];
+        sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
+        sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
+        if (k == 0) wpb = s->tuning.sgm_paths_per_block;
+    }
+    b->sgm_total = sgm_first[count];
+    b->sgm_paths_per_block = wpb ? wpb : 4;
+    int r = CHD_OK;
+    if (r == CHD_OK) r = balloc(b, &b->jobs[0], count);
+    if (r == CHD_OK) r = balloc(b, &b->jobs[1], count);
+    if (r == CHD_OK) r = balloc(b, &b->first, count + 1);
+    if (r == CHD_OK) r = balloc(b, &b->counter, 1);
+    if (r == CHD_OK) r = balloc(b, &b->sgm_jobs, count);
+    if (r == CHD_OK) r = balloc(b, &b->sgm_first, count + 1);
+    if (r != CHD_OK) return bail(r);
+    hipStream_t st = c->stream;
+    hipError_t e = hipMemcpyAsync(b->jobs[0], plain.data(), count * sizeof(CostJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->first, first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->sgm_jobs, sgm.data(), count * sizeof(SgmJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->sgm_first, sgm_first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e)));
+    *out = b;
+    return CHD_OK;
+}
+
+int chd_batch_run(chd_batch *b, uint32_t stages) {
+    if (!b) return fail(CHD_ERR_ARGUMENT, "null batch");
+    CHD_HIP(hipSetDevice(b->context->device));
+    for (chd_session *s : b->sessions) {
+        const uint32_t done = s->stages_done | (stages & (CHD_STAGE_COST | CHD_STAGE_SGM | CHD_STAGE_DIRECT));
+        if ((stages & CHD_STAGE_SGM) && !(done & CHD_STAGE_COST)) return fail(CHD_ERR_STATE, "SGM before cost volume");
+        if ((stages & CHD_STAGE_DIRECT) && !(done & CHD_STAGE_SGM)) return fail(CHD_ERR_STATE, "direct pass before SGM");
+        if ((stages & CHD_STAGE_TARGETS) && !(done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "targets before direct pass");
+    }
+    hipStream_t st = b->context->stream;
+    if ((stages & CHD_STAGE_KEEP_VIEW_SCORES) && !b->views_ready) {
+        // the per-view slices are allocated on first request, outside any capture
+        std::vector<CostJob> views;
+        for (chd_session *s : b->sessions) {
+            float *vs = nullptr;
+            CHD_CHECK(view_score_buffer(s, true, &vs));
+            views.push_back(cost_job(s, vs));
+        }
+        CHD_HIP(hipMemcpyAsync(b->jobs[1], views.data(), views.size() * sizeof(CostJob),
+                               hipMemcpyHostToDevice, st));
+        CHD_HIP(hipStreamSynchronize(st));
+        b->views_ready = true;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : b->graphs)
+        if (g.first == stages) exec = g.second;
+    if (!exec) {
+        std::vector<bool> timing;
+        for (chd_session *s : b->sessions) {
+            timing.push_back(s->timing);
+            s->timing = false;  // no event records inside the graph
+        }
+        CHD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int r = batch_enqueue(b, stages);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        for (size_t k = 0; k < b->sessions.size(); k++) b->sessions[k]->timing = timing[k];
+        if (r != CHD_OK) {
+            if (graph) hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess) return fail(CHD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(graph);
+            return fail(CHD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        b->templates.push_back(graph);
+        b->graphs.emplace_back(stages, exec);
+    }
+    CHD_HIP(hipGraphLaunch(exec, st));
+    const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
+    const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
+    for (chd_session *s : b->sessions) {
+        if (stages & CHD_STAGE_COST)
+            s->stages_done = (s->stages_done & ~CHD_STAGE_KEEP_VIEW_SCORES) | CHD_STAGE_COST |
+                             (keep_views ? CHD_STAGE_KEEP_VIEW_SCORES : 0);
+        if (stages & CHD_STAGE_SGM)
+            s->stages_done = (s->stages_done & ~CHD_STAGE_KEEP_AGGREGATED) | CHD_STAGE_SGM |
+                             (keep ? CHD_STAGE_KEEP_AGGREGATED : 0);
+        s->stages_done |= stages & (CHD_STAGE_DIRECT | CHD_STAGE_TARGETS);
+    }
+    return CHD_OK;
+}
+
+void chd_batch_destroy(chd_batch *b) {
+    if (!b) return;
+    hipSetDevice(b->context->device);
+    hipStreamSynchronize(b->context->stream);
+    delete b;
 }
 
 }  // extern "C"

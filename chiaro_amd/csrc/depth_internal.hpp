@@ -51,14 +51,53 @@ struct WideEntry {
 // Kernel launchers (the ZNCC stages in depth_zncc.hip, the rest in
 // depth_kernels.hip). All enqueue on `stream` and return the hipError_t of
 // the launch.
-hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
-                              const double *labels, uint32_t label_count,
-                              uint32_t minimum_support, uint32_t best_view_count,
-                              float *scores, float *costs, float *guidance, uint8_t *tested,
-                              unsigned long long *work);
+// One cost-volume job: a capture's coarse grid, labels and outputs, and its
+// launch shape (cost_job_shape fills T, nb, slab, tiles, slabs). Work items
+// are (tile, slab) pairs; every shape computes the same volume.
+struct CostJob {
+    PatchSpec S;
+    GridSpec G;
+    const double *labels;
+    uint32_t L;              // labels
+    uint32_t T;              // taps
+    uint32_t nb;             // nodes per tile
+    uint32_t slab;           // labels per slab
+    uint32_t tiles, slabs;
+    uint32_t minimum_support, best_view_count;
+    float *scores, *costs, *guidance;
+    uint8_t *tested;         // cleared before the launch
+    float *view_scores;      // optional [view slot][node][label] per-view ZNCC, NaN = None
+    unsigned long long *work;
+};
 
+// nodes_per_block / label_slab of 0 pick the defaults.
+void cost_job_shape(CostJob &job, uint32_t nodes_per_block, uint32_t label_slab);
+size_t cost_job_lds(const CostJob &job);
+
+hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job);
+
+// Persistent launch over njobs device-resident jobs; first[] (device, njobs +
+// 1 entries) is the prefix of tiles * slabs; lds must cover every job;
+// counter is a device word the launch clears first.
+hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
+                                    const uint32_t *first, uint32_t njobs, size_t lds,
+                                    uint32_t blocks, uint32_t *counter);
+
+// paths_per_block: wavefronts (one path each) per block, 1, 2 or 4.
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
-                      uint32_t columns, uint32_t rows, uint32_t labels, float *paths);
+                      uint32_t columns, uint32_t rows, uint32_t labels, float *paths,
+                      uint32_t paths_per_block);
+
+// One capture's SGM inputs and path volumes for the batched sweep.
+struct SgmJob {
+    const float *costs, *guidance;
+    uint32_t columns, rows;
+    float *paths;
+};
+uint32_t sgm_waves(uint32_t columns, uint32_t rows);
+hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
+                            uint32_t njobs, uint32_t total_waves, uint32_t labels,
+                            uint32_t paths_per_block);
 
 hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float *scores,
                              const uint8_t *tested, const double *labels, uint32_t columns,
@@ -161,8 +200,6 @@ hipError_t launch_fill_u64(hipStream_t stream, unsigned long long *data,
 hipError_t launch_expf(hipStream_t stream, const float *x, float *y, size_t n);
 hipError_t launch_exp(hipStream_t stream, const double *x, double *y, size_t n);
 
-// Dynamic LDS needed by the patch kernels, for a launch-size decision.
-size_t cost_lds_bytes(uint32_t nodes_per_block, uint32_t views, uint32_t taps,
-                      uint32_t label_count);
+hipError_t launch_fill_u8(hipStream_t stream, uint8_t *data, uint8_t value, size_t n);
 
 }  // namespace chd

@@ -62,11 +62,10 @@ __device__ __forceinline__ float wave_min(float v) {
 constexpr uint32_t kSgmPrefetch = 8;
 
 template <uint32_t J>
-__global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ costs,
-                                                   const float *__restrict__ guidance, uint32_t C,
-                                                   uint32_t R, uint32_t L,
-                                                   float *__restrict__ paths) {
-    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+__device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
+                                         const float *__restrict__ guidance, uint32_t C,
+                                         uint32_t R, uint32_t L, float *__restrict__ paths,
+                                         uint32_t wave) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nH = R, nV = C, nD = C + R - 1;
     uint32_t d, p;
@@ -198,11 +197,69 @@ __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ cos
     }
 }
 
-hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
-                      uint32_t columns, uint32_t rows, uint32_t labels, float *paths) {
+template <uint32_t J>
+__global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ costs,
+                                                   const float *__restrict__ guidance, uint32_t C,
+                                                   uint32_t R, uint32_t L,
+                                                   float *__restrict__ paths) {
+    const uint32_t wave =
+        __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    sgm_path<J>(costs, guidance, C, R, L, paths, wave);
+}
+
+// All path sweeps of several captures in one launch; first[] is the prefix
+// of each job's wavefront count (njobs + 1 entries).
+template <uint32_t J>
+__global__ void __launch_bounds__(256) k_sgm_paths_batch(const SgmJob *__restrict__ jobs,
+                                                         const uint32_t *__restrict__ first,
+                                                         uint32_t njobs, uint32_t L) {
+    const uint32_t wave =
+        __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (wave >= first[njobs]) return;
+    uint32_t j = 0;
+    while (j + 1 < njobs && first[j + 1] <= wave) j++;
+    const SgmJob job = jobs[j];
+    sgm_path<J>(job.costs, job.guidance, job.columns, job.rows, L, job.paths, wave - first[j]);
+}
+
+uint32_t sgm_waves(uint32_t columns, uint32_t rows) {
+    return 2 * rows + 2 * columns + 4 * (columns + rows - 1);
+}
+
+hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
+                            uint32_t njobs, uint32_t total_waves, uint32_t labels,
+                            uint32_t paths_per_block) {
     const uint32_t J = (labels + 63) / 64;
-    const uint64_t waves = 2ull * rows + 2ull * columns + 4ull * (columns + rows - 1);
-    const dim3 blocks((uint32_t)((waves + 3) / 4)), threads(256);
+    const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
+    const dim3 blocks((total_waves + wpb - 1) / wpb), threads(64 * wpb);
+    switch (J) {
+#define CHD_SGM_CASE(n)                                                                        \
+    case n:                                                                                    \
+        hipLaunchKernelGGL(k_sgm_paths_batch<n>, blocks, threads, 0, stream, jobs, first,      \
+                           njobs, labels);                                                     \
+        break;
+        CHD_SGM_CASE(1)
+        CHD_SGM_CASE(2)
+        CHD_SGM_CASE(3)
+        CHD_SGM_CASE(4)
+        CHD_SGM_CASE(5)
+        CHD_SGM_CASE(6)
+        CHD_SGM_CASE(7)
+        CHD_SGM_CASE(8)
+#undef CHD_SGM_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
+                      uint32_t columns, uint32_t rows, uint32_t labels, float *paths,
+                      uint32_t paths_per_block) {
+    const uint32_t J = (labels + 63) / 64;
+    const uint32_t waves = sgm_waves(columns, rows);
+    const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
+    const dim3 blocks((waves + wpb - 1) / wpb), threads(64 * wpb);
     switch (J) {
 #define CHD_SGM_CASE(n)                                                                        \
     case n:                                                                                    \
@@ -1107,6 +1164,18 @@ __global__ void k_fill_u32(uint32_t *data, uint32_t value, size_t n) {
 __global__ void k_fill_u64(unsigned long long *data, unsigned long long value, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) data[i] = value;
+}
+
+__global__ void k_fill_u8(uint8_t *data, uint8_t value, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) data[i] = value;
+}
+
+hipError_t launch_fill_u8(hipStream_t stream, uint8_t *data, uint8_t value, size_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, data,
+                       value, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(hipStream_t stream, uint32_t *data, uint32_t value, size_t n) {

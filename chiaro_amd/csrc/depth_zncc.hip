@@ -410,29 +410,31 @@ __device__ __forceinline__ double2 node_position(const GridSpec &G, uint32_t g) 
 
 // ------------------------------------------------------------ cost volume
 
-__global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, const double *labels,
-                                                     uint32_t L, uint32_t nb, uint32_t T,
-                                                     uint32_t minimum_support,
-                                                     uint32_t best_view_count, float *scores,
-                                                     float *costs, float *guidance,
-                                                     uint8_t *tested, unsigned long long *work) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ unsigned int evaluation_count;
-    const uint32_t V = S.view_count;
+// build_cost_volume (depth.rs:1187-1277) for one work item of job J: the
+// nodes of tile `tile` (J.nb nodes) and the labels of slab `slab` (J.slab
+// labels). Every score is computed as in the single launch; a slab only
+// limits which labels this item writes, so any tiling and slab choice gives
+// the same volume. `tested` must be cleared beforehand: an item sets it to 1
+// for the nodes with a finite score among its labels.
+__device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsigned char *lds,
+                          unsigned int *evaluation_count) {
+    const PatchSpec &S = J.S;
+    const uint32_t V = S.view_count, T = J.T, L = J.L, nb = J.nb;
+    const uint32_t l0 = slab * J.slab, Ls = min(J.slab, L - l0);
     const PatchLayout lay = patch_layout(nb, V, T, true);
     const PatchLds P = bind_patch(lds, lay, true);
-    float *res = (float *)(lds + lay.total);          // [nb][V][L]
-    float *centre_value = res + (size_t)nb * V * L;   // [nb]
+    float *res = (float *)(lds + lay.total);          // [nb][V][Ls]
+    float *centre_value = res + (size_t)nb * V * Ls;  // [nb]
     int *node_tested = (int *)(centre_value + nb);    // [nb]
-    const uint32_t nodes = G.columns * G.rows, node0 = blockIdx.x * nb;
+    const uint32_t nodes = J.G.columns * J.G.rows, node0 = tile * nb;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
 
     for (uint32_t n = tid; n < nb; n += nth) {
-        P.pos[n] = node_position(G, node0 + n);
+        P.pos[n] = node_position(J.G, node0 + n);
         P.node_ok[n] = node0 + n < nodes ? 1 : 0;
         node_tested[n] = 0;
     }
-    if (tid == 0) evaluation_count = 0;
+    if (tid == 0) *evaluation_count = 0;
     __syncthreads();
     patch_nodes(S, P, nb, T, centre_value);
 
@@ -441,28 +443,31 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
-        for (uint32_t i = tid; i < nb * L; i += nth) {
-            const uint32_t n = i / L, l = i % L;
+        for (uint32_t i = tid; i < nb * Ls; i += nth) {
+            const uint32_t n = i / Ls, l = l0 + i % Ls;
             float s = nan_f();
             if (P.node_ok[n] && P.nv_ok[n]) {
                 double dx = 0.0, dy = 0.0;
                 bool ok = true;
-                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], labels[l], dx, dy);
+                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], J.labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
+                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                                           evaluation_count))
                     s = sc;
             }
-            res[((size_t)n * V + a) * L + l] = s;
+            res[((size_t)n * V + a) * Ls + (l - l0)] = s;
+            if (J.view_scores && node0 + n < nodes)
+                J.view_scores[((size_t)a * nodes + node0 + n) * L + l] = s;
         }
         __syncthreads();
     }
 
-    for (uint32_t i = tid; i < nb * L; i += nth) {
-        const uint32_t n = i / L, l = i % L, g = node0 + n;
+    for (uint32_t i = tid; i < nb * Ls; i += nth) {
+        const uint32_t n = i / Ls, l = l0 + i % Ls, g = node0 + n;
         if (g >= nodes) continue;
         float score;
-        if (!aggregate_scores(res + (size_t)n * V * L + l, L, V, minimum_support, best_view_count,
-                              score))
+        if (!aggregate_scores(res + (size_t)n * V * Ls + (l - l0), Ls, V, J.minimum_support,
+                              J.best_view_count, score))
             score = nan_f();
         float cost;
         if (finite_f(score)) {
@@ -473,44 +478,97 @@ __global__ void __launch_bounds__(256) k_cost_volume(PatchSpec S, GridSpec G, co
         } else {
             cost = kMissingCost;
         }
-        scores[(size_t)g * L + l] = score;
-        costs[(size_t)g * L + l] = cost;
+        J.scores[(size_t)g * L + l] = score;
+        J.costs[(size_t)g * L + l] = cost;
     }
     __syncthreads();
     for (uint32_t n = tid; n < nb; n += nth) {
         const uint32_t g = node0 + n;
         if (g >= nodes) continue;
-        guidance[g] = centre_value[n];
-        tested[g] = (uint8_t)node_tested[n];
+        J.guidance[g] = centre_value[n];  // every slab writes the same value
+        if (node_tested[n]) J.tested[g] = 1;
     }
-    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+    if (tid == 0 && J.work && *evaluation_count)
+        atomicAdd(J.work, (unsigned long long)*evaluation_count);
 }
 
-size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t label_count) {
+// One launch over one job: block b is item b (tile b / slabs, slab b % slabs).
+__global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    cost_tile(J, blockIdx.x / J.slabs, blockIdx.x % J.slabs, lds, &evaluation_count);
+}
+
+// Persistent form over several jobs (captures): each block takes the next
+// (job, tile, slab) item from a queue counter until the queue is empty, so
+// captures of uneven cost share the chip without a tail per capture. `first`
+// holds each job's first item index (njobs + 1 entries); `counter` must be 0
+// at launch.
+__global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__restrict__ jobs,
+                                                           const uint32_t *__restrict__ first,
+                                                           uint32_t njobs, uint32_t *counter) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    __shared__ uint32_t next;
+    const uint32_t total = first[njobs];
+    for (;;) {
+        __syncthreads();  // the previous item has finished with `next` and LDS
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = next;
+        if (item >= total) break;
+        uint32_t j = 0;
+        while (j + 1 < njobs && first[j + 1] <= item) j++;
+        const CostJob J = jobs[j];
+        const uint32_t local = item - first[j];
+        cost_tile(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
+    }
+}
+
+size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab) {
     const PatchLayout l = patch_layout(nb, views, taps, true);
-    return (size_t)l.total + (size_t)nb * views * label_count * 4 + (size_t)nb * 8;
+    return (size_t)l.total + (size_t)nb * views * slab * 4 + (size_t)nb * 8;
 }
 
-hipError_t launch_cost_volume(hipStream_t stream, const PatchSpec &patch, const GridSpec &grid,
-                              const double *labels, uint32_t label_count,
-                              uint32_t minimum_support, uint32_t best_view_count,
-                              float *scores, float *costs, float *guidance, uint8_t *tested,
-                              unsigned long long *work) {
-    const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
-    const uint32_t V = patch.view_count;
-    // about 1024 (node, label) units per block, within 40 KB of LDS (4
-    // blocks of 256 threads per CU, the occupancy the tap loop's registers
-    // allow)
-    uint32_t nb = 1024 / label_count;
-    if (nb < 1) nb = 1;
-    if (nb > 16) nb = 16;
-    while (nb > 1 && cost_lds_bytes(nb, V, T, label_count) > 40 * 1024) nb--;
-    const size_t lds = cost_lds_bytes(nb, V, T, label_count);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const uint32_t nodes = grid.columns * grid.rows;
-    hipLaunchKernelGGL(k_cost_volume, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
-                       grid, labels, label_count, nb, T, minimum_support, best_view_count, scores,
-                       costs, guidance, tested, work);
+// Default shape: about 1024 (node, label) units per block, within 40 KB of
+// LDS (4 blocks of 256 threads per CU, the occupancy the tap loop's
+// registers allow), one slab.
+void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
+    const uint32_t V = J.S.view_count;
+    J.T = (2 * J.S.radius + 1) * (2 * J.S.radius + 1);
+    J.slab = label_slab == 0 || label_slab > J.L ? J.L : label_slab;
+    uint32_t nb = nodes_per_block;
+    if (nb == 0) {
+        nb = 1024 / J.slab;
+        if (nb < 1) nb = 1;
+        if (nb > 16) nb = 16;
+        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab) > 40 * 1024) nb--;
+    }
+    J.nb = nb;
+    const uint32_t nodes = J.G.columns * J.G.rows;
+    J.tiles = (nodes + nb - 1) / nb;
+    J.slabs = (J.L + J.slab - 1) / J.slab;
+}
+
+size_t cost_job_lds(const CostJob &J) {
+    return cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab);
+}
+
+hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {
+    const size_t lds = cost_job_lds(job);
+    if (lds > 160 * 1024 || job.nb == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cost_volume, dim3(job.tiles * job.slabs), dim3(256), lds, stream, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
+                                    const uint32_t *first, uint32_t njobs, size_t lds,
+                                    uint32_t blocks, uint32_t *counter) {
+    if (lds > 160 * 1024 || njobs == 0) return hipErrorInvalidValue;
+    hipError_t e = launch_fill_u32(stream, counter, 0u, 1);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cost_volume_queue, dim3(blocks), dim3(256), lds, stream, jobs, first,
+                       njobs, counter);
     return hipGetLastError();
 }
 

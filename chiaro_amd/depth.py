@@ -151,6 +151,7 @@ class Session:
                                                reference_index, C.byref(options.to_c()),
                                                C.byref(handle)), "session create")
         self.handle = handle
+        self._reference = reference_index
         g = _abi.GridSizes()
         self.none = self.lib.chd_session_grid(handle, C.byref(g)) == _abi.CHD_NONE
         self.grid = g
@@ -187,7 +188,34 @@ class Session:
     def reset_kernel_times(self):
         _abi.check(self.lib.chd_session_reset_kernel_times(self.handle), "reset times")
 
+    def set_tuning(self, cost_nodes_per_block: int = 0, cost_label_slab: int = 0,
+                   sgm_paths_per_block: int = 0):
+        t = _abi.Tuning(cost_nodes_per_block, cost_label_slab, sgm_paths_per_block, 0)
+        _abi.check(self.lib.chd_session_set_tuning(self.handle, C.byref(t)), "set tuning")
+
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        _abi.check(self.lib.chd_session_get_tuning(self.handle, C.byref(t)), "get tuning")
+        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_ if name != "reserved"}
+
+    def autotune(self, cache_path: str | None = None) -> dict:
+        t = _abi.Tuning()
+        path = None if cache_path is None else str(cache_path).encode()
+        _abi.check(self.lib.chd_autotune(self.handle, path, C.byref(t)), "autotune")
+        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_ if name != "reserved"}
+
     # --- intermediate results (parity checks)
+    def view_scores(self):
+        """Per-view ZNCC slices of the last run with CHD_STAGE_KEEP_VIEW_SCORES:
+        (views, nodes, labels) f32, views in index order over the targets."""
+        g = self.grid
+        views = sum(1 for i, m in enumerate(self.modules)
+                    if i != self._reference and m.accepted and m.camera is not None)
+        out = np.empty((views, g.coarse_columns * g.coarse_rows, g.labels), np.float32)
+        _abi.check(self.lib.chd_session_read_view_scores(self.handle, _abi.ptr(out)),
+                   "read view scores")
+        return out
+
     def cost_volume(self):
         g = self.grid
         nodes = g.coarse_columns * g.coarse_rows
@@ -262,6 +290,36 @@ class Session:
                 pts, cf, dec = keep[i]
                 out[i] = TargetResult(pts, cf, dec, _report_dict(t.report))
         return dmap, out
+
+
+class Batch:
+    """Several sessions run together (chd_batch): one persistent cost-volume
+    launch over all captures, one SGM sweep launch, then each capture's
+    remaining stages, captured once per stage mask into a hipGraph."""
+
+    def __init__(self, context: Context, sessions):
+        self.context = context
+        self.lib = context.lib
+        self.sessions = list(sessions)
+        arr = (C.c_void_p * len(self.sessions))(*[s.handle.value for s in self.sessions])
+        handle = C.c_void_p()
+        _abi.check(self.lib.chd_batch_create(context.handle, arr, len(self.sessions),
+                                             C.byref(handle)), "batch create")
+        self.handle = handle
+
+    def run(self, stages: int = _abi.CHD_STAGE_ALL) -> int:
+        return _abi.check(self.lib.chd_batch_run(self.handle, stages), "batch run")
+
+    def close(self):
+        if self.handle:
+            self.lib.chd_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def refine_multiview_depth(modules, reference_index: int, options: DepthOptions,

@@ -190,3 +190,54 @@ def config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: floa
                      "views": 5, "planes": planes}, desc)
     cap.raws = raws
     return cap
+
+
+def _small_rotation(rng: np.random.Generator, degrees: float) -> np.ndarray:
+    """Rotation about a uniformly random axis by an angle U[-degrees, degrees]."""
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    a = math.radians(rng.uniform(-degrees, degrees))
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(a) * k + (1.0 - math.cos(a)) * (k @ k)
+
+
+def config2(seed: int = 0, scale: float = 1.0, planes: int = 128,
+            plane_count: int = 16) -> Capture:
+    """One capture of the batched workload: ten modules of uneven size, the
+    reference (4160x3120, f = 3300 px) at the origin plus four more of that
+    kind, three 2080x1560 at f = 4125 px and two 1040x780 at f = 4425 px;
+    each non-reference module at a baseline U[10, 60] mm in a random
+    direction of the sensor plane with a random rotation of up to 1 degree.
+    Rig parameters come from numpy's PCG64 seeded with (seed, 2). 12-bit
+    Bayer codes as in config1, 16 planes, 128 hypotheses; `scale` shrinks
+    every module's raster and focal length together (for small test
+    captures)."""
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 2]))
+    specs = [(3300.0, 4160, 3120)] * 5 + [(4125.0, 2080, 1560)] * 3 + [(4425.0, 1040, 780)] * 2
+
+    def size(n):
+        return max(16, int(round(n * scale)) // 8 * 8)
+
+    modules = []
+    for k, (f, w, h) in enumerate(specs):
+        if k == 0:
+            modules.append(_module(f * scale, size(w), size(h), (0.0, 0.0, 0.0)))
+            continue
+        baseline = rng.uniform(10.0, 60.0)
+        angle = rng.uniform(0.0, 2.0 * math.pi)
+        rotation = _small_rotation(rng, 1.0)
+        modules.append(_module(f * scale, size(w), size(h),
+                               (baseline * math.cos(angle), baseline * math.sin(angle), 0.0),
+                               rotation))
+    desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.05e-3,
+                          inverse_depth_max=1.25e-3, texture_cell_px=32)
+    bayer = _abi.Bayer(**BAYER_12BIT)
+    raws = _render(desc, modules, 0, 0.0, bayer)
+    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
+    options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
+    cap = _assemble(modules, planes_, 0, options,
+                    {"workload": "config2", "seed": seed, "scale": scale, "views": len(specs),
+                     "planes": planes}, desc)
+    cap.raws = raws
+    return cap
+

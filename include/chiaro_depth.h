@@ -255,6 +255,9 @@ void chd_session_destroy(chd_session *session);
 #define CHD_STAGE_ALL 15u
 /* Keep the aggregated SGM volume (debug/parity only; costs HBM traffic). */
 #define CHD_STAGE_KEEP_AGGREGATED 256u
+/* Keep every view's own ZNCC slice of the cost volume (the pairwise scores
+ * before aggregate, depth.rs:1761-1770): views x nodes x labels f32. */
+#define CHD_STAGE_KEEP_VIEW_SCORES 512u
 
 int chd_session_run(chd_session *session, uint32_t stages);
 int chd_session_grid(const chd_session *session, chd_grid_sizes *out);
@@ -297,6 +300,44 @@ int chd_session_enable_timing(chd_session *session, int enable);
 int chd_session_kernel_times(chd_session *session, chd_kernel_time *out,
                              uint32_t capacity, uint32_t *count);
 int chd_session_reset_kernel_times(chd_session *session);
+
+/* Per-view ZNCC slices kept by a run with CHD_STAGE_KEEP_VIEW_SCORES:
+ * [view slot][node][label], view slots in view-index order over the
+ * accepted, calibrated targets (NaN = None). */
+int chd_session_read_view_scores(chd_session *session, float *out);
+
+/* Launch shape of the session's kernels. It changes the schedule only:
+ * every shape computes the same results bit for bit. Zero fields take the
+ * defaults. No counterpart in the reference (CPU threads). */
+typedef struct chd_tuning {
+    uint32_t cost_nodes_per_block; /* coarse nodes per cost-volume work item */
+    uint32_t cost_label_slab;      /* labels per cost-volume work item */
+    uint32_t sgm_paths_per_block;  /* SGM path sweeps (wavefronts) per block: 1, 2 or 4 */
+    uint32_t reserved;
+} chd_tuning;
+int chd_session_set_tuning(chd_session *session, const chd_tuning *tuning);
+int chd_session_get_tuning(const chd_session *session, chd_tuning *out);
+
+/* Times candidate shapes on the session's own data (it runs the cost and
+ * SGM stages several times) and sets the fastest. With cache_path, results
+ * are read from and appended to that text file, keyed by the shape of the
+ * work (views, taps, labels, grid, plane sizes), so a later call on the same
+ * shape only reads it. */
+int chd_autotune(chd_session *session, const char *cache_path, chd_tuning *out);
+
+/* Batched execution of several sessions (captures) on one context: the
+ * cost volumes of all of them in one persistent launch fed by a (capture,
+ * node tile, label slab) work queue, all their SGM path sweeps in one
+ * launch, then each capture's remaining stages; the whole sequence for a
+ * given `stages` mask is captured once into a hipGraph and replayed by every
+ * later run. Results are read per session as after chd_session_run, and are
+ * the same bit for bit. Sessions must share the label count and outlive the
+ * batch; no counterpart in the reference (one capture at a time). */
+typedef struct chd_batch chd_batch;
+int chd_batch_create(chd_context *context, chd_session *const *sessions, uint32_t count,
+                     chd_batch **out);
+int chd_batch_run(chd_batch *batch, uint32_t stages);
+void chd_batch_destroy(chd_batch *batch);
 
 /* Stage entry points on host arrays, for parity checks of one stage. */
 

@@ -1,0 +1,200 @@
+"""Schedules that must not change results: launch shapes (tuning), the
+autotuner and its cache, batched captures (persistent cost queue, batched SGM
+sweeps, hipGraph replay), and the per-view score slices against runs of one
+view pair alone. Every comparison is bit for bit; the uneven-module config 2
+capture is also checked against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from chiaro_amd import _abi, scene
+from chiaro_amd.depth import Batch, Session
+
+from parity import assert_bitwise, assert_nodes
+
+pytestmark = pytest.mark.gpu
+
+ALL_KEEP = _abi.CHD_STAGE_ALL | _abi.CHD_STAGE_KEEP_AGGREGATED
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _small_config0(seed, scale=0.25):
+    w, h = int(1040 * scale) // 8 * 8, int(780 * scale) // 8 * 8
+    return scene.config0(seed=seed, width=w, height=h, focal_px=1600.0 * scale)
+
+
+def _snapshot(sess):
+    """Every output of a finished run, as host arrays."""
+    scores, costs, guidance, tested = sess.cost_volume()
+    out = {"scores": scores, "costs": costs, "guidance": guidance, "tested": tested,
+           "aggregated": sess.aggregated()}
+    field, fillable, best = sess.coarse()
+    out.update(coarse=field, fillable=fillable, best=best)
+    for k in range(4):
+        out[f"direct{k}"] = sess.direct(k)[0]
+    dmap, targets = sess.results()
+    out.update(depth=dmap.depth, confidence=dmap.confidence, provenance=dmap.provenance)
+    for t, r in targets.items():
+        out[f"t{t}.decisions"] = r.decisions
+        out[f"t{t}.points"] = r.points
+        out[f"t{t}.confidence"] = r.confidence
+    return out
+
+
+def _same(label, got, want):
+    assert sorted(got) == sorted(want), f"{label}: outputs differ in keys"
+    for key in want:
+        if want[key].dtype == _abi.NODE_DTYPE:
+            assert_nodes(f"{label} {key}", got[key], want[key])
+        else:
+            assert_bitwise(f"{label} {key}", got[key], want[key])
+
+
+def _eager(ctx, cap):
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    try:
+        sess.run(ALL_KEEP)
+        return _snapshot(sess)
+    finally:
+        sess.close()
+
+
+def test_tuning_shapes_bit_exact(gpu_context):
+    cap = _small_config0(3)
+    _say("tuning shapes on config0 quarter seed 3")
+    want = _eager(gpu_context, cap)
+    sess = Session(gpu_context, cap.modules, cap.reference_index, cap.options)
+    try:
+        for nb, slab, wpb in [(1, 0, 1), (4, 0, 2), (3, 17, 4), (12, 33, 1), (16, 65, 2)]:
+            _say(f"  nodes/block {nb} slab {slab} paths/block {wpb}")
+            sess.set_tuning(nb, slab, wpb)
+            assert sess.tuning() == {"cost_nodes_per_block": nb, "cost_label_slab": slab,
+                                     "sgm_paths_per_block": wpb}
+            sess.run(ALL_KEEP)
+            _same(f"tuning {nb}/{slab}/{wpb}", _snapshot(sess), want)
+    finally:
+        sess.close()
+
+
+def test_tuning_rejects_bad_shapes(gpu_context):
+    cap = _small_config0(3)
+    sess = Session(gpu_context, cap.modules, cap.reference_index, cap.options)
+    try:
+        with pytest.raises(RuntimeError):
+            sess.set_tuning(0, 0, 3)
+        with pytest.raises(RuntimeError):
+            sess.set_tuning(65, 0, 0)
+    finally:
+        sess.close()
+
+
+def test_autotune_and_cache(gpu_context, tmp_path):
+    cap = _small_config0(4)
+    _say("autotune on config0 quarter seed 4")
+    want = _eager(gpu_context, cap)
+    cache = tmp_path / "tuning.txt"
+    sess = Session(gpu_context, cap.modules, cap.reference_index, cap.options)
+    try:
+        tuned = sess.autotune(cache)
+        assert tuned["cost_nodes_per_block"] > 0 and tuned["sgm_paths_per_block"] in (1, 2, 4)
+        lines = cache.read_text().splitlines()
+        assert len(lines) == 1 and lines[0].startswith("chd-tune-1 views=2 taps=81 labels=65")
+        assert sess.autotune(cache) == tuned       # read back, not timed again
+        assert len(cache.read_text().splitlines()) == 1
+        sess.run(ALL_KEEP)
+        _same("autotuned", _snapshot(sess), want)
+    finally:
+        sess.close()
+
+
+def test_batch_matches_single_sessions_on_replays(gpu_context):
+    """Three captures (two sizes) batched; the captured graph is replayed
+    three times with eager work of another session in between, and every
+    replay equals the single-session results."""
+    caps = [_small_config0(5), _small_config0(6, scale=0.3), _small_config0(7)]
+    _say("batch of 3 config0 captures")
+    wants = [_eager(gpu_context, c) for c in caps]
+    sessions = [Session(gpu_context, c.modules, c.reference_index, c.options) for c in caps]
+    sessions[1].set_tuning(6, 33, 2)   # a different shape inside the same batch
+    other_cap = _small_config0(8)
+    other = Session(gpu_context, other_cap.modules, 0, other_cap.options)
+    batch = Batch(gpu_context, sessions)
+    try:
+        for replay in range(3):
+            _say(f"  replay {replay}")
+            batch.run(ALL_KEEP)
+            other.run(_abi.CHD_STAGE_ALL)        # eager work between replays
+            gpu_context.synchronize()
+            for k, (s, want) in enumerate(zip(sessions, wants)):
+                _same(f"replay {replay} capture {k}", _snapshot(s), want)
+    finally:
+        batch.close()
+        for s in sessions:
+            s.close()
+        other.close()
+
+
+def test_batch_stage_order_is_checked(gpu_context):
+    cap = _small_config0(5)
+    sess = Session(gpu_context, cap.modules, 0, cap.options)
+    batch = Batch(gpu_context, [sess])
+    try:
+        with pytest.raises(RuntimeError):
+            batch.run(_abi.CHD_STAGE_DIRECT)
+    finally:
+        batch.close()
+        sess.close()
+
+
+def test_view_scores_equal_single_pair_runs(gpu_context):
+    """Each target's ZNCC slice in a batched run equals that of a session
+    holding only the reference and that target."""
+    cap = _small_config0(10)
+    _say("view scores vs single pairs, config0 quarter seed 10")
+    sess = Session(gpu_context, cap.modules, 0, cap.options)
+    batch = Batch(gpu_context, [sess])
+    try:
+        batch.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_KEEP_VIEW_SCORES)
+        slices = sess.view_scores()
+        assert slices.shape[0] == 2
+        for k, t in enumerate([1, 2]):
+            pair = Session(gpu_context, [cap.modules[0], cap.modules[t]], 0, cap.options)
+            try:
+                pair.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_KEEP_VIEW_SCORES)
+                assert_bitwise(f"view {t} slice", slices[k], pair.view_scores()[0])
+            finally:
+                pair.close()
+        # the aggregate of the slices is the cost volume's score
+        scores = sess.cost_volume()[0]
+        assert np.isfinite(slices).any() and np.array_equal(np.isnan(scores),
+                                                            np.isnan(slices).any(axis=0))
+    finally:
+        batch.close()
+        sess.close()
+
+
+def test_config2_batch_bit_exact_with_oracle(gpu_context):
+    """Two small config 2 captures (ten modules of three sizes, rotated rig)
+    batched: each equals its single-session run, and the first equals the
+    oracle."""
+    from test_gpu_parity import _full_parity
+    caps = [scene.config2(seed=s, scale=0.125) for s in (0, 1)]
+    _say("config2 small: single-session parity with the oracle")
+    _full_parity(gpu_context, caps[0], "config2 scale 0.125 seed 0")
+    wants = [_eager(gpu_context, c) for c in caps]
+    sessions = [Session(gpu_context, c.modules, 0, c.options) for c in caps]
+    batch = Batch(gpu_context, sessions)
+    try:
+        batch.run(ALL_KEEP)
+        gpu_context.synchronize()
+        for k, (s, want) in enumerate(zip(sessions, wants)):
+            _same(f"config2 capture {k}", _snapshot(s), want)
+    finally:
+        batch.close()
+        for s in sessions:
+            s.close()
