@@ -124,7 +124,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["config1", "config0"], default="config1")
+    ap.add_argument("--workload", choices=["config1", "config0", "config2"], default="config1")
+    ap.add_argument("--captures", type=int, default=8,
+                    help="config2: captures per batch (one persistent launch, hipGraph replay)")
+    ap.add_argument("--tuning", choices=["default", "auto"], default="default",
+                    help="launch shapes: the defaults, or chd_autotune on this capture")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--full-steps", type=int, default=2,
                     help="steps of the whole path (direct pass and targets) timed beside")
@@ -152,6 +156,9 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    if args.workload == "config2":
+        return bench_batch(args)
+
     from chiaro_amd import _abi, scene
     from chiaro_amd.depth import Context, Session
 
@@ -164,6 +171,7 @@ def main() -> int:
     g = sess.grid
     units = g.coarse_columns * g.coarse_rows * g.labels
     stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    tuning = sess.autotune() if args.tuning == "auto" else sess.tuning()
 
     # Every kernel of the path runs on the context's stream, so a stream
     # synchronise on both sides brackets the timed region. (torch is not
@@ -273,6 +281,7 @@ def main() -> int:
             "labels": g.labels,
             "views": len(cap.modules),
             "parallelism": "1 GPU",
+            "tuning": tuning,
         },
         "units_per_s": round(units / (ms_per_step / 1000.0), 1),
         "kernels_ms": {k: round(v[1] / args.steps, 4) for k, v in kernels.items()},
@@ -287,6 +296,103 @@ def main() -> int:
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     sess.close()
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+def bench_batch(args) -> int:
+    """Config 2: `captures` captures of ten modules of three sizes run as one
+    batch (one SGM sweep launch over all captures, hipGraph replay); the
+    same batch with the persistent cost-volume queue and the sessions run one
+    after another are timed beside it."""
+    from chiaro_amd import _abi, scene
+    from chiaro_amd.depth import Batch, Context, Session
+
+    t = time.perf_counter()
+    caps = [scene.config2(seed=args.seed + k) for k in range(args.captures)]
+    log(f"[bench] {args.captures} config2 captures generated in {time.perf_counter() - t:.1f} s")
+    ctx = Context(0)
+    sessions = [Session(ctx, c.modules, c.reference_index, c.options) for c in caps]
+    tuning = sessions[0].autotune() if args.tuning == "auto" else sessions[0].tuning()
+    if args.tuning == "auto":
+        for s in sessions[1:]:
+            s.set_tuning(**tuning)
+    batch = Batch(ctx, sessions)
+    queued = Batch(ctx, sessions, persistent=True)
+    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+
+    def timed(fn, steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / steps
+
+    for _ in range(args.warmup):
+        batch.run(stages)
+    log(f"[bench] timing {args.steps} batched steps")
+    ms = timed(lambda: batch.run(stages), args.steps)
+    cost_ms = timed(lambda: batch.run(_abi.CHD_STAGE_COST), args.steps)
+    zncc = sum(s.work()["cost_zncc"] for s in sessions)
+    queued.run(stages)
+    queued_ms = timed(lambda: queued.run(stages), max(1, args.steps // 2))
+
+    def sequential():
+        for s in sessions:
+            s.run(stages)
+
+    sequential()
+    seq_ms = timed(sequential, max(1, args.steps // 2))
+    ref = caps[0].modules[0]
+    planes = caps[0].options.planes
+    pixels = ref.width * ref.height * len(caps)
+    value = pixels * planes / (ms / 1000.0) / 1e6
+    g = sessions[0].grid
+    achieved = zncc * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix·hyp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 coordinates)",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"config2: {len(caps)} captures x 10 modules (5 x 4160x3120 f=3300, "
+                         f"3 x 2080x1560 f=4125, 2 x 1040x780 f=4425), {planes} hypotheses, "
+                         f"8-path SGM, seeds {args.seed}..{args.seed + len(caps) - 1}, batched"),
+            "coarse_grid": [g.coarse_columns, g.coarse_rows, g.coarse_step],
+            "labels": g.labels,
+            "parallelism": "1 GPU",
+            "tuning": tuning,
+        },
+        "persistent_queue_ms_per_step": round(queued_ms, 3),
+        "sequential_ms_per_step": round(seq_ms, 3),
+        "batched_over_sequential": round(seq_ms / ms, 3),
+        "cost_ms_per_step": round(cost_ms, 3),
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "k_cost_volume (one launch per capture)",
+            "zncc_per_launch": zncc,
+        },
+        "cpu_baseline": None,
+    }
+    batch.close()
+    queued.close()
+    for s in sessions:
+        s.close()
     ctx.close()
     print(json.dumps(line))
     return 0

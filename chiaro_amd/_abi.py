@@ -33,6 +33,7 @@ CHD_STAGE_TARGETS = 8
 CHD_STAGE_ALL = 15
 CHD_STAGE_KEEP_AGGREGATED = 256
 CHD_STAGE_KEEP_VIEW_SCORES = 512
+CHD_BATCH_PERSISTENT = 1
 
 WARP_UNDEFINED, WARP_GLOBAL, WARP_REFINED, WARP_OCCLUDED, WARP_BOUNDARY = range(5)
 PROVENANCE_UNSUPPORTED, PROVENANCE_GLOBAL, PROVENANCE_MEASURED, PROVENANCE_REGULARIZED = range(4)
@@ -210,7 +211,7 @@ class KernelTime(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cost_nodes_per_block", C.c_uint32), ("cost_label_slab", C.c_uint32),
-                ("sgm_paths_per_block", C.c_uint32), ("reserved", C.c_uint32)]
+                ("sgm_paths_per_block", C.c_uint32), ("cost_tile_order", C.c_uint32)]
 
 
 # ------------------------------------------------------------- loaders
@@ -287,7 +288,7 @@ DEPTH_SIGNATURES = {
     "chd_session_set_tuning": (C.c_int, VP, P(Tuning)),
     "chd_session_get_tuning": (C.c_int, VP, P(Tuning)),
     "chd_autotune": (C.c_int, VP, C.c_char_p, P(Tuning)),
-    "chd_batch_create": (C.c_int, VP, P(VP), C.c_uint32, P(VP)),
+    "chd_batch_create": (C.c_int, VP, P(VP), C.c_uint32, C.c_uint32, P(VP)),
     "chd_batch_run": (C.c_int, VP, C.c_uint32),
     "chd_batch_destroy": (None, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),

@@ -248,6 +248,7 @@ CostJob cost_job(const chd_session *s, float *view_scores) {
     j.tested = s->tested;
     j.view_scores = view_scores;
     j.work = s->work;
+    j.xcd_order = s->tuning.cost_tile_order;
     cost_job_shape(j, s->tuning.cost_nodes_per_block, s->tuning.cost_label_slab);
     return j;
 }
@@ -906,6 +907,7 @@ int chd_session_set_tuning(chd_session *s, const chd_tuning *t) {
         return fail(CHD_ERR_ARGUMENT, "set_tuning: sgm_paths_per_block must be 1, 2 or 4");
     if (t->cost_nodes_per_block > 64)
         return fail(CHD_ERR_LIMIT, "set_tuning: more than 64 nodes per block");
+    if (t->cost_tile_order > 2) return fail(CHD_ERR_ARGUMENT, "set_tuning: cost_tile_order is 0, 1 or 2");
     if (!s->none) {
         chd_tuning old = s->tuning;
         s->tuning = *t;
@@ -1306,7 +1308,7 @@ namespace {
 std::string shape_key(const chd_session *s) {
     const chd_grid_sizes &g = s->grid;
     const uint32_t r = s->options.patch_radius;
-    std::string key = "chd-tune-1 views=" + std::to_string(s->active.size()) +
+    std::string key = "chd-tune-2 views=" + std::to_string(s->active.size()) +
                       " taps=" + std::to_string((2 * r + 1) * (2 * r + 1)) +
                       " labels=" + std::to_string(g.labels) + " grid=" +
                       std::to_string(g.coarse_columns) + "x" + std::to_string(g.coarse_rows);
@@ -1327,8 +1329,9 @@ bool read_tuning_cache(const char *path, const std::string &key, chd_tuning &out
         if (!tab) continue;
         *tab = 0;
         chd_tuning t{};
-        if (key == line && std::sscanf(tab + 1, "%u %u %u", &t.cost_nodes_per_block,
-                                       &t.cost_label_slab, &t.sgm_paths_per_block) == 3) {
+        if (key == line && std::sscanf(tab + 1, "%u %u %u %u", &t.cost_nodes_per_block,
+                                       &t.cost_label_slab, &t.sgm_paths_per_block,
+                                       &t.cost_tile_order) == 4) {
             out = t;  // the last entry for a key wins
             found = true;
         }
@@ -1341,8 +1344,8 @@ void append_tuning_cache(const char *path, const std::string &key, const chd_tun
     if (!path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s\t%u %u %u\n", key.c_str(), t.cost_nodes_per_block, t.cost_label_slab,
-                 t.sgm_paths_per_block);
+    std::fprintf(f, "%s\t%u %u %u %u\n", key.c_str(), t.cost_nodes_per_block,
+                 t.cost_label_slab, t.sgm_paths_per_block, t.cost_tile_order);
     std::fclose(f);
 }
 
@@ -1388,11 +1391,13 @@ int chd_autotune(chd_session *s, const char *cache_path, chd_tuning *out) {
     int r = CHD_OK;
     const uint32_t slabs[2] = {L, (L + 1) / 2};
     const uint32_t nodes[7] = {2, 4, 6, 8, 10, 12, 16};
+    for (uint32_t oi = 0; r == CHD_OK && oi < 2; oi++)
     for (uint32_t si = 0; r == CHD_OK && si < 2; si++) {
         for (uint32_t ni = 0; r == CHD_OK && ni < 7; ni++) {
             chd_tuning t = s->tuning;
             t.cost_nodes_per_block = nodes[ni];
             t.cost_label_slab = slabs[si];
+            t.cost_tile_order = oi;
             s->tuning = t;
             if (cost_job_lds(cost_job(s, nullptr)) > 64 * 1024) continue;
             float ms = 0.0f;
@@ -1442,7 +1447,8 @@ struct chd_batch {
     uint32_t blocks = 0;
     SgmJob *sgm_jobs = nullptr;
     uint32_t *sgm_first = nullptr;
-    uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4;
+    uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
+    bool persistent = false;  // one queue launch for every cost volume
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
     std::vector<hipGraph_t> templates;
 
@@ -1475,8 +1481,16 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
     const uint32_t n = (uint32_t)b->sessions.size();
     if (stages & CHD_STAGE_COST) {
         for (chd_session *s : b->sessions) CHD_CHECK(prepare_cost(s));
-        CHD_HIP(launch_cost_volume_queue(st, b->jobs[keep_views ? 1 : 0], b->first, n, b->lds,
-                                         b->blocks, b->counter));
+        if (b->persistent) {
+            CHD_HIP(launch_cost_volume_queue(st, b->jobs[keep_views ? 1 : 0], b->first, n,
+                                             b->lds, b->blocks, b->counter, b->xcd_order));
+        } else {
+            for (chd_session *s : b->sessions) {
+                float *vs = nullptr;
+                CHD_CHECK(view_score_buffer(s, keep_views, &vs));
+                CHD_HIP(launch_cost_volume(st, cost_job(s, vs)));
+            }
+        }
     }
     if (stages & CHD_STAGE_SGM) {
         CHD_HIP(launch_sgm_batch(st, b->sgm_jobs, b->sgm_first, n, b->sgm_total, b->labels,
@@ -1494,12 +1508,14 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
 }  // extern "C++"
 
 int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t count,
-                     chd_batch **out) {
+                     uint32_t flags, chd_batch **out) {
     if (!c || !sessions || !out || count == 0) return fail(CHD_ERR_ARGUMENT, "batch: bad argument");
+    if (flags & ~CHD_BATCH_PERSISTENT) return fail(CHD_ERR_ARGUMENT, "batch: unknown flags");
     *out = nullptr;
     CHD_HIP(hipSetDevice(c->device));
     auto *b = new chd_batch;
     b->context = c;
+    b->persistent = (flags & CHD_BATCH_PERSISTENT) != 0;
     auto bail = [&](int code) {
         delete b;
         return code;
@@ -1535,7 +1551,10 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
         chd_session *s = b->sessions[k];
         sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
         sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
-        if (k == 0) wpb = s->tuning.sgm_paths_per_block;
+        if (k == 0) {
+            wpb = s->tuning.sgm_paths_per_block;
+            b->xcd_order = s->tuning.cost_tile_order;
+        }
     }
     b->sgm_total = sgm_first[count];
     b->sgm_paths_per_block = wpb ? wpb : 4;
@@ -1543,7 +1562,7 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
     if (r == CHD_OK) r = balloc(b, &b->jobs[0], count);
     if (r == CHD_OK) r = balloc(b, &b->jobs[1], count);
     if (r == CHD_OK) r = balloc(b, &b->first, count + 1);
-    if (r == CHD_OK) r = balloc(b, &b->counter, 1);
+    if (r == CHD_OK) r = balloc(b, &b->counter, kXcds);
     if (r == CHD_OK) r = balloc(b, &b->sgm_jobs, count);
     if (r == CHD_OK) r = balloc(b, &b->sgm_first, count + 1);
     if (r != CHD_OK) return bail(r);

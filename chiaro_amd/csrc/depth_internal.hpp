@@ -68,7 +68,10 @@ struct CostJob {
     uint8_t *tested;         // cleared before the launch
     float *view_scores;      // optional [view slot][node][label] per-view ZNCC, NaN = None
     unsigned long long *work;
+    uint32_t xcd_order;      // 1: each XCD takes a contiguous run of work items
 };
+
+constexpr uint32_t kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round robin
 
 // nodes_per_block / label_slab of 0 pick the defaults.
 void cost_job_shape(CostJob &job, uint32_t nodes_per_block, uint32_t label_slab);
@@ -78,10 +81,11 @@ hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job);
 
 // Persistent launch over njobs device-resident jobs; first[] (device, njobs +
 // 1 entries) is the prefix of tiles * slabs; lds must cover every job;
-// counter is a device word the launch clears first.
+// counters are kXcds device words the launch clears first. With xcd_order
+// each XCD drains its own contiguous run of items before taking others'.
 hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
                                     const uint32_t *first, uint32_t njobs, size_t lds,
-                                    uint32_t blocks, uint32_t *counter);
+                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order);
 
 // paths_per_block: wavefronts (one path each) per block, 1, 2 or 4.
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,

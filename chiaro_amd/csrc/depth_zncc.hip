@@ -492,11 +492,29 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
         atomicAdd(J.work, (unsigned long long)*evaluation_count);
 }
 
-// One launch over one job: block b is item b (tile b / slabs, slab b % slabs).
+// The contiguous run of `total` items that XCD x takes in XCD order: runs
+// of total / 8 items, the first total % 8 of them one longer.
+__device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &start,
+                                        uint32_t &count) {
+    const uint32_t q = total / kXcds, r = total % kXcds;
+    start = x * q + min(x, r);
+    count = q + (x < r ? 1u : 0u);
+}
+
+// One launch over one job: block b is item b (tile item / slabs, slab item %
+// slabs), or with XCD order item k of XCD b % 8's contiguous run, where k =
+// b / 8 (the hardware hands workgroups to the XCDs round robin, so each L2
+// then serves neighbouring tiles).
 __global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
-    cost_tile(J, blockIdx.x / J.slabs, blockIdx.x % J.slabs, lds, &evaluation_count);
+    uint32_t item = blockIdx.x;
+    if (J.xcd_order == 1) {
+        uint32_t start, count;
+        xcd_run(blockIdx.x % kXcds, gridDim.x, start, count);
+        item = start + blockIdx.x / kXcds;
+    }
+    cost_tile(J, item / J.slabs, item % J.slabs, lds, &evaluation_count);
 }
 
 // Persistent form over several jobs (captures): each block takes the next
@@ -504,16 +522,47 @@ __global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
 // captures of uneven cost share the chip without a tail per capture. `first`
 // holds each job's first item index (njobs + 1 entries); `counter` must be 0
 // at launch.
+// The queue is split into 8 sequences, one per XCD, each with its own
+// counter: by default XCD x takes items x, x + 8, x + 16, ..., the order in
+// which the hardware hands a plain launch's workgroups to the XCDs (measured
+// faster than one shared counter); with xcd_order XCD x takes its contiguous
+// run instead. A block drains its own XCD's sequence, then the others' in
+// turn.
 __global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__restrict__ jobs,
                                                            const uint32_t *__restrict__ first,
-                                                           uint32_t njobs, uint32_t *counter) {
+                                                           uint32_t njobs, uint32_t *counters,
+                                                           uint32_t xcd_order) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     __shared__ uint32_t next;
     const uint32_t total = first[njobs];
+    const uint32_t own = blockIdx.x % kXcds;
+    uint32_t run = 0;  // sequences tried so far
+    uint32_t round = 0;
     for (;;) {
         __syncthreads();  // the previous item has finished with `next` and LDS
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        if (threadIdx.x == 0 && xcd_order == 2) {
+            next = blockIdx.x + round * gridDim.x;  // static: no counter
+            round++;
+        } else if (threadIdx.x == 0) {
+            next = total;
+            while (run < kXcds) {
+                const uint32_t x = (own + run) % kXcds;
+                uint32_t start, count;
+                if (xcd_order) {
+                    xcd_run(x, total, start, count);
+                } else {
+                    start = x;
+                    count = total > x ? (total - x + kXcds - 1) / kXcds : 0;
+                }
+                const uint32_t k = atomicAdd(counters + x, 1u);
+                if (k < count) {
+                    next = xcd_order ? start + k : start + k * kXcds;
+                    break;
+                }
+                run++;
+            }
+        }
         __syncthreads();
         const uint32_t item = next;
         if (item >= total) break;
@@ -563,12 +612,12 @@ hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {
 
 hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
                                     const uint32_t *first, uint32_t njobs, size_t lds,
-                                    uint32_t blocks, uint32_t *counter) {
+                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order) {
     if (lds > 160 * 1024 || njobs == 0) return hipErrorInvalidValue;
-    hipError_t e = launch_fill_u32(stream, counter, 0u, 1);
+    hipError_t e = launch_fill_u32(stream, counters, 0u, kXcds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cost_volume_queue, dim3(blocks), dim3(256), lds, stream, jobs, first,
-                       njobs, counter);
+                       njobs, counters, xcd_order);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@ The arithmetic runs in the HIP kernels; this module only marshals arrays.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, fields
 
 import numpy as np
@@ -84,12 +85,19 @@ class Context:
         handle = C.c_void_p()
         _abi.check(self.lib.chd_context_create(device, C.byref(handle)), "context create")
         self.handle = handle
+        # objects living on this context, closed before it (batches first)
+        self._batches = weakref.WeakSet()
+        self._sessions = weakref.WeakSet()
 
     def synchronize(self):
         _abi.check(self.lib.chd_context_synchronize(self.handle), "synchronize")
 
     def close(self):
         if self.handle:
+            for b in list(self._batches):
+                b.close()
+            for s in list(self._sessions):
+                s.close()
             self.lib.chd_context_destroy(self.handle)
             self.handle = None
 
@@ -151,6 +159,7 @@ class Session:
                                                reference_index, C.byref(options.to_c()),
                                                C.byref(handle)), "session create")
         self.handle = handle
+        context._sessions.add(self)
         self._reference = reference_index
         g = _abi.GridSizes()
         self.none = self.lib.chd_session_grid(handle, C.byref(g)) == _abi.CHD_NONE
@@ -189,20 +198,21 @@ class Session:
         _abi.check(self.lib.chd_session_reset_kernel_times(self.handle), "reset times")
 
     def set_tuning(self, cost_nodes_per_block: int = 0, cost_label_slab: int = 0,
-                   sgm_paths_per_block: int = 0):
-        t = _abi.Tuning(cost_nodes_per_block, cost_label_slab, sgm_paths_per_block, 0)
+                   sgm_paths_per_block: int = 0, cost_tile_order: int = 0):
+        t = _abi.Tuning(cost_nodes_per_block, cost_label_slab, sgm_paths_per_block,
+                        cost_tile_order)
         _abi.check(self.lib.chd_session_set_tuning(self.handle, C.byref(t)), "set tuning")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()
         _abi.check(self.lib.chd_session_get_tuning(self.handle, C.byref(t)), "get tuning")
-        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_ if name != "reserved"}
+        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_}
 
     def autotune(self, cache_path: str | None = None) -> dict:
         t = _abi.Tuning()
         path = None if cache_path is None else str(cache_path).encode()
         _abi.check(self.lib.chd_autotune(self.handle, path, C.byref(t)), "autotune")
-        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_ if name != "reserved"}
+        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_}
 
     # --- intermediate results (parity checks)
     def view_scores(self):
@@ -293,19 +303,22 @@ class Session:
 
 
 class Batch:
-    """Several sessions run together (chd_batch): one persistent cost-volume
-    launch over all captures, one SGM sweep launch, then each capture's
-    remaining stages, captured once per stage mask into a hipGraph."""
+    """Several sessions run together (chd_batch): one SGM sweep launch over
+    all captures, the other stages per capture (with persistent=True the cost
+    volumes as one persistent work-queue launch), captured once per stage mask
+    into a hipGraph."""
 
-    def __init__(self, context: Context, sessions):
+    def __init__(self, context: Context, sessions, persistent: bool = False):
         self.context = context
         self.lib = context.lib
         self.sessions = list(sessions)
         arr = (C.c_void_p * len(self.sessions))(*[s.handle.value for s in self.sessions])
         handle = C.c_void_p()
-        _abi.check(self.lib.chd_batch_create(context.handle, arr, len(self.sessions),
+        flags = _abi.CHD_BATCH_PERSISTENT if persistent else 0
+        _abi.check(self.lib.chd_batch_create(context.handle, arr, len(self.sessions), flags,
                                              C.byref(handle)), "batch create")
         self.handle = handle
+        context._batches.add(self)
 
     def run(self, stages: int = _abi.CHD_STAGE_ALL) -> int:
         return _abi.check(self.lib.chd_batch_run(self.handle, stages), "batch run")

@@ -313,7 +313,10 @@ typedef struct chd_tuning {
     uint32_t cost_nodes_per_block; /* coarse nodes per cost-volume work item */
     uint32_t cost_label_slab;      /* labels per cost-volume work item */
     uint32_t sgm_paths_per_block;  /* SGM path sweeps (wavefronts) per block: 1, 2 or 4 */
-    uint32_t reserved;
+    uint32_t cost_tile_order;      /* 0: work items in launch order; 1: each of the 8 XCDs
+                                      (own L2) takes a contiguous run of node tiles; 2 (the
+                                      persistent queue only): a static stride instead of
+                                      the queue counters */
 } chd_tuning;
 int chd_session_set_tuning(chd_session *session, const chd_tuning *tuning);
 int chd_session_get_tuning(const chd_session *session, chd_tuning *out);
@@ -325,17 +328,19 @@ int chd_session_get_tuning(const chd_session *session, chd_tuning *out);
  * shape only reads it. */
 int chd_autotune(chd_session *session, const char *cache_path, chd_tuning *out);
 
-/* Batched execution of several sessions (captures) on one context: the
- * cost volumes of all of them in one persistent launch fed by a (capture,
- * node tile, label slab) work queue, all their SGM path sweeps in one
- * launch, then each capture's remaining stages; the whole sequence for a
- * given `stages` mask is captured once into a hipGraph and replayed by every
- * later run. Results are read per session as after chd_session_run, and are
+/* Batched execution of several sessions (captures) on one context: all
+ * their SGM path sweeps in one launch, each capture's cost volume and
+ * remaining stages in its own launches, the whole sequence for a given
+ * `stages` mask captured once into a hipGraph and replayed by every later
+ * run. With CHD_BATCH_PERSISTENT the cost volumes of all captures run
+ * instead as one persistent launch fed by a (capture, node tile, label slab)
+ * work queue. Results are read per session as after chd_session_run, and are
  * the same bit for bit. Sessions must share the label count and outlive the
  * batch; no counterpart in the reference (one capture at a time). */
 typedef struct chd_batch chd_batch;
+#define CHD_BATCH_PERSISTENT 1u
 int chd_batch_create(chd_context *context, chd_session *const *sessions, uint32_t count,
-                     chd_batch **out);
+                     uint32_t flags, chd_batch **out);
 int chd_batch_run(chd_batch *batch, uint32_t stages);
 void chd_batch_destroy(chd_batch *batch);
 

@@ -70,13 +70,14 @@ def test_tuning_shapes_bit_exact(gpu_context):
     want = _eager(gpu_context, cap)
     sess = Session(gpu_context, cap.modules, cap.reference_index, cap.options)
     try:
-        for nb, slab, wpb in [(1, 0, 1), (4, 0, 2), (3, 17, 4), (12, 33, 1), (16, 65, 2)]:
-            _say(f"  nodes/block {nb} slab {slab} paths/block {wpb}")
-            sess.set_tuning(nb, slab, wpb)
+        for nb, slab, wpb, order in [(1, 0, 1, 0), (4, 0, 2, 1), (3, 17, 4, 0), (12, 33, 1, 1),
+                                     (16, 65, 2, 0), (7, 0, 4, 1)]:
+            _say(f"  nodes/block {nb} slab {slab} paths/block {wpb} xcd order {order}")
+            sess.set_tuning(nb, slab, wpb, order)
             assert sess.tuning() == {"cost_nodes_per_block": nb, "cost_label_slab": slab,
-                                     "sgm_paths_per_block": wpb}
+                                     "sgm_paths_per_block": wpb, "cost_tile_order": order}
             sess.run(ALL_KEEP)
-            _same(f"tuning {nb}/{slab}/{wpb}", _snapshot(sess), want)
+            _same(f"tuning {nb}/{slab}/{wpb}/{order}", _snapshot(sess), want)
     finally:
         sess.close()
 
@@ -89,6 +90,8 @@ def test_tuning_rejects_bad_shapes(gpu_context):
             sess.set_tuning(0, 0, 3)
         with pytest.raises(RuntimeError):
             sess.set_tuning(65, 0, 0)
+        with pytest.raises(RuntimeError):
+            sess.set_tuning(0, 0, 0, 3)
     finally:
         sess.close()
 
@@ -103,7 +106,7 @@ def test_autotune_and_cache(gpu_context, tmp_path):
         tuned = sess.autotune(cache)
         assert tuned["cost_nodes_per_block"] > 0 and tuned["sgm_paths_per_block"] in (1, 2, 4)
         lines = cache.read_text().splitlines()
-        assert len(lines) == 1 and lines[0].startswith("chd-tune-1 views=2 taps=81 labels=65")
+        assert len(lines) == 1 and lines[0].startswith("chd-tune-2 views=2 taps=81 labels=65")
         assert sess.autotune(cache) == tuned       # read back, not timed again
         assert len(cache.read_text().splitlines()) == 1
         sess.run(ALL_KEEP)
@@ -112,18 +115,20 @@ def test_autotune_and_cache(gpu_context, tmp_path):
         sess.close()
 
 
-def test_batch_matches_single_sessions_on_replays(gpu_context):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_matches_single_sessions_on_replays(gpu_context, persistent):
     """Three captures (two sizes) batched; the captured graph is replayed
     three times with eager work of another session in between, and every
     replay equals the single-session results."""
     caps = [_small_config0(5), _small_config0(6, scale=0.3), _small_config0(7)]
-    _say("batch of 3 config0 captures")
+    _say(f"batch of 3 config0 captures, persistent cost queue {persistent}")
     wants = [_eager(gpu_context, c) for c in caps]
     sessions = [Session(gpu_context, c.modules, c.reference_index, c.options) for c in caps]
-    sessions[1].set_tuning(6, 33, 2)   # a different shape inside the same batch
+    sessions[0].set_tuning(0, 0, 0, 1)  # the queue in XCD order
+    sessions[1].set_tuning(6, 33, 2)    # a different shape inside the same batch
     other_cap = _small_config0(8)
     other = Session(gpu_context, other_cap.modules, 0, other_cap.options)
-    batch = Batch(gpu_context, sessions)
+    batch = Batch(gpu_context, sessions, persistent=persistent)
     try:
         for replay in range(3):
             _say(f"  replay {replay}")
@@ -151,13 +156,14 @@ def test_batch_stage_order_is_checked(gpu_context):
         sess.close()
 
 
-def test_view_scores_equal_single_pair_runs(gpu_context):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_view_scores_equal_single_pair_runs(gpu_context, persistent):
     """Each target's ZNCC slice in a batched run equals that of a session
     holding only the reference and that target."""
     cap = _small_config0(10)
-    _say("view scores vs single pairs, config0 quarter seed 10")
+    _say(f"view scores vs single pairs, config0 quarter seed 10, persistent {persistent}")
     sess = Session(gpu_context, cap.modules, 0, cap.options)
-    batch = Batch(gpu_context, [sess])
+    batch = Batch(gpu_context, [sess], persistent=persistent)
     try:
         batch.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_KEEP_VIEW_SCORES)
         slices = sess.view_scores()
@@ -188,7 +194,7 @@ def test_config2_batch_bit_exact_with_oracle(gpu_context):
     _full_parity(gpu_context, caps[0], "config2 scale 0.125 seed 0")
     wants = [_eager(gpu_context, c) for c in caps]
     sessions = [Session(gpu_context, c.modules, 0, c.options) for c in caps]
-    batch = Batch(gpu_context, sessions)
+    batch = Batch(gpu_context, sessions, persistent=True)
     try:
         batch.run(ALL_KEEP)
         gpu_context.synchronize()
