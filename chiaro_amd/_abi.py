@@ -291,6 +291,17 @@ DEPTH_SIGNATURES = {
     "chd_batch_create": (C.c_int, VP, P(VP), C.c_uint32, C.c_uint32, P(VP)),
     "chd_batch_run": (C.c_int, VP, C.c_uint32),
     "chd_batch_destroy": (None, VP),
+    "chd_proposal_create": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P(VP)),
+    "chd_proposal_destroy": (None, VP),
+    "chd_proposal_parameters": (C.c_int, VP, VP, VP, P(C.c_size_t)),
+    "chd_proposal_set_dataset": (C.c_int, VP, VP, VP, C.c_uint32),
+    "chd_proposal_train": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, VP),
+    "chd_proposal_predict": (C.c_int, VP, VP, C.c_uint32, VP),
+    "chd_proposal_gradients": (C.c_int, VP, VP, VP, C.c_uint32, VP, P(C.c_float)),
+    "chd_proposal_adam_step": (C.c_int, VP, VP, C.c_float),
+    "chd_session_set_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
+                                 P(C.c_uint32), P(C.c_uint32)),
+    "chd_session_clear_proposal": (C.c_int, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),
     "chd_select_depths": (C.c_int, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32, P(Options), VP, VP,
                           VP),

@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <random>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 
 #include "../../include/chiaro_depth.h"
 #include "depth_internal.hpp"
+#include "proposal.hpp"
 
 using namespace chd;
 
@@ -139,6 +141,10 @@ struct chd_session {
     uint8_t *map_provenance = nullptr;
     unsigned long long *work = nullptr;  // cost, direct, wide, target ZNCCs
     float *view_scores = nullptr;        // [view slot][node][label], on request
+    // band sweep proposed by a chd_proposal: label windows per coarse node
+    uint16_t *band_lo = nullptr, *band_hi = nullptr;
+    bool band = false;
+    uint32_t version = 0;  // bumped when launch parameters change (batches recapture)
     chd_tuning tuning{};
     uint32_t stages_done = 0;
     bool timing = false;
@@ -249,6 +255,8 @@ CostJob cost_job(const chd_session *s, float *view_scores) {
     j.view_scores = view_scores;
     j.work = s->work;
     j.xcd_order = s->tuning.cost_tile_order;
+    j.window_lo = s->band ? s->band_lo : nullptr;
+    j.window_hi = s->band ? s->band_hi : nullptr;
     cost_job_shape(j, s->tuning.cost_nodes_per_block, s->tuning.cost_label_slab);
     return j;
 }
@@ -271,6 +279,11 @@ int prepare_cost(chd_session *s) {
     hipStream_t st = s->context->stream;
     CHD_HIP(launch_fill_u64(st, s->work, 0ull, 1));
     CHD_HIP(launch_fill_u8(st, s->tested, 0, (size_t)s->grid.coarse_columns * s->grid.coarse_rows));
+    if (s->band) {  // labels outside the windows: score None, cost +inf
+        const size_t volume = (size_t)s->grid.coarse_columns * s->grid.coarse_rows * s->grid.labels;
+        CHD_HIP(launch_fill_u32(st, (uint32_t *)s->scores, 0x7fc00000u, volume));
+        CHD_HIP(launch_fill_u32(st, (uint32_t *)s->costs, 0x7f800000u, volume));
+    }
     return CHD_OK;
 }
 
@@ -917,6 +930,7 @@ int chd_session_set_tuning(chd_session *s, const chd_tuning *t) {
         }
     }
     s->tuning = *t;
+    s->version++;
     return CHD_OK;
 }
 
@@ -1451,6 +1465,14 @@ struct chd_batch {
     bool persistent = false;  // one queue launch for every cost volume
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
     std::vector<hipGraph_t> templates;
+    std::vector<uint32_t> versions;  // sessions' versions the graphs were captured with
+
+    void drop_graphs() {
+        for (auto &g : graphs) hipGraphExecDestroy(g.second);
+        for (hipGraph_t g : templates) hipGraphDestroy(g);
+        graphs.clear();
+        templates.clear();
+    }
 
     ~chd_batch() {
         for (auto &g : graphs) hipGraphExecDestroy(g.second);
@@ -1507,6 +1529,56 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
 }  // namespace
 }  // extern "C++"
 
+extern "C++" {
+namespace {
+
+// Job tables, item prefixes and launch sizes from the sessions' current
+// launch parameters; drops captured graphs. Synchronous.
+int batch_prepare(chd_batch *b) {
+    chd_context *c = b->context;
+    hipStream_t st = c->stream;
+    CHD_HIP(hipStreamSynchronize(st));
+    b->drop_graphs();
+    b->views_ready = false;
+    const uint32_t count = (uint32_t)b->sessions.size();
+    std::vector<CostJob> plain(count);
+    std::vector<uint32_t> first(count + 1, 0);
+    b->lds = 0;
+    b->versions.clear();
+    for (uint32_t k = 0; k < count; k++) {
+        plain[k] = cost_job(b->sessions[k], nullptr);
+        first[k + 1] = first[k] + plain[k].tiles * plain[k].slabs;
+        b->lds = std::max(b->lds, cost_job_lds(plain[k]));
+        b->versions.push_back(b->sessions[k]->version);
+    }
+    if (b->lds > 160 * 1024) return fail(CHD_ERR_LIMIT, "batch: cost work item above 160 KB of LDS");
+    int cus = 0;
+    CHD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint32_t per_cu = std::max<uint32_t>(
+        1, std::min<uint32_t>(4, (uint32_t)(160 * 1024 / std::max<size_t>(b->lds, 1))));
+    b->blocks = (uint32_t)std::max(cus, 1) * per_cu;
+    std::vector<SgmJob> sgm(count);
+    std::vector<uint32_t> sgm_first(count + 1, 0);
+    for (uint32_t k = 0; k < count; k++) {
+        chd_session *s = b->sessions[k];
+        sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
+        sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
+    }
+    b->sgm_total = sgm_first[count];
+    const uint32_t wpb = b->sessions[0]->tuning.sgm_paths_per_block;
+    b->sgm_paths_per_block = wpb ? wpb : 4;
+    b->xcd_order = b->sessions[0]->tuning.cost_tile_order;
+    CHD_HIP(hipMemcpyAsync(b->jobs[0], plain.data(), count * sizeof(CostJob), hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(b->first, first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(b->sgm_jobs, sgm.data(), count * sizeof(SgmJob), hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(b->sgm_first, sgm_first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
 int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t count,
                      uint32_t flags, chd_batch **out) {
     if (!c || !sessions || !out || count == 0) return fail(CHD_ERR_ARGUMENT, "batch: bad argument");
@@ -1530,34 +1602,6 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
         b->labels = s->grid.labels;
         b->sessions.push_back(s);
     }
-    // cost jobs, the item prefix, and the largest work item
-    std::vector<CostJob> plain(count);
-    std::vector<uint32_t> first(count + 1, 0);
-    for (uint32_t k = 0; k < count; k++) {
-        plain[k] = cost_job(b->sessions[k], nullptr);
-        first[k + 1] = first[k] + plain[k].tiles * plain[k].slabs;
-        b->lds = std::max(b->lds, cost_job_lds(plain[k]));
-    }
-    if (b->lds > 160 * 1024) return bail(fail(CHD_ERR_LIMIT, "batch: cost work item above 160 KB of LDS"));
-    int cus = 0;
-    CHD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(4, (uint32_t)(160 * 1024 / std::max<size_t>(b->lds, 1))));
-    b->blocks = (uint32_t)std::max(cus, 1) * per_cu;
-    // SGM jobs
-    std::vector<SgmJob> sgm(count);
-    std::vector<uint32_t> sgm_first(count + 1, 0);
-    uint32_t wpb = 0;
-    for (uint32_t k = 0; k < count; k++) {
-        chd_session *s = b->sessions[k];
-        sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
-        sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
-        if (k == 0) {
-            wpb = s->tuning.sgm_paths_per_block;
-            b->xcd_order = s->tuning.cost_tile_order;
-        }
-    }
-    b->sgm_total = sgm_first[count];
-    b->sgm_paths_per_block = wpb ? wpb : 4;
     int r = CHD_OK;
     if (r == CHD_OK) r = balloc(b, &b->jobs[0], count);
     if (r == CHD_OK) r = balloc(b, &b->jobs[1], count);
@@ -1565,14 +1609,8 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
     if (r == CHD_OK) r = balloc(b, &b->counter, kXcds);
     if (r == CHD_OK) r = balloc(b, &b->sgm_jobs, count);
     if (r == CHD_OK) r = balloc(b, &b->sgm_first, count + 1);
+    if (r == CHD_OK) r = batch_prepare(b);
     if (r != CHD_OK) return bail(r);
-    hipStream_t st = c->stream;
-    hipError_t e = hipMemcpyAsync(b->jobs[0], plain.data(), count * sizeof(CostJob), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->first, first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->sgm_jobs, sgm.data(), count * sizeof(SgmJob), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->sgm_first, sgm_first.data(), (count + 1) * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e)));
     *out = b;
     return CHD_OK;
 }
@@ -1587,6 +1625,11 @@ int chd_batch_run(chd_batch *b, uint32_t stages) {
         if ((stages & CHD_STAGE_TARGETS) && !(done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "targets before direct pass");
     }
     hipStream_t st = b->context->stream;
+    // a session whose launch parameters changed (tuning, band) needs new
+    // job tables and graphs
+    std::vector<uint32_t> versions;
+    for (chd_session *s : b->sessions) versions.push_back(s->version);
+    if (versions != b->versions) CHD_CHECK(batch_prepare(b));
     if ((stages & CHD_STAGE_KEEP_VIEW_SCORES) && !b->views_ready) {
         // the per-view slices are allocated on first request, outside any capture
         std::vector<CostJob> views;
@@ -1647,6 +1690,424 @@ void chd_batch_destroy(chd_batch *b) {
     hipSetDevice(b->context->device);
     hipStreamSynchronize(b->context->stream);
     delete b;
+}
+
+// ------------------------------------------------------------ proposal
+
+struct chd_proposal {
+    chd_context *context = nullptr;
+    uint32_t width = 0, height = 0, max_batch = 0;
+    ConvLayer layers[7];
+    struct Param {
+        float *w, *b, *mw, *vw, *mb, *vb, *gw, *gb;
+        uint16_t *wk, *wt;
+    } params[7];
+    uint16_t *act[7] = {};  // act[0] the input, act[i] the output of layer i (i <= 6)
+    float *out = nullptr;   // output of layer 7 (f32)
+    float *gact[7] = {};    // d loss / d act[i] (f32), i = 1 .. 6
+    float *gout = nullptr;  // d loss / d out
+    uint16_t *gbf = nullptr;  // BF16 gradient of the layer being back-propagated
+    uint16_t *labels = nullptr;
+    float *totals = nullptr;
+    uint16_t *data_inputs = nullptr, *data_labels = nullptr;
+    uint32_t data_count = 0;
+    uint64_t step = 0;
+    uint32_t *full_tiles = nullptr;
+    std::vector<void *> allocations;
+    std::vector<void *> dataset;  // allocations of the current training set
+
+    ~chd_proposal() {
+        for (void *p : dataset) hipFree(p);
+        for (void *p : allocations) hipFree(p);
+    }
+};
+
+extern "C++" {
+namespace {
+
+template <class T>
+int palloc(chd_proposal *p, T **out, size_t count, std::vector<void *> *list = nullptr) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(CHD_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    (list ? *list : p->allocations).push_back(q);
+    *out = (T *)q;
+    return CHD_OK;
+}
+
+// The network: 3x3 convolutions, stride 2 down twice, nearest x2 up twice.
+void proposal_layers(uint32_t w, uint32_t h, ConvLayer *L) {
+    const uint32_t w2 = (w + 1) / 2, h2 = (h + 1) / 2, w4 = (w2 + 1) / 2, h4 = (h2 + 1) / 2;
+    struct Spec { uint32_t cin, cout, hin, win, hout, wout, stride, up, relu; };
+    const Spec spec[7] = {
+        {3, 32, h, w, h, w, 1, 0, 1},       {32, 32, h, w, h2, w2, 2, 0, 1},
+        {32, 32, h2, w2, h2, w2, 1, 0, 1},  {32, 32, h2, w2, h4, w4, 2, 0, 1},
+        {32, 32, h4, w4, h4, w4, 1, 0, 1},  {32, 32, h4, w4, h2, w2, 1, 1, 1},
+        {32, 3, h2, w2, h, w, 1, 1, 0},
+    };
+    for (int i = 0; i < 7; i++) {
+        const Spec &s = spec[i];
+        L[i] = ConvLayer{s.cin, (s.cin + 15) / 16 * 16, s.cout, (s.cout + 15) / 16 * 16,
+                         s.hin, s.win, s.hout, s.wout, s.stride, s.up, s.relu, 0};
+        L[i].k = 9 * L[i].cin_p;
+    }
+}
+
+size_t layer_out(const ConvLayer &L, uint32_t batch) {
+    return (size_t)batch * L.hout * L.wout * L.cout_p;
+}
+
+int proposal_refresh_weights(chd_proposal *p) {
+    for (int i = 0; i < 7; i++)
+        CHD_HIP(launch_weights_bf16(p->context->stream, p->layers[i], p->params[i].w,
+                                    p->params[i].wk, p->params[i].wt));
+    return CHD_OK;
+}
+
+// Forward pass over `batch` inputs already in act[0].
+int proposal_forward(chd_proposal *p, uint32_t batch) {
+    hipStream_t st = p->context->stream;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        void *out = i == 6 ? (void *)p->out : (void *)p->act[i + 1];
+        CHD_HIP(launch_conv_fwd(st, L, batch, p->act[i], p->params[i].wt, p->params[i].b, out,
+                                i == 6));
+    }
+    return CHD_OK;
+}
+
+// Loss, backward pass and gradients (gw, gb) for `batch` inputs/labels.
+int proposal_backward(chd_proposal *p, uint32_t batch) {
+    hipStream_t st = p->context->stream;
+    const ConvLayer &last = p->layers[6];
+    const uint32_t M = batch * last.hout * last.wout;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        CHD_HIP(launch_fill_u32(st, (uint32_t *)p->params[i].gw, 0u, (size_t)L.k * L.cout_p));
+        CHD_HIP(launch_fill_u32(st, (uint32_t *)p->params[i].gb, 0u, L.cout_p));
+    }
+    CHD_HIP(launch_proposal_loss(st, p->out, p->labels, M, last.cout_p, (float)M, p->gout,
+                                 p->totals));
+    for (int i = 6; i >= 0; i--) {
+        const ConvLayer &L = p->layers[i];
+        const size_t n = layer_out(L, batch);
+        const float *g = i == 6 ? p->gout : p->gact[i + 1];
+        CHD_HIP(launch_grad_bf16(st, g, i == 6 ? nullptr : p->act[i + 1], n, i != 6, p->gbf));
+        CHD_HIP(launch_conv_dw(st, L, batch, p->act[i], p->gbf, p->params[i].gw));
+        CHD_HIP(launch_bias_grad(st, p->gbf, batch * L.hout * L.wout, L.cout_p, p->params[i].gb));
+        if (i > 0) {
+            const size_t nin = (size_t)batch * L.hin * L.win * L.cin_p;
+            CHD_HIP(launch_fill_u32(st, (uint32_t *)p->gact[i], 0u, nin));
+            CHD_HIP(launch_conv_dx(st, L, batch, p->gbf, p->params[i].wk, p->gact[i]));
+        }
+    }
+    return CHD_OK;
+}
+
+int proposal_adam(chd_proposal *p, float lr) {
+    hipStream_t st = p->context->stream;
+    p->step++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.0f - std::pow(b1, (float)p->step), bc2 = 1.0f - std::pow(b2, (float)p->step);
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        auto &q = p->params[i];
+        CHD_HIP(launch_adam(st, q.w, q.gw, q.mw, q.vw, L.k * L.cout_p, lr, b1, b2, eps, bc1, bc2));
+        CHD_HIP(launch_adam(st, q.b, q.gb, q.mb, q.vb, L.cout_p, lr, b1, b2, eps, bc1, bc2));
+    }
+    return proposal_refresh_weights(p);
+}
+
+size_t proposal_parameter_count(const chd_proposal *p) {
+    size_t n = 0;
+    for (int i = 0; i < 7; i++) n += (size_t)p->layers[i].k * p->layers[i].cout_p + p->layers[i].cout_p;
+    return n;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_t max_batch,
+                        uint64_t seed, chd_proposal **out) {
+    if (!c || !out || width < 4 || height < 4 || max_batch == 0)
+        return fail(CHD_ERR_ARGUMENT, "proposal: bad argument");
+    if ((size_t)width * height * max_batch > ((size_t)1 << 26))
+        return fail(CHD_ERR_LIMIT, "proposal: batch x width x height above 2^26 pixels");
+    *out = nullptr;
+    CHD_HIP(hipSetDevice(c->device));
+    auto *p = new chd_proposal;
+    p->context = c;
+    p->width = width;
+    p->height = height;
+    p->max_batch = max_batch;
+    proposal_layers(width, height, p->layers);
+    auto bail = [&](int code) {
+        delete p;
+        return code;
+    };
+    int r = CHD_OK;
+    // He-normal weights from a fixed generator, zero biases
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> normal(0.0f, 1.0f);
+    for (int i = 0; i < 7 && r == CHD_OK; i++) {
+        const ConvLayer &L = p->layers[i];
+        auto &q = p->params[i];
+        const size_t nw = (size_t)L.k * L.cout_p;
+        for (float **f : {&q.w, &q.mw, &q.vw, &q.gw})
+            if (r == CHD_OK) r = palloc(p, f, nw);
+        for (float **f : {&q.b, &q.mb, &q.vb, &q.gb})
+            if (r == CHD_OK) r = palloc(p, f, L.cout_p);
+        if (r == CHD_OK) r = palloc(p, &q.wk, nw);
+        if (r == CHD_OK) r = palloc(p, &q.wt, nw);
+        if (r != CHD_OK) break;
+        std::vector<float> w(nw, 0.0f), zeros(nw, 0.0f);
+        const float scale = std::sqrt(2.0f / (9.0f * (float)L.cin));
+        for (size_t k = 0; k < nw; k++) w[k] = normal(rng) * scale;
+        hipStream_t st = c->stream;
+        hipError_t e = hipMemcpyAsync(q.w, w.data(), nw * 4, hipMemcpyHostToDevice, st);
+        for (float *f : {q.mw, q.vw})
+            if (e == hipSuccess) e = hipMemcpyAsync(f, zeros.data(), nw * 4, hipMemcpyHostToDevice, st);
+        for (float *f : {q.b, q.mb, q.vb})
+            if (e == hipSuccess) e = hipMemcpyAsync(f, zeros.data(), L.cout_p * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) r = fail(CHD_ERR_HIP, std::string("proposal init: ") + hipGetErrorString(e));
+    }
+    const ConvLayer *L = p->layers;
+    if (r == CHD_OK) r = palloc(p, &p->act[0], (size_t)max_batch * L[0].hin * L[0].win * L[0].cin_p);
+    for (int i = 1; i < 7 && r == CHD_OK; i++) {
+        r = palloc(p, &p->act[i], layer_out(L[i - 1], max_batch));
+        if (r == CHD_OK) r = palloc(p, &p->gact[i], layer_out(L[i - 1], max_batch));
+    }
+    size_t largest = 0;
+    for (int i = 0; i < 7; i++) largest = std::max(largest, layer_out(L[i], max_batch));
+    if (r == CHD_OK) r = palloc(p, &p->out, layer_out(L[6], max_batch));
+    if (r == CHD_OK) r = palloc(p, &p->gout, layer_out(L[6], max_batch));
+    if (r == CHD_OK) r = palloc(p, &p->gbf, largest);
+    if (r == CHD_OK) r = palloc(p, &p->labels, (size_t)max_batch * width * height);
+    if (r == CHD_OK) r = palloc(p, &p->totals, 2);
+    if (r == CHD_OK) r = palloc(p, &p->full_tiles, 1);
+    if (r == CHD_OK) r = proposal_refresh_weights(p);
+    if (r != CHD_OK) return bail(r);
+    // act[0]'s padding channels must read zero
+    CHD_HIP(launch_fill_u32(c->stream, (uint32_t *)p->act[0], 0u,
+                            (size_t)max_batch * width * height * 16 / 2));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    *out = p;
+    return CHD_OK;
+}
+
+void chd_proposal_destroy(chd_proposal *p) {
+    if (!p) return;
+    hipSetDevice(p->context->device);
+    hipStreamSynchronize(p->context->stream);
+    delete p;
+}
+
+int chd_proposal_parameters(chd_proposal *p, float *get, const float *set, size_t *count) {
+    if (!p || !count) return fail(CHD_ERR_ARGUMENT, "proposal_parameters: bad argument");
+    const size_t n = proposal_parameter_count(p);
+    if (!get && !set) {
+        *count = n;
+        return CHD_OK;
+    }
+    if (*count != n) return fail(CHD_ERR_ARGUMENT, "proposal_parameters: wrong count");
+    hipStream_t st = p->context->stream;
+    size_t at = 0;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        const size_t nw = (size_t)L.k * L.cout_p;
+        if (get) {
+            CHD_HIP(hipMemcpyAsync(get + at, p->params[i].w, nw * 4, hipMemcpyDeviceToHost, st));
+            CHD_HIP(hipMemcpyAsync(get + at + nw, p->params[i].b, L.cout_p * 4, hipMemcpyDeviceToHost, st));
+        } else {
+            CHD_HIP(hipMemcpyAsync(p->params[i].w, set + at, nw * 4, hipMemcpyHostToDevice, st));
+            CHD_HIP(hipMemcpyAsync(p->params[i].b, set + at + nw, L.cout_p * 4, hipMemcpyHostToDevice, st));
+        }
+        at += nw + L.cout_p;
+    }
+    if (set) CHD_CHECK(proposal_refresh_weights(p));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_proposal_set_dataset(chd_proposal *p, const uint16_t *inputs, const uint16_t *labels,
+                             uint32_t count) {
+    if (!p || !inputs || !labels || count == 0)
+        return fail(CHD_ERR_ARGUMENT, "proposal_set_dataset: bad argument");
+    CHD_HIP(hipStreamSynchronize(p->context->stream));
+    for (void *q : p->dataset) hipFree(q);
+    p->dataset.clear();
+    p->data_count = 0;
+    const size_t pixels = (size_t)p->width * p->height;
+    CHD_CHECK(palloc(p, &p->data_inputs, pixels * 16 * count, &p->dataset));
+    CHD_CHECK(palloc(p, &p->data_labels, pixels * count, &p->dataset));
+    CHD_HIP(hipMemcpy(p->data_inputs, inputs, pixels * 16 * count * 2, hipMemcpyHostToDevice));
+    CHD_HIP(hipMemcpy(p->data_labels, labels, pixels * count * 2, hipMemcpyHostToDevice));
+    p->data_count = count;
+    return CHD_OK;
+}
+
+int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float learning_rate,
+                       uint64_t shuffle_seed, float *epoch_loss) {
+    if (!p || batch == 0 || batch > p->max_batch)
+        return fail(CHD_ERR_ARGUMENT, "proposal_train: bad argument");
+    if (p->data_count == 0) return fail(CHD_ERR_STATE, "proposal_train: no training set");
+    CHD_HIP(hipSetDevice(p->context->device));
+    hipStream_t st = p->context->stream;
+    const size_t pixels = (size_t)p->width * p->height;
+    std::vector<uint32_t> order(p->data_count);
+    for (uint32_t e = 0; e < epochs; e++) {
+        for (uint32_t k = 0; k < p->data_count; k++) order[k] = k;
+        std::mt19937_64 rng(shuffle_seed + e);
+        std::shuffle(order.begin(), order.end(), rng);
+        CHD_HIP(launch_fill_u32(st, (uint32_t *)p->totals, 0u, 2));
+        for (uint32_t first = 0; first < p->data_count; first += batch) {
+            const uint32_t n = std::min(batch, p->data_count - first);
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t s = order[first + k];
+                CHD_HIP(hipMemcpyAsync(p->act[0] + k * pixels * 16, p->data_inputs + s * pixels * 16,
+                                       pixels * 32, hipMemcpyDeviceToDevice, st));
+                CHD_HIP(hipMemcpyAsync(p->labels + k * pixels, p->data_labels + s * pixels,
+                                       pixels * 2, hipMemcpyDeviceToDevice, st));
+            }
+            CHD_CHECK(proposal_forward(p, n));
+            CHD_CHECK(proposal_backward(p, n));
+            CHD_CHECK(proposal_adam(p, learning_rate));
+        }
+        float totals[2] = {0.0f, 0.0f};
+        CHD_HIP(hipMemcpyAsync(totals, p->totals, 8, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipStreamSynchronize(st));
+        if (epoch_loss) epoch_loss[e] = totals[1] > 0.0f ? totals[0] / totals[1] : 0.0f;
+    }
+    return CHD_OK;
+}
+
+int chd_proposal_predict(chd_proposal *p, const uint16_t *inputs, uint32_t count,
+                         float *outputs) {
+    if (!p || !inputs || !outputs || count == 0) return fail(CHD_ERR_ARGUMENT, "proposal_predict: bad argument");
+    CHD_HIP(hipSetDevice(p->context->device));
+    hipStream_t st = p->context->stream;
+    const size_t pixels = (size_t)p->width * p->height;
+    std::vector<float> raw(pixels * 16 * p->max_batch);
+    for (uint32_t first = 0; first < count; first += p->max_batch) {
+        const uint32_t n = std::min(p->max_batch, count - first);
+        CHD_HIP(hipMemcpyAsync(p->act[0], inputs + (size_t)first * pixels * 16, n * pixels * 32,
+                               hipMemcpyHostToDevice, st));
+        CHD_CHECK(proposal_forward(p, n));
+        CHD_HIP(hipMemcpyAsync(raw.data(), p->out, n * pixels * 16 * 4, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < n * pixels; i++) {
+            float *o = outputs + ((size_t)first * pixels + i) * 3;
+            o[0] = 64.0f + 16.0f * raw[i * 16];
+            o[1] = raw[i * 16 + 1];
+            o[2] = 1.0f / (1.0f + std::exp(-raw[i * 16 + 2]));
+        }
+    }
+    return CHD_OK;
+}
+
+int chd_proposal_gradients(chd_proposal *p, const uint16_t *inputs, const uint16_t *labels,
+                           uint32_t count, float *gradients, float *loss) {
+    if (!p || !inputs || !labels || !gradients || count == 0 || count > p->max_batch)
+        return fail(CHD_ERR_ARGUMENT, "proposal_gradients: bad argument");
+    CHD_HIP(hipSetDevice(p->context->device));
+    hipStream_t st = p->context->stream;
+    const size_t pixels = (size_t)p->width * p->height;
+    CHD_HIP(hipMemcpyAsync(p->act[0], inputs, count * pixels * 32, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(p->labels, labels, count * pixels * 2, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_fill_u32(st, (uint32_t *)p->totals, 0u, 2));
+    CHD_CHECK(proposal_forward(p, count));
+    CHD_CHECK(proposal_backward(p, count));
+    size_t at = 0;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        const size_t nw = (size_t)L.k * L.cout_p;
+        CHD_HIP(hipMemcpyAsync(gradients + at, p->params[i].gw, nw * 4, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipMemcpyAsync(gradients + at + nw, p->params[i].gb, L.cout_p * 4, hipMemcpyDeviceToHost, st));
+        at += nw + L.cout_p;
+    }
+    float totals[2];
+    CHD_HIP(hipMemcpyAsync(totals, p->totals, 8, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = totals[0];
+    return CHD_OK;
+}
+
+int chd_proposal_adam_step(chd_proposal *p, const float *gradients, float learning_rate) {
+    if (!p || !gradients) return fail(CHD_ERR_ARGUMENT, "proposal_adam_step: bad argument");
+    hipStream_t st = p->context->stream;
+    size_t at = 0;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        const size_t nw = (size_t)L.k * L.cout_p;
+        CHD_HIP(hipMemcpyAsync(p->params[i].gw, gradients + at, nw * 4, hipMemcpyHostToDevice, st));
+        CHD_HIP(hipMemcpyAsync(p->params[i].gb, gradients + at + nw, L.cout_p * 4, hipMemcpyHostToDevice, st));
+        at += nw + L.cout_p;
+    }
+    CHD_CHECK(proposal_adam(p, learning_rate));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
+                             uint32_t half_width, uint32_t tile, uint32_t *full_tiles,
+                             uint32_t *tiles) {
+    if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
+        return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
+    if (s->none) return CHD_NONE;
+    if (s->context != p->context) return fail(CHD_ERR_ARGUMENT, "set_proposal: another context");
+    if (s->active.empty()) return fail(CHD_ERR_STATE, "set_proposal: no target view");
+    const auto ref = s->plane_sizes[s->reference];
+    if (ref.first / 2 != p->width || ref.second / 2 != p->height)
+        return fail(CHD_ERR_ARGUMENT, "set_proposal: the network's grid is not the reference plane at half size");
+    CHD_HIP(hipSetDevice(s->context->device));
+    hipStream_t st = s->context->stream;
+    const size_t nodes = (size_t)s->grid.coarse_columns * s->grid.coarse_rows;
+    if (!s->band_lo) {
+        CHD_CHECK(dalloc(s, &s->band_lo, nodes));
+        CHD_CHECK(dalloc(s, &s->band_hi, nodes));
+    }
+    // input: the reference and the first two targets, halved again
+    std::vector<ViewDev> views(s->count);
+    CHD_HIP(hipMemcpyAsync(views.data(), s->views, s->count * sizeof(ViewDev), hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    const float *planes[3] = {views[s->reference].plane, nullptr, nullptr};
+    uint32_t pitches[3] = {views[s->reference].plane_pitch, 0, 0};
+    for (size_t k = 0; k < 2 && k < s->active.size(); k++) {
+        const ViewDev &v = views[s->active[k]];
+        if (v.plane_width != ref.first || v.plane_height != ref.second)
+            return fail(CHD_ERR_ARGUMENT, "set_proposal: target planes differ in size from the reference");
+        planes[1 + k] = v.plane;
+        pitches[1 + k] = v.plane_pitch;
+    }
+    CHD_HIP(launch_quarter_input(st, planes, pitches, ref.first, ref.second, p->width, p->height,
+                                 p->act[0]));
+    CHD_CHECK(proposal_forward(p, 1));
+    CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
+    CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,
+                                s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
+                                threshold, half_width, tile, s->band_lo, s->band_hi,
+                                p->full_tiles));
+    if (!s->band) {
+        s->band = true;
+        s->version++;
+    }
+    if (tiles)
+        *tiles = ((s->grid.coarse_columns + tile - 1) / tile) * ((s->grid.coarse_rows + tile - 1) / tile);
+    if (full_tiles) {
+        CHD_HIP(hipMemcpyAsync(full_tiles, p->full_tiles, 4, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipStreamSynchronize(st));
+    }
+    return CHD_OK;
+}
+
+int chd_session_clear_proposal(chd_session *s) {
+    if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
+    if (s->band) {
+        s->band = false;
+        s->version++;
+    }
+    return CHD_OK;
 }
 
 }  // extern "C"

@@ -69,6 +69,9 @@ struct CostJob {
     float *view_scores;      // optional [view slot][node][label] per-view ZNCC, NaN = None
     unsigned long long *work;
     uint32_t xcd_order;      // 1: each XCD takes a contiguous run of work items
+    // optional per-node label windows (band sweep): label 0 plus lo .. hi;
+    // with them a work item is one tile (one slab)
+    const uint16_t *window_lo, *window_hi;
 };
 
 constexpr uint32_t kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round robin

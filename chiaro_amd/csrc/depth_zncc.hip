@@ -501,6 +501,104 @@ __device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &st
     count = q + (x < r ? 1u : 0u);
 }
 
+// The band form of cost_tile: each node sweeps label 0 (the global warp)
+// and its window lo .. hi only (window_lo/hi; a full window is 1 .. L - 1),
+// as proposed by the learned band proposal. Scores and costs of the labels
+// outside the window are left as filled beforehand (NaN, +inf), so the SGM
+// that follows never selects them; inside the window every value is the
+// full sweep's. One slab; items are (node, label) over the nodes' windows.
+__device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *lds,
+                               unsigned int *evaluation_count) {
+    const PatchSpec &S = J.S;
+    const uint32_t V = S.view_count, T = J.T, L = J.L, nb = J.nb;
+    const PatchLayout lay = patch_layout(nb, V, T, true);
+    const PatchLds P = bind_patch(lds, lay, true);
+    float *res = (float *)(lds + lay.total);          // [nb][V][L] (windows are <= L)
+    float *centre_value = res + (size_t)nb * V * L;   // [nb]
+    int *node_tested = (int *)(centre_value + nb);    // [nb]
+    uint32_t *base = (uint32_t *)(node_tested + nb);  // [nb + 1]
+    uint32_t *lo = base + nb + 1;                     // [nb]
+    uint8_t *item_node = (uint8_t *)(lo + nb);        // [nb * L]
+    const uint32_t nodes = J.G.columns * J.G.rows, node0 = tile * nb;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+
+    for (uint32_t n = tid; n < nb; n += nth) {
+        const uint32_t g = node0 + n;
+        P.pos[n] = node_position(J.G, g);
+        P.node_ok[n] = g < nodes ? 1 : 0;
+        node_tested[n] = 0;
+        lo[n] = g < nodes ? J.window_lo[g] : 1;
+    }
+    if (tid == 0) {
+        *evaluation_count = 0;
+        uint32_t b = 0;
+        for (uint32_t n = 0; n < nb; n++) {
+            const uint32_t g = node0 + n;
+            base[n] = b;
+            if (g < nodes) b += 1 + (uint32_t)J.window_hi[g] + 1 - J.window_lo[g];
+        }
+        base[nb] = b;
+    }
+    __syncthreads();
+    patch_nodes(S, P, nb, T, centre_value);
+    for (uint32_t n = tid; n < nb; n += nth)
+        for (uint32_t k = base[n]; k < base[n + 1]; k++) item_node[k] = (uint8_t)n;
+    __syncthreads();
+    const uint32_t items = base[nb];
+
+    for (uint32_t a = 0; a < V; a++) {
+        patch_view(S, P, nb, T, a);
+        const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
+        const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
+        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+        for (uint32_t i = tid; i < items; i += nth) {
+            const uint32_t n = item_node[i], j = i - base[n];
+            const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
+            float s = nan_f();
+            if (P.node_ok[n] && P.nv_ok[n]) {
+                double dx = 0.0, dy = 0.0;
+                bool ok = true;
+                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], J.labels[l], dx, dy);
+                float sc;
+                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                                           evaluation_count))
+                    s = sc;
+            }
+            res[((size_t)n * V + a) * L + j] = s;
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t i = tid; i < items; i += nth) {
+        const uint32_t n = item_node[i], j = i - base[n], g = node0 + n;
+        const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
+        float score;
+        if (!aggregate_scores(res + (size_t)n * V * L + j, L, V, J.minimum_support,
+                              J.best_view_count, score))
+            score = nan_f();
+        float cost;
+        if (finite_f(score)) {
+            node_tested[n] = 1;
+            const float near_prior =
+                l == 0 ? 0.0f : kNearDepthPrior * (float)(l - 1) / (float)(L - 2);
+            cost = rclampf(1.0f - score, 0.0f, 2.0f) + near_prior;
+        } else {
+            cost = kMissingCost;
+        }
+        J.scores[(size_t)g * L + l] = score;
+        J.costs[(size_t)g * L + l] = cost;
+    }
+    __syncthreads();
+    for (uint32_t n = tid; n < nb; n += nth) {
+        const uint32_t g = node0 + n;
+        if (g >= nodes) continue;
+        J.guidance[g] = centre_value[n];
+        if (node_tested[n]) J.tested[g] = 1;
+    }
+    if (tid == 0 && J.work && *evaluation_count)
+        atomicAdd(J.work, (unsigned long long)*evaluation_count);
+}
+
 // One launch over one job: block b is item b (tile item / slabs, slab item %
 // slabs), or with XCD order item k of XCD b % 8's contiguous run, where k =
 // b / 8 (the hardware hands workgroups to the XCDs round robin, so each L2
@@ -514,7 +612,10 @@ __global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
         xcd_run(blockIdx.x % kXcds, gridDim.x, start, count);
         item = start + blockIdx.x / kXcds;
     }
-    cost_tile(J, item / J.slabs, item % J.slabs, lds, &evaluation_count);
+    if (J.window_lo)
+        cost_tile_band(J, item, lds, &evaluation_count);
+    else
+        cost_tile(J, item / J.slabs, item % J.slabs, lds, &evaluation_count);
 }
 
 // Persistent form over several jobs (captures): each block takes the next
@@ -570,7 +671,10 @@ __global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__rest
         while (j + 1 < njobs && first[j + 1] <= item) j++;
         const CostJob J = jobs[j];
         const uint32_t local = item - first[j];
-        cost_tile(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
+        if (J.window_lo)
+            cost_tile_band(J, local, lds, &evaluation_count);
+        else
+            cost_tile(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
     }
 }
 
@@ -585,7 +689,7 @@ size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab)
 void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     const uint32_t V = J.S.view_count;
     J.T = (2 * J.S.radius + 1) * (2 * J.S.radius + 1);
-    J.slab = label_slab == 0 || label_slab > J.L ? J.L : label_slab;
+    J.slab = label_slab == 0 || label_slab > J.L || J.window_lo ? J.L : label_slab;
     uint32_t nb = nodes_per_block;
     if (nb == 0) {
         nb = 1024 / J.slab;
@@ -600,7 +704,9 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
 }
 
 size_t cost_job_lds(const CostJob &J) {
-    return cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab);
+    size_t bytes = cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab);
+    if (J.window_lo) bytes += (size_t)(2 * J.nb + 1) * 4 + (size_t)J.nb * J.L;  // band tables
+    return bytes;
 }
 
 hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {

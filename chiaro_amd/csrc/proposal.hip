@@ -1,0 +1,453 @@
+// proposal.hip - the learned hypothesis-band proposal (gfx950).
+//
+// Build-only component with no counterpart in the reference (the reference
+// sweeps every hypothesis on the CPU): a 7-layer 3x3 convolutional
+// encoder-decoder at quarter resolution reads the reference and two target
+// planes and predicts, per pixel, a centre label, a log band half-width and
+// a confidence. The cost volume then sweeps only +-8 labels around the
+// centre where the tile's confidence is high and falls back to the full
+// sweep elsewhere (the band windows below feed cost_tile).
+//
+// Arithmetic: BF16 weights and activations, FP32 accumulation on the matrix
+// cores (v_mfma_f32_16x16x16_bf16). Every convolution is an implicit GEMM:
+// rows are output pixels, columns output channels, and the reduction runs
+// over (tap, input channel) with k = tap * cin_p + c. Operand layout of the
+// 16x16x16 instruction, lane l = 16 * q + r (q = l / 16, r = l % 16):
+//   A: row r, k = 4q .. 4q + 3;  B: column r, k = 4q .. 4q + 3;
+//   D: rows 4q .. 4q + 3 of column r.
+// Backward: dX_col = dY W^T on the matrix cores, scattered to dX with f32
+// atomics (col2im); dW = X_col^T dY per pixel chunk, accumulated with f32
+// atomics; Adam on f32 master weights, then BF16 copies in both layouts.
+#include <hip/hip_runtime.h>
+
+#include "proposal.hpp"
+
+namespace chd {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Input pixel of output pixel (oy, ox) at tap (ky, kx); false in the zero
+// padding.
+__device__ __forceinline__ bool conv_source(const ConvLayer &L, uint32_t oy, uint32_t ox,
+                                            uint32_t ky, uint32_t kx, uint32_t &iy,
+                                            uint32_t &ix) {
+    if (L.upsample) {
+        const int uy = (int)oy + (int)ky - 1, ux = (int)ox + (int)kx - 1;
+        if (uy < 0 || ux < 0 || uy >= (int)(2 * L.hin) || ux >= (int)(2 * L.win)) return false;
+        iy = (uint32_t)uy >> 1;
+        ix = (uint32_t)ux >> 1;
+        return true;
+    }
+    const int y = (int)(oy * L.stride) + (int)ky - 1, x = (int)(ox * L.stride) + (int)kx - 1;
+    if (y < 0 || x < 0 || y >= (int)L.hin || x >= (int)L.win) return false;
+    iy = (uint32_t)y;
+    ix = (uint32_t)x;
+    return true;
+}
+
+// ------------------------------------------------------------- forward
+
+// One wavefront per 16 output pixels; all cout_p / 16 column tiles.
+template <bool OutF32>
+__global__ void __launch_bounds__(256) k_conv_fwd(ConvLayer L, uint32_t batch,
+                                                  const uint16_t *__restrict__ in,
+                                                  const uint16_t *__restrict__ wt,
+                                                  const float *__restrict__ bias, void *out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t plane = L.hout * L.wout, M = batch * plane;
+    const uint32_t m0 = wave * 16;
+    if (m0 >= M) return;
+    const uint32_t q = lane >> 4, r = lane & 15u;
+    const uint32_t m = m0 + r;
+    const bool mv = m < M;
+    const uint32_t n = mv ? m / plane : 0, rem = mv ? m % plane : 0;
+    const uint32_t oy = rem / L.wout, ox = rem % L.wout;
+    const uint32_t K = L.k, tiles = L.cout_p / 16;
+    f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    for (uint32_t tap = 0; tap < 9; tap++) {
+        uint32_t iy = 0, ix = 0;
+        const bool valid = mv && conv_source(L, oy, ox, tap / 3, tap % 3, iy, ix);
+        const uint16_t *px = in + (((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p;
+        for (uint32_t c0 = 0; c0 < L.cin_p; c0 += 16) {
+            bf16x4 a = {0, 0, 0, 0};
+            if (valid) a = *(const bf16x4 *)(px + c0 + 4 * q);
+            const uint32_t k0 = tap * L.cin_p + c0;
+            for (uint32_t t = 0; t < tiles; t++) {
+                const bf16x4 b = *(const bf16x4 *)(wt + (size_t)(t * 16 + r) * K + k0 + 4 * q);
+                acc[t] = mfma16(a, b, acc[t]);
+            }
+        }
+    }
+    for (uint32_t t = 0; t < tiles; t++) {
+        const uint32_t ch = t * 16 + r;
+        for (uint32_t v = 0; v < 4; v++) {
+            const uint32_t p = m0 + 4 * q + v;
+            if (p >= M) continue;
+            float y = 0.0f;
+            if (ch < L.cout) {
+                y = acc[t][v] + bias[ch];
+                if (L.relu) y = fmaxf(y, 0.0f);
+            }
+            if (OutF32)
+                ((float *)out)[(size_t)p * L.cout_p + ch] = y;
+            else
+                ((uint16_t *)out)[(size_t)p * L.cout_p + ch] = f2bf(y);
+        }
+    }
+}
+
+// ------------------------------------------------------------ backward
+
+// dy (f32) -> BF16, zero where the layer's ReLU output was zero.
+__global__ void k_grad_bf16(const float *__restrict__ dy, const uint16_t *__restrict__ y,
+                            size_t n, int relu, uint16_t *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = relu && !(bf2f(y[i]) > 0.0f) ? 0.0f : dy[i];
+    out[i] = f2bf(g);
+}
+
+// dX_col = dY W^T per 16 pixels x 16 k, scattered into dX (f32).
+__global__ void __launch_bounds__(256) k_conv_dx(ConvLayer L, uint32_t batch,
+                                                 const uint16_t *__restrict__ dy,
+                                                 const uint16_t *__restrict__ wk,
+                                                 float *__restrict__ dx) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t plane = L.hout * L.wout, M = batch * plane;
+    const uint32_t ktiles = L.k / 16, mtiles = (M + 15) / 16;
+    if (wave >= mtiles * ktiles) return;
+    const uint32_t m0 = (wave / ktiles) * 16, k0 = (wave % ktiles) * 16;
+    const uint32_t q = lane >> 4, r = lane & 15u;
+    const uint32_t m = m0 + r;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t c0 = 0; c0 < L.cout_p; c0 += 16) {
+        bf16x4 a = {0, 0, 0, 0};
+        if (m < M) a = *(const bf16x4 *)(dy + (size_t)m * L.cout_p + c0 + 4 * q);
+        const bf16x4 b = *(const bf16x4 *)(wk + (size_t)(k0 + r) * L.cout_p + c0 + 4 * q);
+        acc = mfma16(a, b, acc);
+    }
+    const uint32_t k = k0 + r, tap = k / L.cin_p, c = k % L.cin_p;
+    if (c >= L.cin) return;
+    for (uint32_t v = 0; v < 4; v++) {
+        const uint32_t p = m0 + 4 * q + v;
+        if (p >= M) continue;
+        const uint32_t n = p / plane, rem = p % plane;
+        uint32_t iy, ix;
+        if (!conv_source(L, rem / L.wout, rem % L.wout, tap / 3, tap % 3, iy, ix)) continue;
+        atomicAdd(dx + (((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p + c, acc[v]);
+    }
+}
+
+// dW = X_col^T dY over chunks of `chunk` pixels (a multiple of 16) per
+// wavefront; each wavefront owns a 16 k x 16 output-channel tile.
+__global__ void __launch_bounds__(256) k_conv_dw(ConvLayer L, uint32_t batch,
+                                                 const uint16_t *__restrict__ in,
+                                                 const uint16_t *__restrict__ dy, uint32_t chunk,
+                                                 float *__restrict__ dw) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t plane = L.hout * L.wout, M = batch * plane;
+    const uint32_t ktiles = L.k / 16, ctiles = L.cout_p / 16, chunks = (M + chunk - 1) / chunk;
+    if (wave >= ktiles * ctiles * chunks) return;
+    const uint32_t kt = wave % ktiles, ct = (wave / ktiles) % ctiles, mc = wave / (ktiles * ctiles);
+    const uint32_t q = lane >> 4, r = lane & 15u;
+    const uint32_t k = kt * 16 + r, tap = k / L.cin_p, c = k % L.cin_p, co = ct * 16 + r;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t mend = min(M, (mc + 1) * chunk);
+    for (uint32_t m0 = mc * chunk; m0 < mend; m0 += 16) {
+        bf16x4 a, b;
+        for (uint32_t v = 0; v < 4; v++) {
+            const uint32_t p = m0 + 4 * q + v;
+            short av = 0, bv = 0;
+            if (p < mend) {
+                const uint32_t n = p / plane, rem = p % plane;
+                uint32_t iy, ix;
+                if (c < L.cin &&
+                    conv_source(L, rem / L.wout, rem % L.wout, tap / 3, tap % 3, iy, ix))
+                    av = (short)in[(((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p + c];
+                bv = (short)dy[(size_t)p * L.cout_p + co];
+            }
+            a[v] = av;
+            b[v] = bv;
+        }
+        acc = mfma16(a, b, acc);
+    }
+    for (uint32_t v = 0; v < 4; v++)
+        atomicAdd(dw + (size_t)(kt * 16 + 4 * q + v) * L.cout_p + co, acc[v]);
+}
+
+// db[c] += sum over pixels of the (masked, BF16) gradient dy[p][c], one
+// thread per (chunk, c).
+__global__ void k_bias_grad(const uint16_t *__restrict__ dy, uint32_t M, uint32_t cout_p,
+                            uint32_t chunk, float *__restrict__ db) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = i % cout_p, mc = i / cout_p;
+    if (mc * chunk >= M) return;
+    float s = 0.0f;
+    const uint32_t end = min(M, (mc + 1) * chunk);
+    for (uint32_t p = mc * chunk; p < end; p++) s += bf2f(dy[(size_t)p * cout_p + c]);
+    atomicAdd(db + c, s);
+}
+
+// ------------------------------------------------------------ loss
+
+// Per pixel of the last layer: outputs o0 (centre, label units = 64 + 16 o0),
+// o1 (log band half-width s), o2 (confidence logit z); label t (0 = no
+// label). Loss = |t - c| e^-s + s (Laplace likelihood) + BCE(z, |t - c| <=
+// 2), averaged over `norm` pixels; writes d loss / d outputs.
+__global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *__restrict__ labels,
+                                uint32_t M, uint32_t cout_p, float norm, float *__restrict__ dout,
+                                float *__restrict__ totals) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    float loss = 0.0f, valid = 0.0f;
+    if (p < M) {
+        const float *o = out + (size_t)p * cout_p;
+        float *d = dout + (size_t)p * cout_p;
+        for (uint32_t c = 0; c < cout_p; c++) d[c] = 0.0f;
+        const uint32_t t = labels[p];
+        if (t > 0) {
+            const float centre = 64.0f + 16.0f * o[0];
+            const float s = fminf(fmaxf(o[1], -3.0f), 5.0f);
+            const float z = o[2];
+            const float e = __expf(-s), diff = centre - (float)t, ad = fabsf(diff);
+            const float y = ad <= 2.0f ? 1.0f : 0.0f;
+            const float sig = 1.0f / (1.0f + __expf(-z));
+            loss = ad * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
+            d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * e * 16.0f / norm;
+            d[1] = (o[1] > -3.0f && o[1] < 5.0f) ? (1.0f - ad * e) / norm : 0.0f;
+            d[2] = (sig - y) / norm;
+            valid = 1.0f;
+        }
+    }
+    // block sums of loss and labelled pixels
+    __shared__ float sl[256], sv[256];
+    sl[threadIdx.x] = loss;
+    sv[threadIdx.x] = valid;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sl[threadIdx.x] += sl[threadIdx.x + s];
+            sv[threadIdx.x] += sv[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(totals, sl[0]);
+        atomicAdd(totals + 1, sv[0]);
+    }
+}
+
+// ------------------------------------------------------------ Adam
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, uint32_t n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// BF16 copies of the f32 master weights [k][cout_p]: wk = [k][cout_p] and
+// wt = [cout_p][k]; padded input channels and output channels stay zero.
+__global__ void k_weights_bf16(ConvLayer L, const float *__restrict__ w, uint16_t *__restrict__ wk,
+                               uint16_t *__restrict__ wt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.k * L.cout_p) return;
+    const uint32_t k = i / L.cout_p, co = i % L.cout_p;
+    const bool live = (k % L.cin_p) < L.cin && co < L.cout;
+    const uint16_t h = live ? f2bf(w[i]) : 0;
+    wk[i] = h;
+    wt[(size_t)co * L.k + k] = h;
+}
+
+// ------------------------------------------------------------ inputs
+
+// Quarter-resolution network input from the half-resolution planes of up
+// to three views (reference first): the mean of each 2x2 block, scaled by
+// 1/8 (log-luminance lies in [0, ln 1001]), as BF16 channels 0..2 of a
+// 16-channel pixel.
+__global__ void k_quarter_input(const float *__restrict__ p0, const float *__restrict__ p1,
+                                const float *__restrict__ p2, uint32_t pitch0, uint32_t pitch1,
+                                uint32_t pitch2, uint32_t plane_w, uint32_t plane_h, uint32_t w,
+                                uint32_t h, uint16_t *__restrict__ out) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const float *planes[3] = {p0, p1, p2};
+    const uint32_t pitches[3] = {pitch0, pitch1, pitch2};
+    uint16_t *o = out + ((size_t)y * w + x) * 16;
+    for (uint32_t c = 0; c < 16; c++) {
+        float v = 0.0f;
+        if (c < 3 && planes[c]) {
+            const uint32_t x0 = min(2 * x, plane_w - 1), x1 = min(2 * x + 1, plane_w - 1);
+            const uint32_t y0 = min(2 * y, plane_h - 1), y1 = min(2 * y + 1, plane_h - 1);
+            const float *p = planes[c];
+            const uint32_t pt = pitches[c];
+            v = 0.125f * 0.25f *
+                (p[(size_t)y0 * pt + x0] + p[(size_t)y0 * pt + x1] + p[(size_t)y1 * pt + x0] +
+                 p[(size_t)y1 * pt + x1]);
+        }
+        o[c] = f2bf(v);
+    }
+}
+
+// ------------------------------------------------------------ band windows
+
+// One block per tile x tile coarse nodes: the tile's mean confidence decides
+// between the band (centre +-half labels, clamped to 1 .. L-1; label 0, the
+// global warp, is always swept) and the full sweep (lo = 1, hi = L - 1).
+// Node (c, r) reads the network pixel (min(c, w - 1), min(r, h - 1)).
+__global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ out,
+                                                      uint32_t cout_p, uint32_t w, uint32_t h,
+                                                      uint32_t columns, uint32_t rows,
+                                                      uint32_t L, float threshold, uint32_t half,
+                                                      uint32_t tile, uint16_t *__restrict__ lo,
+                                                      uint16_t *__restrict__ hi,
+                                                      uint32_t *__restrict__ full_tiles) {
+    __shared__ float sum[256];
+    __shared__ float count[256];
+    const uint32_t tiles_x = (columns + tile - 1) / tile;
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    float s = 0.0f, cnt = 0.0f;
+    for (uint32_t i = threadIdx.x; i < tile * tile; i += blockDim.x) {
+        const uint32_t c = tx * tile + i % tile, r = ty * tile + i / tile;
+        if (c >= columns || r >= rows) continue;
+        const float *o = out + ((size_t)min(r, h - 1) * w + min(c, w - 1)) * cout_p;
+        s += 1.0f / (1.0f + __expf(-o[2]));
+        cnt += 1.0f;
+    }
+    sum[threadIdx.x] = s;
+    count[threadIdx.x] = cnt;
+    __syncthreads();
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            sum[threadIdx.x] += sum[threadIdx.x + k];
+            count[threadIdx.x] += count[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    const bool full = !(sum[0] >= threshold * count[0]);
+    if (threadIdx.x == 0 && full) atomicAdd(full_tiles, 1u);
+    for (uint32_t i = threadIdx.x; i < tile * tile; i += blockDim.x) {
+        const uint32_t c = tx * tile + i % tile, r = ty * tile + i / tile;
+        if (c >= columns || r >= rows) continue;
+        const size_t node = (size_t)r * columns + c;
+        if (full) {
+            lo[node] = 1;
+            hi[node] = (uint16_t)(L - 1);
+            continue;
+        }
+        const float *o = out + ((size_t)min(r, h - 1) * w + min(c, w - 1)) * cout_p;
+        const float centre = rintf(64.0f + 16.0f * o[0]);
+        const int ctr = (int)fminf(fmaxf(centre, 1.0f), (float)(L - 1));
+        lo[node] = (uint16_t)max(1, ctr - (int)half);
+        hi[node] = (uint16_t)min((int)L - 1, ctr + (int)half);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+hipError_t launch_conv_fwd(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *in,
+                           const uint16_t *wt, const float *bias, void *out, bool out_f32) {
+    const uint32_t M = batch * L.hout * L.wout;
+    const dim3 grid((M + 63) / 64);
+    if (out_f32)
+        hipLaunchKernelGGL(k_conv_fwd<true>, grid, dim3(256), 0, s, L, batch, in, wt, bias, out);
+    else
+        hipLaunchKernelGGL(k_conv_fwd<false>, grid, dim3(256), 0, s, L, batch, in, wt, bias, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_bf16(hipStream_t s, const float *dy, const uint16_t *y, size_t n, bool relu,
+                            uint16_t *out) {
+    hipLaunchKernelGGL(k_grad_bf16, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, dy, y, n,
+                       relu ? 1 : 0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_dx(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *dy,
+                          const uint16_t *wk, float *dx) {
+    const uint32_t M = batch * L.hout * L.wout;
+    const uint32_t waves = ((M + 15) / 16) * (L.k / 16);
+    hipLaunchKernelGGL(k_conv_dx, dim3((waves + 3) / 4), dim3(256), 0, s, L, batch, dy, wk, dx);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_dw(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *in,
+                          const uint16_t *dy, float *dw) {
+    const uint32_t M = batch * L.hout * L.wout, chunk = 1024;
+    const uint32_t waves = (L.k / 16) * (L.cout_p / 16) * ((M + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_conv_dw, dim3((waves + 3) / 4), dim3(256), 0, s, L, batch, in, dy, chunk,
+                       dw);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(hipStream_t s, const uint16_t *dy, uint32_t M, uint32_t cout_p,
+                            float *db) {
+    const uint32_t chunk = 256, threads = ((M + chunk - 1) / chunk) * cout_p;
+    hipLaunchKernelGGL(k_bias_grad, dim3((threads + 255) / 256), dim3(256), 0, s, dy, M, cout_p,
+                       chunk, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_proposal_loss(hipStream_t s, const float *out, const uint16_t *labels,
+                                uint32_t M, uint32_t cout_p, float norm, float *dout,
+                                float *totals) {
+    hipLaunchKernelGGL(k_proposal_loss, dim3((M + 255) / 256), dim3(256), 0, s, out, labels, M,
+                       cout_p, norm, dout, totals);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(hipStream_t s, float *w, const float *g, float *m, float *v, uint32_t n,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2) {
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2,
+                       eps, bc1, bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_weights_bf16(hipStream_t s, const ConvLayer &L, const float *w, uint16_t *wk,
+                               uint16_t *wt) {
+    const uint32_t n = L.k * L.cout_p;
+    hipLaunchKernelGGL(k_weights_bf16, dim3((n + 255) / 256), dim3(256), 0, s, L, w, wk, wt);
+    return hipGetLastError();
+}
+
+hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
+                                const uint32_t pitches[3], uint32_t plane_w, uint32_t plane_h,
+                                uint32_t w, uint32_t h, uint16_t *out) {
+    hipLaunchKernelGGL(k_quarter_input, dim3((w + 127) / 128, h), dim3(128), 0, s, planes[0],
+                       planes[1], planes[2], pitches[0], pitches[1], pitches[2], plane_w, plane_h,
+                       w, h, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
+                               uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
+                               float threshold, uint32_t half, uint32_t tile, uint16_t *lo,
+                               uint16_t *hi, uint32_t *full_tiles) {
+    const uint32_t tiles = ((columns + tile - 1) / tile) * ((rows + tile - 1) / tile);
+    hipLaunchKernelGGL(k_band_windows, dim3(tiles), dim3(256), 0, s, out, cout_p, w, h, columns,
+                       rows, L, threshold, half, tile, lo, hi, full_tiles);
+    return hipGetLastError();
+}
+
+}  // namespace chd

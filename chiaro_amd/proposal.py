@@ -1,0 +1,378 @@
+"""Learned band proposal (chd_proposal): dataset building, training and the
+band sweep it drives. No counterpart in the reference, which sweeps every
+hypothesis; the plain full sweep stays the judge of its output.
+
+Network (matching chiaro_amd/csrc/proposal.hip): seven 3x3 convolutions at
+the reference plane's half size (the "quarter resolution" of the raster):
+3->32, 32->32 stride 2, 32->32, 32->32 stride 2, 32->32, up x2 + 32->32,
+up x2 + 32->3; ReLU after all but the last; BF16 weights and activations,
+FP32 accumulation. Outputs per pixel: centre label = 64 + 16 o0, log band
+half-width o1, confidence sigmoid(o2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _abi
+from .depth import Context, DepthOptions, Session
+
+
+# ---------------------------------------------------------------- bf16
+
+def to_bf16(x: np.ndarray) -> np.ndarray:
+    """float32 -> BF16 bits (uint16), round to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return r
+
+
+def from_bf16(h: np.ndarray) -> np.ndarray:
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def quarter_input(planes) -> np.ndarray:
+    """(h, w, 16) BF16 network input from up to three half-resolution planes
+    (reference first): 2x2 means scaled by 1/8, as the device computes it."""
+    ph, pw = planes[0].shape
+    h, w = ph // 2, pw // 2
+    out = np.zeros((h, w, 16), np.float32)
+    ys0 = np.minimum(2 * np.arange(h), ph - 1)
+    ys1 = np.minimum(2 * np.arange(h) + 1, ph - 1)
+    xs0 = np.minimum(2 * np.arange(w), pw - 1)
+    xs1 = np.minimum(2 * np.arange(w) + 1, pw - 1)
+    for c, p in enumerate(planes[:3]):
+        p = np.asarray(p, np.float32)
+        s = ((p[np.ix_(ys0, xs0)] + p[np.ix_(ys0, xs1)]) + p[np.ix_(ys1, xs0)]) + p[np.ix_(ys1, xs1)]
+        out[:, :, c] = np.float32(0.125 * 0.25) * s
+    return to_bf16(out)
+
+
+# ---------------------------------------------------------------- layers
+
+def layer_specs(width: int, height: int):
+    """(cin, cout, hin, win, hout, wout, stride, upsample, relu) per layer."""
+    w2, h2 = (width + 1) // 2, (height + 1) // 2
+    w4, h4 = (w2 + 1) // 2, (h2 + 1) // 2
+    return [
+        (3, 32, height, width, height, width, 1, 0, 1),
+        (32, 32, height, width, h2, w2, 2, 0, 1),
+        (32, 32, h2, w2, h2, w2, 1, 0, 1),
+        (32, 32, h2, w2, h4, w4, 2, 0, 1),
+        (32, 32, h4, w4, h4, w4, 1, 0, 1),
+        (32, 32, h4, w4, h2, w2, 1, 1, 1),
+        (32, 3, h2, w2, height, width, 1, 1, 0),
+    ]
+
+
+def _pad16(c):
+    return (c + 15) // 16 * 16
+
+
+# ---------------------------------------------------------------- reference
+
+class ReferenceNet:
+    """FP64 numpy restatement of the network's forward and backward passes
+    (the checker of the GPU kernels). Parameters use the device layout:
+    per layer W [9 * cin_p][cout_p] (k = tap * cin_p + c), b [cout_p]. With
+    bf16=True activations are rounded to BF16 after each layer and weights
+    are taken at BF16 precision, as on the device."""
+
+    def __init__(self, width, height, params, bf16=True):
+        self.specs = layer_specs(width, height)
+        self.bf16 = bf16
+        self.W, self.b = [], []
+        at = 0
+        for (cin, cout, *_rest) in self.specs:
+            k, cp = 9 * _pad16(cin), _pad16(cout)
+            w = params[at:at + k * cp].reshape(k, cp).astype(np.float64)
+            at += k * cp
+            self.b.append(params[at:at + cp].astype(np.float64))
+            at += cp
+            if bf16:
+                w = from_bf16(to_bf16(w.astype(np.float32))).astype(np.float64)
+                mask = np.zeros_like(w)
+                for t in range(9):
+                    mask[t * _pad16(cin):t * _pad16(cin) + cin, :cout] = 1.0
+                w = w * mask
+            self.W.append(w)
+
+    def _round(self, x):
+        return from_bf16(to_bf16(x.astype(np.float32))).astype(np.float64) if self.bf16 else x
+
+    @staticmethod
+    def _source(spec, oy, ox, ky, kx):
+        cin, cout, hin, win, hout, wout, stride, up, relu = spec
+        if up:
+            uy, ux = oy + ky - 1, ox + kx - 1
+            ok = (uy >= 0) & (ux >= 0) & (uy < 2 * hin) & (ux < 2 * win)
+            return ok, np.clip(uy, 0, None) >> 1, np.clip(ux, 0, None) >> 1
+        y, x = oy * stride + ky - 1, ox * stride + kx - 1
+        ok = (y >= 0) & (x >= 0) & (y < hin) & (x < win)
+        return ok, np.clip(y, 0, hin - 1), np.clip(x, 0, win - 1)
+
+    def _im2col(self, spec, x):
+        """x: (n, hin, win, cin_p) -> (n * hout * wout, 9 * cin_p)"""
+        cin, cout, hin, win, hout, wout, stride, up, relu = spec
+        cp = x.shape[-1]
+        oy, ox = np.meshgrid(np.arange(hout), np.arange(wout), indexing="ij")
+        cols = np.zeros((x.shape[0], hout, wout, 9, cp))
+        for t in range(9):
+            ok, iy, ix = self._source(spec, oy, ox, t // 3, t % 3)
+            cols[:, :, :, t, :] = x[:, np.minimum(iy, hin - 1), np.minimum(ix, win - 1), :] * ok[None, :, :, None]
+        return cols.reshape(-1, 9 * cp)
+
+    def forward(self, x):
+        """x: (n, h, w, 16) float -> list of activations; the last is the
+        f32 output (n, h, w, 16)."""
+        acts = [x.astype(np.float64)]
+        for i, spec in enumerate(self.specs):
+            cin, cout, hin, win, hout, wout, stride, up, relu = spec
+            col = self._im2col(spec, acts[-1])
+            y = col @ self.W[i] + self.b[i][None, :]
+            y[:, cout:] = 0.0
+            if relu:
+                y = np.maximum(y, 0.0)
+            y = y.reshape(x.shape[0], hout, wout, -1)
+            acts.append(self._round(y) if i < 6 else y)
+        return acts
+
+    @staticmethod
+    def loss_grad(out, labels):
+        """Loss (sum) and d loss / d out, normalised by the pixel count, as
+        k_proposal_loss computes them (exp/log in f64 here)."""
+        o = out.reshape(-1, out.shape[-1])
+        t = labels.reshape(-1).astype(np.float64)
+        m = t > 0
+        d = np.zeros_like(o)
+        centre = 64.0 + 16.0 * o[:, 0]
+        s = np.clip(o[:, 1], -3.0, 5.0)
+        z = o[:, 2]
+        e = np.exp(-s)
+        diff = centre - t
+        ad = np.abs(diff)
+        y = (ad <= 2.0).astype(np.float64)
+        sig = 1.0 / (1.0 + np.exp(-z))
+        loss = ad * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+        n = o.shape[0]
+        d[:, 0] = np.sign(diff) * e * 16.0 / n
+        d[:, 1] = np.where((o[:, 1] > -3.0) & (o[:, 1] < 5.0), (1.0 - ad * e) / n, 0.0)
+        d[:, 2] = (sig - y) / n
+        d[~m] = 0.0
+        return float(loss[m].sum()), d.reshape(out.shape)
+
+    def backward(self, acts, dout):
+        """Parameter gradients (same order as the parameters) for
+        d loss / d out = dout."""
+        grads_w, grads_b = [None] * 7, [None] * 7
+        g = dout
+        for i in range(6, -1, -1):
+            spec = self.specs[i]
+            cin, cout, hin, win, hout, wout, stride, up, relu = spec
+            g2 = g.reshape(-1, g.shape[-1]).copy()
+            if relu:
+                g2 = g2 * (acts[i + 1].reshape(-1, g.shape[-1]) > 0)
+            if self.bf16:
+                g2 = from_bf16(to_bf16(g2.astype(np.float32))).astype(np.float64)
+            col = self._im2col(spec, acts[i])
+            grads_w[i] = col.T @ g2
+            grads_b[i] = g2.sum(axis=0)
+            if i > 0:
+                dcol = g2 @ self.W[i].T  # (m, 9 cin_p)
+                cp = acts[i].shape[-1]
+                dx = np.zeros_like(acts[i])
+                oy, ox = np.meshgrid(np.arange(hout), np.arange(wout), indexing="ij")
+                dcol = dcol.reshape(acts[i].shape[0], hout, wout, 9, cp)
+                for t in range(9):
+                    ok, iy, ix = self._source(spec, oy, ox, t // 3, t % 3)
+                    for n in range(acts[i].shape[0]):
+                        np.add.at(dx[n], (iy[ok], ix[ok]), dcol[n][ok][:, t, :])
+                dx[..., cin:] = 0.0
+                g = dx
+        out = []
+        for i in range(7):
+            out.append(grads_w[i].reshape(-1))
+            out.append(grads_b[i])
+        return np.concatenate(out)
+
+
+def adam_reference(w, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in f32 arithmetic, as k_adam computes it."""
+    f = np.float32
+    m = f(b1) * m + f(1 - b1) * g
+    v = f(b2) * v + f(1 - b2) * g * g
+    bc1 = f(1 - np.float32(b1) ** np.float32(step))
+    bc2 = f(1 - np.float32(b2) ** np.float32(step))
+    w = w - f(lr) * (m / bc1) / (np.sqrt(v / bc2) + f(eps))
+    return w.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+
+
+# ---------------------------------------------------------------- device net
+
+class Proposal:
+    """chd_proposal on one context."""
+
+    def __init__(self, context: Context, width: int, height: int, max_batch: int = 32,
+                 seed: int = 0):
+        self.context = context
+        self.lib = context.lib
+        self.width, self.height = width, height
+        handle = C.c_void_p()
+        _abi.check(self.lib.chd_proposal_create(context.handle, width, height, max_batch, seed,
+                                                C.byref(handle)), "proposal create")
+        self.handle = handle
+        n = C.c_size_t()
+        _abi.check(self.lib.chd_proposal_parameters(handle, None, None, C.byref(n)), "parameters")
+        self.parameter_count = n.value
+
+    def close(self):
+        if self.handle:
+            self.lib.chd_proposal_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def parameters(self) -> np.ndarray:
+        out = np.empty(self.parameter_count, np.float32)
+        n = C.c_size_t(self.parameter_count)
+        _abi.check(self.lib.chd_proposal_parameters(self.handle, out.ctypes.data, None, C.byref(n)),
+                   "parameters")
+        return out
+
+    def set_parameters(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, np.float32)
+        n = C.c_size_t(self.parameter_count)
+        _abi.check(self.lib.chd_proposal_parameters(self.handle, None, v.ctypes.data, C.byref(n)),
+                   "set parameters")
+
+    def set_dataset(self, inputs: np.ndarray, labels: np.ndarray):
+        x = np.ascontiguousarray(inputs, np.uint16)
+        y = np.ascontiguousarray(labels, np.uint16)
+        _abi.check(self.lib.chd_proposal_set_dataset(self.handle, x.ctypes.data, y.ctypes.data,
+                                                     x.shape[0]), "set dataset")
+
+    def train(self, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0) -> list:
+        losses = np.zeros(epochs, np.float32)
+        _abi.check(self.lib.chd_proposal_train(self.handle, epochs, batch, lr, seed,
+                                               losses.ctypes.data), "train")
+        return [float(v) for v in losses]
+
+    def predict(self, inputs: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, np.uint16)
+        out = np.empty((x.shape[0], self.height, self.width, 3), np.float32)
+        _abi.check(self.lib.chd_proposal_predict(self.handle, x.ctypes.data, x.shape[0],
+                                                 out.ctypes.data), "predict")
+        return out
+
+    def gradients(self, inputs: np.ndarray, labels: np.ndarray):
+        x = np.ascontiguousarray(inputs, np.uint16)
+        y = np.ascontiguousarray(labels, np.uint16)
+        g = np.empty(self.parameter_count, np.float32)
+        loss = C.c_float()
+        _abi.check(self.lib.chd_proposal_gradients(self.handle, x.ctypes.data, y.ctypes.data,
+                                                   x.shape[0], g.ctypes.data, C.byref(loss)),
+                   "gradients")
+        return g, loss.value
+
+    def adam_step(self, gradients: np.ndarray, lr: float):
+        g = np.ascontiguousarray(gradients, np.float32)
+        _abi.check(self.lib.chd_proposal_adam_step(self.handle, g.ctypes.data, lr), "adam step")
+
+    def apply(self, session: Session, threshold: float = 0.5, half_width: int = 8,
+              tile: int = 8, report: bool = True):
+        """Band windows for `session` from its planes; returns (fallback
+        tiles, tiles) when report, else None (no synchronisation)."""
+        full, tiles = C.c_uint32(), C.c_uint32()
+        _abi.check(self.lib.chd_session_set_proposal(
+            session.handle, self.handle, threshold, half_width, tile,
+            C.byref(full) if report else None, C.byref(tiles)), "set proposal")
+        return (full.value, tiles.value) if report else None
+
+
+def clear(session: Session):
+    _abi.check(session.lib.chd_session_clear_proposal(session.handle), "clear proposal")
+
+
+# ---------------------------------------------------------------- data
+
+def capture_inputs(cap) -> np.ndarray:
+    """Network input of a capture: reference and first two targets."""
+    planes = [cap.modules[cap.reference_index].luminance]
+    planes += [m.luminance for i, m in enumerate(cap.modules) if i != cap.reference_index][:2]
+    return quarter_input(planes)
+
+
+def full_sweep_labels(ctx: Context, cap, width: int, height: int):
+    """The plain full sweep's WTA label per network pixel (0 where the node
+    was not tested), and the session's grid."""
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    try:
+        sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+        field, fillable, best = sess.coarse()
+        _, _, _, tested = sess.cost_volume()
+        g = sess.grid
+        labels = np.where(tested > 0, best, 0).astype(np.uint16).reshape(g.coarse_rows, g.coarse_columns)
+        return np.ascontiguousarray(labels[:height, :width])
+    finally:
+        sess.close()
+
+
+def build_dataset(ctx: Context, seeds, make_capture):
+    """(inputs, labels) for the given seeds; make_capture(seed) -> Capture."""
+    xs, ys = [], []
+    for s in seeds:
+        cap = make_capture(s)
+        x = capture_inputs(cap)
+        h, w = x.shape[:2]
+        xs.append(x)
+        ys.append(full_sweep_labels(ctx, cap, w, h))
+    return np.stack(xs), np.stack(ys)
+
+
+def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_width: int = 8,
+              tile: int = 8, steps: int = 3):
+    """Full sweep vs band sweep on one capture: the fraction of nodes tested
+    by the full sweep whose band label is within +-1, the agreement of the
+    tested masks, the fallback-tile fraction, and both times (cost + SGM,
+    the band time including the network and the window proposal)."""
+    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    try:
+        sess.run(stages)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sess.run(stages)
+        ctx.synchronize()
+        full_ms = (time.perf_counter() - t0) * 1000 / steps
+        _, _, full_best = sess.coarse()
+        full_tested = sess.cost_volume()[3]
+        full_tiles, tiles = proposal.apply(sess, threshold, half_width, tile)
+        sess.run(stages)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            proposal.apply(sess, threshold, half_width, tile, report=False)
+            sess.run(stages)
+        ctx.synchronize()
+        band_ms = (time.perf_counter() - t0) * 1000 / steps
+        _, _, band_best = sess.coarse()
+        band_tested = sess.cost_volume()[3]
+        clear(sess)
+    finally:
+        sess.close()
+    valid = full_tested > 0
+    close = np.abs(band_best.astype(np.int32) - full_best.astype(np.int32)) <= 1
+    return {
+        "agreement": float(close[valid].mean()) if valid.any() else 1.0,
+        "mask_agreement": float((band_tested == full_tested).mean()),
+        "fallback_fraction": full_tiles / max(tiles, 1),
+        "full_ms": full_ms,
+        "band_ms": band_ms,
+    }

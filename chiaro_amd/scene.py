@@ -241,3 +241,14 @@ def config2(seed: int = 0, scale: float = 1.0, planes: int = 128,
     cap.raws = raws
     return cap
 
+
+
+def config3(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float = 1600.0) -> Capture:
+    """The learned-proposal workload: config0's three-view scene at 1040x780
+    swept with 128 hypotheses on a grid of step 4, so that the coarse grid
+    (261 x 196 nodes) matches the network's 260 x 195 pixels (the reference
+    plane at half size)."""
+    cap = config0(seed=seed, width=width, height=height, focal_px=focal_px, planes=128)
+    cap.options.grid_step = 4
+    cap.description["workload"] = "config3"
+    return cap

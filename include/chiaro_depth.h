@@ -344,6 +344,54 @@ int chd_batch_create(chd_context *context, chd_session *const *sessions, uint32_
 int chd_batch_run(chd_batch *batch, uint32_t stages);
 void chd_batch_destroy(chd_batch *batch);
 
+/* Learned band proposal (no counterpart in the reference, which sweeps
+ * every hypothesis): a 7-layer 3x3 convolutional encoder-decoder with BF16
+ * weights and activations and FP32 accumulation on the matrix cores. Its
+ * input is width x height x 16 BF16 (channels 0..2: the reference and the
+ * first two target planes at half the plane resolution, scaled by 1/8; the
+ * rest zero); per pixel it predicts a centre label, a log band half-width and
+ * a confidence. Trained with Adam on full-sweep WTA label maps (0 = none). */
+typedef struct chd_proposal chd_proposal;
+int chd_proposal_create(chd_context *context, uint32_t width, uint32_t height,
+                        uint32_t max_batch, uint64_t seed, chd_proposal **out);
+void chd_proposal_destroy(chd_proposal *proposal);
+/* All parameters as f32 (per layer: weights [9 * cin_p][cout_p], then
+ * biases [cout_p]): with get and set NULL, *count receives the count; else
+ * *count must equal it and get receives or set replaces them. */
+int chd_proposal_parameters(chd_proposal *proposal, float *get, const float *set,
+                            size_t *count);
+/* Training set (host arrays): inputs count x height x width x 16 BF16 bits,
+ * labels count x height x width. */
+int chd_proposal_set_dataset(chd_proposal *proposal, const uint16_t *inputs,
+                             const uint16_t *labels, uint32_t count);
+/* Epochs of shuffled mini-batches with Adam (beta 0.9, 0.999; eps 1e-8);
+ * epoch_loss (optional, `epochs` entries) receives the mean loss per
+ * labelled pixel of each epoch. */
+int chd_proposal_train(chd_proposal *proposal, uint32_t epochs, uint32_t batch,
+                       float learning_rate, uint64_t shuffle_seed, float *epoch_loss);
+/* Per pixel of `count` inputs: centre label, log half-width, confidence. */
+int chd_proposal_predict(chd_proposal *proposal, const uint16_t *inputs, uint32_t count,
+                         float *outputs);
+/* Parameter gradients of the summed loss (divided by the batch's pixel
+ * count) for one batch, in chd_proposal_parameters order; *loss receives the
+ * summed loss. For checks of the backward pass. */
+int chd_proposal_gradients(chd_proposal *proposal, const uint16_t *inputs,
+                           const uint16_t *labels, uint32_t count, float *gradients,
+                           float *loss);
+/* One Adam update from the given gradients (for checks of the optimiser). */
+int chd_proposal_adam_step(chd_proposal *proposal, const float *gradients,
+                           float learning_rate);
+/* Runs the network on the session's own planes (on the device) and makes
+ * its next cost volumes sweep label 0 and centre +- half_width only; tiles
+ * of tile x tile coarse nodes whose mean confidence is below `threshold`
+ * keep the full sweep. The network grid must be the reference plane at half
+ * size. full_tiles / tiles (optional) receive the fallback count and the
+ * tile count (full_tiles synchronises). clear restores the full sweep. */
+int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float threshold,
+                             uint32_t half_width, uint32_t tile, uint32_t *full_tiles,
+                             uint32_t *tiles);
+int chd_session_clear_proposal(chd_session *session);
+
 /* Stage entry points on host arrays, for parity checks of one stage. */
 
 /* semi_global_costs (depth.rs:1279-1311). */

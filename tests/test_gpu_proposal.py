@@ -1,0 +1,153 @@
+"""The learned band proposal on the GPU against its FP64 restatement
+(forward, gradients, Adam), and the band sweep it drives: with every tile
+falling back it equals the plain full sweep bit for bit; with bands it runs
+and is scored against the full sweep."""
+import sys
+
+import numpy as np
+import pytest
+
+from chiaro_amd import _abi, scene
+from chiaro_amd.depth import Session
+from chiaro_amd.proposal import (Proposal, ReferenceNet, adam_reference, agreement,
+                                 build_dataset, clear, from_bf16, layer_specs, to_bf16)
+
+from parity import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _params(width, height, rng, scale=0.2):
+    out = []
+    for (cin, cout, *_r) in layer_specs(width, height):
+        cp_in, cp_out = (cin + 15) // 16 * 16, (cout + 15) // 16 * 16
+        w = np.zeros((9 * cp_in, cp_out), np.float32)
+        for t in range(9):
+            w[t * cp_in:t * cp_in + cin, :cout] = rng.normal(size=(cin, cout)) * scale
+        b = np.zeros(cp_out, np.float32)
+        b[:cout] = rng.normal(size=cout) * 0.1
+        out += [w.reshape(-1), b]
+    return np.concatenate(out).astype(np.float32)
+
+
+def _inputs(rng, n, h, w):
+    x = np.zeros((n, h, w, 16), np.float32)
+    x[..., :3] = rng.uniform(0, 1, size=(n, h, w, 3))
+    return to_bf16(x)
+
+
+def test_forward_matches_fp64_restatement(gpu_context):
+    rng = np.random.default_rng(0)
+    w, h = 33, 25
+    net = Proposal(gpu_context, w, h, max_batch=2, seed=1)
+    try:
+        params = _params(w, h, rng)
+        net.set_parameters(params)
+        assert np.array_equal(net.parameters(), params)
+        x = _inputs(rng, 2, h, w)
+        got = net.predict(x)
+        ref = ReferenceNet(w, h, params, bf16=True).forward(from_bf16(x).astype(np.float64))[-1]
+        centre = 64.0 + 16.0 * ref[..., 0]
+        scale = np.abs(ref[..., :3]).max()
+        _say(f"forward: max |o| {scale:.3f}")
+        assert np.abs((got[..., 0] - 64.0) / 16.0 - ref[..., 0]).max() <= 0.02 * scale + 1e-3
+        assert np.abs(got[..., 1] - ref[..., 1]).max() <= 0.02 * scale + 1e-3
+        assert np.abs(got[..., 2] - 1 / (1 + np.exp(-ref[..., 2]))).max() <= 0.01 + 0.005 * scale
+        assert np.abs(got[..., 0] - centre).max() <= 16 * (0.02 * scale + 1e-3)
+    finally:
+        net.close()
+
+
+def test_gradients_match_fp64_restatement(gpu_context):
+    rng = np.random.default_rng(2)
+    w, h = 17, 13
+    net = Proposal(gpu_context, w, h, max_batch=2, seed=3)
+    try:
+        params = _params(w, h, rng)
+        net.set_parameters(params)
+        x = _inputs(rng, 2, h, w)
+        labels = rng.integers(1, 129, size=(2, h, w)).astype(np.uint16)
+        labels[1, :2] = 0
+        g, loss = net.gradients(x, labels)
+        ref = ReferenceNet(w, h, params, bf16=True)
+        acts = ref.forward(from_bf16(x).astype(np.float64))
+        want_loss, dout = ref.loss_grad(acts[-1], labels)
+        want = ref.backward(acts, dout)
+        cos = float(np.dot(g, want) / (np.linalg.norm(g) * np.linalg.norm(want)))
+        rel = float(np.linalg.norm(g - want) / np.linalg.norm(want))
+        _say(f"gradients: cosine {cos:.5f}, relative error {rel:.4f}, loss {loss:.3f} vs {want_loss:.3f}")
+        assert abs(loss - want_loss) <= 1e-2 * abs(want_loss)
+        assert cos > 0.995 and rel < 0.06
+    finally:
+        net.close()
+
+
+def test_adam_step_matches_reference(gpu_context):
+    rng = np.random.default_rng(4)
+    w, h = 16, 12
+    net = Proposal(gpu_context, w, h, max_batch=1, seed=5)
+    try:
+        p0 = net.parameters()
+        m = np.zeros_like(p0)
+        v = np.zeros_like(p0)
+        p = p0
+        for step in (1, 2, 3):
+            g = rng.normal(size=p0.shape).astype(np.float32)
+            net.adam_step(g, 1e-3)
+            p, m, v = adam_reference(p, g, m, v, step, 1e-3)
+            got = net.parameters()
+            assert np.allclose(got, p, rtol=2e-6, atol=1e-7), f"step {step}"
+    finally:
+        net.close()
+
+
+def _cap(seed):
+    return scene.config3(seed=seed, width=520, height=392, focal_px=800.0)
+
+
+def test_training_and_band_sweep(gpu_context):
+    """Train on a few small config 3 scenes: the loss falls; every tile in
+    fallback gives the plain full sweep bit for bit; bands run and agree
+    with the full sweep on most nodes."""
+    _say("building 12 small config3 scenes")
+    x, y = build_dataset(gpu_context, range(12), _cap)
+    h, w = x.shape[1:3]
+    net = Proposal(gpu_context, w, h, max_batch=4, seed=7)
+    try:
+        net.set_dataset(x, y)
+        losses = net.train(epochs=8, batch=4, lr=1e-3, seed=1)
+        _say(f"losses {['%.3f' % l for l in losses]}")
+        assert np.isfinite(losses).all() and losses[-1] < losses[0]
+        cap = _cap(100)
+        full = Session(gpu_context, cap.modules, 0, cap.options)
+        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
+        want_scores, want_costs, _, want_tested = full.cost_volume()
+        want_best = full.coarse()[2]
+        # every tile in fallback: the plain sweep
+        fb, tiles = net.apply(full, threshold=2.0)
+        assert fb == tiles
+        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+        s, c, _, t = full.cost_volume()
+        assert_bitwise("fallback scores", s, want_scores)
+        assert_bitwise("fallback costs", c, want_costs)
+        assert_bitwise("fallback tested", t, want_tested)
+        assert_bitwise("fallback labels", full.coarse()[2], want_best)
+        # bands everywhere: windows of +-8 around the centre
+        fb, tiles = net.apply(full, threshold=0.0)
+        assert fb == 0
+        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+        costs = full.cost_volume()[1]
+        swept = np.isfinite(costs).sum(axis=1)
+        assert swept.max() <= 18 and swept.min() >= 1
+        clear(full)
+        full.close()
+        stats = agreement(gpu_context, _cap(101), net, threshold=0.5)
+        _say(f"band vs full: {stats}")
+        assert 0.0 <= stats["agreement"] <= 1.0 and stats["band_ms"] > 0
+    finally:
+        net.close()
