@@ -691,7 +691,13 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     J.T = (2 * J.S.radius + 1) * (2 * J.S.radius + 1);
     J.slab = label_slab == 0 || label_slab > J.L || J.window_lo ? J.L : label_slab;
     uint32_t nb = nodes_per_block;
-    if (nb == 0) {
+    if (nb == 0 && J.window_lo) {
+        // band windows hold ~18 labels: more nodes per item keep the threads
+        // busy (the LDS still covers full-sweep nodes in fallback tiles)
+        nb = 16;
+        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab) + (2 * nb + 1) * 4 + nb * J.L > 64 * 1024)
+            nb--;
+    } else if (nb == 0) {
         nb = 1024 / J.slab;
         if (nb < 1) nb = 1;
         if (nb > 16) nb = 16;

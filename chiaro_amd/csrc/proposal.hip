@@ -208,8 +208,11 @@ __global__ void k_bias_grad(const uint16_t *__restrict__ dy, uint32_t M, uint32_
 
 // Per pixel of the last layer: outputs o0 (centre, label units = 64 + 16 o0),
 // o1 (log band half-width s), o2 (confidence logit z); label t (0 = no
-// label). Loss = |t - c| e^-s + s (Laplace likelihood) + BCE(z, |t - c| <=
-// 2), averaged over `norm` pixels; writes d loss / d outputs.
+// label). Loss = |t - c| / 8 (the centre) + a e^-s + s with a = |t - c|
+// held constant (the Laplace scale of the centre's error, which does not
+// pull on the centre) + BCE(z, |t - c| <= 6): the confidence learns whether
+// a band of +-8 labels around the centre covers the full sweep's label with
+// a margin of 2. Divided by `norm` pixels; writes d loss / d outputs.
 __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *__restrict__ labels,
                                 uint32_t M, uint32_t cout_p, float norm, float *__restrict__ dout,
                                 float *__restrict__ totals) {
@@ -225,10 +228,10 @@ __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *_
             const float s = fminf(fmaxf(o[1], -3.0f), 5.0f);
             const float z = o[2];
             const float e = __expf(-s), diff = centre - (float)t, ad = fabsf(diff);
-            const float y = ad <= 2.0f ? 1.0f : 0.0f;
+            const float y = ad <= 6.0f ? 1.0f : 0.0f;
             const float sig = 1.0f / (1.0f + __expf(-z));
-            loss = ad * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
-            d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * e * 16.0f / norm;
+            loss = ad * 0.125f + ad * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
+            d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * 2.0f / norm;
             d[1] = (o[1] > -3.0f && o[1] < 5.0f) ? (1.0f - ad * e) / norm : 0.0f;
             d[2] = (sig - y) / norm;
             valid = 1.0f;

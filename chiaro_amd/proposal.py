@@ -153,11 +153,11 @@ class ReferenceNet:
         e = np.exp(-s)
         diff = centre - t
         ad = np.abs(diff)
-        y = (ad <= 2.0).astype(np.float64)
+        y = (ad <= 6.0).astype(np.float64)
         sig = 1.0 / (1.0 + np.exp(-z))
-        loss = ad * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+        loss = ad * 0.125 + ad * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
         n = o.shape[0]
-        d[:, 0] = np.sign(diff) * e * 16.0 / n
+        d[:, 0] = np.sign(diff) * 2.0 / n
         d[:, 1] = np.where((o[:, 1] > -3.0) & (o[:, 1] < 5.0), (1.0 - ad * e) / n, 0.0)
         d[:, 2] = (sig - y) / n
         d[~m] = 0.0
@@ -369,10 +369,30 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
         sess.close()
     valid = full_tested > 0
     close = np.abs(band_best.astype(np.int32) - full_best.astype(np.int32)) <= 1
-    return {
+    stats = {
         "agreement": float(close[valid].mean()) if valid.any() else 1.0,
         "mask_agreement": float((band_tested == full_tested).mean()),
         "fallback_fraction": full_tiles / max(tiles, 1),
         "full_ms": full_ms,
         "band_ms": band_ms,
     }
+    # where the band disagrees: is the full sweep's label inside the node's
+    # window (then the difference comes from SGM paths crossing windows) or
+    # outside it (the centre missed)?
+    g = sess.grid
+    pred = proposal.predict(capture_inputs(cap)[None])[0]
+    rows, cols = g.coarse_rows, g.coarse_columns
+    r = np.minimum(np.arange(rows), pred.shape[0] - 1)
+    c = np.minimum(np.arange(cols), pred.shape[1] - 1)
+    centre = np.clip(np.rint(pred[np.ix_(r, c)][..., 0]), 1, g.labels - 1).reshape(-1)
+    conf = pred[np.ix_(r, c)][..., 2]
+    tr, tc = np.arange(rows) // tile, np.arange(cols) // tile
+    tile_id = (tr[:, None] * ((cols + tile - 1) // tile) + tc[None, :]).reshape(-1)
+    means = np.bincount(tile_id, conf.reshape(-1)) / np.bincount(tile_id)
+    banded = means[tile_id] >= threshold
+    inside = np.abs(full_best.astype(np.int64) - centre) <= half_width
+    wrong = valid & ~close & banded
+    stats["banded_nodes"] = float((valid & banded).mean())
+    stats["window_covers_label"] = float(inside[valid & banded].mean()) if (valid & banded).any() else 1.0
+    stats["disagree_inside_window"] = float((wrong & inside).sum() / max(wrong.sum(), 1))
+    return stats

@@ -21,10 +21,21 @@ def _params(width, height, rng, scale=0.2):
     return np.concatenate(out)
 
 
-def _loss(net, x, labels):
+def _loss(net, x, labels, frozen=None):
+    """The training loss per pixel; the scale term's |t - c| is the value
+    `frozen` (the unperturbed error), as the gradient treats it."""
     acts = net.forward(x)
-    total, _ = net.loss_grad(acts[-1], labels)
-    return total / labels.size
+    out = acts[-1].reshape(-1, acts[-1].shape[-1])
+    t = labels.reshape(-1).astype(np.float64)
+    m = t > 0
+    c = 64.0 + 16.0 * out[:, 0]
+    s = np.clip(out[:, 1], -3.0, 5.0)
+    z = out[:, 2]
+    ad = np.abs(c - t)
+    a = ad if frozen is None else frozen
+    y = (a <= 6.0).astype(np.float64)
+    loss = ad * 0.125 + a * np.exp(-s) + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+    return float(loss[m].sum()) / labels.size, ad
 
 
 def test_backward_matches_finite_differences():
@@ -39,14 +50,17 @@ def test_backward_matches_finite_differences():
     acts = net.forward(x)
     _, dout = net.loss_grad(acts[-1], labels)
     grads = net.backward(acts, dout)
+    base, frozen = _loss(net, x, labels)
+    total, _ = net.loss_grad(acts[-1], labels)
+    assert abs(total / labels.size - base) <= 1e-9 * abs(base)
     live = np.flatnonzero(np.abs(params) > 0)
     for i in rng.choice(live, size=40, replace=False):
         eps = 1e-5
         p1, p2 = params.copy(), params.copy()
         p1[i] += eps
         p2[i] -= eps
-        fd = (_loss(ReferenceNet(w, h, p1, bf16=False), x, labels) -
-              _loss(ReferenceNet(w, h, p2, bf16=False), x, labels)) / (2 * eps)
+        fd = (_loss(ReferenceNet(w, h, p1, bf16=False), x, labels, frozen)[0] -
+              _loss(ReferenceNet(w, h, p2, bf16=False), x, labels, frozen)[0]) / (2 * eps)
         assert abs(fd - grads[i]) <= 1e-5 + 1e-3 * abs(fd), (i, fd, grads[i])
 
 
