@@ -124,14 +124,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["config1", "config0", "config2", "config3"],
+    ap.add_argument("--workload", choices=["config1", "config0", "config2", "config3", "config4"],
                     default="config1")
     ap.add_argument("--train-scenes", type=int, default=512,
                     help="config3: training scenes for the learned band proposal")
     ap.add_argument("--eval-scenes", type=int, default=24, help="config3: held-out scenes")
     ap.add_argument("--epochs", type=int, default=10, help="config3: training epochs")
-    ap.add_argument("--captures", type=int, default=8,
-                    help="config2: captures per batch (one persistent launch, hipGraph replay)")
+    ap.add_argument("--captures", type=int, default=0,
+                    help="config2/config4: captures per batch (default 8 and 4)")
     ap.add_argument("--tuning", choices=["default", "auto"], default="default",
                     help="launch shapes: the defaults, or chd_autotune on this capture")
     ap.add_argument("--seed", type=int, default=0)
@@ -161,7 +161,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    if args.workload == "config2":
+    if args.workload in ("config2", "config4"):
         return bench_batch(args)
     if args.workload == "config3":
         return bench_learned(args)
@@ -309,16 +309,22 @@ def main() -> int:
 
 
 def bench_batch(args) -> int:
-    """Config 2: `captures` captures of ten modules of three sizes run as one
-    batch (one SGM sweep launch over all captures, hipGraph replay); the
-    same batch with the persistent cost-volume queue and the sessions run one
-    after another are timed beside it."""
+    """Config 2 (`captures` captures of ten modules of three sizes) or
+    config 4 (captures of sixteen 4160x3120 wide, mid and tele modules at 256
+    hypotheses) run as one batch (one SGM sweep launch over all captures,
+    hipGraph replay); the same batch with the persistent cost-volume queue
+    and the sessions run one after another are timed beside it."""
     from chiaro_amd import _abi, scene
     from chiaro_amd.depth import Batch, Context, Session
 
+    count = args.captures or (8 if args.workload == "config2" else 4)
+    make = scene.config2 if args.workload == "config2" else scene.config4
     t = time.perf_counter()
-    caps = [scene.config2(seed=args.seed + k) for k in range(args.captures)]
-    log(f"[bench] {args.captures} config2 captures generated in {time.perf_counter() - t:.1f} s")
+    caps = []
+    for k in range(count):
+        caps.append(make(seed=args.seed + k))
+        caps[-1].raws = None          # only the luminance planes are used here
+    log(f"[bench] {count} {args.workload} captures generated in {time.perf_counter() - t:.1f} s")
     ctx = Context(0)
     sessions = [Session(ctx, c.modules, c.reference_index, c.options) for c in caps]
     tuning = sessions[0].autotune() if args.tuning == "auto" else sessions[0].tuning()
@@ -355,6 +361,14 @@ def bench_batch(args) -> int:
     ref = caps[0].modules[0]
     planes = caps[0].options.planes
     pixels = ref.width * ref.height * len(caps)
+    if args.workload == "config2":
+        workload = (f"config2: {len(caps)} captures x 10 modules (5 x 4160x3120 f=3300, "
+                    f"3 x 2080x1560 f=4125, 2 x 1040x780 f=4425), {planes} hypotheses, "
+                    f"8-path SGM, seeds {args.seed}..{args.seed + len(caps) - 1}, batched")
+    else:
+        workload = (f"config4: {len(caps)} captures x 16 modules 4160x3120 (5 wide f=3300, "
+                    f"5 mid f=8250, 6 tele f=17700), 24-plane scenes, {planes} hypotheses, "
+                    f"8-path SGM, seeds {args.seed}..{args.seed + len(caps) - 1}, batched")
     value = pixels * planes / (ms / 1000.0) / 1e6
     g = sessions[0].grid
     achieved = zncc * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
@@ -372,9 +386,7 @@ def bench_batch(args) -> int:
         "dtype": "f32 (f64 coordinates)",
         "data": "synthetic",
         "config": {
-            "workload": (f"config2: {len(caps)} captures x 10 modules (5 x 4160x3120 f=3300, "
-                         f"3 x 2080x1560 f=4125, 2 x 1040x780 f=4425), {planes} hypotheses, "
-                         f"8-path SGM, seeds {args.seed}..{args.seed + len(caps) - 1}, batched"),
+            "workload": workload,
             "coarse_grid": [g.coarse_columns, g.coarse_rows, g.coarse_step],
             "labels": g.labels,
             "parallelism": "1 GPU",
@@ -410,14 +422,15 @@ def bench_learned(args) -> int:
     full sweep's label maps of `train_scenes` seeded 1040x780 three-view
     scenes (128 hypotheses, grid step 4), then on `eval_scenes` held-out
     seeds times the full sweep (cost + SGM) against the band sweep (network +
-    windows + cost + SGM) at several confidence thresholds, and reports the
-    fastest threshold whose +-1-hypothesis agreement with the full sweep is
-    at least 98 %."""
+    windows + cost + SGM) at several confidence thresholds, tile sizes and
+    band-width caps (bands of +-8 labels, or widened to +-ceil(4 b) for the
+    predicted scale b up to the cap), and reports the fastest setting whose
+    +-1-hypothesis agreement with the full sweep is at least 98 %."""
     import numpy as np
 
     from chiaro_amd import scene
     from chiaro_amd.depth import Context
-    from chiaro_amd.proposal import Proposal, agreement, build_dataset
+    from chiaro_amd.proposal import Proposal, agreement, build_dataset, constant_proposal
 
     ctx = Context(0)
     t = time.perf_counter()
@@ -432,15 +445,30 @@ def bench_learned(args) -> int:
     log(f"[bench] trained {args.epochs} epochs in {train_s:.1f} s, losses {losses}")
     evals = [scene.config3(seed=100000 + k) for k in range(args.eval_scenes)]
     rows = []
-    for tile in (4, 8):
-        for thr in (0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9):
-            stats = [agreement(ctx, cap, net, thr, tile=tile) for cap in evals]
-            row = {k: float(np.mean([st[k] for st in stats])) for k in stats[0]}
-            row["threshold"] = thr
-            row["tile"] = tile
-            row["speedup"] = row["full_ms"] / row["band_ms"]
-            rows.append(row)
-            log(f"[bench] tile {tile} threshold {thr}: {row}")
+    for max_half in (8, 16, 24, 32):
+        for tile in (4, 8):
+            for thr in (0.0, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9):
+                stats = [agreement(ctx, cap, net, thr, tile=tile, max_half_width=max_half)
+                         for cap in evals]
+                row = {k: float(np.mean([st[k] for st in stats])) for k in stats[0]}
+                row["threshold"] = thr
+                row["tile"] = tile
+                row["max_half_width"] = max_half
+                row["speedup"] = row["full_ms"] / row["band_ms"]
+                rows.append(row)
+                log(f"[bench] max half {max_half} tile {tile} threshold {thr}: "
+                    f"agreement {row['agreement']:.4f} fallback {row['fallback_fraction']:.3f} "
+                    f"speedup {row['speedup']:.3f} covers {row['window_covers_label']:.3f}")
+    # ablation at an equal budget (+-8 labels everywhere, no fallback): the
+    # network's centres against one centre, the training set's median label
+    const = constant_proposal(ctx, w, h, float(np.median(y[y > 0])))
+    ablation = {}
+    for name, prop in (("network", net), ("constant_centre", const)):
+        stats = [agreement(ctx, cap, prop, 0.0, tile=8) for cap in evals]
+        ablation[name] = {k: round(float(np.mean([st[k] for st in stats])), 5)
+                          for k in ("agreement", "window_covers_label", "band_ms")}
+    const.close()
+    log(f"[bench] ablation {ablation}")
     ok = [r for r in rows if r["agreement"] >= 0.98]
     best = max(ok, key=lambda r: r["speedup"]) if ok else None
     ref = evals[0].modules[0]
@@ -472,12 +500,14 @@ def bench_learned(args) -> int:
             "met_98_percent": best is not None,
             "threshold": chosen["threshold"],
             "tile": chosen["tile"],
+            "max_half_width": chosen["max_half_width"],
             "speedup": round(chosen["speedup"], 3),
             "agreement": round(chosen["agreement"], 5),
             "mask_agreement": round(chosen["mask_agreement"], 5),
             "fallback_fraction": round(chosen["fallback_fraction"], 4),
             "thresholds": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
                            for r in rows],
+            "ablation_equal_budget": ablation,
             "train_seconds": round(train_s, 2),
             "losses": [round(l, 4) for l in losses],
         },

@@ -300,7 +300,7 @@ DEPTH_SIGNATURES = {
     "chd_proposal_gradients": (C.c_int, VP, VP, VP, C.c_uint32, VP, P(C.c_float)),
     "chd_proposal_adam_step": (C.c_int, VP, VP, C.c_float),
     "chd_session_set_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
-                                 P(C.c_uint32), P(C.c_uint32)),
+                                 C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
     "chd_session_clear_proposal": (C.c_int, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),
     "chd_select_depths": (C.c_int, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32, P(Options), VP, VP,

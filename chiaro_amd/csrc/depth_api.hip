@@ -111,6 +111,7 @@ struct chd_session {
     std::vector<uint32_t> active;  // accepted, calibrated, non-reference views
     std::vector<std::pair<uint32_t, uint32_t>> plane_sizes;  // per view
     ViewDev *views = nullptr;
+    std::vector<ViewDev> host_views;  // what `views` holds
     uint32_t *vlist = nullptr;     // device copy of `active`
     double *labels = nullptr;
     float *scores = nullptr, *costs = nullptr, *guidance = nullptr;
@@ -660,6 +661,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
         }
         if (any) CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
     }
+    s->host_views = host_views;
     std::vector<double> labels = label_depths(*options);
     CHD_ALLOC(s->labels, labels.size());
     CHD_UPLOAD(s->labels, labels.data(), labels.size() * sizeof(double));
@@ -1956,6 +1958,9 @@ int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float l
     hipStream_t st = p->context->stream;
     const size_t pixels = (size_t)p->width * p->height;
     std::vector<uint32_t> order(p->data_count);
+    // cosine decay from learning_rate to 5% of it over the whole run
+    const double total = (double)epochs * ((p->data_count + batch - 1) / batch);
+    double step = 0.0;
     for (uint32_t e = 0; e < epochs; e++) {
         for (uint32_t k = 0; k < p->data_count; k++) order[k] = k;
         std::mt19937_64 rng(shuffle_seed + e);
@@ -1972,7 +1977,9 @@ int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float l
             }
             CHD_CHECK(proposal_forward(p, n));
             CHD_CHECK(proposal_backward(p, n));
-            CHD_CHECK(proposal_adam(p, learning_rate));
+            const double decay = 0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * step / total));
+            step += 1.0;
+            CHD_CHECK(proposal_adam(p, (float)(learning_rate * decay)));
         }
         float totals[2] = {0.0f, 0.0f};
         CHD_HIP(hipMemcpyAsync(totals, p->totals, 8, hipMemcpyDeviceToHost, st));
@@ -2050,8 +2057,8 @@ int chd_proposal_adam_step(chd_proposal *p, const float *gradients, float learni
 }
 
 int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
-                             uint32_t half_width, uint32_t tile, uint32_t *full_tiles,
-                             uint32_t *tiles) {
+                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
+                             uint32_t *full_tiles, uint32_t *tiles) {
     if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
         return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
     if (s->none) return CHD_NONE;
@@ -2068,9 +2075,7 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
         CHD_CHECK(dalloc(s, &s->band_hi, nodes));
     }
     // input: the reference and the first two targets, halved again
-    std::vector<ViewDev> views(s->count);
-    CHD_HIP(hipMemcpyAsync(views.data(), s->views, s->count * sizeof(ViewDev), hipMemcpyDeviceToHost, st));
-    CHD_HIP(hipStreamSynchronize(st));
+    const std::vector<ViewDev> &views = s->host_views;
     const float *planes[3] = {views[s->reference].plane, nullptr, nullptr};
     uint32_t pitches[3] = {views[s->reference].plane_pitch, 0, 0};
     for (size_t k = 0; k < 2 && k < s->active.size(); k++) {
@@ -2086,8 +2091,8 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
     CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
     CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,
                                 s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
-                                threshold, half_width, tile, s->band_lo, s->band_hi,
-                                p->full_tiles));
+                                threshold, half_width, std::max(half_width, max_half_width),
+                                tile, s->band_lo, s->band_hi, p->full_tiles));
     if (!s->band) {
         s->band = true;
         s->version++;

@@ -228,11 +228,14 @@ __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *_
             const float s = fminf(fmaxf(o[1], -3.0f), 5.0f);
             const float z = o[2];
             const float e = __expf(-s), diff = centre - (float)t, ad = fabsf(diff);
+            // the scale term sees the error capped at 32 labels, so one far
+            // miss cannot swamp the width gradient
+            const float al = fminf(ad, 32.0f);
             const float y = ad <= 6.0f ? 1.0f : 0.0f;
             const float sig = 1.0f / (1.0f + __expf(-z));
-            loss = ad * 0.125f + ad * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
+            loss = ad * 0.125f + al * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
             d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * 2.0f / norm;
-            d[1] = (o[1] > -3.0f && o[1] < 5.0f) ? (1.0f - ad * e) / norm : 0.0f;
+            d[1] = (o[1] > -3.0f && o[1] < 5.0f) ? (1.0f - al * e) / norm : 0.0f;
             d[2] = (sig - y) / norm;
             valid = 1.0f;
         }
@@ -316,14 +319,18 @@ __global__ void k_quarter_input(const float *__restrict__ p0, const float *__res
 // ------------------------------------------------------------ band windows
 
 // One block per tile x tile coarse nodes: the tile's mean confidence decides
-// between the band (centre +-half labels, clamped to 1 .. L-1; label 0, the
-// global warp, is always swept) and the full sweep (lo = 1, hi = L - 1).
-// Node (c, r) reads the network pixel (min(c, w - 1), min(r, h - 1)).
+// between the band and the full sweep (lo = 1, hi = L - 1). A banded node
+// sweeps centre +- w labels, clamped to 1 .. L-1 (label 0, the global warp,
+// is always swept), with w = ceil(4 b) for the predicted Laplace scale b =
+// exp(log half-width) - the +-4b interval holds 98 % of a Laplace error -
+// clamped to [half, max_half]. Node (c, r) reads the network pixel
+// (min(c, w - 1), min(r, h - 1)).
 __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ out,
                                                       uint32_t cout_p, uint32_t w, uint32_t h,
                                                       uint32_t columns, uint32_t rows,
                                                       uint32_t L, float threshold, uint32_t half,
-                                                      uint32_t tile, uint16_t *__restrict__ lo,
+                                                      uint32_t max_half, uint32_t tile,
+                                                      uint16_t *__restrict__ lo,
                                                       uint16_t *__restrict__ hi,
                                                       uint32_t *__restrict__ full_tiles) {
     __shared__ float sum[256];
@@ -362,8 +369,10 @@ __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ 
         const float *o = out + ((size_t)min(r, h - 1) * w + min(c, w - 1)) * cout_p;
         const float centre = rintf(64.0f + 16.0f * o[0]);
         const int ctr = (int)fminf(fmaxf(centre, 1.0f), (float)(L - 1));
-        lo[node] = (uint16_t)max(1, ctr - (int)half);
-        hi[node] = (uint16_t)min((int)L - 1, ctr + (int)half);
+        const float b = __expf(fminf(fmaxf(o[1], -3.0f), 5.0f));
+        const int w = (int)fminf(fmaxf(ceilf(4.0f * b), (float)half), (float)max_half);
+        lo[node] = (uint16_t)max(1, ctr - w);
+        hi[node] = (uint16_t)min((int)L - 1, ctr + w);
     }
 }
 
@@ -445,11 +454,11 @@ hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
 
 hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
-                               float threshold, uint32_t half, uint32_t tile, uint16_t *lo,
-                               uint16_t *hi, uint32_t *full_tiles) {
+                               float threshold, uint32_t half, uint32_t max_half, uint32_t tile,
+                               uint16_t *lo, uint16_t *hi, uint32_t *full_tiles) {
     const uint32_t tiles = ((columns + tile - 1) / tile) * ((rows + tile - 1) / tile);
     hipLaunchKernelGGL(k_band_windows, dim3(tiles), dim3(256), 0, s, out, cout_p, w, h, columns,
-                       rows, L, threshold, half, tile, lo, hi, full_tiles);
+                       rows, L, threshold, half, max_half, tile, lo, hi, full_tiles);
     return hipGetLastError();
 }
 

@@ -153,12 +153,13 @@ class ReferenceNet:
         e = np.exp(-s)
         diff = centre - t
         ad = np.abs(diff)
+        al = np.minimum(ad, 32.0)
         y = (ad <= 6.0).astype(np.float64)
         sig = 1.0 / (1.0 + np.exp(-z))
-        loss = ad * 0.125 + ad * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+        loss = ad * 0.125 + al * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
         n = o.shape[0]
         d[:, 0] = np.sign(diff) * 2.0 / n
-        d[:, 1] = np.where((o[:, 1] > -3.0) & (o[:, 1] < 5.0), (1.0 - ad * e) / n, 0.0)
+        d[:, 1] = np.where((o[:, 1] > -3.0) & (o[:, 1] < 5.0), (1.0 - al * e) / n, 0.0)
         d[:, 2] = (sig - y) / n
         d[~m] = 0.0
         return float(loss[m].sum()), d.reshape(out.shape)
@@ -285,14 +286,29 @@ class Proposal:
         _abi.check(self.lib.chd_proposal_adam_step(self.handle, g.ctypes.data, lr), "adam step")
 
     def apply(self, session: Session, threshold: float = 0.5, half_width: int = 8,
-              tile: int = 8, report: bool = True):
-        """Band windows for `session` from its planes; returns (fallback
-        tiles, tiles) when report, else None (no synchronisation)."""
+              tile: int = 8, report: bool = True, max_half_width: int = 0):
+        """Band windows for `session` from its planes (centre +- half_width,
+        or +- ceil(4 b) clamped to [half_width, max_half_width] when
+        max_half_width is larger); returns (fallback tiles, tiles) when
+        report, else None (no synchronisation)."""
         full, tiles = C.c_uint32(), C.c_uint32()
         _abi.check(self.lib.chd_session_set_proposal(
-            session.handle, self.handle, threshold, half_width, tile,
+            session.handle, self.handle, threshold, half_width, max_half_width, tile,
             C.byref(full) if report else None, C.byref(tiles)), "set proposal")
         return (full.value, tiles.value) if report else None
+
+
+def constant_proposal(context: Context, width: int, height: int, centre_label: float) -> Proposal:
+    """The method without the network at the same hypothesis budget: zero
+    weights, so every pixel gets the output biases - one centre label for the
+    whole image at full confidence (every tile banded)."""
+    p = Proposal(context, width, height, max_batch=1)
+    params = np.zeros(p.parameter_count, np.float32)
+    last = params[-16:]               # biases of the output layer (cout_p = 16)
+    last[0] = (centre_label - 64.0) / 16.0
+    last[2] = 30.0                    # sigmoid(30) = 1 in f32
+    p.set_parameters(params)
+    return p
 
 
 def clear(session: Session):
@@ -336,7 +352,7 @@ def build_dataset(ctx: Context, seeds, make_capture):
 
 
 def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_width: int = 8,
-              tile: int = 8, steps: int = 3):
+              tile: int = 8, steps: int = 3, max_half_width: int = 0):
     """Full sweep vs band sweep on one capture: the fraction of nodes tested
     by the full sweep whose band label is within +-1, the agreement of the
     tested masks, the fallback-tile fraction, and both times (cost + SGM,
@@ -353,12 +369,14 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
         full_ms = (time.perf_counter() - t0) * 1000 / steps
         _, _, full_best = sess.coarse()
         full_tested = sess.cost_volume()[3]
-        full_tiles, tiles = proposal.apply(sess, threshold, half_width, tile)
+        full_tiles, tiles = proposal.apply(sess, threshold, half_width, tile,
+                                           max_half_width=max_half_width)
         sess.run(stages)
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            proposal.apply(sess, threshold, half_width, tile, report=False)
+            proposal.apply(sess, threshold, half_width, tile, report=False,
+                           max_half_width=max_half_width)
             sess.run(stages)
         ctx.synchronize()
         band_ms = (time.perf_counter() - t0) * 1000 / steps
@@ -386,11 +404,13 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
     c = np.minimum(np.arange(cols), pred.shape[1] - 1)
     centre = np.clip(np.rint(pred[np.ix_(r, c)][..., 0]), 1, g.labels - 1).reshape(-1)
     conf = pred[np.ix_(r, c)][..., 2]
+    scale = np.exp(np.clip(pred[np.ix_(r, c)][..., 1], -3.0, 5.0)).reshape(-1)
+    width = np.clip(np.ceil(4.0 * scale), half_width, max(half_width, max_half_width))
     tr, tc = np.arange(rows) // tile, np.arange(cols) // tile
     tile_id = (tr[:, None] * ((cols + tile - 1) // tile) + tc[None, :]).reshape(-1)
     means = np.bincount(tile_id, conf.reshape(-1)) / np.bincount(tile_id)
     banded = means[tile_id] >= threshold
-    inside = np.abs(full_best.astype(np.int64) - centre) <= half_width
+    inside = np.abs(full_best.astype(np.int64) - centre) <= width
     wrong = valid & ~close & banded
     stats["banded_nodes"] = float((valid & banded).mean())
     stats["window_covers_label"] = float(inside[valid & banded].mean()) if (valid & banded).any() else 1.0

@@ -252,3 +252,38 @@ def config3(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float
     cap.options.grid_step = 4
     cap.description["workload"] = "config3"
     return cap
+
+
+def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
+            plane_count: int = 24) -> Capture:
+    """One capture of the full-capture workload: sixteen 4160x3120 modules
+    viewing nested fields of one 24-plane scene - five wide (f = 3300 px: the
+    reference at the origin and four on a 30 mm ring at 0/90/180/270 deg),
+    five mid (f = 8250 px, on a 20 mm ring at 36 + 72k deg) and six tele
+    (f = 17700 px, on a 45 mm ring at 30 + 60k deg), all with the reference's
+    orientation. Inverse depths U[0.02, 1.25] 1/m, 12-bit Bayer codes as in
+    config1, 256 hypotheses, near 800 mm, far 50000 mm. `scale` shrinks every
+    raster and focal length together (for small test captures)."""
+    def size(n):
+        return max(16, int(round(n * scale)) // 8 * 8)
+
+    w, h = size(4160), size(3120)
+    modules = [_module(3300.0 * scale, w, h, (0.0, 0.0, 0.0))]
+    rings = [(3300.0, 30.0, 0.0, 4), (8250.0, 20.0, 36.0, 5), (17700.0, 45.0, 30.0, 6)]
+    for f, radius, first, count in rings:
+        step = 90.0 if count == 4 else 360.0 / count
+        for k in range(count):
+            a = math.radians(first + step * k)
+            modules.append(_module(f * scale, w, h, (radius * math.cos(a), radius * math.sin(a), 0.0)))
+    desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.02e-3,
+                          inverse_depth_max=1.25e-3, texture_cell_px=32)
+    bayer = _abi.Bayer(**BAYER_12BIT)
+    raws = _render(desc, modules, 0, 0.0, bayer)
+    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
+    options = DepthOptions(near_depth=800.0, far_depth=50000.0, planes=planes)
+    names = [f"W{k}" for k in range(5)] + [f"M{k}" for k in range(5)] + [f"T{k}" for k in range(6)]
+    cap = _assemble(modules, planes_, 0, options,
+                    {"workload": "config4", "seed": seed, "scale": scale, "views": len(modules),
+                     "planes": planes}, desc, names)
+    cap.raws = raws
+    return cap

@@ -364,8 +364,9 @@ int chd_proposal_parameters(chd_proposal *proposal, float *get, const float *set
  * labels count x height x width. */
 int chd_proposal_set_dataset(chd_proposal *proposal, const uint16_t *inputs,
                              const uint16_t *labels, uint32_t count);
-/* Epochs of shuffled mini-batches with Adam (beta 0.9, 0.999; eps 1e-8);
- * epoch_loss (optional, `epochs` entries) receives the mean loss per
+/* Epochs of shuffled mini-batches with Adam (beta 0.9, 0.999; eps 1e-8),
+ * the step size decaying from learning_rate to 5% of it on a cosine over
+ * the run; epoch_loss (optional, `epochs` entries) receives the mean loss per
  * labelled pixel of each epoch. */
 int chd_proposal_train(chd_proposal *proposal, uint32_t epochs, uint32_t batch,
                        float learning_rate, uint64_t shuffle_seed, float *epoch_loss);
@@ -382,14 +383,16 @@ int chd_proposal_gradients(chd_proposal *proposal, const uint16_t *inputs,
 int chd_proposal_adam_step(chd_proposal *proposal, const float *gradients,
                            float learning_rate);
 /* Runs the network on the session's own planes (on the device) and makes
- * its next cost volumes sweep label 0 and centre +- half_width only; tiles
- * of tile x tile coarse nodes whose mean confidence is below `threshold`
- * keep the full sweep. The network grid must be the reference plane at half
- * size. full_tiles / tiles (optional) receive the fallback count and the
- * tile count (full_tiles synchronises). clear restores the full sweep. */
+ * its next cost volumes sweep label 0 and centre +- w only, w = ceil(4 b)
+ * for the predicted band scale b, clamped to [half_width, max_half_width]
+ * (max_half_width <= half_width: always half_width); tiles of tile x tile
+ * coarse nodes whose mean confidence is below `threshold` keep the full
+ * sweep. The network grid must be the reference plane at half size.
+ * full_tiles / tiles (optional) receive the fallback count and the tile
+ * count (full_tiles synchronises). clear restores the full sweep. */
 int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float threshold,
-                             uint32_t half_width, uint32_t tile, uint32_t *full_tiles,
-                             uint32_t *tiles);
+                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
+                             uint32_t *full_tiles, uint32_t *tiles);
 int chd_session_clear_proposal(chd_session *session);
 
 /* Stage entry points on host arrays, for parity checks of one stage. */

@@ -204,3 +204,25 @@ def test_config2_batch_bit_exact_with_oracle(gpu_context):
         batch.close()
         for s in sessions:
             s.close()
+
+
+def test_config4_batch_bit_exact_with_oracle(gpu_context):
+    """Two small config 4 captures (sixteen modules: wide, mid and tele
+    rings with nested fields of view, 256 hypotheses) batched: each equals
+    its single-session run, and the first equals the oracle."""
+    from test_gpu_parity import _full_parity
+    caps = [scene.config4(seed=s, scale=0.125) for s in (0, 1)]
+    _say("config4 small: single-session parity with the oracle")
+    _full_parity(gpu_context, caps[0], "config4 scale 0.125 seed 0")
+    wants = [_eager(gpu_context, c) for c in caps]
+    sessions = [Session(gpu_context, c.modules, 0, c.options) for c in caps]
+    batch = Batch(gpu_context, sessions)
+    try:
+        batch.run(ALL_KEEP)
+        gpu_context.synchronize()
+        for k, (s, want) in enumerate(zip(sessions, wants)):
+            _same(f"config4 capture {k}", _snapshot(s), want)
+    finally:
+        batch.close()
+        for s in sessions:
+            s.close()

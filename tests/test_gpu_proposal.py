@@ -10,7 +10,8 @@ import pytest
 from chiaro_amd import _abi, scene
 from chiaro_amd.depth import Session
 from chiaro_amd.proposal import (Proposal, ReferenceNet, adam_reference, agreement,
-                                 build_dataset, clear, from_bf16, layer_specs, to_bf16)
+                                 build_dataset, clear, constant_proposal, from_bf16,
+                                 layer_specs, to_bf16)
 
 from parity import assert_bitwise
 
@@ -144,6 +145,11 @@ def test_training_and_band_sweep(gpu_context):
         costs = full.cost_volume()[1]
         swept = np.isfinite(costs).sum(axis=1)
         assert swept.max() <= 18 and swept.min() >= 1
+        # widths from the predicted scale: +-8 .. +-24
+        net.apply(full, threshold=0.0, max_half_width=24)
+        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+        swept = np.isfinite(full.cost_volume()[1]).sum(axis=1)
+        assert swept.max() <= 50 and swept.min() >= 1
         clear(full)
         full.close()
         stats = agreement(gpu_context, _cap(101), net, threshold=0.5)
@@ -151,3 +157,34 @@ def test_training_and_band_sweep(gpu_context):
         assert 0.0 <= stats["agreement"] <= 1.0 and stats["band_ms"] > 0
     finally:
         net.close()
+
+
+def test_ablation_network_beats_constant_band(gpu_context):
+    """Ablation at an equal hypothesis budget (+-8 labels at every node, no
+    fallback): the band centred by the trained network against one centred
+    on the training set's median label. The network must cover the full
+    sweep's label, and reproduce it within +-1, far more often."""
+    _say("building 160 small config3 scenes")
+    x, y = build_dataset(gpu_context, range(1000, 1160), _cap)
+    h, w = x.shape[1:3]
+    net = Proposal(gpu_context, w, h, max_batch=32, seed=11)
+    const = constant_proposal(gpu_context, w, h, float(np.median(y[y > 0])))
+    try:
+        net.set_dataset(x, y)
+        losses = net.train(epochs=40, batch=32, lr=1e-3, seed=2)
+        _say(f"losses {losses[0]:.3f} -> {losses[-1]:.3f}")
+        with_net, without = [], []
+        for seed in (2000, 2001, 2002):
+            cap = _cap(seed)
+            with_net.append(agreement(gpu_context, cap, net, threshold=0.0, steps=1))
+            without.append(agreement(gpu_context, cap, const, threshold=0.0, steps=1))
+        mean = lambda rows, k: float(np.mean([r[k] for r in rows]))
+        a_net, a_const = mean(with_net, "agreement"), mean(without, "agreement")
+        c_net, c_const = mean(with_net, "window_covers_label"), mean(without, "window_covers_label")
+        _say(f"network: agreement {a_net:.3f} coverage {c_net:.3f}; "
+             f"constant centre: agreement {a_const:.3f} coverage {c_const:.3f}")
+        assert all(r["fallback_fraction"] == 0.0 for r in with_net + without)
+        assert c_net > 1.5 * c_const and a_net > 1.5 * a_const
+    finally:
+        net.close()
+        const.close()

@@ -22,8 +22,9 @@ def _params(width, height, rng, scale=0.2):
 
 
 def _loss(net, x, labels, frozen=None):
-    """The training loss per pixel; the scale term's |t - c| is the value
-    `frozen` (the unperturbed error), as the gradient treats it."""
+    """The training loss per pixel; the scale term's |t - c| (capped at 32
+    labels) is the value `frozen` (the unperturbed error), as the gradient
+    treats it."""
     acts = net.forward(x)
     out = acts[-1].reshape(-1, acts[-1].shape[-1])
     t = labels.reshape(-1).astype(np.float64)
@@ -34,7 +35,8 @@ def _loss(net, x, labels, frozen=None):
     ad = np.abs(c - t)
     a = ad if frozen is None else frozen
     y = (a <= 6.0).astype(np.float64)
-    loss = ad * 0.125 + a * np.exp(-s) + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+    loss = (ad * 0.125 + np.minimum(a, 32.0) * np.exp(-s) + s
+            + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z)))))
     return float(loss[m].sum()) / labels.size, ad
 
 
