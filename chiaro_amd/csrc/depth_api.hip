@@ -312,7 +312,8 @@ int run_sgm_rest(chd_session *s, bool keep_aggregated) {
         return launch_sgm_select(st, s->paths, s->scores, s->tested, s->labels, g.coarse_columns,
                                  g.coarse_rows, g.labels, s->options,
                                  keep_aggregated ? s->aggregated : nullptr, s->coarse, s->fillable,
-                                 s->best_label);
+                                 s->best_label, s->band ? s->band_lo : nullptr,
+                                 s->band ? s->band_hi : nullptr);
     }));
     if (s->options.completion_iterations > 0) {
         const size_t n = (size_t)g.coarse_columns * g.coarse_rows;
@@ -1848,7 +1849,10 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
         return code;
     };
     int r = CHD_OK;
-    // He-normal weights from a fixed generator, zero biases
+    // He-normal weights from a fixed generator and zero biases for the hidden
+    // layers; the output layer starts at zero weights and a band scale of 8
+    // labels (log-scale bias ln 8), so that the first steps' large errors
+    // cannot drive the hidden ReLUs dead
     std::mt19937_64 rng(seed);
     std::normal_distribution<float> normal(0.0f, 1.0f);
     for (int i = 0; i < 7 && r == CHD_OK; i++) {
@@ -1862,14 +1866,19 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
         if (r == CHD_OK) r = palloc(p, &q.wk, nw);
         if (r == CHD_OK) r = palloc(p, &q.wt, nw);
         if (r != CHD_OK) break;
-        std::vector<float> w(nw, 0.0f), zeros(nw, 0.0f);
+        const bool output = i == 6;
+        std::vector<float> w(nw, 0.0f), zeros(nw, 0.0f), b(L.cout_p, 0.0f);
         const float scale = std::sqrt(2.0f / (9.0f * (float)L.cin));
-        for (size_t k = 0; k < nw; k++) w[k] = normal(rng) * scale;
+        if (!output)
+            for (size_t k = 0; k < nw; k++) w[k] = normal(rng) * scale;
+        else
+            b[1] = std::log(8.0f);
         hipStream_t st = c->stream;
         hipError_t e = hipMemcpyAsync(q.w, w.data(), nw * 4, hipMemcpyHostToDevice, st);
         for (float *f : {q.mw, q.vw})
             if (e == hipSuccess) e = hipMemcpyAsync(f, zeros.data(), nw * 4, hipMemcpyHostToDevice, st);
-        for (float *f : {q.b, q.mb, q.vb})
+        if (e == hipSuccess) e = hipMemcpyAsync(q.b, b.data(), L.cout_p * 4, hipMemcpyHostToDevice, st);
+        for (float *f : {q.mb, q.vb})
             if (e == hipSuccess) e = hipMemcpyAsync(f, zeros.data(), L.cout_p * 4, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) r = fail(CHD_ERR_HIP, std::string("proposal init: ") + hipGetErrorString(e));

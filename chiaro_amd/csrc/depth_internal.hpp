@@ -110,7 +110,8 @@ hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float
                              const uint8_t *tested, const double *labels, uint32_t columns,
                              uint32_t rows, uint32_t label_count, const chd_options &options,
                              float *aggregated, chd_node_depth *field, uint8_t *fillable,
-                             uint16_t *best_label);
+                             uint16_t *best_label, const uint16_t *window_lo = nullptr,
+                             const uint16_t *window_hi = nullptr);  // band windows or null
 
 hipError_t launch_select_depths(hipStream_t stream, const float *regularised,
                                 const float *scores, const uint8_t *tested,

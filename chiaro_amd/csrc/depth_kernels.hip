@@ -61,6 +61,23 @@ __device__ __forceinline__ float wave_min(float v) {
 // which the shifts turn into the reference's missing neighbours.
 constexpr uint32_t kSgmPrefetch = 8;
 
+// Label set of a node: every label, or with band windows (lo, hi non-null)
+// label 0 and lo[node] .. hi[node]. The costs outside the set are +inf, so
+// every path value there is +inf too (the sweep runs over the whole volume,
+// as cheap as a windowed sweep: it is bound by its serial steps, not by
+// memory); the select reads the set only and takes +inf elsewhere.
+struct LabelSet {
+    uint32_t lo, hi;  // lo = 1, hi = L - 1 without windows
+    __device__ __forceinline__ bool has(uint32_t l) const { return l == 0 || (l >= lo && l <= hi); }
+};
+__device__ __forceinline__ LabelSet label_set(const uint16_t *lo, const uint16_t *hi, size_t node,
+                                              uint32_t L) {
+    LabelSet w;
+    w.lo = lo ? lo[node] : 1u;
+    w.hi = lo ? hi[node] : L - 1;
+    return w;
+}
+
 template <uint32_t J>
 __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
                                          const float *__restrict__ guidance, uint32_t C,
@@ -437,11 +454,14 @@ __global__ void __launch_bounds__(256) k_sgm_select(const float *__restrict__ pa
                                                     uint32_t nodes, uint32_t L, uint32_t J,
                                                     chd_options o, float *aggregated,
                                                     chd_node_depth *field, uint8_t *fillable,
-                                                    uint16_t *best_label) {
+                                                    uint16_t *best_label,
+                                                    const uint16_t *__restrict__ win_lo,
+                                                    const uint16_t *__restrict__ win_hi) {
     const uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63u;
     if (node >= nodes) return;
     const size_t volume = (size_t)nodes * L;
+    const LabelSet w = label_set(win_lo, win_hi, node, L);
     float s[kSgmMaxChunks];
 #pragma unroll
     for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
@@ -450,9 +470,12 @@ __global__ void __launch_bounds__(256) k_sgm_select(const float *__restrict__ pa
             const uint32_t l = lane + 64 * j;
             if (l < L) {
                 const size_t at = (size_t)node * L + l;
-                float acc = 0.0f;
+                float acc = __int_as_float(0x7f800000);  // outside the set: 8 x +inf
+                if (w.has(l)) {
+                    acc = 0.0f;
 #pragma unroll
-                for (uint32_t d = 0; d < 8; d++) acc += paths[d * volume + at];
+                    for (uint32_t d = 0; d < 8; d++) acc += paths[d * volume + at];
+                }
                 s[j] = acc;
                 if (aggregated) aggregated[at] = acc;
             }
@@ -465,13 +488,15 @@ hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float
                              const uint8_t *tested, const double *labels, uint32_t columns,
                              uint32_t rows, uint32_t label_count, const chd_options &options,
                              float *aggregated, chd_node_depth *field, uint8_t *fillable,
-                             uint16_t *best_label) {
+                             uint16_t *best_label, const uint16_t *window_lo,
+                             const uint16_t *window_hi) {
     const uint32_t J = (label_count + 63) / 64;
     if (J > kSgmMaxChunks) return hipErrorInvalidValue;
     const uint32_t nodes = columns * rows;
     dim3 blocks((nodes + 3) / 4);
     hipLaunchKernelGGL(k_sgm_select, blocks, dim3(256), 0, stream, paths, scores, tested, labels,
-                       nodes, label_count, J, options, aggregated, field, fillable, best_label);
+                       nodes, label_count, J, options, aggregated, field, fillable, best_label,
+                       window_lo, window_hi);
     return hipGetLastError();
 }
 

@@ -13,11 +13,15 @@
 //    and w * r, and the label- and view-independent sums sum(w), sum(w r),
 //    sum(w r^2) accumulated in tap order (the reference keeps each of its
 //    five sums in its own accumulator, so hoisting these three is exact);
-//  * per view, rebuilt inside the view loop (patch_view): the global-warp
-//    tap positions, the centre's global point, the label-independent half of
-//    map_at_depth (the corrected reference ray) and the min/max warp tap,
-//    which decides Plane::sample's bounds test for all taps at once because
-//    ((f64(g) + delta) - 0.5) * 0.5 -> f32 is monotone in g.
+//  * per view: the centre's global point and the label-independent half of
+//    map_at_depth (the corrected reference ray), computed for a group of up
+//    to kRayViews views at once (one thread per node and view, so the long
+//    f64 chains run side by side), and, rebuilt inside the view loop
+//    (patch_view), the global-warp tap positions with their min/max, which
+//    decides Plane::sample's bounds test for all taps at once because
+//    ((f64(g) + delta) - 0.5) * 0.5 -> f32 is monotone in g. A group of
+//    lanes per node computes the taps and reduces the min/max (exact in any
+//    order) with lane shuffles.
 // A work item is one (node, hypothesis) ZNCC for the current view, looping
 // the taps in the reference's dy-major order. With the view loop outermost
 // the target plane, its buffer resource and the camera are uniform across
@@ -36,7 +40,11 @@ constexpr uint32_t kMaxCandidates = 32;  // direct_depth_candidates bound (depth
 struct PatchLayout {
     uint32_t cams, pos, ray, ext, gc, gq, wr, sums, centre, spatial, node_ok, nv_ok, gc_ok,
         ray_ok, total;
+    uint32_t ray_views;  // views per ray group: the per-view arrays hold [ray_views][nb]
 };
+
+// Views whose centre points and corrected rays are computed together.
+constexpr uint32_t kRayViews = 4;
 
 __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
@@ -45,21 +53,23 @@ __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15
 __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t,
                                                     bool f64_taps) {
     PatchLayout l;
+    const uint32_t rv = v < kRayViews ? (v > 0 ? v : 1u) : kRayViews;
+    l.ray_views = rv;
     uint32_t o = 0;
     l.cams = o; o = align16(o + v * (uint32_t)sizeof(chd_camera));
     l.pos = o; o = align16(o + nb * 16u);
-    l.ray = o; o = align16(o + nb * 6u * 8u);
+    l.ray = o; o = align16(o + rv * nb * 6u * 8u);
     l.ext = o; o = align16(o + nb * 16u);
-    l.gc = o; o = align16(o + nb * 8u);
+    l.gc = o; o = align16(o + rv * nb * 8u);
     l.gq = o; o = align16(o + nb * t * (f64_taps ? 16u : 8u));  // first holds the reference samples
     l.wr = o; o = align16(o + nb * t * 8u);
     l.sums = o; o = align16(o + nb * 3u * 4u);
     l.centre = o; o = align16(o + nb * 4u);
     l.spatial = o; o = align16(o + t * 4u);
     l.node_ok = o; o = align16(o + nb * 4u);
-    l.nv_ok = o; o = align16(o + nb * 4u);
-    l.gc_ok = o; o = align16(o + nb * 4u);
-    l.ray_ok = o; o = align16(o + nb * 4u);
+    l.nv_ok = o; o = align16(o + rv * nb * 4u);
+    l.gc_ok = o; o = align16(o + rv * nb * 4u);
+    l.ray_ok = o; o = align16(o + rv * nb * 4u);
     l.total = o;
     return l;
 }
@@ -67,6 +77,8 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
 struct PatchLds {
     chd_camera *cams;  // cameras of the scored views, in view-slot order
     double2 *pos;      // raster position of each node
+    // ray, gc, nv_ok, gc_ok, ray_ok: [ray group slot][node] in the layout;
+    // patch_view returns a copy pointing at the current view's slot
     double *ray;       // [node][6]: corrected reference ray (current view)
     float4 *ext;       // [node]: min x, max x, min y, max y warp tap (current view)
     float2 *gc;        // [node]: global point of the centre (current view)
@@ -83,6 +95,7 @@ struct PatchLds {
     int *nv_ok;        // current view: global warp defined at the centre and every tap
     int *gc_ok;        // current view: global warp defined at the centre
     int *ray_ok;       // current view: corrected reference ray exists
+    uint32_t ray_views;
 };
 
 __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l, bool f64_taps) {
@@ -104,6 +117,7 @@ __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l,
     p.nv_ok = (int *)(base + l.nv_ok);
     p.gc_ok = (int *)(base + l.gc_ok);
     p.ray_ok = (int *)(base + l.ray_ok);
+    p.ray_views = l.ray_views;
     return p;
 }
 
@@ -139,11 +153,58 @@ __device__ __forceinline__ DevWarp view_warp(const ViewDev &v) {
     return w;
 }
 
+// Centre global point and corrected reference ray of nodes 0 .. nb-1 for the
+// views first .. first + count - 1 (count <= ray_views), one (node, view)
+// per thread (the block's last threads, which have the fewest of the
+// accompanying sample items), into ray-group slots 0 .. count - 1; nv_ok
+// starts as gc_ok (patch_view clears it where a tap's warp is undefined).
+// Reads the cameras from global memory (the LDS copy may still be in flight).
+__device__ void patch_rays(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t first,
+                           uint32_t count) {
+    const chd_camera &reference = *S.views[S.reference].camera;
+    for (uint32_t i = blockDim.x - 1 - threadIdx.x; i < nb * count; i += blockDim.x) {
+        const uint32_t slot = i / nb, n = i % nb, k = slot * nb + n;
+        const ViewDev &view = S.views[S.vlist[first + slot]];
+        const double2 p = P.pos[n];
+        float2 gc;
+        if (!warp_map(view_warp(view), (float)p.x, (float)p.y, gc)) {
+            P.nv_ok[k] = 0;
+            P.gc_ok[k] = 0;
+            P.ray_ok[k] = 0;
+            continue;
+        }
+        P.nv_ok[k] = 1;
+        P.gc_ok[k] = 1;
+        P.gc[k] = gc;
+        // corrected_reference = reference.map_from(target, gc, 1e8)
+        // (depth.rs:1752-1756), then the reference ray through it
+        d3 o, d;
+        pixel_to_ray(*view.camera, (double)gc.x, (double)gc.y, o, d);
+        const d3 far{o.x + d.x * kInfinityDepth, o.y + d.y * kInfinityDepth,
+                     o.z + d.z * kInfinityDepth};
+        double cx, cy;
+        if (!project(reference, far, cx, cy)) {
+            P.ray_ok[k] = 0;
+            continue;
+        }
+        P.ray_ok[k] = 1;
+        pixel_to_ray(reference, cx, cy, o, d);
+        double *ray = P.ray + (size_t)k * 6;
+        ray[0] = o.x;
+        ray[1] = o.y;
+        ray[2] = o.z;
+        ray[3] = d.x;
+        ray[4] = d.y;
+        ray[5] = d.z;
+    }
+}
+
 // Node half of the patch context. On entry P.pos[n] and P.node_ok[n] (node
 // exists) are set and visible. On exit node_ok[n] is 0 when the centre or any
 // tap of the reference patch falls outside the reference plane (every ZNCC of
 // the node is then None), centre_value[n] holds reference_guidance
-// (depth.rs:1824-1832), and the cameras of all scored views are in LDS.
+// (depth.rs:1824-1832), the cameras of all scored views are in LDS, and the
+// first ray group (views 0 .. ray_views - 1) is computed.
 __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             float *centre_value) {
     const uint32_t V = S.view_count, tid = threadIdx.x, nth = blockDim.x;
@@ -156,8 +217,7 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
         const uint32_t *src = (const uint32_t *)S.views[S.vlist[i / words]].camera;
         ((uint32_t *)P.cams)[i] = src[i % words];
     }
-    for (uint32_t n = tid; n < nb; n += nth) centre_value[n] = 0.0f;
-    __syncthreads();
+    if (V > 0) patch_rays(S, P, nb, 0, V < P.ray_views ? V : P.ray_views);
     for (uint32_t t = tid; t < T; t += nth) {
         const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
         const float distance_sq = (float)(dx * dx + dy * dy);
@@ -170,9 +230,11 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             P.centre[n] = c;
             centre_value[n] = c;
         } else {
+            centre_value[n] = 0.0f;
             P.node_ok[n] = 0;
         }
     }
+#pragma unroll 3
     for (uint32_t i = tid; i < nb * T; i += nth) {
         const uint32_t n = i / T, t = i % T;
         const double2 p = P.pos[n];
@@ -211,83 +273,66 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
     __syncthreads();
 }
 
-// View half of the patch context for view slot `a`. Must be entered by the
-// whole block after every read of the previous view's context has finished.
-__device__ void patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
-                           uint32_t a) {
+// View half of the patch context for view slot `a`; returns the context with
+// the per-view pointers at a's ray-group slot. Must be entered by the whole
+// block after every read of the previous view's context has finished.
+__device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
+                               uint32_t a) {
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const int r = (int)S.radius;
     const uint32_t width = 2u * S.radius + 1u;
-    const ViewDev &view = S.views[S.vlist[a]];
-    const DevWarp gw = view_warp(view);
-    const chd_camera &reference = *S.views[S.reference].camera;
-    for (uint32_t n = tid; n < nb; n += nth) {
-        P.nv_ok[n] = 1;
-        P.gc_ok[n] = 1;
-        P.ray_ok[n] = 1;
+    const uint32_t V = S.view_count, R = P.ray_views, slot = a % R;
+    if (a > 0 && slot == 0) {
+        patch_rays(S, P, nb, a, V - a < R ? V - a : R);
+        __syncthreads();
+    }
+    PatchLds Q = P;
+    Q.ray = P.ray + (size_t)slot * nb * 6;
+    Q.gc = P.gc + slot * nb;
+    Q.nv_ok = P.nv_ok + slot * nb;
+    Q.gc_ok = P.gc_ok + slot * nb;
+    Q.ray_ok = P.ray_ok + slot * nb;
+    const DevWarp gw = view_warp(S.views[S.vlist[a]]);
+    // G lanes per node (a power of two dividing the wavefront) compute the
+    // node's tap positions and reduce their min/max
+    const uint32_t G = nb <= 4 ? 64u : nb <= 8 ? 32u : 16u;
+    const uint32_t per_pass = nth / G, lane = tid % G;
+    for (uint32_t n0 = 0; n0 < nb; n0 += per_pass) {
+        const uint32_t n = n0 + tid / G;
+        float x0 = __int_as_float(0x7f800000), y0 = x0;
+        float x1 = -x0, y1 = -x0;
+        if (n < nb) {
+            const double2 p = P.pos[n];
+#pragma unroll 3
+            for (uint32_t t = lane; t < T; t += G) {
+                const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+                float2 q;
+                if (!warp_map(gw, (float)(p.x + (double)dx * 2.0), (float)(p.y + (double)dy * 2.0),
+                              q)) {
+                    Q.nv_ok[n] = 0;
+                    q = make_float2(0.0f, 0.0f);
+                }
+                const uint32_t i = n * T + t;
+                if (P.f64_taps)
+                    P.gq[i] = make_double2((double)q.x, (double)q.y);
+                else
+                    P.gqf[i] = q;
+                x0 = fminf(x0, q.x);
+                x1 = fmaxf(x1, q.x);
+                y0 = fminf(y0, q.y);
+                y1 = fmaxf(y1, q.y);
+            }
+        }
+        for (uint32_t m = G >> 1; m > 0; m >>= 1) {
+            x0 = fminf(x0, __shfl_xor(x0, (int)m, (int)G));
+            x1 = fmaxf(x1, __shfl_xor(x1, (int)m, (int)G));
+            y0 = fminf(y0, __shfl_xor(y0, (int)m, (int)G));
+            y1 = fmaxf(y1, __shfl_xor(y1, (int)m, (int)G));
+        }
+        if (n < nb && lane == 0) P.ext[n] = make_float4(x0, x1, y0, y1);
     }
     __syncthreads();
-    for (uint32_t i = tid; i < nb * T; i += nth) {
-        const uint32_t n = i / T, t = i % T;
-        const double2 p = P.pos[n];
-        const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
-        float2 q;
-        if (!warp_map(gw, (float)(p.x + (double)dx * 2.0), (float)(p.y + (double)dy * 2.0), q)) {
-            P.nv_ok[n] = 0;
-            q = make_float2(0.0f, 0.0f);
-        }
-        if (P.f64_taps)
-            P.gq[i] = make_double2((double)q.x, (double)q.y);
-        else
-            P.gqf[i] = q;
-    }
-    for (uint32_t n = tid; n < nb; n += nth) {
-        const double2 p = P.pos[n];
-        float2 gc;
-        if (!warp_map(gw, (float)p.x, (float)p.y, gc)) {
-            P.nv_ok[n] = 0;
-            P.gc_ok[n] = 0;
-            P.ray_ok[n] = 0;
-            continue;
-        }
-        P.gc[n] = gc;
-        // corrected_reference = reference.map_from(target, gc, 1e8)
-        // (depth.rs:1752-1756), then the reference ray through it
-        d3 o, d;
-        pixel_to_ray(P.cams[a], (double)gc.x, (double)gc.y, o, d);
-        const d3 far{o.x + d.x * kInfinityDepth, o.y + d.y * kInfinityDepth,
-                     o.z + d.z * kInfinityDepth};
-        double cx, cy;
-        if (!project(reference, far, cx, cy)) {
-            P.ray_ok[n] = 0;
-            continue;
-        }
-        pixel_to_ray(reference, cx, cy, o, d);
-        double *ray = P.ray + (size_t)n * 6;
-        ray[0] = o.x;
-        ray[1] = o.y;
-        ray[2] = o.z;
-        ray[3] = d.x;
-        ray[4] = d.y;
-        ray[5] = d.z;
-    }
-    __syncthreads();
-    for (uint32_t n = tid; n < nb; n += nth) {
-        if (!P.nv_ok[n]) continue;
-        float x0 = 0.0f, x1 = 0.0f, y0 = 0.0f, y1 = 0.0f;
-        for (uint32_t t = 0; t < T; t++) {
-            // f64-held positions are exact f32 values
-            const size_t i = (size_t)n * T + t;
-            const float gx = P.f64_taps ? (float)P.gq[i].x : P.gqf[i].x;
-            const float gy = P.f64_taps ? (float)P.gq[i].y : P.gqf[i].y;
-            x0 = t == 0 ? gx : fminf(x0, gx);
-            x1 = t == 0 ? gx : fmaxf(x1, gx);
-            y0 = t == 0 ? gy : fminf(y0, gy);
-            y1 = t == 0 ? gy : fmaxf(y1, gy);
-        }
-        P.ext[n] = make_float4(x0, x1, y0, y1);
-    }
-    __syncthreads();
+    return Q;
 }
 
 // The three target sums of warped_patch_zncc over the taps in the
@@ -439,19 +484,19 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
     patch_nodes(S, P, nb, T, centre_value);
 
     for (uint32_t a = 0; a < V; a++) {
-        patch_view(S, P, nb, T, a);
+        const PatchLds Q = patch_view(S, P, nb, T, a);
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         for (uint32_t i = tid; i < nb * Ls; i += nth) {
             const uint32_t n = i / Ls, l = l0 + i % Ls;
             float s = nan_f();
-            if (P.node_ok[n] && P.nv_ok[n]) {
+            if (Q.node_ok[n] && Q.nv_ok[n]) {
                 double dx = 0.0, dy = 0.0;
                 bool ok = true;
-                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], J.labels[l], dx, dy);
+                if (l > 0) ok = delta_at_depth(Q, n, Q.cams[a], J.labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                if (ok && patch_zncc<true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
                                            evaluation_count))
                     s = sc;
             }
@@ -507,12 +552,14 @@ __device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &st
 // outside the window are left as filled beforehand (NaN, +inf), so the SGM
 // that follows never selects them; inside the window every value is the
 // full sweep's. One slab; items are (node, label) over the nodes' windows.
+// The tap positions are held as f32 (half the LDS of the full sweep's f64
+// layout), which keeps four blocks per CU at 12 nodes per item.
 __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *lds,
                                unsigned int *evaluation_count) {
     const PatchSpec &S = J.S;
     const uint32_t V = S.view_count, T = J.T, L = J.L, nb = J.nb;
-    const PatchLayout lay = patch_layout(nb, V, T, true);
-    const PatchLds P = bind_patch(lds, lay, true);
+    const PatchLayout lay = patch_layout(nb, V, T, false);
+    const PatchLds P = bind_patch(lds, lay, false);
     float *res = (float *)(lds + lay.total);          // [nb][V][L] (windows are <= L)
     float *centre_value = res + (size_t)nb * V * L;   // [nb]
     int *node_tested = (int *)(centre_value + nb);    // [nb]
@@ -547,7 +594,7 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
     const uint32_t items = base[nb];
 
     for (uint32_t a = 0; a < V; a++) {
-        patch_view(S, P, nb, T, a);
+        const PatchLds Q = patch_view(S, P, nb, T, a);
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
@@ -555,12 +602,12 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
             const uint32_t n = item_node[i], j = i - base[n];
             const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
             float s = nan_f();
-            if (P.node_ok[n] && P.nv_ok[n]) {
+            if (Q.node_ok[n] && Q.nv_ok[n]) {
                 double dx = 0.0, dy = 0.0;
                 bool ok = true;
-                if (l > 0) ok = delta_at_depth(P, n, P.cams[a], J.labels[l], dx, dy);
+                if (l > 0) ok = delta_at_depth(Q, n, Q.cams[a], J.labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc<true>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
                                            evaluation_count))
                     s = sc;
             }
@@ -678,9 +725,12 @@ __global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__rest
     }
 }
 
-size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab) {
-    const PatchLayout l = patch_layout(nb, views, taps, true);
-    return (size_t)l.total + (size_t)nb * views * slab * 4 + (size_t)nb * 8;
+size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
+                      bool band = false) {
+    const PatchLayout l = patch_layout(nb, views, taps, !band);
+    size_t bytes = (size_t)l.total + (size_t)nb * views * slab * 4 + (size_t)nb * 8;
+    if (band) bytes += (size_t)(2 * nb + 1) * 4 + (size_t)nb * slab;  // base, lo, item_node
+    return bytes;
 }
 
 // Default shape: about 1024 (node, label) units per block, within 40 KB of
@@ -692,11 +742,11 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     J.slab = label_slab == 0 || label_slab > J.L || J.window_lo ? J.L : label_slab;
     uint32_t nb = nodes_per_block;
     if (nb == 0 && J.window_lo) {
-        // band windows hold ~18 labels: more nodes per item keep the threads
-        // busy (the LDS still covers full-sweep nodes in fallback tiles)
+        // band windows hold 18 .. 66 labels: more nodes per item keep the
+        // threads busy; within 40 KB of LDS (four blocks per CU; the LDS
+        // still covers full-sweep nodes in fallback tiles)
         nb = 16;
-        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab) + (2 * nb + 1) * 4 + nb * J.L > 64 * 1024)
-            nb--;
+        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab, true) > 40 * 1024) nb--;
     } else if (nb == 0) {
         nb = 1024 / J.slab;
         if (nb < 1) nb = 1;
@@ -710,9 +760,7 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
 }
 
 size_t cost_job_lds(const CostJob &J) {
-    size_t bytes = cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab);
-    if (J.window_lo) bytes += (size_t)(2 * J.nb + 1) * 4 + (size_t)J.nb * J.L;  // band tables
-    return bytes;
+    return cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab, J.window_lo != nullptr);
 }
 
 hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {
@@ -918,8 +966,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (uint32_t k = 0; node0 + n < nodes && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
     __syncthreads();
     for (uint32_t a = 0; a < V; a++) {
-        patch_view(S, P, nb, T, a);
-        score_slots(S, P, nb, T, a, cand, base, item_node, kMaxCandidates, res,
+        const PatchLds Q = patch_view(S, P, nb, T, a);
+        score_slots(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
                     &evaluation_count);
         __syncthreads();
     }
@@ -1080,8 +1128,8 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
                 item_node[base[n] + k] = (uint8_t)n;
         __syncthreads();
         for (uint32_t a = 0; a < V; a++) {
-            patch_view(S, P, nb, T, a);
-            score_slots(S, P, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
+            const PatchLds Q = patch_view(S, P, nb, T, a);
+            score_slots(S, Q, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
                         &evaluation_count);
             __syncthreads();
         }
@@ -1168,7 +1216,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     patch_nodes(S, P, nb, T, centre_value);
 
     for (uint32_t a = 0; a < V; a++) {
-        patch_view(S, P, nb, T, a);
+        const PatchLds Q = patch_view(S, P, nb, T, a);
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
@@ -1176,7 +1224,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         // Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
-            state[n] = g < nodes && P.gc_ok[n] && field[g].valid ? 1 : 0;
+            state[n] = g < nodes && Q.gc_ok[n] && field[g].valid ? 1 : 0;
         }
         __syncthreads();
         // phase A: offsets k < 5 (score_one_view_at_point, depth.rs:1710-1736)
@@ -1186,8 +1234,8 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             if (state[n] == 0) continue;
             float s = nan_f(), sc;
             if (k == 5) {
-                if (P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc,
+                if (Q.node_ok[n] && Q.nv_ok[n] &&
+                    patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc,
                                       &evaluation_count))
                     s = sc;
                 baseline[n] = s;
@@ -1199,18 +1247,18 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
                 rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
             const double depth = 1.0 / inverse;
             double qx = 0.0, qy = 0.0;
-            bool ok = P.ray_ok[n] != 0;
+            bool ok = Q.ray_ok[n] != 0;
             if (ok) {
-                const double *r = P.ray + (size_t)n * 6;
+                const double *r = Q.ray + (size_t)n * 6;
                 const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
-                ok = project(P.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
+                ok = project(Q.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
             }
             const float2 q = make_float2((float)qx, (float)qy);
             points[n * 5 + k] = q;
             if (ok) {
-                const double dx = (double)(q.x - P.gc[n].x), dy = (double)(q.y - P.gc[n].y);
-                if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                    patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                const double dx = (double)(q.x - Q.gc[n].x), dy = (double)(q.y - Q.gc[n].y);
+                if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && Q.node_ok[n] && Q.nv_ok[n] &&
+                    patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
                                       &evaluation_count))
                     s = sc;
             }
@@ -1253,10 +1301,10 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             const uint32_t cell = k < 4 ? k : k + 1;
             const float2 point = make_float2(best_point[n].x + steps[cell % 3],
                                              best_point[n].y + steps[cell / 3]);
-            const double dx = (double)(point.x - P.gc[n].x), dy = (double)(point.y - P.gc[n].y);
+            const double dx = (double)(point.x - Q.gc[n].x), dy = (double)(point.y - Q.gc[n].y);
             float s = nan_f(), sc;
-            if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && P.node_ok[n] && P.nv_ok[n] &&
-                patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
+            if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && Q.node_ok[n] && Q.nv_ok[n] &&
+                patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
                 s = sc;
             phase[n * 8 + k] = s;
         }
@@ -1270,10 +1318,10 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             d.point[0] = d.point[1] = 0.0f;
             d.confidence = 0.0f;
             d.measured = 0;
-            if (P.gc_ok[n]) {
+            if (Q.gc_ok[n]) {
                 d.code = CHD_WARP_GLOBAL;
-                d.global[0] = P.gc[n].x;
-                d.global[1] = P.gc[n].y;
+                d.global[0] = Q.gc[n].x;
+                d.global[1] = Q.gc[n].y;
             }
             if (state[n] != 0) {
                 bool has_selected = false;

@@ -61,7 +61,10 @@ __device__ __forceinline__ bool conv_source(const ConvLayer &L, uint32_t oy, uin
 // ------------------------------------------------------------- forward
 
 // One wavefront per 16 output pixels; all cout_p / 16 column tiles.
-template <bool OutF32>
+// Chunks = cin_p / 16 and Tiles = cout_p / 16 are template parameters, so
+// the tap and chunk loops unroll and every operand load of the wave is in
+// flight before the first product; the products run in (tap, chunk) order.
+template <bool OutF32, uint32_t Chunks, uint32_t Tiles>
 __global__ void __launch_bounds__(256) k_conv_fwd(ConvLayer L, uint32_t batch,
                                                   const uint16_t *__restrict__ in,
                                                   const uint16_t *__restrict__ wt,
@@ -76,23 +79,36 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvLayer L, uint32_t batch,
     const bool mv = m < M;
     const uint32_t n = mv ? m / plane : 0, rem = mv ? m % plane : 0;
     const uint32_t oy = rem / L.wout, ox = rem % L.wout;
-    const uint32_t K = L.k, tiles = L.cout_p / 16;
-    f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    constexpr uint32_t cin_p = Chunks * 16, K = 9 * cin_p;
+    bf16x4 a[9][Chunks];
+#pragma unroll
     for (uint32_t tap = 0; tap < 9; tap++) {
         uint32_t iy = 0, ix = 0;
         const bool valid = mv && conv_source(L, oy, ox, tap / 3, tap % 3, iy, ix);
-        const uint16_t *px = in + (((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p;
-        for (uint32_t c0 = 0; c0 < L.cin_p; c0 += 16) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (valid) a = *(const bf16x4 *)(px + c0 + 4 * q);
-            const uint32_t k0 = tap * L.cin_p + c0;
-            for (uint32_t t = 0; t < tiles; t++) {
+        const uint16_t *px = in + (((size_t)n * L.hin + iy) * L.win + ix) * cin_p;
+#pragma unroll
+        for (uint32_t c = 0; c < Chunks; c++) {
+            a[tap][c] = bf16x4{0, 0, 0, 0};
+            if (valid) a[tap][c] = *(const bf16x4 *)(px + c * 16 + 4 * q);
+        }
+    }
+    f32x4 acc[Tiles];
+#pragma unroll
+    for (uint32_t t = 0; t < Tiles; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (uint32_t tap = 0; tap < 9; tap++) {
+#pragma unroll
+        for (uint32_t c = 0; c < Chunks; c++) {
+            const uint32_t k0 = tap * cin_p + c * 16;
+#pragma unroll
+            for (uint32_t t = 0; t < Tiles; t++) {
                 const bf16x4 b = *(const bf16x4 *)(wt + (size_t)(t * 16 + r) * K + k0 + 4 * q);
-                acc[t] = mfma16(a, b, acc[t]);
+                acc[t] = mfma16(a[tap][c], b, acc[t]);
             }
         }
     }
-    for (uint32_t t = 0; t < tiles; t++) {
+#pragma unroll
+    for (uint32_t t = 0; t < Tiles; t++) {
         const uint32_t ch = t * 16 + r;
         for (uint32_t v = 0; v < 4; v++) {
             const uint32_t p = m0 + 4 * q + v;
@@ -103,9 +119,9 @@ __global__ void __launch_bounds__(256) k_conv_fwd(ConvLayer L, uint32_t batch,
                 if (L.relu) y = fmaxf(y, 0.0f);
             }
             if (OutF32)
-                ((float *)out)[(size_t)p * L.cout_p + ch] = y;
+                ((float *)out)[(size_t)p * (Tiles * 16) + ch] = y;
             else
-                ((uint16_t *)out)[(size_t)p * L.cout_p + ch] = f2bf(y);
+                ((uint16_t *)out)[(size_t)p * (Tiles * 16) + ch] = f2bf(y);
         }
     }
 }
@@ -382,11 +398,17 @@ hipError_t launch_conv_fwd(hipStream_t s, const ConvLayer &L, uint32_t batch, co
                            const uint16_t *wt, const float *bias, void *out, bool out_f32) {
     const uint32_t M = batch * L.hout * L.wout;
     const dim3 grid((M + 63) / 64);
-    if (out_f32)
-        hipLaunchKernelGGL(k_conv_fwd<true>, grid, dim3(256), 0, s, L, batch, in, wt, bias, out);
-    else
-        hipLaunchKernelGGL(k_conv_fwd<false>, grid, dim3(256), 0, s, L, batch, in, wt, bias, out);
-    return hipGetLastError();
+    const uint32_t chunks = L.cin_p / 16, tiles = L.cout_p / 16;
+#define CHD_CONV(F, C, T)                                                                     \
+    if (out_f32 == F && chunks == C && tiles == T) {                                          \
+        hipLaunchKernelGGL((k_conv_fwd<F, C, T>), grid, dim3(256), 0, s, L, batch, in, wt, bias, \
+                           out);                                                              \
+        return hipGetLastError();                                                             \
+    }
+    CHD_CONV(false, 1, 1) CHD_CONV(false, 1, 2) CHD_CONV(false, 2, 1) CHD_CONV(false, 2, 2)
+    CHD_CONV(true, 1, 1) CHD_CONV(true, 1, 2) CHD_CONV(true, 2, 1) CHD_CONV(true, 2, 2)
+#undef CHD_CONV
+    return hipErrorInvalidValue;  // channel counts above 32
 }
 
 hipError_t launch_grad_bf16(hipStream_t s, const float *dy, const uint16_t *y, size_t n, bool relu,

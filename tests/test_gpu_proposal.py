@@ -111,10 +111,11 @@ def _cap(seed):
     return scene.config3(seed=seed, width=520, height=392, focal_px=800.0)
 
 
-def test_training_and_band_sweep(gpu_context):
+def test_training_and_band_sweep(gpu_context, oracle_lib):
     """Train on a few small config 3 scenes: the loss falls; every tile in
-    fallback gives the plain full sweep bit for bit; bands run and agree
-    with the full sweep on most nodes."""
+    fallback gives the plain full sweep bit for bit; with bands the cost
+    volume holds +inf outside each node's window and the windowed SGM equals
+    the oracle's full SGM over that volume bit for bit."""
     _say("building 12 small config3 scenes")
     x, y = build_dataset(gpu_context, range(12), _cap)
     h, w = x.shape[1:3]
@@ -138,18 +139,20 @@ def test_training_and_band_sweep(gpu_context):
         assert_bitwise("fallback costs", c, want_costs)
         assert_bitwise("fallback tested", t, want_tested)
         assert_bitwise("fallback labels", full.coarse()[2], want_best)
-        # bands everywhere: windows of +-8 around the centre
-        fb, tiles = net.apply(full, threshold=0.0)
-        assert fb == 0
-        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
-        costs = full.cost_volume()[1]
-        swept = np.isfinite(costs).sum(axis=1)
-        assert swept.max() <= 18 and swept.min() >= 1
-        # widths from the predicted scale: +-8 .. +-24
-        net.apply(full, threshold=0.0, max_half_width=24)
-        full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
-        swept = np.isfinite(full.cost_volume()[1]).sum(axis=1)
-        assert swept.max() <= 50 and swept.min() >= 1
+        # bands everywhere: windows of +-8 around the centre, then widths
+        # from the predicted scale (+-8 .. +-24)
+        for max_half, most in ((0, 18), (24, 50)):
+            fb, tiles = net.apply(full, threshold=0.0, max_half_width=max_half)
+            assert fb == 0
+            full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
+            _, costs, guidance, _ = full.cost_volume()
+            swept = np.isfinite(costs).sum(axis=1)
+            assert swept.max() <= most and swept.min() >= 1
+            g = full.grid
+            want = np.empty_like(costs)
+            oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
+                                            g.coarse_rows, g.labels, _abi.ptr(want))
+            assert_bitwise(f"band SGM (max half {max_half})", full.aggregated(), want)
         clear(full)
         full.close()
         stats = agreement(gpu_context, _cap(101), net, threshold=0.5)
@@ -164,17 +167,17 @@ def test_ablation_network_beats_constant_band(gpu_context):
     fallback): the band centred by the trained network against one centred
     on the training set's median label. The network must cover the full
     sweep's label, and reproduce it within +-1, far more often."""
-    _say("building 160 small config3 scenes")
-    x, y = build_dataset(gpu_context, range(1000, 1160), _cap)
+    _say("building 1000 small config3 scenes")
+    x, y = build_dataset(gpu_context, range(1000, 2000), _cap)
     h, w = x.shape[1:3]
-    net = Proposal(gpu_context, w, h, max_batch=32, seed=11)
+    net = Proposal(gpu_context, w, h, max_batch=32, seed=12)
     const = constant_proposal(gpu_context, w, h, float(np.median(y[y > 0])))
     try:
         net.set_dataset(x, y)
-        losses = net.train(epochs=40, batch=32, lr=1e-3, seed=2)
+        losses = net.train(epochs=60, batch=32, lr=1e-3, seed=12)
         _say(f"losses {losses[0]:.3f} -> {losses[-1]:.3f}")
         with_net, without = [], []
-        for seed in (2000, 2001, 2002):
+        for seed in (3000, 3001, 3002):
             cap = _cap(seed)
             with_net.append(agreement(gpu_context, cap, net, threshold=0.0, steps=1))
             without.append(agreement(gpu_context, cap, const, threshold=0.0, steps=1))
@@ -184,7 +187,7 @@ def test_ablation_network_beats_constant_band(gpu_context):
         _say(f"network: agreement {a_net:.3f} coverage {c_net:.3f}; "
              f"constant centre: agreement {a_const:.3f} coverage {c_const:.3f}")
         assert all(r["fallback_fraction"] == 0.0 for r in with_net + without)
-        assert c_net > 1.5 * c_const and a_net > 1.5 * a_const
+        assert c_net > 2.0 * c_const and a_net > 1.5 * a_const
     finally:
         net.close()
         const.close()
