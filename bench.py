@@ -404,7 +404,7 @@ def bench_batch(args) -> int:
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": None,
             "kernel": "k_cost_volume (one launch per capture)",
-            "zncc_per_launch": zncc,
+            "zncc_per_step": zncc,
         },
         "cpu_baseline": None,
     }
@@ -430,14 +430,18 @@ def bench_learned(args) -> int:
 
     from chiaro_amd import scene
     from chiaro_amd.depth import Context
-    from chiaro_amd.proposal import Proposal, agreement, build_dataset, constant_proposal
+    from chiaro_amd.proposal import (Proposal, agreement, build_dataset, constant_proposal,
+                                     input_transform)
 
     ctx = Context(0)
     t = time.perf_counter()
-    x, y = build_dataset(ctx, range(args.train_scenes), lambda s: scene.config3(seed=s))
+    offset, scale = input_transform([scene.config3(seed=s) for s in range(16)])
+    x, y = build_dataset(ctx, range(args.train_scenes), lambda s: scene.config3(seed=s), offset,
+                         scale)
     log(f"[bench] {args.train_scenes} config3 scenes and labels in {time.perf_counter() - t:.1f} s")
     h, w = x.shape[1:3]
     net = Proposal(ctx, w, h, max_batch=32, seed=args.seed)
+    net.set_input_transform(offset, scale)
     net.set_dataset(x, y)
     t = time.perf_counter()
     losses = net.train(epochs=args.epochs, batch=32, lr=1e-3, seed=args.seed)
@@ -508,6 +512,7 @@ def bench_learned(args) -> int:
             "thresholds": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
                            for r in rows],
             "ablation_equal_budget": ablation,
+            "input_transform": {"offset": offset, "scale": scale},
             "train_seconds": round(train_s, 2),
             "losses": [round(l, 4) for l in losses],
         },

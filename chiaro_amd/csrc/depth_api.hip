@@ -280,12 +280,7 @@ int prepare_cost(chd_session *s) {
     hipStream_t st = s->context->stream;
     CHD_HIP(launch_fill_u64(st, s->work, 0ull, 1));
     CHD_HIP(launch_fill_u8(st, s->tested, 0, (size_t)s->grid.coarse_columns * s->grid.coarse_rows));
-    if (s->band) {  // labels outside the windows: score None, cost +inf
-        const size_t volume = (size_t)s->grid.coarse_columns * s->grid.coarse_rows * s->grid.labels;
-        CHD_HIP(launch_fill_u32(st, (uint32_t *)s->scores, 0x7fc00000u, volume));
-        CHD_HIP(launch_fill_u32(st, (uint32_t *)s->costs, 0x7f800000u, volume));
-    }
-    return CHD_OK;
+    return CHD_OK;  // band items write NaN / +inf outside their windows themselves
 }
 
 int run_cost(chd_session *s, bool keep_view_scores) {
@@ -1700,6 +1695,7 @@ void chd_batch_destroy(chd_batch *b) {
 struct chd_proposal {
     chd_context *context = nullptr;
     uint32_t width = 0, height = 0, max_batch = 0;
+    float input_offset = 0.0f, input_scale = 0.125f;  // input = (2x2 mean - offset) * scale
     ConvLayer layers[7];
     struct Param {
         float *w, *b, *mw, *vw, *mb, *vb, *gw, *gb;
@@ -1850,9 +1846,9 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
     };
     int r = CHD_OK;
     // He-normal weights from a fixed generator and zero biases for the hidden
-    // layers; the output layer starts at zero weights and a band scale of 8
-    // labels (log-scale bias ln 8), so that the first steps' large errors
-    // cannot drive the hidden ReLUs dead
+    // layers; the output layer starts at a tenth of that scale and a band
+    // scale of 8 labels (log-scale bias ln 8), so that the first steps'
+    // large errors cannot drive the hidden ReLUs dead
     std::mt19937_64 rng(seed);
     std::normal_distribution<float> normal(0.0f, 1.0f);
     for (int i = 0; i < 7 && r == CHD_OK; i++) {
@@ -1869,10 +1865,8 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
         const bool output = i == 6;
         std::vector<float> w(nw, 0.0f), zeros(nw, 0.0f), b(L.cout_p, 0.0f);
         const float scale = std::sqrt(2.0f / (9.0f * (float)L.cin));
-        if (!output)
-            for (size_t k = 0; k < nw; k++) w[k] = normal(rng) * scale;
-        else
-            b[1] = std::log(8.0f);
+        for (size_t k = 0; k < nw; k++) w[k] = normal(rng) * (output ? 0.1f * scale : scale);
+        if (output) b[1] = std::log(8.0f);
         hipStream_t st = c->stream;
         hipError_t e = hipMemcpyAsync(q.w, w.data(), nw * 4, hipMemcpyHostToDevice, st);
         for (float *f : {q.mw, q.vw})
@@ -1938,6 +1932,21 @@ int chd_proposal_parameters(chd_proposal *p, float *get, const float *set, size_
     }
     if (set) CHD_CHECK(proposal_refresh_weights(p));
     CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_proposal_set_input_transform(chd_proposal *p, float offset, float scale) {
+    if (!p || !(scale > 0.0f) || !std::isfinite(scale) || !std::isfinite(offset))
+        return fail(CHD_ERR_ARGUMENT, "proposal_set_input_transform: bad argument");
+    p->input_offset = offset;
+    p->input_scale = scale;
+    return CHD_OK;
+}
+
+int chd_proposal_get_input_transform(const chd_proposal *p, float *offset, float *scale) {
+    if (!p || !offset || !scale) return fail(CHD_ERR_ARGUMENT, "proposal_get_input_transform: bad argument");
+    *offset = p->input_offset;
+    *scale = p->input_scale;
     return CHD_OK;
 }
 
@@ -2095,7 +2104,7 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
         pitches[1 + k] = v.plane_pitch;
     }
     CHD_HIP(launch_quarter_input(st, planes, pitches, ref.first, ref.second, p->width, p->height,
-                                 p->act[0]));
+                                 p->input_offset, p->input_scale, p->act[0]));
     CHD_CHECK(proposal_forward(p, 1));
     CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
     CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,

@@ -356,6 +356,7 @@ struct DevPlane {
 struct DevWarp {
     const float2 *points;
     uint32_t step, columns, rows;
+    float inverse_step;  // 1 / step when step is a power of two (then exact), else 0
 };
 
 // Plane::sample (image.rs:451-466) on a padded row-major plane.
@@ -416,7 +417,11 @@ __device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc
 
 // Warp::map (align.rs:84-111)
 __device__ __forceinline__ bool warp_map(const DevWarp &w, float x, float y, float2 &out) {
-    float fx = x / (float)w.step, fy = y / (float)w.step;
+    // x / step; a power-of-two step divides exactly as a multiplication by
+    // its reciprocal (both round the same exact quotient)
+    const bool scaled = w.inverse_step != 0.0f;
+    float fx = scaled ? x * w.inverse_step : x / (float)w.step;
+    float fy = scaled ? y * w.inverse_step : y / (float)w.step;
     if (fx < 0.0f || fy < 0.0f) return false;
     uint32_t c0 = min(sat_u32(floorf(fx)), w.columns - 1);
     uint32_t r0 = min(sat_u32(floorf(fy)), w.rows - 1);

@@ -150,6 +150,7 @@ __device__ __forceinline__ DevWarp view_warp(const ViewDev &v) {
     w.step = v.warp_step;
     w.columns = v.warp_columns;
     w.rows = v.warp_rows;
+    w.inverse_step = w.step != 0 && (w.step & (w.step - 1)) == 0 ? 1.0f / (float)w.step : 0.0f;
     return w;
 }
 
@@ -549,9 +550,8 @@ __device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &st
 // The band form of cost_tile: each node sweeps label 0 (the global warp)
 // and its window lo .. hi only (window_lo/hi; a full window is 1 .. L - 1),
 // as proposed by the learned band proposal. Scores and costs of the labels
-// outside the window are left as filled beforehand (NaN, +inf), so the SGM
-// that follows never selects them; inside the window every value is the
-// full sweep's. One slab; items are (node, label) over the nodes' windows.
+// outside the window are written as NaN and +inf, so the SGM that follows
+// never selects them; inside the window every value is the full sweep's. One slab; items are (node, label) over the nodes' windows.
 // The tap positions are held as f32 (half the LDS of the full sweep's f64
 // layout), which keeps four blocks per CU at 12 nodes per item.
 __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *lds,
@@ -576,15 +576,18 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
         node_tested[n] = 0;
         lo[n] = g < nodes ? J.window_lo[g] : 1;
     }
-    if (tid == 0) {
-        *evaluation_count = 0;
-        uint32_t b = 0;
-        for (uint32_t n = 0; n < nb; n++) {
-            const uint32_t g = node0 + n;
-            base[n] = b;
-            if (g < nodes) b += 1 + (uint32_t)J.window_hi[g] + 1 - J.window_lo[g];
+    if (tid < 64) {
+        // wavefront 0: exclusive prefix of the nodes' item counts (nb <= 64)
+        const uint32_t g = node0 + tid;
+        const uint32_t c = tid < nb && g < nodes ? 2u + J.window_hi[g] - J.window_lo[g] : 0u;
+        uint32_t incl = c;
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (tid >= o) incl += v;
         }
-        base[nb] = b;
+        if (tid < nb) base[tid] = incl - c;
+        if (tid + 1 == nb) base[nb] = incl;
+        if (tid == 0) *evaluation_count = 0;
     }
     __syncthreads();
     patch_nodes(S, P, nb, T, centre_value);
@@ -634,6 +637,14 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
         }
         J.scores[(size_t)g * L + l] = score;
         J.costs[(size_t)g * L + l] = cost;
+    }
+    // labels outside the windows: score None, cost +inf
+    for (uint32_t i = tid; i < nb * L; i += nth) {
+        const uint32_t n = i / L, l = i % L, g = node0 + n;
+        if (g >= nodes || l == 0 || (l >= lo[n] && l < lo[n] + (base[n + 1] - base[n]) - 1))
+            continue;
+        J.scores[(size_t)g * L + l] = nan_f();
+        J.costs[(size_t)g * L + l] = __int_as_float(0x7f800000);
     }
     __syncthreads();
     for (uint32_t n = tid; n < nb; n += nth) {

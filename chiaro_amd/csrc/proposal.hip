@@ -311,7 +311,7 @@ __global__ void k_weights_bf16(ConvLayer L, const float *__restrict__ w, uint16_
 __global__ void k_quarter_input(const float *__restrict__ p0, const float *__restrict__ p1,
                                 const float *__restrict__ p2, uint32_t pitch0, uint32_t pitch1,
                                 uint32_t pitch2, uint32_t plane_w, uint32_t plane_h, uint32_t w,
-                                uint32_t h, uint16_t *__restrict__ out) {
+                                uint32_t h, float offset, float scale, uint16_t *__restrict__ out) {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w || y >= h) return;
     const float *planes[3] = {p0, p1, p2};
@@ -324,9 +324,9 @@ __global__ void k_quarter_input(const float *__restrict__ p0, const float *__res
             const uint32_t y0 = min(2 * y, plane_h - 1), y1 = min(2 * y + 1, plane_h - 1);
             const float *p = planes[c];
             const uint32_t pt = pitches[c];
-            v = 0.125f * 0.25f *
-                (p[(size_t)y0 * pt + x0] + p[(size_t)y0 * pt + x1] + p[(size_t)y1 * pt + x0] +
-                 p[(size_t)y1 * pt + x1]);
+            const float mean = 0.25f * (p[(size_t)y0 * pt + x0] + p[(size_t)y0 * pt + x1] +
+                                        p[(size_t)y1 * pt + x0] + p[(size_t)y1 * pt + x1]);
+            v = (mean - offset) * scale;
         }
         o[c] = f2bf(v);
     }
@@ -467,10 +467,11 @@ hipError_t launch_weights_bf16(hipStream_t s, const ConvLayer &L, const float *w
 
 hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
                                 const uint32_t pitches[3], uint32_t plane_w, uint32_t plane_h,
-                                uint32_t w, uint32_t h, uint16_t *out) {
+                                uint32_t w, uint32_t h, float offset, float scale,
+                                uint16_t *out) {
     hipLaunchKernelGGL(k_quarter_input, dim3((w + 127) / 128, h), dim3(128), 0, s, planes[0],
                        planes[1], planes[2], pitches[0], pitches[1], pitches[2], plane_w, plane_h,
-                       w, h, out);
+                       w, h, offset, scale, out);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ hipError_t launch_weights_bf16(hipStream_t s, const ConvLayer &L, const float *w
                                uint16_t *wt);
 hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
                                 const uint32_t pitches[3], uint32_t plane_w, uint32_t plane_h,
-                                uint32_t w, uint32_t h, uint16_t *out);
+                                uint32_t w, uint32_t h, float offset, float scale, uint16_t *out);
 hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
                                float threshold, uint32_t half, uint32_t max_half, uint32_t tile,

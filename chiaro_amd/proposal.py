@@ -33,20 +33,30 @@ def from_bf16(h: np.ndarray) -> np.ndarray:
     return (h.astype(np.uint32) << 16).view(np.float32)
 
 
-def quarter_input(planes) -> np.ndarray:
-    """(h, w, 16) BF16 network input from up to three half-resolution planes
-    (reference first): 2x2 means scaled by 1/8, as the device computes it."""
+def quarter_means(planes) -> np.ndarray:
+    """(h, w, len(planes)) f32 2x2 means of up to three half-resolution
+    planes, summed in the device's order."""
     ph, pw = planes[0].shape
     h, w = ph // 2, pw // 2
-    out = np.zeros((h, w, 16), np.float32)
     ys0 = np.minimum(2 * np.arange(h), ph - 1)
     ys1 = np.minimum(2 * np.arange(h) + 1, ph - 1)
     xs0 = np.minimum(2 * np.arange(w), pw - 1)
     xs1 = np.minimum(2 * np.arange(w) + 1, pw - 1)
+    out = np.zeros((h, w, min(len(planes), 3)), np.float32)
     for c, p in enumerate(planes[:3]):
         p = np.asarray(p, np.float32)
         s = ((p[np.ix_(ys0, xs0)] + p[np.ix_(ys0, xs1)]) + p[np.ix_(ys1, xs0)]) + p[np.ix_(ys1, xs1)]
-        out[:, :, c] = np.float32(0.125 * 0.25) * s
+        out[:, :, c] = np.float32(0.25) * s
+    return out
+
+
+def quarter_input(planes, offset: float = 0.0, scale: float = 0.125) -> np.ndarray:
+    """(h, w, 16) BF16 network input from up to three half-resolution planes
+    (reference first): (2x2 mean - offset) * scale in f32, as the device
+    computes it (k_quarter_input)."""
+    m = quarter_means(planes)
+    out = np.zeros(m.shape[:2] + (16,), np.float32)
+    out[:, :, :m.shape[2]] = (m - np.float32(offset)) * np.float32(scale)
     return to_bf16(out)
 
 
@@ -227,6 +237,17 @@ class Proposal:
         n = C.c_size_t()
         _abi.check(self.lib.chd_proposal_parameters(handle, None, None, C.byref(n)), "parameters")
         self.parameter_count = n.value
+        self.input_offset, self.input_scale = 0.0, 0.125
+
+    def set_input_transform(self, offset: float, scale: float):
+        """Inputs are (2x2 mean - offset) * scale; capture_inputs must use the
+        same pair for the data given to this network."""
+        _abi.check(self.lib.chd_proposal_set_input_transform(self.handle, offset, scale),
+                   "input transform")
+        o, s = C.c_float(), C.c_float()
+        _abi.check(self.lib.chd_proposal_get_input_transform(self.handle, C.byref(o), C.byref(s)),
+                   "input transform")
+        self.input_offset, self.input_scale = o.value, s.value
 
     def close(self):
         if self.handle:
@@ -317,11 +338,25 @@ def clear(session: Session):
 
 # ---------------------------------------------------------------- data
 
-def capture_inputs(cap) -> np.ndarray:
+def capture_inputs(cap, offset: float = 0.0, scale: float = 0.125) -> np.ndarray:
     """Network input of a capture: reference and first two targets."""
     planes = [cap.modules[cap.reference_index].luminance]
     planes += [m.luminance for i, m in enumerate(cap.modules) if i != cap.reference_index][:2]
-    return quarter_input(planes)
+    return quarter_input(planes, offset, scale)
+
+
+def input_transform(caps):
+    """(offset, scale) that standardise the 2x2 means of the captures' input
+    planes: mean 0, standard deviation 1 (the planes hold log luminance with
+    a large common offset, which BF16 would otherwise spend its precision
+    on)."""
+    values = []
+    for cap in caps:
+        planes = [cap.modules[cap.reference_index].luminance]
+        planes += [m.luminance for i, m in enumerate(cap.modules) if i != cap.reference_index][:2]
+        values.append(quarter_means(planes).astype(np.float64).ravel())
+    v = np.concatenate(values)
+    return float(np.float32(v.mean())), float(np.float32(1.0 / max(v.std(), 1e-6)))
 
 
 def full_sweep_labels(ctx: Context, cap, width: int, height: int):
@@ -339,12 +374,13 @@ def full_sweep_labels(ctx: Context, cap, width: int, height: int):
         sess.close()
 
 
-def build_dataset(ctx: Context, seeds, make_capture):
-    """(inputs, labels) for the given seeds; make_capture(seed) -> Capture."""
+def build_dataset(ctx: Context, seeds, make_capture, offset: float = 0.0, scale: float = 0.125):
+    """(inputs, labels) for the given seeds; make_capture(seed) -> Capture;
+    inputs through the input transform (offset, scale)."""
     xs, ys = [], []
     for s in seeds:
         cap = make_capture(s)
-        x = capture_inputs(cap)
+        x = capture_inputs(cap, offset, scale)
         h, w = x.shape[:2]
         xs.append(x)
         ys.append(full_sweep_labels(ctx, cap, w, h))
@@ -398,7 +434,7 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
     # window (then the difference comes from SGM paths crossing windows) or
     # outside it (the centre missed)?
     g = sess.grid
-    pred = proposal.predict(capture_inputs(cap)[None])[0]
+    pred = proposal.predict(capture_inputs(cap, proposal.input_offset, proposal.input_scale)[None])[0]
     rows, cols = g.coarse_rows, g.coarse_columns
     r = np.minimum(np.arange(rows), pred.shape[0] - 1)
     c = np.minimum(np.arange(cols), pred.shape[1] - 1)

@@ -348,8 +348,8 @@ void chd_batch_destroy(chd_batch *batch);
  * every hypothesis): a 7-layer 3x3 convolutional encoder-decoder with BF16
  * weights and activations and FP32 accumulation on the matrix cores. Its
  * input is width x height x 16 BF16 (channels 0..2: the reference and the
- * first two target planes at half the plane resolution, scaled by 1/8; the
- * rest zero); per pixel it predicts a centre label, a log band half-width and
+ * first two target planes at half the plane resolution, through the input
+ * transform below; the rest zero); per pixel it predicts a centre label, a log band half-width and
  * a confidence. Trained with Adam on full-sweep WTA label maps (0 = none). */
 typedef struct chd_proposal chd_proposal;
 int chd_proposal_create(chd_context *context, uint32_t width, uint32_t height,
@@ -360,6 +360,12 @@ void chd_proposal_destroy(chd_proposal *proposal);
  * *count must equal it and get receives or set replaces them. */
 int chd_proposal_parameters(chd_proposal *proposal, float *get, const float *set,
                             size_t *count);
+/* The input transform: each input value is (2x2 mean of a plane - offset) *
+ * scale in f32, rounded to BF16 (default offset 0, scale 1/8). Inputs given
+ * to set_dataset / predict / gradients must have been made with the same
+ * transform; set_proposal applies it on the device. */
+int chd_proposal_set_input_transform(chd_proposal *proposal, float offset, float scale);
+int chd_proposal_get_input_transform(const chd_proposal *proposal, float *offset, float *scale);
 /* Training set (host arrays): inputs count x height x width x 16 BF16 bits,
  * labels count x height x width. */
 int chd_proposal_set_dataset(chd_proposal *proposal, const uint16_t *inputs,

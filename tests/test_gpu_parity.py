@@ -195,14 +195,14 @@ def _camera_variants(oracle_lib, cap):
     return cams
 
 
-def _oracle_warp(oracle_lib, ref_cam, cam, w, h):
-    columns, rows = -(-w // 32) + 1, -(-h // 32) + 1
+def _oracle_warp(oracle_lib, ref_cam, cam, w, h, step=32):
+    columns, rows = -(-w // step) + 1, -(-h // step) + 1
     pts = np.empty((rows, columns, 2), np.float32)
     conf = np.empty((rows, columns), np.float32)
     wp = _abi.Warp()
     wp.points = pts.ctypes.data_as(C.POINTER(C.c_float))
     wp.confidence = conf.ctypes.data_as(C.POINTER(C.c_float))
-    oracle_lib.or_global_warp(C.byref(ref_cam), C.byref(cam), w, h, 32, C.byref(wp))
+    oracle_lib.or_global_warp(C.byref(ref_cam), C.byref(cam), w, h, step, C.byref(wp))
     return pts
 
 
@@ -220,6 +220,19 @@ def test_camera_model_branches_bit_exact(gpu_context, oracle_lib):
                                       warp_step=32))
     cap.modules = mods
     _full_parity(gpu_context, cap, "distortion / mirror / orientation cameras")
+
+
+def test_warp_steps_bit_exact(gpu_context, oracle_lib):
+    """Global warps on grids of 24 and 16 px (Warp::map divides by the step:
+    the kernels multiply by the reciprocal only for power-of-two steps)."""
+    for step in (24, 16):
+        cap = _small_config0(7)
+        ref = cap.modules[0]
+        for m in cap.modules[1:]:
+            m.warp_points = _oracle_warp(oracle_lib, ref.camera, m.camera, ref.width, ref.height,
+                                         step)
+            m.warp_step = step
+        _full_parity(gpu_context, cap, f"global warps with a {step} px grid")
 
 
 def test_skipped_views_and_undefined_warps_bit_exact(gpu_context):

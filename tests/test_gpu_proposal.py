@@ -10,8 +10,8 @@ import pytest
 from chiaro_amd import _abi, scene
 from chiaro_amd.depth import Session
 from chiaro_amd.proposal import (Proposal, ReferenceNet, adam_reference, agreement,
-                                 build_dataset, clear, constant_proposal, from_bf16,
-                                 layer_specs, to_bf16)
+                                 build_dataset, capture_inputs, clear, constant_proposal,
+                                 from_bf16, input_transform, layer_specs, to_bf16)
 
 from parity import assert_bitwise
 
@@ -117,10 +117,12 @@ def test_training_and_band_sweep(gpu_context, oracle_lib):
     volume holds +inf outside each node's window and the windowed SGM equals
     the oracle's full SGM over that volume bit for bit."""
     _say("building 12 small config3 scenes")
-    x, y = build_dataset(gpu_context, range(12), _cap)
+    offset, scale = input_transform([_cap(s) for s in range(4)])
+    x, y = build_dataset(gpu_context, range(12), _cap, offset, scale)
     h, w = x.shape[1:3]
     net = Proposal(gpu_context, w, h, max_batch=4, seed=7)
     try:
+        net.set_input_transform(offset, scale)
         net.set_dataset(x, y)
         losses = net.train(epochs=8, batch=4, lr=1e-3, seed=1)
         _say(f"losses {['%.3f' % l for l in losses]}")
@@ -148,6 +150,21 @@ def test_training_and_band_sweep(gpu_context, oracle_lib):
             _, costs, guidance, _ = full.cost_volume()
             swept = np.isfinite(costs).sum(axis=1)
             assert swept.max() <= most and swept.min() >= 1
+            if max_half == 0:
+                # the device builds the network input from the session's planes
+                # exactly as capture_inputs does on the host: the windows equal
+                # the ones the host-side prediction gives
+                g = full.grid
+                pred = net.predict(capture_inputs(cap, offset, scale)[None])[0]
+                r = np.minimum(np.arange(g.coarse_rows), pred.shape[0] - 1)
+                c = np.minimum(np.arange(g.coarse_columns), pred.shape[1] - 1)
+                centre = np.clip(np.rint(pred[np.ix_(r, c)][..., 0]), 1, g.labels - 1)
+                want_lo = np.maximum(1, centre - 8).reshape(-1)
+                want_hi = np.minimum(g.labels - 1, centre + 8).reshape(-1)
+                fin = np.isfinite(costs[:, 1:])
+                got_lo = 1 + np.argmax(fin, axis=1)
+                got_hi = g.labels - 1 - np.argmax(fin[:, ::-1], axis=1)
+                assert np.array_equal(got_lo, want_lo) and np.array_equal(got_hi, want_hi)
             g = full.grid
             want = np.empty_like(costs)
             oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
@@ -168,13 +185,15 @@ def test_ablation_network_beats_constant_band(gpu_context):
     on the training set's median label. The network must cover the full
     sweep's label, and reproduce it within +-1, far more often."""
     _say("building 1000 small config3 scenes")
-    x, y = build_dataset(gpu_context, range(1000, 2000), _cap)
+    offset, scale = input_transform([_cap(s) for s in range(1000, 1008)])
+    x, y = build_dataset(gpu_context, range(1000, 2000), _cap, offset, scale)
     h, w = x.shape[1:3]
-    net = Proposal(gpu_context, w, h, max_batch=32, seed=12)
+    net = Proposal(gpu_context, w, h, max_batch=32, seed=11)
     const = constant_proposal(gpu_context, w, h, float(np.median(y[y > 0])))
     try:
+        net.set_input_transform(offset, scale)
         net.set_dataset(x, y)
-        losses = net.train(epochs=60, batch=32, lr=1e-3, seed=12)
+        losses = net.train(epochs=60, batch=32, lr=1e-3, seed=11)
         _say(f"losses {losses[0]:.3f} -> {losses[-1]:.3f}")
         with_net, without = [], []
         for seed in (3000, 3001, 3002):
