@@ -1695,7 +1695,8 @@ void chd_batch_destroy(chd_batch *b) {
 struct chd_proposal {
     chd_context *context = nullptr;
     uint32_t width = 0, height = 0, max_batch = 0;
-    float input_offset = 0.0f, input_scale = 0.125f;  // input = (2x2 mean - offset) * scale
+    float input_offset = 0.0f, input_scale = 0.125f;  // input = (cell mean - offset) * scale
+    uint32_t label_count = 129;  // centre label = (L - 1) / 2 + (L - 1) / 8 * o0
     ConvLayer layers[7];
     struct Param {
         float *w, *b, *mw, *vw, *mb, *vb, *gw, *gb;
@@ -1785,8 +1786,8 @@ int proposal_backward(chd_proposal *p, uint32_t batch) {
         CHD_HIP(launch_fill_u32(st, (uint32_t *)p->params[i].gw, 0u, (size_t)L.k * L.cout_p));
         CHD_HIP(launch_fill_u32(st, (uint32_t *)p->params[i].gb, 0u, L.cout_p));
     }
-    CHD_HIP(launch_proposal_loss(st, p->out, p->labels, M, last.cout_p, (float)M, p->gout,
-                                 p->totals));
+    CHD_HIP(launch_proposal_loss(st, p->out, p->labels, M, last.cout_p, p->label_count, (float)M,
+                                 p->gout, p->totals));
     for (int i = 6; i >= 0; i--) {
         const ConvLayer &L = p->layers[i];
         const size_t n = layer_out(L, batch);
@@ -1943,6 +1944,13 @@ int chd_proposal_set_input_transform(chd_proposal *p, float offset, float scale)
     return CHD_OK;
 }
 
+int chd_proposal_set_label_count(chd_proposal *p, uint32_t label_count) {
+    if (!p || label_count < 3 || label_count > 512)
+        return fail(CHD_ERR_ARGUMENT, "proposal_set_label_count: bad argument");
+    p->label_count = label_count;
+    return CHD_OK;
+}
+
 int chd_proposal_get_input_transform(const chd_proposal *p, float *offset, float *scale) {
     if (!p || !offset || !scale) return fail(CHD_ERR_ARGUMENT, "proposal_get_input_transform: bad argument");
     *offset = p->input_offset;
@@ -2023,7 +2031,8 @@ int chd_proposal_predict(chd_proposal *p, const uint16_t *inputs, uint32_t count
         CHD_HIP(hipStreamSynchronize(st));
         for (size_t i = 0; i < n * pixels; i++) {
             float *o = outputs + ((size_t)first * pixels + i) * 3;
-            o[0] = 64.0f + 16.0f * raw[i * 16];
+            o[0] = (float)(p->label_count - 1) * 0.5f +
+                   (float)(p->label_count - 1) * 0.125f * raw[i * 16];
             o[1] = raw[i * 16 + 1];
             o[2] = 1.0f / (1.0f + std::exp(-raw[i * 16 + 2]));
         }
@@ -2083,8 +2092,13 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
     if (s->context != p->context) return fail(CHD_ERR_ARGUMENT, "set_proposal: another context");
     if (s->active.empty()) return fail(CHD_ERR_STATE, "set_proposal: no target view");
     const auto ref = s->plane_sizes[s->reference];
-    if (ref.first / 2 != p->width || ref.second / 2 != p->height)
-        return fail(CHD_ERR_ARGUMENT, "set_proposal: the network's grid is not the reference plane at half size");
+    // one network pixel per coarse node: cells of step / 2 plane pixels
+    const uint32_t factor = s->grid.coarse_step / 2;
+    if (s->grid.coarse_step % 2 != 0 || ref.first / factor != p->width ||
+        ref.second / factor != p->height)
+        return fail(CHD_ERR_ARGUMENT, "set_proposal: the network's grid is not the reference plane in cells of half the grid step");
+    if (s->grid.labels != p->label_count)
+        return fail(CHD_ERR_ARGUMENT, "set_proposal: the network's label count differs from the session's");
     CHD_HIP(hipSetDevice(s->context->device));
     hipStream_t st = s->context->stream;
     const size_t nodes = (size_t)s->grid.coarse_columns * s->grid.coarse_rows;
@@ -2104,7 +2118,7 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
         pitches[1 + k] = v.plane_pitch;
     }
     CHD_HIP(launch_quarter_input(st, planes, pitches, ref.first, ref.second, p->width, p->height,
-                                 p->input_offset, p->input_scale, p->act[0]));
+                                 factor, p->input_offset, p->input_scale, p->act[0]));
     CHD_CHECK(proposal_forward(p, 1));
     CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
     CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,

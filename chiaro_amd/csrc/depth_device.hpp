@@ -393,6 +393,36 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const DevPlane &p) 
 // or, on a transposed plane, (y0, y0 + 1) of columns x0 and x0 + 1. The
 // interpolation is the same either way. For x >= 0, truncation is floor and
 // x - floor(x) is exact, which is what the hardware fract computes.
+// The two halves of plane_sample_inside, for callers that issue the loads of
+// several samples before interpolating any of them.
+struct PendingSample {
+    float2 a, b;
+    float tx, ty;
+};
+template <bool Transposed>
+__device__ __forceinline__ PendingSample plane_sample_issue(__amdgpu_buffer_rsrc_t rsrc,
+                                                           uint32_t pitch, uint32_t line_bytes,
+                                                           float x, float y) {
+    const uint32_t line = Transposed ? (uint32_t)x : (uint32_t)y;
+    const uint32_t along = Transposed ? (uint32_t)y : (uint32_t)x;
+    const uint32_t offset = (__umul24(line, pitch) + along) * 4u;
+    PendingSample p;
+    p.a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
+    p.b = __builtin_bit_cast(float2,
+                             __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, line_bytes, 0));
+    p.tx = __builtin_amdgcn_fractf(x);
+    p.ty = __builtin_amdgcn_fractf(y);
+    return p;
+}
+template <bool Transposed>
+__device__ __forceinline__ float plane_sample_finish(const PendingSample &p) {
+    const float p00 = p.a.x, p10 = Transposed ? p.b.x : p.a.y;
+    const float p01 = Transposed ? p.a.y : p.b.x, p11 = p.b.y;
+    const float top = p00 * (1.0f - p.tx) + p10 * p.tx;
+    const float bottom = p01 * (1.0f - p.tx) + p11 * p.tx;
+    return top * (1.0f - p.ty) + bottom * p.ty;
+}
+
 template <bool Transposed>
 __device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc, uint32_t pitch,
                                                      uint32_t line_bytes, float x, float y) {

@@ -64,6 +64,7 @@ struct CostJob {
     uint32_t slab;           // labels per slab
     uint32_t tiles, slabs;
     uint32_t minimum_support, best_view_count;
+    uint32_t topk;           // views aggregated as a running top-K (set by cost_job_shape)
     float *scores, *costs, *guidance;
     uint8_t *tested;         // cleared before the launch
     float *view_scores;      // optional [view slot][node][label] per-view ZNCC, NaN = None

@@ -56,7 +56,7 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
     const uint32_t rv = v < kRayViews ? (v > 0 ? v : 1u) : kRayViews;
     l.ray_views = rv;
     uint32_t o = 0;
-    l.cams = o; o = align16(o + v * (uint32_t)sizeof(chd_camera));
+    l.cams = o; o = align16(o + (uint32_t)sizeof(chd_camera));  // the current view's
     l.pos = o; o = align16(o + nb * 16u);
     l.ray = o; o = align16(o + rv * nb * 6u * 8u);
     l.ext = o; o = align16(o + nb * 16u);
@@ -75,7 +75,7 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
 }
 
 struct PatchLds {
-    chd_camera *cams;  // cameras of the scored views, in view-slot order
+    chd_camera *cam;   // camera of the current view (copied by patch_view)
     double2 *pos;      // raster position of each node
     // ray, gc, nv_ok, gc_ok, ray_ok: [ray group slot][node] in the layout;
     // patch_view returns a copy pointing at the current view's slot
@@ -102,7 +102,7 @@ __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l,
     PatchLds p;
     p.f64_taps = f64_taps;
     p.gqf = (float2 *)(base + l.gq);
-    p.cams = (chd_camera *)(base + l.cams);
+    p.cam = (chd_camera *)(base + l.cams);
     p.pos = (double2 *)(base + l.pos);
     p.ray = (double *)(base + l.ray);
     p.ext = (float4 *)(base + l.ext);
@@ -204,8 +204,8 @@ __device__ void patch_rays(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
 // exists) are set and visible. On exit node_ok[n] is 0 when the centre or any
 // tap of the reference patch falls outside the reference plane (every ZNCC of
 // the node is then None), centre_value[n] holds reference_guidance
-// (depth.rs:1824-1832), the cameras of all scored views are in LDS, and the
-// first ray group (views 0 .. ray_views - 1) is computed.
+// (depth.rs:1824-1832), and the first ray group (views 0 .. ray_views - 1)
+// is computed.
 __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             float *centre_value) {
     const uint32_t V = S.view_count, tid = threadIdx.x, nth = blockDim.x;
@@ -213,11 +213,6 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
     const uint32_t width = 2u * S.radius + 1u;
     const DevPlane rp = view_plane(S.views[S.reference], false);
     const float sigma = rmaxf((float)S.radius * 0.75f, 1.0f);
-    constexpr uint32_t words = sizeof(chd_camera) / 4;
-    for (uint32_t i = tid; i < V * words; i += nth) {
-        const uint32_t *src = (const uint32_t *)S.views[S.vlist[i / words]].camera;
-        ((uint32_t *)P.cams)[i] = src[i % words];
-    }
     if (V > 0) patch_rays(S, P, nb, 0, V < P.ray_views ? V : P.ray_views);
     for (uint32_t t = tid; t < T; t += nth) {
         const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
@@ -276,14 +271,16 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
 
 // View half of the patch context for view slot `a`; returns the context with
 // the per-view pointers at a's ray-group slot. Must be entered by the whole
-// block after every read of the previous view's context has finished.
+// block after every read of the previous view's context has finished. The
+// first ray group comes from patch_nodes unless first_group_ready is false
+// (a second pass over the views).
 __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
-                               uint32_t a) {
+                               uint32_t a, bool first_group_ready = true) {
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const int r = (int)S.radius;
     const uint32_t width = 2u * S.radius + 1u;
     const uint32_t V = S.view_count, R = P.ray_views, slot = a % R;
-    if (a > 0 && slot == 0) {
+    if (slot == 0 && (a > 0 || !first_group_ready)) {
         patch_rays(S, P, nb, a, V - a < R ? V - a : R);
         __syncthreads();
     }
@@ -294,6 +291,11 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
     Q.gc_ok = P.gc_ok + slot * nb;
     Q.ray_ok = P.ray_ok + slot * nb;
     const DevWarp gw = view_warp(S.views[S.vlist[a]]);
+    {  // the view's camera into LDS (the previous view's readers are done)
+        constexpr uint32_t words = sizeof(chd_camera) / 4;
+        const uint32_t *src = (const uint32_t *)S.views[S.vlist[a]].camera;
+        for (uint32_t i = tid; i < words; i += nth) ((uint32_t *)P.cam)[i] = src[i];
+    }
     // G lanes per node (a power of two dividing the wavefront) compute the
     // node's tap positions and reduce their min/max
     const uint32_t G = nb <= 4 ? 64u : nb <= 8 ? 32u : 16u;
@@ -337,34 +339,48 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
 }
 
 // The three target sums of warped_patch_zncc over the taps in the
-// reference's dy-major order, each in its own f32 accumulator.
+// reference's dy-major order, each in its own f32 accumulator. Taps go in
+// batches of nine: all eighteen plane loads of a batch are issued before the
+// first interpolation (the sums still run in tap order).
 template <bool Transposed, bool F64Taps>
 __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
                                          double dx, double dy, float &st, float &stt, float &sp) {
+    constexpr uint32_t B = 9;
     const float2 *wr = P.wr + (size_t)n * T;
     const double2 *gq = P.gq + (size_t)n * T;
     const float2 *gqf = P.gqf + (size_t)n * T;
-#pragma unroll 9
-    for (uint32_t t = 0; t < T; t++) {
-        const float2 w = wr[t];
-        double qx, qy;
-        if constexpr (F64Taps) {
-            const double2 q = gq[t];
-            qx = q.x;
-            qy = q.y;
-        } else {
-            const float2 q = gqf[t];
-            qx = (double)q.x;
-            qy = (double)q.y;
+    for (uint32_t t0 = 0; t0 < T; t0 += B) {
+        PendingSample ps[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            const uint32_t t = t0 + k < T ? t0 + k : T - 1;
+            double qx, qy;
+            if constexpr (F64Taps) {
+                const double2 q = gq[t];
+                qx = q.x;
+                qy = q.y;
+            } else {
+                const float2 q = gqf[t];
+                qx = (double)q.x;
+                qy = (double)q.y;
+            }
+            ps[k] = plane_sample_issue<Transposed>(rsrc, pitch, line_bytes, to_plane(qx + dx),
+                                                   to_plane(qy + dy));
         }
-        const float x = to_plane(qx + dx);
-        const float y = to_plane(qy + dy);
-        const float tv = plane_sample_inside<Transposed>(rsrc, pitch, line_bytes, x, y);
-        const float wv = w.x * tv;
-        st += wv;
-        stt += wv * tv;
-        sp += w.y * tv;
+        // keep the batch's loads ahead of its interpolations (the scheduler
+        // would otherwise sink each load next to its use)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            if (t0 + k >= T) break;
+            const float2 w = wr[t0 + k];
+            const float tv = plane_sample_finish<Transposed>(ps[k]);
+            const float wv = w.x * tv;
+            st += wv;
+            stt += wv * tv;
+            sp += w.y * tv;
+        }
     }
 }
 
@@ -447,6 +463,44 @@ __device__ __forceinline__ bool aggregate_scores(const float *res, uint32_t stri
     return true;
 }
 
+// aggregate over the views as a running top-K, for kernels whose view loop
+// is outermost: slot[0] holds the count of scored views (as bits), slot[1..K]
+// the K largest scores so far in descending total_cmp order (an equal score
+// stays behind the earlier view's, as the reference's first-maximum pick
+// has it; equal keys are equal bits, so the order among them cannot change
+// the sum). With K = min(best_view_count, V) the sum below takes the same
+// values in the same order as aggregate_scores.
+__device__ __forceinline__ void topk_insert(float *slot, uint32_t K, float s) {
+    if (s != s) return;
+    const uint32_t c = __float_as_uint(slot[0]);
+    const int32_t key = tkey32(s);
+    float *v = slot + 1;
+    uint32_t j = c < K ? c : K;
+    while (j > 0 && tkey32(v[j - 1]) < key) {
+        if (j < K) v[j] = v[j - 1];
+        j--;
+    }
+    if (j < K) v[j] = s;
+    slot[0] = __uint_as_float(c + 1);
+}
+
+__device__ __forceinline__ bool topk_aggregate(const float *slot, uint32_t minimum_support,
+                                               uint32_t best_view_count, float &out) {
+    const uint32_t count = __float_as_uint(slot[0]);
+    if (count < minimum_support) return false;
+    const uint32_t k = best_view_count < count ? best_view_count : count;
+    if (k < minimum_support) return false;
+    float sum = -0.0f;
+    for (uint32_t i = 0; i < k; i++) sum += slot[1 + i];
+    out = sum / (float)k;
+    return true;
+}
+
+__host__ __device__ __forceinline__ uint32_t topk_slots(uint32_t best_view_count, uint32_t V) {
+    const uint32_t k = best_view_count < V ? best_view_count : V;
+    return 1 + (k > 0 ? k : 1);
+}
+
 __device__ __forceinline__ float nan_f() { return __int_as_float(0x7fc00000); }
 
 __device__ __forceinline__ double2 node_position(const GridSpec &G, uint32_t g) {
@@ -462,6 +516,9 @@ __device__ __forceinline__ double2 node_position(const GridSpec &G, uint32_t g) 
 // limits which labels this item writes, so any tiling and slab choice gives
 // the same volume. `tested` must be cleared beforehand: an item sets it to 1
 // for the nodes with a finite score among its labels.
+// TopK: the views' scores of an item are kept as a running top-K (when K <
+// V this takes less LDS); else all V are kept and aggregate_scores picks.
+template <bool TopK>
 __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsigned char *lds,
                           unsigned int *evaluation_count) {
     const PatchSpec &S = J.S;
@@ -469,8 +526,9 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
     const uint32_t l0 = slab * J.slab, Ls = min(J.slab, L - l0);
     const PatchLayout lay = patch_layout(nb, V, T, true);
     const PatchLds P = bind_patch(lds, lay, true);
-    float *res = (float *)(lds + lay.total);          // [nb][V][Ls]
-    float *centre_value = res + (size_t)nb * V * Ls;  // [nb]
+    const uint32_t KS = TopK ? topk_slots(J.best_view_count, V) : V;
+    float *res = (float *)(lds + lay.total);  // [nb * Ls][KS] running top-K, or [V][nb * Ls]
+    float *centre_value = res + (size_t)nb * Ls * KS;  // [nb]
     int *node_tested = (int *)(centre_value + nb);    // [nb]
     const uint32_t nodes = J.G.columns * J.G.rows, node0 = tile * nb;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
@@ -481,6 +539,9 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
         node_tested[n] = 0;
     }
     if (tid == 0) *evaluation_count = 0;
+    // each item's slot is only touched by the thread that owns the item
+    if (TopK)
+        for (uint32_t i = tid; i < nb * Ls; i += nth) res[(size_t)i * KS] = __uint_as_float(0u);
     __syncthreads();
     patch_nodes(S, P, nb, T, centre_value);
 
@@ -495,13 +556,16 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
             if (Q.node_ok[n] && Q.nv_ok[n]) {
                 double dx = 0.0, dy = 0.0;
                 bool ok = true;
-                if (l > 0) ok = delta_at_depth(Q, n, Q.cams[a], J.labels[l], dx, dy);
+                if (l > 0) ok = delta_at_depth(Q, n, *Q.cam, J.labels[l], dx, dy);
                 float sc;
                 if (ok && patch_zncc<true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
                                            evaluation_count))
                     s = sc;
             }
-            res[((size_t)n * V + a) * Ls + (l - l0)] = s;
+            if (TopK)
+                topk_insert(res + (size_t)i * KS, KS - 1, s);
+            else
+                res[(size_t)a * nb * Ls + i] = s;
             if (J.view_scores && node0 + n < nodes)
                 J.view_scores[((size_t)a * nodes + node0 + n) * L + l] = s;
         }
@@ -512,9 +576,11 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
         const uint32_t n = i / Ls, l = l0 + i % Ls, g = node0 + n;
         if (g >= nodes) continue;
         float score;
-        if (!aggregate_scores(res + (size_t)n * V * Ls + (l - l0), Ls, V, J.minimum_support,
-                              J.best_view_count, score))
-            score = nan_f();
+        const bool scored =
+            TopK ? topk_aggregate(res + (size_t)i * KS, J.minimum_support, J.best_view_count, score)
+                 : aggregate_scores(res + i, nb * Ls, V, J.minimum_support, J.best_view_count,
+                                    score);
+        if (!scored) score = nan_f();
         float cost;
         if (finite_f(score)) {
             node_tested[n] = 1;
@@ -554,14 +620,17 @@ __device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &st
 // never selects them; inside the window every value is the full sweep's. One slab; items are (node, label) over the nodes' windows.
 // The tap positions are held as f32 (half the LDS of the full sweep's f64
 // layout), which keeps four blocks per CU at 12 nodes per item.
+template <bool TopK>
 __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *lds,
                                unsigned int *evaluation_count) {
     const PatchSpec &S = J.S;
     const uint32_t V = S.view_count, T = J.T, L = J.L, nb = J.nb;
     const PatchLayout lay = patch_layout(nb, V, T, false);
     const PatchLds P = bind_patch(lds, lay, false);
-    float *res = (float *)(lds + lay.total);          // [nb][V][L] (windows are <= L)
-    float *centre_value = res + (size_t)nb * V * L;   // [nb]
+    const uint32_t KS = TopK ? topk_slots(J.best_view_count, V) : V;
+    const uint32_t C = J.slab;                        // items per pass over the views
+    float *res = (float *)(lds + lay.total);          // [C][KS] running top-K, or [V][C]
+    float *centre_value = res + (size_t)C * KS;       // [nb]
     int *node_tested = (int *)(centre_value + nb);    // [nb]
     uint32_t *base = (uint32_t *)(node_tested + nb);  // [nb + 1]
     uint32_t *lo = base + nb + 1;                     // [nb]
@@ -596,47 +665,60 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
     __syncthreads();
     const uint32_t items = base[nb];
 
-    for (uint32_t a = 0; a < V; a++) {
-        const PatchLds Q = patch_view(S, P, nb, T, a);
-        const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
-        const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
-        for (uint32_t i = tid; i < items; i += nth) {
-            const uint32_t n = item_node[i], j = i - base[n];
-            const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
-            float s = nan_f();
-            if (Q.node_ok[n] && Q.nv_ok[n]) {
-                double dx = 0.0, dy = 0.0;
-                bool ok = true;
-                if (l > 0) ok = delta_at_depth(Q, n, Q.cams[a], J.labels[l], dx, dy);
-                float sc;
-                if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
-                                           evaluation_count))
-                    s = sc;
+    // passes of at most C items (one for banded tiles; wide or fallback
+    // windows take several, each rebuilding the view contexts). A slot is
+    // only touched by the thread that owns its item.
+    for (uint32_t c0 = 0; c0 < items; c0 += C) {
+        const uint32_t cn = min(C, items - c0);
+        if (TopK)
+            for (uint32_t i = tid; i < cn; i += nth) res[(size_t)i * KS] = __uint_as_float(0u);
+        for (uint32_t a = 0; a < V; a++) {
+            const PatchLds Q = patch_view(S, P, nb, T, a, c0 == 0);
+            const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
+            const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
+            const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
+            for (uint32_t i = tid; i < cn; i += nth) {
+                const uint32_t it = c0 + i, n = item_node[it], j = it - base[n];
+                const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
+                float s = nan_f();
+                if (Q.node_ok[n] && Q.nv_ok[n]) {
+                    double dx = 0.0, dy = 0.0;
+                    bool ok = true;
+                    if (l > 0) ok = delta_at_depth(Q, n, *Q.cam, J.labels[l], dx, dy);
+                    float sc;
+                    if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                                               evaluation_count))
+                        s = sc;
+                }
+                if (TopK)
+                    topk_insert(res + (size_t)i * KS, KS - 1, s);
+                else
+                    res[(size_t)a * C + i] = s;
             }
-            res[((size_t)n * V + a) * L + j] = s;
+            __syncthreads();
         }
-        __syncthreads();
-    }
-
-    for (uint32_t i = tid; i < items; i += nth) {
-        const uint32_t n = item_node[i], j = i - base[n], g = node0 + n;
-        const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
-        float score;
-        if (!aggregate_scores(res + (size_t)n * V * L + j, L, V, J.minimum_support,
-                              J.best_view_count, score))
-            score = nan_f();
-        float cost;
-        if (finite_f(score)) {
-            node_tested[n] = 1;
-            const float near_prior =
-                l == 0 ? 0.0f : kNearDepthPrior * (float)(l - 1) / (float)(L - 2);
-            cost = rclampf(1.0f - score, 0.0f, 2.0f) + near_prior;
-        } else {
-            cost = kMissingCost;
+        for (uint32_t i = tid; i < cn; i += nth) {
+            const uint32_t it = c0 + i, n = item_node[it], j = it - base[n], g = node0 + n;
+            const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
+            float score;
+            const bool scored =
+                TopK ? topk_aggregate(res + (size_t)i * KS, J.minimum_support, J.best_view_count,
+                                      score)
+                     : aggregate_scores(res + i, C, V, J.minimum_support, J.best_view_count,
+                                        score);
+            if (!scored) score = nan_f();
+            float cost;
+            if (finite_f(score)) {
+                node_tested[n] = 1;
+                const float near_prior =
+                    l == 0 ? 0.0f : kNearDepthPrior * (float)(l - 1) / (float)(L - 2);
+                cost = rclampf(1.0f - score, 0.0f, 2.0f) + near_prior;
+            } else {
+                cost = kMissingCost;
+            }
+            J.scores[(size_t)g * L + l] = score;
+            J.costs[(size_t)g * L + l] = cost;
         }
-        J.scores[(size_t)g * L + l] = score;
-        J.costs[(size_t)g * L + l] = cost;
     }
     // labels outside the windows: score None, cost +inf
     for (uint32_t i = tid; i < nb * L; i += nth) {
@@ -661,7 +743,12 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
 // slabs), or with XCD order item k of XCD b % 8's contiguous run, where k =
 // b / 8 (the hardware hands workgroups to the XCDs round robin, so each L2
 // then serves neighbouring tiles).
-__global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
+// At most four waves per SIMD fit anyway (LDS and the 256-thread blocks), so
+// the tap loop may spend up to 128 VGPRs keeping a step's 18 plane loads in
+// flight (with a higher occupancy target the compiler serialises them).
+template <bool TopK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_cost_volume(CostJob J) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     uint32_t item = blockIdx.x;
@@ -670,10 +757,23 @@ __global__ void __launch_bounds__(256) k_cost_volume(CostJob J) {
         xcd_run(blockIdx.x % kXcds, gridDim.x, start, count);
         item = start + blockIdx.x / kXcds;
     }
-    if (J.window_lo)
-        cost_tile_band(J, item, lds, &evaluation_count);
-    else
-        cost_tile(J, item / J.slabs, item % J.slabs, lds, &evaluation_count);
+    cost_tile<TopK>(J, item / J.slabs, item % J.slabs, lds, &evaluation_count);
+}
+
+// The band sweep's launch (its own kernel, so that the full sweep's register
+// allocation does not carry the band's bookkeeping).
+template <bool TopK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_cost_volume_band(CostJob J) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int evaluation_count;
+    uint32_t item = blockIdx.x;
+    if (J.xcd_order == 1) {
+        uint32_t start, count;
+        xcd_run(blockIdx.x % kXcds, gridDim.x, start, count);
+        item = start + blockIdx.x / kXcds;
+    }
+    cost_tile_band<TopK>(J, item, lds, &evaluation_count);
 }
 
 // Persistent form over several jobs (captures): each block takes the next
@@ -729,18 +829,26 @@ __global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__rest
         while (j + 1 < njobs && first[j + 1] <= item) j++;
         const CostJob J = jobs[j];
         const uint32_t local = item - first[j];
-        if (J.window_lo)
-            cost_tile_band(J, local, lds, &evaluation_count);
+        if (J.window_lo && J.topk)
+            cost_tile_band<true>(J, local, lds, &evaluation_count);
+        else if (J.window_lo)
+            cost_tile_band<false>(J, local, lds, &evaluation_count);
+        else if (J.topk)
+            cost_tile<true>(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
         else
-            cost_tile(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
+            cost_tile<false>(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
     }
 }
 
+// slab: labels per item, or for band items the item capacity of one pass
+// (band items also hold base, lo and item_node for up to L labels a node)
 size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
-                      bool band = false) {
+                      uint32_t best_view_count, bool band = false, uint32_t labels = 0) {
     const PatchLayout l = patch_layout(nb, views, taps, !band);
-    size_t bytes = (size_t)l.total + (size_t)nb * views * slab * 4 + (size_t)nb * 8;
-    if (band) bytes += (size_t)(2 * nb + 1) * 4 + (size_t)nb * slab;  // base, lo, item_node
+    const uint32_t k = topk_slots(best_view_count, views);
+    const size_t slots = (size_t)(band ? slab : nb * slab) * (k < views ? k : views);
+    size_t bytes = (size_t)l.total + slots * 4 + (size_t)nb * 8;
+    if (band) bytes += (size_t)(2 * nb + 1) * 4 + (size_t)nb * labels;
     return bytes;
 }
 
@@ -750,19 +858,34 @@ size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
 void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     const uint32_t V = J.S.view_count;
     J.T = (2 * J.S.radius + 1) * (2 * J.S.radius + 1);
-    J.slab = label_slab == 0 || label_slab > J.L || J.window_lo ? J.L : label_slab;
+    J.slab = label_slab == 0 || label_slab > J.L ? J.L : label_slab;
+    J.topk = topk_slots(J.best_view_count, V) < V ? 1u : 0u;
     uint32_t nb = nodes_per_block;
-    if (nb == 0 && J.window_lo) {
-        // band windows hold 18 .. 66 labels: more nodes per item keep the
-        // threads busy; within 40 KB of LDS (four blocks per CU; the LDS
-        // still covers full-sweep nodes in fallback tiles)
-        nb = 16;
-        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab, true) > 40 * 1024) nb--;
+    if (J.window_lo) {
+        // band windows hold 18 .. 66 labels: up to 14 nodes per item keep
+        // the threads busy; within 40 KB of LDS (four blocks per CU) with
+        // room for +-16 windows in one pass, the rest of the 40 KB holding
+        // more items per pass (fallback tiles take several passes)
+        const uint32_t KS = J.topk ? topk_slots(J.best_view_count, V) : V;
+        if (nb == 0) {
+            nb = 14;  // 14 x 18 items of +-8 windows fill the block's 256 threads
+            while (nb > 1 &&
+                   cost_lds_bytes(nb, V, J.T, nb * 34, J.best_view_count, true, J.L) > 40 * 1024)
+                nb--;
+        }
+        const size_t fixed = cost_lds_bytes(nb, V, J.T, 0, J.best_view_count, true, J.L);
+        size_t cap = fixed < 40 * 1024 ? (40 * 1024 - fixed) / ((size_t)KS * 4) : 0;
+        if (cap > (size_t)nb * J.L) cap = (size_t)nb * J.L;
+        J.slab = cap > 64 ? (uint32_t)cap : 64u;
+        J.nb = nb;
+        J.tiles = (J.G.columns * J.G.rows + nb - 1) / nb;
+        J.slabs = 1;
+        return;
     } else if (nb == 0) {
         nb = 1024 / J.slab;
         if (nb < 1) nb = 1;
         if (nb > 16) nb = 16;
-        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab) > 40 * 1024) nb--;
+        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab, J.best_view_count) > 40 * 1024) nb--;
     }
     J.nb = nb;
     const uint32_t nodes = J.G.columns * J.G.rows;
@@ -771,13 +894,23 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
 }
 
 size_t cost_job_lds(const CostJob &J) {
-    return cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab, J.window_lo != nullptr);
+    return cost_lds_bytes(J.nb, J.S.view_count, J.T, J.slab, J.best_view_count,
+                          J.window_lo != nullptr, J.L);
 }
 
 hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {
     const size_t lds = cost_job_lds(job);
     if (lds > 160 * 1024 || job.nb == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cost_volume, dim3(job.tiles * job.slabs), dim3(256), lds, stream, job);
+    if (job.window_lo && job.topk)
+        hipLaunchKernelGGL(k_cost_volume_band<true>, dim3(job.tiles), dim3(256), lds, stream, job);
+    else if (job.window_lo)
+        hipLaunchKernelGGL(k_cost_volume_band<false>, dim3(job.tiles), dim3(256), lds, stream, job);
+    else if (job.topk)
+        hipLaunchKernelGGL(k_cost_volume<true>, dim3(job.tiles * job.slabs), dim3(256), lds, stream,
+                           job);
+    else
+        hipLaunchKernelGGL(k_cost_volume<false>, dim3(job.tiles * job.slabs), dim3(256), lds,
+                           stream, job);
     return hipGetLastError();
 }
 
@@ -920,7 +1053,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             double dx = 0.0, dy = 0.0;
             bool ok = true;
             if (slot > 0)
-                ok = delta_at_depth(P, n, P.cams[a], cand[n * stride + slot - 1], dx, dy);
+                ok = delta_at_depth(P, n, *P.cam, cand[n * stride + slot - 1], dx, dy);
             float sc;
             if (ok && patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, evaluations)) s = sc;
         }
@@ -1262,7 +1395,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             if (ok) {
                 const double *r = Q.ray + (size_t)n * 6;
                 const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
-                ok = project(Q.cams[a], point, qx, qy) && finite_d(qx) && finite_d(qy);
+                ok = project(*Q.cam, point, qx, qy) && finite_d(qx) && finite_d(qy);
             }
             const float2 q = make_float2((float)qx, (float)qy);
             points[n * 5 + k] = q;

@@ -222,7 +222,8 @@ __global__ void k_bias_grad(const uint16_t *__restrict__ dy, uint32_t M, uint32_
 
 // ------------------------------------------------------------ loss
 
-// Per pixel of the last layer: outputs o0 (centre, label units = 64 + 16 o0),
+// Per pixel of the last layer: outputs o0 (centre, label units = half + k o0
+// with half = (L - 1) / 2 and k = (L - 1) / 8 for L labels: 64 + 16 o0 at 129),
 // o1 (log band half-width s), o2 (confidence logit z); label t (0 = no
 // label). Loss = |t - c| / 8 (the centre) + a e^-s + s with a = |t - c|
 // held constant (the Laplace scale of the centre's error, which does not
@@ -230,8 +231,8 @@ __global__ void k_bias_grad(const uint16_t *__restrict__ dy, uint32_t M, uint32_
 // a band of +-8 labels around the centre covers the full sweep's label with
 // a margin of 2. Divided by `norm` pixels; writes d loss / d outputs.
 __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *__restrict__ labels,
-                                uint32_t M, uint32_t cout_p, float norm, float *__restrict__ dout,
-                                float *__restrict__ totals) {
+                                uint32_t M, uint32_t cout_p, float norm, float half, float k,
+                                float *__restrict__ dout, float *__restrict__ totals) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     float loss = 0.0f, valid = 0.0f;
     if (p < M) {
@@ -240,7 +241,7 @@ __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *_
         for (uint32_t c = 0; c < cout_p; c++) d[c] = 0.0f;
         const uint32_t t = labels[p];
         if (t > 0) {
-            const float centre = 64.0f + 16.0f * o[0];
+            const float centre = half + k * o[0];
             const float s = fminf(fmaxf(o[1], -3.0f), 5.0f);
             const float z = o[2];
             const float e = __expf(-s), diff = centre - (float)t, ad = fabsf(diff);
@@ -250,7 +251,7 @@ __global__ void k_proposal_loss(const float *__restrict__ out, const uint16_t *_
             const float y = ad <= 6.0f ? 1.0f : 0.0f;
             const float sig = 1.0f / (1.0f + __expf(-z));
             loss = ad * 0.125f + al * e + s + (fmaxf(z, 0.0f) - z * y + __logf(1.0f + __expf(-fabsf(z))));
-            d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * 2.0f / norm;
+            d[0] = (diff > 0.0f ? 1.0f : diff < 0.0f ? -1.0f : 0.0f) * (k * 0.125f) / norm;
             d[1] = (o[1] > -3.0f && o[1] < 5.0f) ? (1.0f - al * e) / norm : 0.0f;
             d[2] = (sig - y) / norm;
             valid = 1.0f;
@@ -311,7 +312,8 @@ __global__ void k_weights_bf16(ConvLayer L, const float *__restrict__ w, uint16_
 __global__ void k_quarter_input(const float *__restrict__ p0, const float *__restrict__ p1,
                                 const float *__restrict__ p2, uint32_t pitch0, uint32_t pitch1,
                                 uint32_t pitch2, uint32_t plane_w, uint32_t plane_h, uint32_t w,
-                                uint32_t h, float offset, float scale, uint16_t *__restrict__ out) {
+                                uint32_t h, uint32_t factor, float offset, float scale,
+                                uint16_t *__restrict__ out) {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w || y >= h) return;
     const float *planes[3] = {p0, p1, p2};
@@ -320,12 +322,16 @@ __global__ void k_quarter_input(const float *__restrict__ p0, const float *__res
     for (uint32_t c = 0; c < 16; c++) {
         float v = 0.0f;
         if (c < 3 && planes[c]) {
-            const uint32_t x0 = min(2 * x, plane_w - 1), x1 = min(2 * x + 1, plane_w - 1);
-            const uint32_t y0 = min(2 * y, plane_h - 1), y1 = min(2 * y + 1, plane_h - 1);
+            // factor x factor cell mean, summed row by row (clamped at the edges)
             const float *p = planes[c];
             const uint32_t pt = pitches[c];
-            const float mean = 0.25f * (p[(size_t)y0 * pt + x0] + p[(size_t)y0 * pt + x1] +
-                                        p[(size_t)y1 * pt + x0] + p[(size_t)y1 * pt + x1]);
+            float sum = 0.0f;
+            for (uint32_t dy = 0; dy < factor; dy++) {
+                const uint32_t yy = min(factor * y + dy, plane_h - 1);
+                for (uint32_t dx = 0; dx < factor; dx++)
+                    sum += p[(size_t)yy * pt + min(factor * x + dx, plane_w - 1)];
+            }
+            const float mean = (1.0f / (float)(factor * factor)) * sum;
             v = (mean - offset) * scale;
         }
         o[c] = f2bf(v);
@@ -344,8 +350,9 @@ __global__ void k_quarter_input(const float *__restrict__ p0, const float *__res
 __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ out,
                                                       uint32_t cout_p, uint32_t w, uint32_t h,
                                                       uint32_t columns, uint32_t rows,
-                                                      uint32_t L, float threshold, uint32_t half,
-                                                      uint32_t max_half, uint32_t tile,
+                                                      uint32_t L, float centre_half,
+                                                      float centre_k, float threshold,
+                                                      uint32_t half, uint32_t max_half, uint32_t tile,
                                                       uint16_t *__restrict__ lo,
                                                       uint16_t *__restrict__ hi,
                                                       uint32_t *__restrict__ full_tiles) {
@@ -383,7 +390,7 @@ __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ 
             continue;
         }
         const float *o = out + ((size_t)min(r, h - 1) * w + min(c, w - 1)) * cout_p;
-        const float centre = rintf(64.0f + 16.0f * o[0]);
+        const float centre = rintf(centre_half + centre_k * o[0]);
         const int ctr = (int)fminf(fmaxf(centre, 1.0f), (float)(L - 1));
         const float b = __expf(fminf(fmaxf(o[1], -3.0f), 5.0f));
         const int w = (int)fminf(fmaxf(ceilf(4.0f * b), (float)half), (float)max_half);
@@ -444,10 +451,11 @@ hipError_t launch_bias_grad(hipStream_t s, const uint16_t *dy, uint32_t M, uint3
 }
 
 hipError_t launch_proposal_loss(hipStream_t s, const float *out, const uint16_t *labels,
-                                uint32_t M, uint32_t cout_p, float norm, float *dout,
-                                float *totals) {
+                                uint32_t M, uint32_t cout_p, uint32_t label_count, float norm,
+                                float *dout, float *totals) {
+    const float half = (float)(label_count - 1) * 0.5f, k = (float)(label_count - 1) * 0.125f;
     hipLaunchKernelGGL(k_proposal_loss, dim3((M + 255) / 256), dim3(256), 0, s, out, labels, M,
-                       cout_p, norm, dout, totals);
+                       cout_p, norm, half, k, dout, totals);
     return hipGetLastError();
 }
 
@@ -467,11 +475,11 @@ hipError_t launch_weights_bf16(hipStream_t s, const ConvLayer &L, const float *w
 
 hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
                                 const uint32_t pitches[3], uint32_t plane_w, uint32_t plane_h,
-                                uint32_t w, uint32_t h, float offset, float scale,
-                                uint16_t *out) {
+                                uint32_t w, uint32_t h, uint32_t factor, float offset,
+                                float scale, uint16_t *out) {
     hipLaunchKernelGGL(k_quarter_input, dim3((w + 127) / 128, h), dim3(128), 0, s, planes[0],
                        planes[1], planes[2], pitches[0], pitches[1], pitches[2], plane_w, plane_h,
-                       w, h, offset, scale, out);
+                       w, h, factor, offset, scale, out);
     return hipGetLastError();
 }
 
@@ -479,9 +487,11 @@ hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
                                float threshold, uint32_t half, uint32_t max_half, uint32_t tile,
                                uint16_t *lo, uint16_t *hi, uint32_t *full_tiles) {
+    const float centre_half = (float)(L - 1) * 0.5f, centre_k = (float)(L - 1) * 0.125f;
     const uint32_t tiles = ((columns + tile - 1) / tile) * ((rows + tile - 1) / tile);
     hipLaunchKernelGGL(k_band_windows, dim3(tiles), dim3(256), 0, s, out, cout_p, w, h, columns,
-                       rows, L, threshold, half, max_half, tile, lo, hi, full_tiles);
+                       rows, L, centre_half, centre_k, threshold, half, max_half, tile, lo, hi,
+                       full_tiles);
     return hipGetLastError();
 }
 

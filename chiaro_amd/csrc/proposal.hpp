@@ -29,16 +29,18 @@ hipError_t launch_conv_dw(hipStream_t s, const ConvLayer &L, uint32_t batch, con
                           const uint16_t *dy, float *dw);
 hipError_t launch_bias_grad(hipStream_t s, const uint16_t *dy, uint32_t M, uint32_t cout_p,
                             float *db);
+// Labels 1 .. label_count - 1 (0 = unlabelled); centre = (L-1)/2 + (L-1)/8 o0.
 hipError_t launch_proposal_loss(hipStream_t s, const float *out, const uint16_t *labels,
-                                uint32_t M, uint32_t cout_p, float norm, float *dout,
-                                float *totals);
+                                uint32_t M, uint32_t cout_p, uint32_t label_count, float norm,
+                                float *dout, float *totals);
 hipError_t launch_adam(hipStream_t s, float *w, const float *g, float *m, float *v, uint32_t n,
                        float lr, float b1, float b2, float eps, float bc1, float bc2);
 hipError_t launch_weights_bf16(hipStream_t s, const ConvLayer &L, const float *w, uint16_t *wk,
                                uint16_t *wt);
 hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
                                 const uint32_t pitches[3], uint32_t plane_w, uint32_t plane_h,
-                                uint32_t w, uint32_t h, float offset, float scale, uint16_t *out);
+                                uint32_t w, uint32_t h, uint32_t factor, float offset, float scale,
+                                uint16_t *out);
 hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
                                float threshold, uint32_t half, uint32_t max_half, uint32_t tile,

@@ -88,6 +88,7 @@ typedef struct {
     double canvas_x0, canvas_y0;
     uint32_t canvas_w, canvas_h;
     float *canvas;
+    uint8_t *cells; /* Voronoi cell of each integer canvas point (or NULL) */
 } scene_state;
 
 static uint32_t voronoi_cell(const scene_state *s, double x, double y) {
@@ -102,6 +103,22 @@ static uint32_t voronoi_cell(const scene_state *s, double x, double y) {
         }
     }
     return best;
+}
+
+/* voronoi_cell through the canvas's cell map: a point whose four enclosing
+ * grid points share one cell lies in that cell (cells are convex), else the
+ * sites are searched. Same result as voronoi_cell, much cheaper. */
+static uint32_t cell_at(const scene_state *s, double x, double y) {
+    const double cx = x - s->canvas_x0, cy = y - s->canvas_y0;
+    if (s->cells && cx >= 0.0 && cy >= 0.0 && cx < (double)(s->canvas_w - 1) &&
+        cy < (double)(s->canvas_h - 1)) {
+        const size_t i = (size_t)cx, j = (size_t)cy, w = s->canvas_w;
+        const uint8_t c = s->cells[j * w + i];
+        if (s->cells[j * w + i + 1] == c && s->cells[(j + 1) * w + i] == c &&
+            s->cells[(j + 1) * w + i + 1] == c)
+            return c;
+    }
+    return voronoi_cell(s, x, y);
 }
 
 static double value_noise(uint64_t seed, uint32_t octave, double x, double y, double cell) {
@@ -157,6 +174,7 @@ static int build_scene(const chs_scene *sc, const chs_module *ref, uint32_t marg
     s->canvas_w = ref->width + 2 * margin;
     s->canvas_h = ref->height + 2 * margin;
     s->canvas = (float *)malloc((size_t)s->canvas_w * s->canvas_h * sizeof(float));
+    s->cells = NULL;
     return s->canvas != NULL;
 }
 
@@ -214,7 +232,7 @@ static float radiance(const render_job *j, uint32_t view, uint32_t px, uint32_t 
         double rx = r->focal_px * X / z + rcx, ry = r->focal_px * Y / z + rcy;
         last_x = rx;
         last_y = ry;
-        if (voronoi_cell(s, rx, ry) == i) return canvas_sample(s, rx, ry);
+        if (cell_at(s, rx, ry) == i) return canvas_sample(s, rx, ry);
     }
     return canvas_sample(s, last_x, last_y); /* farthest plane as backdrop */
 }
@@ -224,9 +242,12 @@ static void *render_worker(void *arg) {
     if (j->canvas_pass) {
         const scene_state *s = j->s;
         for (uint32_t y = j->row_begin; y < j->row_end; y++)
-            for (uint32_t x = 0; x < s->canvas_w; x++)
-                s->canvas[(size_t)y * s->canvas_w + x] =
-                    (float)texture(j->sc, s->canvas_x0 + x, s->canvas_y0 + y);
+            for (uint32_t x = 0; x < s->canvas_w; x++) {
+                const size_t at = (size_t)y * s->canvas_w + x;
+                s->canvas[at] = (float)texture(j->sc, s->canvas_x0 + x, s->canvas_y0 + y);
+                if (s->cells)
+                    s->cells[at] = (uint8_t)voronoi_cell(s, s->canvas_x0 + x, s->canvas_y0 + y);
+            }
         return NULL;
     }
     const chs_module *m = &j->modules[j->view];
@@ -307,6 +328,7 @@ int chs_render(const chs_scene *sc, const chs_module *modules, uint32_t count, u
     if (!sc || !modules || ref >= count) return -1;
     scene_state s;
     if (!build_scene(sc, &modules[ref], rig_margin(sc, modules, count, ref), &s)) return -1;
+    s.cells = (uint8_t *)malloc((size_t)s.canvas_w * s.canvas_h);
     render_job proto;
     memset(&proto, 0, sizeof proto);
     proto.sc = sc;
@@ -324,11 +346,13 @@ int chs_render(const chs_scene *sc, const chs_module *modules, uint32_t count, u
         proto.raw = raws ? raws[v] : NULL;
         if ((bayer && !proto.raw) || (!bayer && !proto.image)) {
             free(s.canvas);
+            free(s.cells);
             return -1;
         }
         run_rows(&proto, modules[v].height, threads);
     }
     free(s.canvas);
+    free(s.cells);
     return 0;
 }
 

@@ -238,6 +238,14 @@ class Session:
             "read cost volume")
         return scores, costs, guidance, tested
 
+    def tested(self):
+        """The cost volume's tested flags alone (the volumes stay on the device)."""
+        g = self.grid
+        tested = np.empty(g.coarse_columns * g.coarse_rows, np.uint8)
+        _abi.check(self.lib.chd_session_read_cost_volume(self.handle, None, None, None,
+                                                         _abi.ptr(tested)), "read tested")
+        return tested
+
     def aggregated(self):
         g = self.grid
         out = np.empty((g.coarse_columns * g.coarse_rows, g.labels), np.float32)

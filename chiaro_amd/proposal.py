@@ -2,11 +2,13 @@
 band sweep it drives. No counterpart in the reference, which sweeps every
 hypothesis; the plain full sweep stays the judge of its output.
 
-Network (matching chiaro_amd/csrc/proposal.hip): seven 3x3 convolutions at
-the reference plane's half size (the "quarter resolution" of the raster):
-3->32, 32->32 stride 2, 32->32, 32->32 stride 2, 32->32, up x2 + 32->32,
-up x2 + 32->3; ReLU after all but the last; BF16 weights and activations,
-FP32 accumulation. Outputs per pixel: centre label = 64 + 16 o0, log band
+Network (matching chiaro_amd/csrc/proposal.hip): seven 3x3 convolutions on
+a grid of one pixel per coarse node (the reference plane in cells of half the
+grid step: the plane at half size, the raster's "quarter resolution", for a
+4 px grid): 3->32, 32->32 stride 2, 32->32, 32->32 stride 2, 32->32, up x2 +
+32->32, up x2 + 32->3; ReLU after all but the last; BF16 weights and
+activations, FP32 accumulation. Outputs per pixel: centre label =
+(L - 1) / 2 + (L - 1) / 8 o0 for L labels (64 + 16 o0 at 129), log band
 half-width o1, confidence sigmoid(o2).
 """
 from __future__ import annotations
@@ -33,28 +35,30 @@ def from_bf16(h: np.ndarray) -> np.ndarray:
     return (h.astype(np.uint32) << 16).view(np.float32)
 
 
-def quarter_means(planes) -> np.ndarray:
-    """(h, w, len(planes)) f32 2x2 means of up to three half-resolution
-    planes, summed in the device's order."""
+def quarter_means(planes, factor: int = 2) -> np.ndarray:
+    """(h, w, len(planes)) f32 means over factor x factor cells of up to three
+    planes (edges clamped), summed row by row from 0 as the device does."""
     ph, pw = planes[0].shape
-    h, w = ph // 2, pw // 2
-    ys0 = np.minimum(2 * np.arange(h), ph - 1)
-    ys1 = np.minimum(2 * np.arange(h) + 1, ph - 1)
-    xs0 = np.minimum(2 * np.arange(w), pw - 1)
-    xs1 = np.minimum(2 * np.arange(w) + 1, pw - 1)
+    h, w = ph // factor, pw // factor
     out = np.zeros((h, w, min(len(planes), 3)), np.float32)
     for c, p in enumerate(planes[:3]):
         p = np.asarray(p, np.float32)
-        s = ((p[np.ix_(ys0, xs0)] + p[np.ix_(ys0, xs1)]) + p[np.ix_(ys1, xs0)]) + p[np.ix_(ys1, xs1)]
-        out[:, :, c] = np.float32(0.25) * s
+        s = np.zeros((h, w), np.float32)
+        for dy in range(factor):
+            ys = np.minimum(factor * np.arange(h) + dy, ph - 1)
+            for dx in range(factor):
+                xs = np.minimum(factor * np.arange(w) + dx, pw - 1)
+                s = s + p[np.ix_(ys, xs)]
+        out[:, :, c] = np.float32(1.0 / (factor * factor)) * s
     return out
 
 
-def quarter_input(planes, offset: float = 0.0, scale: float = 0.125) -> np.ndarray:
-    """(h, w, 16) BF16 network input from up to three half-resolution planes
-    (reference first): (2x2 mean - offset) * scale in f32, as the device
-    computes it (k_quarter_input)."""
-    m = quarter_means(planes)
+def quarter_input(planes, offset: float = 0.0, scale: float = 0.125,
+                  factor: int = 2) -> np.ndarray:
+    """(h, w, 16) BF16 network input from up to three planes (reference
+    first): (cell mean - offset) * scale in f32, as the device computes it
+    (k_quarter_input); factor 2 gives the plane at half size."""
+    m = quarter_means(planes, factor)
     out = np.zeros(m.shape[:2] + (16,), np.float32)
     out[:, :, :m.shape[2]] = (m - np.float32(offset)) * np.float32(scale)
     return to_bf16(out)
@@ -150,14 +154,15 @@ class ReferenceNet:
         return acts
 
     @staticmethod
-    def loss_grad(out, labels):
+    def loss_grad(out, labels, label_count: int = 129):
         """Loss (sum) and d loss / d out, normalised by the pixel count, as
         k_proposal_loss computes them (exp/log in f64 here)."""
         o = out.reshape(-1, out.shape[-1])
         t = labels.reshape(-1).astype(np.float64)
         m = t > 0
         d = np.zeros_like(o)
-        centre = 64.0 + 16.0 * o[:, 0]
+        half, k = (label_count - 1) * 0.5, (label_count - 1) * 0.125
+        centre = half + k * o[:, 0]
         s = np.clip(o[:, 1], -3.0, 5.0)
         z = o[:, 2]
         e = np.exp(-s)
@@ -168,7 +173,7 @@ class ReferenceNet:
         sig = 1.0 / (1.0 + np.exp(-z))
         loss = ad * 0.125 + al * e + s + (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
         n = o.shape[0]
-        d[:, 0] = np.sign(diff) * 2.0 / n
+        d[:, 0] = np.sign(diff) * (k * 0.125) / n
         d[:, 1] = np.where((o[:, 1] > -3.0) & (o[:, 1] < 5.0), (1.0 - al * e) / n, 0.0)
         d[:, 2] = (sig - y) / n
         d[~m] = 0.0
@@ -238,6 +243,12 @@ class Proposal:
         _abi.check(self.lib.chd_proposal_parameters(handle, None, None, C.byref(n)), "parameters")
         self.parameter_count = n.value
         self.input_offset, self.input_scale = 0.0, 0.125
+        self.label_count = 129
+
+    def set_label_count(self, label_count: int):
+        """Labels of the sessions served (centre = (L-1)/2 + (L-1)/8 o0)."""
+        _abi.check(self.lib.chd_proposal_set_label_count(self.handle, label_count), "label count")
+        self.label_count = label_count
 
     def set_input_transform(self, offset: float, scale: float):
         """Inputs are (2x2 mean - offset) * scale; capture_inputs must use the
@@ -319,14 +330,16 @@ class Proposal:
         return (full.value, tiles.value) if report else None
 
 
-def constant_proposal(context: Context, width: int, height: int, centre_label: float) -> Proposal:
+def constant_proposal(context: Context, width: int, height: int, centre_label: float,
+                      label_count: int = 129) -> Proposal:
     """The method without the network at the same hypothesis budget: zero
     weights, so every pixel gets the output biases - one centre label for the
     whole image at full confidence (every tile banded)."""
     p = Proposal(context, width, height, max_batch=1)
+    p.set_label_count(label_count)
     params = np.zeros(p.parameter_count, np.float32)
     last = params[-16:]               # biases of the output layer (cout_p = 16)
-    last[0] = (centre_label - 64.0) / 16.0
+    last[0] = (centre_label - (label_count - 1) * 0.5) / ((label_count - 1) * 0.125)
     last[2] = 30.0                    # sigmoid(30) = 1 in f32
     p.set_parameters(params)
     return p
@@ -338,11 +351,21 @@ def clear(session: Session):
 
 # ---------------------------------------------------------------- data
 
-def capture_inputs(cap, offset: float = 0.0, scale: float = 0.125) -> np.ndarray:
-    """Network input of a capture: reference and first two targets."""
+def grid_factor(cap) -> int:
+    """Plane pixels per network pixel: one network pixel per coarse node."""
+    return max(cap.options.grid_step, 4) // 2
+
+
+def _input_planes(cap):
     planes = [cap.modules[cap.reference_index].luminance]
     planes += [m.luminance for i, m in enumerate(cap.modules) if i != cap.reference_index][:2]
-    return quarter_input(planes, offset, scale)
+    return planes
+
+
+def capture_inputs(cap, offset: float = 0.0, scale: float = 0.125) -> np.ndarray:
+    """Network input of a capture: reference and first two targets, one
+    pixel per coarse node."""
+    return quarter_input(_input_planes(cap), offset, scale, grid_factor(cap))
 
 
 def input_transform(caps):
@@ -352,9 +375,7 @@ def input_transform(caps):
     on)."""
     values = []
     for cap in caps:
-        planes = [cap.modules[cap.reference_index].luminance]
-        planes += [m.luminance for i, m in enumerate(cap.modules) if i != cap.reference_index][:2]
-        values.append(quarter_means(planes).astype(np.float64).ravel())
+        values.append(quarter_means(_input_planes(cap), grid_factor(cap)).astype(np.float64).ravel())
     v = np.concatenate(values)
     return float(np.float32(v.mean())), float(np.float32(1.0 / max(v.std(), 1e-6)))
 
@@ -452,3 +473,61 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
     stats["window_covers_label"] = float(inside[valid & banded].mean()) if (valid & banded).any() else 1.0
     stats["disagree_inside_window"] = float((wrong & inside).sum() / max(wrong.sum(), 1))
     return stats
+
+
+def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int = 3):
+    """The captures run as one batch (chd_batch): the full sweep against the
+    band sweep for each (threshold, tile, max_half_width) in `settings`
+    (half_width 8). Per setting: the fraction of nodes tested by the full
+    sweep whose band label is within +-1, mask agreement, fallback-tile
+    fraction and both times per batch (the band time includes the network
+    and the windows of every capture)."""
+    from .depth import Batch
+    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    sessions = [Session(ctx, c.modules, c.reference_index, c.options) for c in caps]
+    batch = Batch(ctx, sessions)
+    rows = []
+    try:
+        batch.run(stages)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.run(stages)
+        ctx.synchronize()
+        full_ms = (time.perf_counter() - t0) * 1000 / steps
+        full_best = [s.coarse()[2] for s in sessions]
+        full_tested = [s.tested() for s in sessions]
+        for threshold, tile, max_half in settings:
+            fallback = tiles = 0
+            for s in sessions:
+                f, t = proposal.apply(s, threshold, 8, tile, max_half_width=max_half)
+                fallback += f
+                tiles += t
+            batch.run(stages)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                for s in sessions:
+                    proposal.apply(s, threshold, 8, tile, report=False, max_half_width=max_half)
+                batch.run(stages)
+            ctx.synchronize()
+            band_ms = (time.perf_counter() - t0) * 1000 / steps
+            close, valid, mask = 0, 0, 0.0
+            for s, fb, ft in zip(sessions, full_best, full_tested):
+                bb = s.coarse()[2]
+                bt = s.tested()
+                v = ft > 0
+                close += int((np.abs(bb.astype(np.int32) - fb.astype(np.int32)) <= 1)[v].sum())
+                valid += int(v.sum())
+                mask += float((bt == ft).mean()) / len(sessions)
+            rows.append({"threshold": threshold, "tile": tile, "max_half_width": max_half,
+                         "agreement": close / max(valid, 1), "mask_agreement": mask,
+                         "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
+                         "band_ms": band_ms, "speedup": full_ms / band_ms})
+    finally:
+        for s in sessions:
+            clear(s)
+        batch.close()
+        for s in sessions:
+            s.close()
+    return rows

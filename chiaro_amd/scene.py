@@ -263,7 +263,9 @@ def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
     (f = 17700 px, on a 45 mm ring at 30 + 60k deg), all with the reference's
     orientation. Inverse depths U[0.02, 1.25] 1/m, 12-bit Bayer codes as in
     config1, 256 hypotheses, near 800 mm, far 50000 mm. `scale` shrinks every
-    raster and focal length together (for small test captures)."""
+    raster, focal length and the texture's cell size together (small test
+    captures; at 0.5 with a 4 px grid the coarse grid matches the full
+    capture's at 8 px)."""
     def size(n):
         return max(16, int(round(n * scale)) // 8 * 8)
 
@@ -276,7 +278,8 @@ def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
             a = math.radians(first + step * k)
             modules.append(_module(f * scale, w, h, (radius * math.cos(a), radius * math.sin(a), 0.0)))
     desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.02e-3,
-                          inverse_depth_max=1.25e-3, texture_cell_px=32)
+                          inverse_depth_max=1.25e-3,
+                          texture_cell_px=max(4, int(round(32 * scale))))
     bayer = _abi.Bayer(**BAYER_12BIT)
     raws = _render(desc, modules, 0, 0.0, bayer)
     planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]

@@ -264,7 +264,8 @@ int chd_session_grid(const chd_session *session, chd_grid_sizes *out);
 int chd_session_read_results(chd_session *session, chd_depth_map *map,
                              chd_target_result *targets);
 /* Cost volume, label-contiguous [node][label] (depth.rs:1270-1276). NaN
- * scores are the reference's f32::NAN for unsupported labels. */
+ * scores are the reference's f32::NAN for unsupported labels. NULL outputs
+ * are skipped (the reads below do the same). */
 int chd_session_read_cost_volume(chd_session *session, float *scores, float *costs,
                                  float *guidance, uint8_t *tested);
 int chd_session_read_aggregated(chd_session *session, float *aggregated);
@@ -348,8 +349,8 @@ void chd_batch_destroy(chd_batch *batch);
  * every hypothesis): a 7-layer 3x3 convolutional encoder-decoder with BF16
  * weights and activations and FP32 accumulation on the matrix cores. Its
  * input is width x height x 16 BF16 (channels 0..2: the reference and the
- * first two target planes at half the plane resolution, through the input
- * transform below; the rest zero); per pixel it predicts a centre label, a log band half-width and
+ * first two target planes as means over cells of (grid step / 2)^2 plane
+ * pixels, through the input transform below; the rest zero); per pixel it predicts a centre label, a log band half-width and
  * a confidence. Trained with Adam on full-sweep WTA label maps (0 = none). */
 typedef struct chd_proposal chd_proposal;
 int chd_proposal_create(chd_context *context, uint32_t width, uint32_t height,
@@ -360,11 +361,15 @@ void chd_proposal_destroy(chd_proposal *proposal);
  * *count must equal it and get receives or set replaces them. */
 int chd_proposal_parameters(chd_proposal *proposal, float *get, const float *set,
                             size_t *count);
-/* The input transform: each input value is (2x2 mean of a plane - offset) *
+/* The input transform: each input value is (cell mean of a plane - offset) *
  * scale in f32, rounded to BF16 (default offset 0, scale 1/8). Inputs given
  * to set_dataset / predict / gradients must have been made with the same
  * transform; set_proposal applies it on the device. */
 int chd_proposal_set_input_transform(chd_proposal *proposal, float offset, float scale);
+/* Hypothesis labels L of the sessions the network serves (default 129 = 128
+ * planes + the global warp): its centre output maps to (L - 1) / 2 +
+ * (L - 1) / 8 * o0 labels. */
+int chd_proposal_set_label_count(chd_proposal *proposal, uint32_t label_count);
 int chd_proposal_get_input_transform(const chd_proposal *proposal, float *offset, float *scale);
 /* Training set (host arrays): inputs count x height x width x 16 BF16 bits,
  * labels count x height x width. */
@@ -395,7 +400,10 @@ int chd_proposal_adam_step(chd_proposal *proposal, const float *gradients,
  * coarse nodes whose mean confidence is below `threshold` keep the full
  * sweep. The network grid must be the reference plane at half size.
  * full_tiles / tiles (optional) receive the fallback count and the tile
- * count (full_tiles synchronises). clear restores the full sweep. */
+ * count (full_tiles synchronises). clear restores the full sweep. The
+ * network's grid has one pixel per coarse node: the reference plane in cells
+ * of (grid step / 2) x (grid step / 2) plane pixels (the grid step must be
+ * even), and its label count must equal the session's. */
 int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float threshold,
                              uint32_t half_width, uint32_t max_half_width, uint32_t tile,
                              uint32_t *full_tiles, uint32_t *tiles);

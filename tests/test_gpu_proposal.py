@@ -210,3 +210,41 @@ def test_ablation_network_beats_constant_band(gpu_context):
     finally:
         net.close()
         const.close()
+
+
+def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
+    """A network on the coarse grid of a config 4 capture (8 px step: 4x4
+    plane cells per network pixel, 257 labels): the device-built input gives
+    the host's windows exactly and the band SGM equals the oracle's."""
+    cap = scene.config4(seed=3, scale=0.125)
+    sess = Session(gpu_context, cap.modules, 0, cap.options)
+    g = sess.grid
+    ref = cap.modules[0].luminance
+    h, w = ref.shape[0] // 4, ref.shape[1] // 4
+    net = Proposal(gpu_context, w, h, max_batch=1, seed=5)
+    try:
+        net.set_label_count(g.labels)
+        offset, scale = input_transform([cap])
+        net.set_input_transform(offset, scale)
+        params = _params(w, h, np.random.default_rng(6), scale=0.3)
+        net.set_parameters(params)
+        fb, tiles = net.apply(sess, threshold=0.0)
+        assert fb == 0 and tiles > 0
+        sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
+        _, costs, guidance, _ = sess.cost_volume()
+        pred = net.predict(capture_inputs(cap, offset, scale)[None])[0]
+        r = np.minimum(np.arange(g.coarse_rows), h - 1)
+        c = np.minimum(np.arange(g.coarse_columns), w - 1)
+        centre = np.clip(np.rint(pred[np.ix_(r, c)][..., 0]), 1, g.labels - 1)
+        fin = np.isfinite(costs[:, 1:])
+        assert np.array_equal(1 + np.argmax(fin, axis=1), np.maximum(1, centre - 8).reshape(-1))
+        assert np.array_equal(g.labels - 1 - np.argmax(fin[:, ::-1], axis=1),
+                              np.minimum(g.labels - 1, centre + 8).reshape(-1))
+        want = np.empty_like(costs)
+        oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
+                                        g.coarse_rows, g.labels, _abi.ptr(want))
+        assert_bitwise("band SGM at 257 labels", sess.aggregated(), want)
+        clear(sess)
+    finally:
+        net.close()
+        sess.close()
