@@ -161,10 +161,12 @@ __device__ __forceinline__ DevWarp view_warp(const ViewDev &v) {
 // starts as gc_ok (patch_view clears it where a tap's warp is undefined).
 // Reads the cameras from global memory (the LDS copy may still be in flight).
 __device__ void patch_rays(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t first,
-                           uint32_t count) {
+                           uint32_t count, uint32_t n_begin = 0, uint32_t n_end = ~0u) {
     const chd_camera &reference = *S.views[S.reference].camera;
-    for (uint32_t i = blockDim.x - 1 - threadIdx.x; i < nb * count; i += blockDim.x) {
-        const uint32_t slot = i / nb, n = i % nb, k = slot * nb + n;
+    if (n_end > nb) n_end = nb;
+    const uint32_t m = n_end > n_begin ? n_end - n_begin : 0;
+    for (uint32_t i = blockDim.x - 1 - threadIdx.x; i < m * count; i += blockDim.x) {
+        const uint32_t slot = i / m, n = n_begin + i % m, k = slot * nb + n;
         const ViewDev &view = S.views[S.vlist[first + slot]];
         const double2 p = P.pos[n];
         float2 gc;
@@ -275,13 +277,15 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
 // first ray group comes from patch_nodes unless first_group_ready is false
 // (a second pass over the views).
 __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
-                               uint32_t a, bool first_group_ready = true) {
+                               uint32_t a, bool first_group_ready = true, uint32_t n_begin = 0,
+                               uint32_t n_end = ~0u) {
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const int r = (int)S.radius;
     const uint32_t width = 2u * S.radius + 1u;
     const uint32_t V = S.view_count, R = P.ray_views, slot = a % R;
+    if (n_end > nb) n_end = nb;
     if (slot == 0 && (a > 0 || !first_group_ready)) {
-        patch_rays(S, P, nb, a, V - a < R ? V - a : R);
+        patch_rays(S, P, nb, a, V - a < R ? V - a : R, n_begin, n_end);
         __syncthreads();
     }
     PatchLds Q = P;
@@ -297,14 +301,16 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
         for (uint32_t i = tid; i < words; i += nth) ((uint32_t *)P.cam)[i] = src[i];
     }
     // G lanes per node (a power of two dividing the wavefront) compute the
-    // node's tap positions and reduce their min/max
-    const uint32_t G = nb <= 4 ? 64u : nb <= 8 ? 32u : 16u;
+    // node's tap positions and reduce their min/max; only nodes n_begin ..
+    // n_end - 1 (the others' contexts are not read until rebuilt)
+    const uint32_t m = n_end > n_begin ? n_end - n_begin : 0;
+    const uint32_t G = m <= 4 ? 64u : m <= 8 ? 32u : 16u;
     const uint32_t per_pass = nth / G, lane = tid % G;
-    for (uint32_t n0 = 0; n0 < nb; n0 += per_pass) {
+    for (uint32_t n0 = n_begin; n0 < n_end; n0 += per_pass) {
         const uint32_t n = n0 + tid / G;
         float x0 = __int_as_float(0x7f800000), y0 = x0;
         float x1 = -x0, y1 = -x0;
-        if (n < nb) {
+        if (n < n_end) {
             const double2 p = P.pos[n];
 #pragma unroll 3
             for (uint32_t t = lane; t < T; t += G) {
@@ -332,7 +338,7 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
             y0 = fminf(y0, __shfl_xor(y0, (int)m, (int)G));
             y1 = fmaxf(y1, __shfl_xor(y1, (int)m, (int)G));
         }
-        if (n < nb && lane == 0) P.ext[n] = make_float4(x0, x1, y0, y1);
+        if (n < n_end && lane == 0) P.ext[n] = make_float4(x0, x1, y0, y1);
     }
     __syncthreads();
     return Q;
@@ -673,7 +679,8 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
         if (TopK)
             for (uint32_t i = tid; i < cn; i += nth) res[(size_t)i * KS] = __uint_as_float(0u);
         for (uint32_t a = 0; a < V; a++) {
-            const PatchLds Q = patch_view(S, P, nb, T, a, c0 == 0);
+            const PatchLds Q = patch_view(S, P, nb, T, a, c0 == 0, item_node[c0],
+                                          item_node[c0 + cn - 1] + 1u);
             const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
             const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
             const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
