@@ -542,10 +542,15 @@ def bench_learned4(args) -> int:
     from chiaro_amd.depth import Context
     from chiaro_amd.proposal import (Proposal, batch_agreement, build_dataset, input_transform)
 
+    made = [0]
+
     def train_capture(seed):
         cap = scene.config4(seed=seed, scale=0.5)
         cap.options.grid_step = 4
         cap.raws = None
+        made[0] += 1
+        if made[0] % 40 == 0:
+            log(f"[bench] {made[0]} training captures")
         return cap
 
     ctx = Context(0)
