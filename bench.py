@@ -115,7 +115,10 @@ def load_traffic(kernel: str, workload: str):
         data = json.loads(path.read_text())
     except (OSError, ValueError):
         return None
-    entry = data.get(workload, {}).get(kernel)
+    table = data.get(workload, {})
+    # a templated kernel is listed under its instantiated name
+    entry = table.get(kernel) or next((v for k, v in table.items() if k.startswith(kernel + "<")),
+                                      None)
     return None if entry is None else entry.get("bytes_per_launch")
 
 
