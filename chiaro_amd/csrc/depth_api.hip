@@ -1461,6 +1461,7 @@ struct chd_batch {
     uint32_t *sgm_first = nullptr;
     uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
     bool persistent = false;  // one queue launch for every cost volume
+    int queue_mode = 4;       // the jobs' common kind for the queue kernel (4: mixed)
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
     std::vector<hipGraph_t> templates;
     std::vector<uint32_t> versions;  // sessions' versions the graphs were captured with
@@ -1503,7 +1504,8 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
         for (chd_session *s : b->sessions) CHD_CHECK(prepare_cost(s));
         if (b->persistent) {
             CHD_HIP(launch_cost_volume_queue(st, b->jobs[keep_views ? 1 : 0], b->first, n,
-                                             b->lds, b->blocks, b->counter, b->xcd_order));
+                                             b->lds, b->blocks, b->counter, b->xcd_order,
+                                             b->queue_mode));
         } else {
             for (chd_session *s : b->sessions) {
                 float *vs = nullptr;
@@ -1543,9 +1545,12 @@ int batch_prepare(chd_batch *b) {
     std::vector<uint32_t> first(count + 1, 0);
     b->lds = 0;
     b->versions.clear();
+    b->queue_mode = -1;
     for (uint32_t k = 0; k < count; k++) {
         plain[k] = cost_job(b->sessions[k], nullptr);
         first[k + 1] = first[k] + plain[k].tiles * plain[k].slabs;
+        const int mode = cost_job_mode(plain[k]);
+        b->queue_mode = b->queue_mode < 0 || b->queue_mode == mode ? mode : 4;
         b->lds = std::max(b->lds, cost_job_lds(plain[k]));
         b->versions.push_back(b->sessions[k]->version);
     }

@@ -87,9 +87,12 @@ hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job);
 // 1 entries) is the prefix of tiles * slabs; lds must cover every job;
 // counters are kXcds device words the launch clears first. With xcd_order
 // each XCD drains its own contiguous run of items before taking others'.
+// mode: cost_job_mode of every job when they agree, else 4 (mixed).
+int cost_job_mode(const CostJob &job);
 hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
                                     const uint32_t *first, uint32_t njobs, size_t lds,
-                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order);
+                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order,
+                                    int mode);
 
 // paths_per_block: wavefronts (one path each) per block, 1, 2 or 4.
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,

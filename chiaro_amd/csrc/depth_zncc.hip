@@ -794,10 +794,14 @@ k_cost_volume_band(CostJob J) {
 // faster than one shared counter); with xcd_order XCD x takes its contiguous
 // run instead. A block drains its own XCD's sequence, then the others' in
 // turn.
-__global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__restrict__ jobs,
-                                                           const uint32_t *__restrict__ first,
-                                                           uint32_t njobs, uint32_t *counters,
-                                                           uint32_t xcd_order) {
+// Mode: the kind of every job (0 full, 1 full with top-K, 2 band, 3 band
+// with top-K), or kQueueMixed to decide per job (one kernel then carries the
+// register needs of all four).
+constexpr int kQueueMixed = 4;
+template <int Mode>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_cost_volume_queue(const CostJob *__restrict__ jobs, const uint32_t *__restrict__ first,
+                    uint32_t njobs, uint32_t *counters, uint32_t xcd_order) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     __shared__ uint32_t next;
@@ -836,13 +840,14 @@ __global__ void __launch_bounds__(256) k_cost_volume_queue(const CostJob *__rest
         while (j + 1 < njobs && first[j + 1] <= item) j++;
         const CostJob J = jobs[j];
         const uint32_t local = item - first[j];
-        if (J.window_lo && J.topk)
+        const int mode = Mode == kQueueMixed ? (J.window_lo ? 2 : 0) + (J.topk ? 1 : 0) : Mode;
+        if ((Mode == kQueueMixed || Mode == 3) && mode == 3)
             cost_tile_band<true>(J, local, lds, &evaluation_count);
-        else if (J.window_lo)
+        else if ((Mode == kQueueMixed || Mode == 2) && mode == 2)
             cost_tile_band<false>(J, local, lds, &evaluation_count);
-        else if (J.topk)
+        else if ((Mode == kQueueMixed || Mode == 1) && mode == 1)
             cost_tile<true>(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
-        else
+        else if (Mode == kQueueMixed || Mode == 0)
             cost_tile<false>(J, local / J.slabs, local % J.slabs, lds, &evaluation_count);
     }
 }
@@ -921,14 +926,30 @@ hipError_t launch_cost_volume(hipStream_t stream, const CostJob &job) {
     return hipGetLastError();
 }
 
+int cost_job_mode(const CostJob &job) { return (job.window_lo ? 2 : 0) + (job.topk ? 1 : 0); }
+
 hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
                                     const uint32_t *first, uint32_t njobs, size_t lds,
-                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order) {
+                                    uint32_t blocks, uint32_t *counters, uint32_t xcd_order,
+                                    int mode) {
     if (lds > 160 * 1024 || njobs == 0) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(stream, counters, 0u, kXcds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cost_volume_queue, dim3(blocks), dim3(256), lds, stream, jobs, first,
-                       njobs, counters, xcd_order);
+    switch (mode) {
+#define CHD_QUEUE(m)                                                                           \
+    case m:                                                                                    \
+        hipLaunchKernelGGL(k_cost_volume_queue<m>, dim3(blocks), dim3(256), lds, stream, jobs,  \
+                           first, njobs, counters, xcd_order);                                 \
+        break;
+        CHD_QUEUE(0)
+        CHD_QUEUE(1)
+        CHD_QUEUE(2)
+        CHD_QUEUE(3)
+        CHD_QUEUE(kQueueMixed)
+#undef CHD_QUEUE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
