@@ -226,3 +226,64 @@ def test_config4_batch_bit_exact_with_oracle(gpu_context):
         batch.close()
         for s in sessions:
             s.close()
+
+
+def _banded_session(ctx, cap, seed):
+    """A session with band windows from a seeded random proposal (about half
+    of the tiles banded)."""
+    from chiaro_amd.proposal import Proposal, input_transform, layer_specs
+    sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
+    factor = max(cap.options.grid_step, 4) // 2
+    h, w = (n // factor for n in cap.modules[cap.reference_index].luminance.shape)
+    net = Proposal(ctx, w, h, max_batch=1, seed=seed)
+    net.set_label_count(sess.grid.labels)
+    net.set_input_transform(*input_transform([cap]))
+    rng = np.random.default_rng(seed)
+    parts = []
+    for (cin, cout, *_r) in layer_specs(w, h):
+        cp_in, cp_out = (cin + 15) // 16 * 16, (cout + 15) // 16 * 16
+        wt = np.zeros((9 * cp_in, cp_out), np.float32)
+        for t in range(9):
+            wt[t * cp_in:t * cp_in + cin, :cout] = rng.normal(size=(cin, cout)) * 0.3
+        parts += [wt.reshape(-1), np.zeros(cp_out, np.float32)]
+    net.set_parameters(np.concatenate(parts))
+    net.apply(sess, threshold=0.5, half_width=8, tile=4, max_half_width=24)
+    return sess, net
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_band_batches_match_single_sessions(gpu_context, persistent):
+    """Band sweeps (windows from a proposal) batched equal the same sessions
+    run alone: two captures of two views (band items with all views kept),
+    two of nine views (band items with a running top-K) and the mixed pair
+    (with the persistent queue: its per-kind and mixed kernels)."""
+    small3 = [scene.config3(seed=s, width=520, height=392, focal_px=800.0) for s in (40, 41)]
+    small2 = []
+    for s in (42, 43):
+        c = scene.config2(seed=s, scale=0.125)
+        c.options.grid_step = 4
+        small2.append(c)
+    for label, caps in (("two views", small3), ("nine views", small2),
+                        ("mixed", [small3[0], small2[0]])):
+        _say(f"band batch ({label}), persistent {persistent}")
+        wants = []
+        for k, cap in enumerate(caps):
+            sess, net = _banded_session(gpu_context, cap, 7 + k)
+            try:
+                sess.run(ALL_KEEP)
+                wants.append(_snapshot(sess))
+            finally:
+                net.close()
+                sess.close()
+        pairs = [_banded_session(gpu_context, cap, 7 + k) for k, cap in enumerate(caps)]
+        batch = Batch(gpu_context, [s for s, _ in pairs], persistent=persistent)
+        try:
+            batch.run(ALL_KEEP)
+            gpu_context.synchronize()
+            for k, ((s, _), want) in enumerate(zip(pairs, wants)):
+                _same(f"band batch {label} capture {k}", _snapshot(s), want)
+        finally:
+            batch.close()
+            for s, net in pairs:
+                net.close()
+                s.close()
