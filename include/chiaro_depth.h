@@ -312,7 +312,8 @@ int chd_session_read_view_scores(chd_session *session, float *out);
  * defaults. No counterpart in the reference (CPU threads). */
 typedef struct chd_tuning {
     uint32_t cost_nodes_per_block; /* coarse nodes per cost-volume work item */
-    uint32_t cost_label_slab;      /* labels per cost-volume work item */
+    uint32_t cost_label_slab;      /* labels per cost-volume work item (band sweeps size
+                                      their passes from the LDS instead) */
     uint32_t sgm_paths_per_block;  /* SGM path sweeps (wavefronts) per block: 1, 2 or 4 */
     uint32_t cost_tile_order;      /* 0: work items in launch order; 1: each of the 8 XCDs
                                       (own L2) takes a contiguous run of node tiles; 2 (the
@@ -350,8 +351,9 @@ void chd_batch_destroy(chd_batch *batch);
  * weights and activations and FP32 accumulation on the matrix cores. Its
  * input is width x height x 16 BF16 (channels 0..2: the reference and the
  * first two target planes as means over cells of (grid step / 2)^2 plane
- * pixels, through the input transform below; the rest zero); per pixel it predicts a centre label, a log band half-width and
- * a confidence. Trained with Adam on full-sweep WTA label maps (0 = none). */
+ * pixels, through the input transform below; the rest zero); per pixel it
+ * predicts a centre label, a log band half-width and a confidence. Trained
+ * with Adam on full-sweep WTA label maps (0 = none). */
 typedef struct chd_proposal chd_proposal;
 int chd_proposal_create(chd_context *context, uint32_t width, uint32_t height,
                         uint32_t max_batch, uint64_t seed, chd_proposal **out);
