@@ -145,6 +145,7 @@ struct chd_session {
     // band sweep proposed by a chd_proposal: label windows per coarse node
     uint16_t *band_lo = nullptr, *band_hi = nullptr;
     bool band = false;
+    uint32_t band_window_max = 0;  // labels of the widest band window (2 max_half + 2)
     uint32_t version = 0;  // bumped when launch parameters change (batches recapture)
     chd_tuning tuning{};
     uint32_t stages_done = 0;
@@ -258,6 +259,7 @@ CostJob cost_job(const chd_session *s, float *view_scores) {
     j.xcd_order = s->tuning.cost_tile_order;
     j.window_lo = s->band ? s->band_lo : nullptr;
     j.window_hi = s->band ? s->band_hi : nullptr;
+    j.window_max = s->band ? s->band_window_max : 0;
     cost_job_shape(j, s->tuning.cost_nodes_per_block, s->tuning.cost_label_slab);
     return j;
 }
@@ -2130,8 +2132,10 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
                                 s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
                                 threshold, half_width, std::max(half_width, max_half_width),
                                 tile, s->band_lo, s->band_hi, p->full_tiles));
-    if (!s->band) {
+    const uint32_t window_max = 2 * std::max(half_width, max_half_width) + 2;
+    if (!s->band || s->band_window_max != window_max) {
         s->band = true;
+        s->band_window_max = window_max;  // shapes the band items
         s->version++;
     }
     if (tiles)

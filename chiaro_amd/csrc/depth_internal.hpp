@@ -65,6 +65,7 @@ struct CostJob {
     uint32_t tiles, slabs;
     uint32_t minimum_support, best_view_count;
     uint32_t topk;           // views aggregated as a running top-K (set by cost_job_shape)
+    uint32_t window_max;     // band sweep: labels of the widest window (0: unknown)
     float *scores, *costs, *guidance;
     uint8_t *tested;         // cleared before the launch
     float *view_scores;      // optional [view slot][node][label] per-view ZNCC, NaN = None

@@ -874,13 +874,16 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     J.topk = topk_slots(J.best_view_count, V) < V ? 1u : 0u;
     uint32_t nb = nodes_per_block;
     if (J.window_lo) {
-        // band windows hold 18 .. 66 labels: up to 14 nodes per item keep
-        // the threads busy; within 40 KB of LDS (four blocks per CU) with
+        // band windows hold 18 .. 66 labels: about 512 items per block (two
+        // rounds of the 256 threads) at the widest window, 4 .. 14 nodes
+        // (measured on config 4: 14 nodes best at +-8 and +-16, 6 at +-32 and
+        // for fallback tiles); within 40 KB of LDS (four blocks per CU) with
         // room for +-16 windows in one pass, the rest of the 40 KB holding
         // more items per pass (fallback tiles take several passes)
         const uint32_t KS = J.topk ? topk_slots(J.best_view_count, V) : V;
         if (nb == 0) {
-            nb = 14;  // 14 x 18 items of +-8 windows fill the block's 256 threads
+            nb = J.window_max ? 512 / J.window_max : 14;
+            nb = nb < 4 ? 4 : nb > 14 ? 14 : nb;
             while (nb > 1 &&
                    cost_lds_bytes(nb, V, J.T, nb * 34, J.best_view_count, true, J.L) > 40 * 1024)
                 nb--;
