@@ -12,8 +12,12 @@ value = reference raster pixels x planes / step time / 1e6. The full path
 (direct pass, filters and per-target decisions added) is timed as well and
 reported beside it.
 
-Prints one JSON line. --gpus above 1 is rejected: this build measures one
-GPU (the path shards by capture).
+With --learned-scenes > 0 (the default) the line also carries
+"learned_band": the learned band proposal trained and measured on config 3
+at a reduced size (--workload config3 runs the full grid).
+
+Prints one JSON line. --gpus above 1 is rejected: one GPU is measured (the
+path shards by capture, whole captures per GPU).
 """
 from __future__ import annotations
 
@@ -134,6 +138,10 @@ def main() -> int:
                     help="config3: training scenes for the learned band proposal")
     ap.add_argument("--eval-scenes", type=int, default=24, help="config3: held-out scenes")
     ap.add_argument("--epochs", type=int, default=10, help="config3: training epochs")
+    ap.add_argument("--learned-scenes", type=int, default=1000,
+                    help="config1: training scenes of the learned-band line beside it (0 skips)")
+    ap.add_argument("--learned-epochs", type=int, default=60)
+    ap.add_argument("--learned-eval", type=int, default=4)
     ap.add_argument("--captures", type=int, default=0,
                     help="config2/config4: captures per batch (default 8 and 4)")
     ap.add_argument("--tuning", choices=["default", "auto"], default="default",
@@ -309,6 +317,16 @@ def main() -> int:
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     sess.close()
+    if args.learned_scenes > 0 and args.workload == "config1":
+        # the metric's second half, at a reduced size: the learned band on
+        # config 3 (the full grid and sizes: --workload config3)
+        grid = [(mh, 8, thr) for mh in (16, 24, 32) for thr in (0.2, 0.3, 0.4, 0.5)]
+        res = learned_config3(ctx, args.learned_scenes, args.learned_epochs, args.learned_eval,
+                              args.seed, grid, ablation=False)
+        line["learned_band"] = learned_summary(
+            res, f"config3 (1040x780, 3 views, 128 hypotheses): {args.learned_scenes} training "
+                 f"scenes x {args.learned_epochs} epochs, {args.learned_eval} held-out scenes, "
+                 f"8x8 tiles")
     ctx.close()
     print(json.dumps(line))
     return 0
@@ -423,68 +441,111 @@ def bench_batch(args) -> int:
     return 0
 
 
-def bench_learned(args) -> int:
-    """Config 3: the learned band proposal. Trains the network on the plain
-    full sweep's label maps of `train_scenes` seeded 1040x780 three-view
-    scenes (128 hypotheses, grid step 4), then on `eval_scenes` held-out
-    seeds times the full sweep (cost + SGM) against the band sweep (network +
-    windows + cost + SGM) at several confidence thresholds, tile sizes and
-    band-width caps (bands of +-8 labels, or widened to +-ceil(4 b) for the
-    predicted scale b up to the cap), and reports the fastest setting whose
-    +-1-hypothesis agreement with the full sweep is at least 98 %."""
+def learned_config3(ctx, train_scenes: int, epochs: int, eval_scenes: int, seed: int,
+                    grid, ablation: bool = True) -> dict:
+    """Trains the band proposal on the plain full sweep's label maps of
+    `train_scenes` seeded config 3 scenes and measures, on `eval_scenes`
+    held-out seeds, the full sweep (cost + SGM) against the band sweep
+    (network + windows + cost + SGM) for each (width cap, tile, threshold) in
+    `grid`; the fastest setting with at least 98 % +-1-hypothesis agreement
+    is `best`."""
     import numpy as np
 
     from chiaro_amd import scene
-    from chiaro_amd.depth import Context
     from chiaro_amd.proposal import (Proposal, agreement, build_dataset, constant_proposal,
                                      input_transform)
 
-    ctx = Context(0)
     t = time.perf_counter()
     offset, scale = input_transform([scene.config3(seed=s) for s in range(16)])
-    x, y = build_dataset(ctx, range(args.train_scenes), lambda s: scene.config3(seed=s), offset,
-                         scale)
-    log(f"[bench] {args.train_scenes} config3 scenes and labels in {time.perf_counter() - t:.1f} s")
+    x, y = build_dataset(ctx, range(train_scenes), lambda s: scene.config3(seed=s), offset, scale)
+    log(f"[bench] {train_scenes} config3 scenes and labels in {time.perf_counter() - t:.1f} s")
     h, w = x.shape[1:3]
-    net = Proposal(ctx, w, h, max_batch=32, seed=args.seed)
+    net = Proposal(ctx, w, h, max_batch=32, seed=seed)
     net.set_input_transform(offset, scale)
     net.set_dataset(x, y)
     t = time.perf_counter()
-    losses = net.train(epochs=args.epochs, batch=32, lr=1e-3, seed=args.seed)
+    losses = net.train(epochs=epochs, batch=32, lr=1e-3, seed=seed)
     train_s = time.perf_counter() - t
-    log(f"[bench] trained {args.epochs} epochs in {train_s:.1f} s, losses {losses}")
-    evals = [scene.config3(seed=100000 + k) for k in range(args.eval_scenes)]
+    log(f"[bench] trained {epochs} epochs in {train_s:.1f} s, last losses {losses[-3:]}")
+    evals = [scene.config3(seed=100000 + k) for k in range(eval_scenes)]
     rows = []
-    for max_half in (8, 16, 24, 32):
-        for tile in (4, 8):
-            for thr in (0.0, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9):
-                stats = [agreement(ctx, cap, net, thr, tile=tile, max_half_width=max_half)
-                         for cap in evals]
-                row = {k: float(np.mean([st[k] for st in stats])) for k in stats[0]}
-                row["threshold"] = thr
-                row["tile"] = tile
-                row["max_half_width"] = max_half
-                row["speedup"] = row["full_ms"] / row["band_ms"]
-                rows.append(row)
-                log(f"[bench] max half {max_half} tile {tile} threshold {thr}: "
-                    f"agreement {row['agreement']:.4f} fallback {row['fallback_fraction']:.3f} "
-                    f"speedup {row['speedup']:.3f} covers {row['window_covers_label']:.3f}")
-    # ablation at an equal budget (+-8 labels everywhere, no fallback): the
-    # network's centres against one centre, the training set's median label
-    const = constant_proposal(ctx, w, h, float(np.median(y[y > 0])))
-    ablation = {}
-    for name, prop in (("network", net), ("constant_centre", const)):
-        stats = [agreement(ctx, cap, prop, 0.0, tile=8) for cap in evals]
-        ablation[name] = {k: round(float(np.mean([st[k] for st in stats])), 5)
-                          for k in ("agreement", "window_covers_label", "band_ms")}
-    const.close()
-    log(f"[bench] ablation {ablation}")
+    for max_half, tile, thr in grid:
+        stats = [agreement(ctx, cap, net, thr, tile=tile, max_half_width=max_half) for cap in evals]
+        row = {k: float(np.mean([st[k] for st in stats])) for k in stats[0]}
+        row["threshold"] = thr
+        row["tile"] = tile
+        row["max_half_width"] = max_half
+        row["speedup"] = row["full_ms"] / row["band_ms"]
+        rows.append(row)
+        log(f"[bench] max half {max_half} tile {tile} threshold {thr}: "
+            f"agreement {row['agreement']:.4f} fallback {row['fallback_fraction']:.3f} "
+            f"speedup {row['speedup']:.3f} covers {row['window_covers_label']:.3f}")
+    out = {}
+    if ablation:
+        # equal budget (+-8 labels everywhere, no fallback): the network's
+        # centres against one centre, the training set's median label
+        const = constant_proposal(ctx, w, h, float(np.median(y[y > 0])))
+        for name, prop in (("network", net), ("constant_centre", const)):
+            stats = [agreement(ctx, cap, prop, 0.0, tile=8) for cap in evals]
+            out[name] = {k: round(float(np.mean([st[k] for st in stats])), 5)
+                         for k in ("agreement", "window_covers_label", "band_ms")}
+        const.close()
+        log(f"[bench] ablation {out}")
+    net.close()
     ok = [r for r in rows if r["agreement"] >= 0.98]
-    best = max(ok, key=lambda r: r["speedup"]) if ok else None
     ref = evals[0].modules[0]
-    pixels_hyp = ref.width * ref.height * evals[0].options.planes
-    full_ms = float(np.mean([r["full_ms"] for r in rows]))
-    chosen = best or rows[-1]
+    return {
+        "rows": rows,
+        "best": max(ok, key=lambda r: r["speedup"]) if ok else None,
+        "ablation": out,
+        "losses": losses,
+        "train_s": train_s,
+        "offset": offset,
+        "scale": scale,
+        "pixels_hyp": ref.width * ref.height * evals[0].options.planes,
+        "full_ms": float(np.mean([r["full_ms"] for r in rows])),
+    }
+
+
+def learned_summary(res: dict, workload: str) -> dict:
+    chosen = res["best"] or max(res["rows"], key=lambda r: r["agreement"])
+    return {
+        "workload": workload,
+        "met_98_percent": res["best"] is not None,
+        "threshold": chosen["threshold"],
+        "tile": chosen["tile"],
+        "max_half_width": chosen["max_half_width"],
+        "speedup": round(chosen["speedup"], 3),
+        "agreement": round(chosen["agreement"], 5),
+        "mask_agreement": round(chosen["mask_agreement"], 5),
+        "fallback_fraction": round(chosen["fallback_fraction"], 4),
+        "full_ms": round(chosen["full_ms"], 4),
+        "band_ms": round(chosen["band_ms"], 4),
+        "ablation_equal_budget": res["ablation"],
+        "input_transform": {"offset": res["offset"], "scale": res["scale"]},
+        "train_seconds": round(res["train_s"], 2),
+    }
+
+
+def bench_learned(args) -> int:
+    """Config 3: the learned band proposal over the full grid of settings
+    (width caps 8..32, tiles 4 and 8, thresholds 0..0.9), with the
+    equal-budget ablation; see learned_config3."""
+    from chiaro_amd.depth import Context
+
+    ctx = Context(0)
+    grid = [(mh, tile, thr) for mh in (8, 16, 24, 32) for tile in (4, 8)
+            for thr in (0.0, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)]
+    res = learned_config3(ctx, args.train_scenes, args.epochs, args.eval_scenes, args.seed, grid)
+    chosen = res["best"] or max(res["rows"], key=lambda r: r["agreement"])
+    pixels_hyp, full_ms = res["pixels_hyp"], res["full_ms"]
+    workload = (f"config3: 3 views 1040x780, 128 hypotheses, grid step 4; network trained on "
+                f"{args.train_scenes} scenes x {args.epochs} epochs, evaluated on "
+                f"{args.eval_scenes} held-out scenes")
+    summary = learned_summary(res, workload)
+    summary["thresholds"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
+                             for r in res["rows"]]
+    summary["losses"] = [round(l, 4) for l in res["losses"]]
     line = {
         "metric": METRIC,
         "value": round(pixels_hyp / (chosen["band_ms"] / 1000.0) / 1e6, 2),
@@ -498,34 +559,13 @@ def bench_learned(args) -> int:
         "vs_baseline": None,
         "dtype": "f32 sweep, bf16 network",
         "data": "synthetic",
-        "config": {
-            "workload": (f"config3: 3 views 1040x780, 128 hypotheses, grid step 4; network "
-                         f"trained on {args.train_scenes} scenes x {args.epochs} epochs, "
-                         f"evaluated on {args.eval_scenes} held-out scenes"),
-            "parallelism": "1 GPU",
-        },
+        "config": {"workload": workload, "parallelism": "1 GPU"},
         "full_sweep_ms": round(full_ms, 4),
         "full_sweep_value": round(pixels_hyp / (full_ms / 1000.0) / 1e6, 2),
-        "learned_band": {
-            "met_98_percent": best is not None,
-            "threshold": chosen["threshold"],
-            "tile": chosen["tile"],
-            "max_half_width": chosen["max_half_width"],
-            "speedup": round(chosen["speedup"], 3),
-            "agreement": round(chosen["agreement"], 5),
-            "mask_agreement": round(chosen["mask_agreement"], 5),
-            "fallback_fraction": round(chosen["fallback_fraction"], 4),
-            "thresholds": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
-                           for r in rows],
-            "ablation_equal_budget": ablation,
-            "input_transform": {"offset": offset, "scale": scale},
-            "train_seconds": round(train_s, 2),
-            "losses": [round(l, 4) for l in losses],
-        },
+        "learned_band": summary,
         "roofline": None,
         "cpu_baseline": None,
     }
-    net.close()
     ctx.close()
     print(json.dumps(line))
     return 0

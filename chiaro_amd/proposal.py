@@ -1,6 +1,6 @@
 """Learned band proposal (chd_proposal): dataset building, training and the
 band sweep it drives. No counterpart in the reference, which sweeps every
-hypothesis; the plain full sweep stays the judge of its output.
+hypothesis; its output is scored against the plain full sweep.
 
 Network (matching chiaro_amd/csrc/proposal.hip): seven 3x3 convolutions on
 a grid of one pixel per coarse node (the reference plane in cells of half the
