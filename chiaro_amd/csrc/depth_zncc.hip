@@ -390,21 +390,23 @@ __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t
     }
 }
 
-// warped_patch_zncc (depth.rs:1772-1822) of node n against the current view
-// displaced by the raster delta (dx, dy); false is the reference's None.
-// F64Taps must match the layout's f64_taps.
-template <bool F64Taps>
-__device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
-                                           const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
-                                           uint32_t line_bytes, double dx, double dy, float &score,
-                                           unsigned int *evaluations) {
+// warped_patch_zncc (depth.rs:1772-1822) in two halves. The bounds half: every tap's bilinear footprint lies inside the plane
+// (the extremes of the warped taps decide it, since to_plane is monotone).
+__device__ __forceinline__ bool zncc_inside(const PatchLds &P, uint32_t n, const DevPlane &tp,
+                                            double dx, double dy) {
     const float4 e = P.ext[n];
     const float xmin = to_plane((double)e.x + dx), xmax = to_plane((double)e.y + dx);
     const float ymin = to_plane((double)e.z + dy), ymax = to_plane((double)e.w + dy);
-    if (!(xmin >= 0.0f && ymin >= 0.0f && xmax <= (float)(tp.width - 1) &&
-          ymax <= (float)(tp.height - 1)))
-        return false;
-    atomicAdd(evaluations, 1u);
+    return xmin >= 0.0f && ymin >= 0.0f && xmax <= (float)(tp.width - 1) &&
+           ymax <= (float)(tp.height - 1);
+}
+
+// The sums half, for a displacement zncc_inside accepted.
+template <bool F64Taps>
+__device__ __forceinline__ bool zncc_taps(const PatchLds &P, uint32_t n, uint32_t T,
+                                          const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
+                                          uint32_t line_bytes, double dx, double dy,
+                                          float &score) {
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
     if (tp.transposed)
         tap_sums<true, F64Taps>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
@@ -418,6 +420,20 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
     if (!(denominator > 1.0e-6f)) return false;
     score = rclampf(covariance / denominator, -1.0f, 1.0f);
     return true;
+}
+
+// warped_patch_zncc (depth.rs:1772-1822) of node n against the current view
+// displaced by the raster delta (dx, dy); false is the reference's None.
+// Evaluations count the displacements that pass the bounds check.
+// F64Taps must match the layout's f64_taps.
+template <bool F64Taps>
+__device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
+                                           const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
+                                           uint32_t line_bytes, double dx, double dy, float &score,
+                                           unsigned int *evaluations) {
+    if (!zncc_inside(P, n, tp, dx, dy)) return false;
+    atomicAdd(evaluations, 1u);
+    return zncc_taps<F64Taps>(P, n, T, tp, rsrc, line_bytes, dx, dy, score);
 }
 
 // map_at_depth (depth.rs:1738-1759) from the cached corrected ray of node n,
@@ -1368,7 +1384,8 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
                                                  uint32_t nb, chd_node_warp *decisions,
                                                  unsigned long long *work) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ unsigned int evaluation_count;
+    __shared__ unsigned int list_count;
+    __shared__ uint16_t list[kTargetNodes * 8];  // items of the phase that pass every gate
     const uint32_t V = S.view_count;
     const PatchLayout lay = patch_layout(nb, V, T, false);
     const PatchLds P = bind_patch(lds, lay, false);
@@ -1386,7 +1403,8 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         P.pos[n] = node_position(F, node0 + n);
         P.node_ok[n] = node0 + n < nodes ? 1 : 0;
     }
-    if (tid == 0) evaluation_count = 0;
+    if (tid == 0) list_count = 0;
+    unsigned int evaluations = 0;  // thread 0's tally of ZNCCs past the bounds check
     __syncthreads();
     patch_nodes(S, P, nb, T, centre_value);
 
@@ -1403,44 +1421,65 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         }
         __syncthreads();
         // phase A: offsets k < 5 (score_one_view_at_point, depth.rs:1710-1736)
-        // and the baseline k = 5 (score_one_view with no depth: delta 0)
+        // and the baseline k = 5 (score_one_view with no depth: delta 0).
+        // Every item first records None and its point; the items that pass
+        // the gates (projection, parallax, patch and bounds) join a list whose
+        // ZNCCs then run on consecutive threads, so no lane idles in the taps.
         for (uint32_t i = tid; i < nb * 6; i += blockDim.x) {
             const uint32_t n = i / 6, k = i % 6;
             if (state[n] == 0) continue;
-            float s = nan_f(), sc;
+            bool run = false;
             if (k == 5) {
-                if (Q.node_ok[n] && Q.nv_ok[n] &&
-                    patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, 0.0, 0.0, sc,
-                                      &evaluation_count))
-                    s = sc;
-                baseline[n] = s;
-                continue;
+                baseline[n] = nan_f();
+                run = Q.node_ok[n] && Q.nv_ok[n] && zncc_inside(Q, n, tp, 0.0, 0.0);
+            } else {
+                const double offsets[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
+                const double shared_inverse = 1.0 / field[node0 + n].depth;
+                const double inverse =
+                    rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
+                const double depth = 1.0 / inverse;
+                double qx = 0.0, qy = 0.0;
+                bool ok = Q.ray_ok[n] != 0;
+                if (ok) {
+                    const double *r = Q.ray + (size_t)n * 6;
+                    const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
+                    ok = project(*Q.cam, point, qx, qy) && finite_d(qx) && finite_d(qy);
+                }
+                const float2 q = make_float2((float)qx, (float)qy);
+                points[n * 5 + k] = q;
+                phase[n * 8 + k] = nan_f();
+                if (ok) {
+                    const double dx = (double)(q.x - Q.gc[n].x), dy = (double)(q.y - Q.gc[n].y);
+                    run = !(fabs(dx) > 128.0 || fabs(dy) > 128.0) && Q.node_ok[n] &&
+                          Q.nv_ok[n] && zncc_inside(Q, n, tp, dx, dy);
+                }
             }
-            const double offsets[5] = {-1.0, -0.5, 0.0, 0.5, 1.0};
-            const double shared_inverse = 1.0 / field[node0 + n].depth;
-            const double inverse =
-                rclamp(shared_inverse + offsets[k] * inverse_step, min_inverse, max_inverse);
-            const double depth = 1.0 / inverse;
-            double qx = 0.0, qy = 0.0;
-            bool ok = Q.ray_ok[n] != 0;
-            if (ok) {
-                const double *r = Q.ray + (size_t)n * 6;
-                const d3 point{r[0] + r[3] * depth, r[1] + r[4] * depth, r[2] + r[5] * depth};
-                ok = project(*Q.cam, point, qx, qy) && finite_d(qx) && finite_d(qy);
+            if (run) list[atomicAdd(&list_count, 1u)] = (uint16_t)i;
+        }
+        __syncthreads();
+        {
+            const uint32_t count = list_count;
+            if (tid == 0) evaluations += count;
+            for (uint32_t j = tid; j < count; j += blockDim.x) {
+                const uint32_t i = list[j], n = i / 6, k = i % 6;
+                double dx = 0.0, dy = 0.0;
+                if (k < 5) {
+                    const float2 q = points[n * 5 + k];
+                    dx = (double)(q.x - Q.gc[n].x);
+                    dy = (double)(q.y - Q.gc[n].y);
+                }
+                float sc;
+                if (zncc_taps<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc)) {
+                    if (k == 5)
+                        baseline[n] = sc;
+                    else
+                        phase[n * 8 + k] = sc;
+                }
             }
-            const float2 q = make_float2((float)qx, (float)qy);
-            points[n * 5 + k] = q;
-            if (ok) {
-                const double dx = (double)(q.x - Q.gc[n].x), dy = (double)(q.y - Q.gc[n].y);
-                if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && Q.node_ok[n] && Q.nv_ok[n] &&
-                    patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
-                                      &evaluation_count))
-                    s = sc;
-            }
-            phase[n * 8 + k] = s;
         }
         __syncthreads();
         // the best shared-depth offset: first maximum of score - |offset| * 0.002
+        if (tid == 0) list_count = 0;
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             if (state[n] != 1) continue;
             const float magnitude[5] = {1.0f, 0.5f, 0.0f, 0.5f, 1.0f};
@@ -1468,7 +1507,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             best_point[n] = bp;
         }
         __syncthreads();
-        // phase B: residual offsets, dy-major
+        // phase B: residual offsets, dy-major, gated and listed as in phase A
         const float steps[3] = {-1.5f, 0.0f, 1.5f};
         for (uint32_t i = tid; i < nb * 8; i += blockDim.x) {
             const uint32_t n = i / 8, k = i % 8;
@@ -1477,13 +1516,29 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             const float2 point = make_float2(best_point[n].x + steps[cell % 3],
                                              best_point[n].y + steps[cell / 3]);
             const double dx = (double)(point.x - Q.gc[n].x), dy = (double)(point.y - Q.gc[n].y);
-            float s = nan_f(), sc;
+            phase[n * 8 + k] = nan_f();
             if (!(fabs(dx) > 128.0 || fabs(dy) > 128.0) && Q.node_ok[n] && Q.nv_ok[n] &&
-                patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc, &evaluation_count))
-                s = sc;
-            phase[n * 8 + k] = s;
+                zncc_inside(Q, n, tp, dx, dy))
+                list[atomicAdd(&list_count, 1u)] = (uint16_t)i;
         }
         __syncthreads();
+        {
+            const uint32_t count = list_count;
+            if (tid == 0) evaluations += count;
+            for (uint32_t j = tid; j < count; j += blockDim.x) {
+                const uint32_t i = list[j], n = i / 8, k = i % 8;
+                const uint32_t cell = k < 4 ? k : k + 1;
+                const float2 point = make_float2(best_point[n].x + steps[cell % 3],
+                                                 best_point[n].y + steps[cell / 3]);
+                const double dx = (double)(point.x - Q.gc[n].x);
+                const double dy = (double)(point.y - Q.gc[n].y);
+                float sc;
+                if (zncc_taps<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc))
+                    phase[n * 8 + k] = sc;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) list_count = 0;
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
             if (g >= nodes) continue;
@@ -1550,7 +1605,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         }
         __syncthreads();
     }
-    if (tid == 0 && work && evaluation_count) atomicAdd(work, (unsigned long long)evaluation_count);
+    if (tid == 0 && work && evaluations) atomicAdd(work, (unsigned long long)evaluations);
 }
 
 size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
