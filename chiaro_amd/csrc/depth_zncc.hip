@@ -344,50 +344,65 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
     return Q;
 }
 
+// One batch of kTapBatch taps from t0 of the three target sums of
+// warped_patch_zncc: all eighteen plane loads are issued before the first
+// interpolation, then the sums run in tap order. Tail batches (t0 + kTapBatch
+// > T) re-read the last tap for the missing ones and drop them; full batches
+// need no clamp, so their LDS reads take constant offsets from one address.
+constexpr uint32_t kTapBatch = 9;
+template <bool Transposed, bool F64Taps, bool Tail>
+__device__ __forceinline__ void tap_batch(const float2 *wr, const double2 *gq, const float2 *gqf,
+                                          uint32_t t0, uint32_t T, uint32_t pitch,
+                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
+                                          double dx, double dy, float &st, float &stt, float &sp) {
+    PendingSample ps[kTapBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kTapBatch; k++) {
+        const uint32_t t = Tail ? (t0 + k < T ? t0 + k : T - 1) : t0 + k;
+        double qx, qy;
+        if constexpr (F64Taps) {
+            const double2 q = gq[t];
+            qx = q.x;
+            qy = q.y;
+        } else {
+            const float2 q = gqf[t];
+            qx = (double)q.x;
+            qy = (double)q.y;
+        }
+        ps[k] = plane_sample_issue<Transposed>(rsrc, pitch, line_bytes, to_plane(qx + dx),
+                                               to_plane(qy + dy));
+    }
+    // keep the batch's loads ahead of its interpolations (the scheduler
+    // would otherwise sink each load next to its use)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (uint32_t k = 0; k < kTapBatch; k++) {
+        if (Tail && t0 + k >= T) break;
+        const float2 w = wr[t0 + k];
+        const float tv = plane_sample_finish<Transposed>(ps[k]);
+        const float wv = w.x * tv;
+        st += wv;
+        stt += wv * tv;
+        sp += w.y * tv;
+    }
+}
+
 // The three target sums of warped_patch_zncc over the taps in the
-// reference's dy-major order, each in its own f32 accumulator. Taps go in
-// batches of nine: all eighteen plane loads of a batch are issued before the
-// first interpolation (the sums still run in tap order).
+// reference's dy-major order, each in its own f32 accumulator.
 template <bool Transposed, bool F64Taps>
 __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
                                          double dx, double dy, float &st, float &stt, float &sp) {
-    constexpr uint32_t B = 9;
     const float2 *wr = P.wr + (size_t)n * T;
     const double2 *gq = P.gq + (size_t)n * T;
     const float2 *gqf = P.gqf + (size_t)n * T;
-    for (uint32_t t0 = 0; t0 < T; t0 += B) {
-        PendingSample ps[B];
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) {
-            const uint32_t t = t0 + k < T ? t0 + k : T - 1;
-            double qx, qy;
-            if constexpr (F64Taps) {
-                const double2 q = gq[t];
-                qx = q.x;
-                qy = q.y;
-            } else {
-                const float2 q = gqf[t];
-                qx = (double)q.x;
-                qy = (double)q.y;
-            }
-            ps[k] = plane_sample_issue<Transposed>(rsrc, pitch, line_bytes, to_plane(qx + dx),
-                                                   to_plane(qy + dy));
-        }
-        // keep the batch's loads ahead of its interpolations (the scheduler
-        // would otherwise sink each load next to its use)
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (uint32_t k = 0; k < B; k++) {
-            if (t0 + k >= T) break;
-            const float2 w = wr[t0 + k];
-            const float tv = plane_sample_finish<Transposed>(ps[k]);
-            const float wv = w.x * tv;
-            st += wv;
-            stt += wv * tv;
-            sp += w.y * tv;
-        }
-    }
+    uint32_t t0 = 0;
+    for (; t0 + kTapBatch <= T; t0 += kTapBatch)
+        tap_batch<Transposed, F64Taps, false>(wr, gq, gqf, t0, T, pitch, rsrc, line_bytes, dx, dy,
+                                              st, stt, sp);
+    if (t0 < T)
+        tap_batch<Transposed, F64Taps, true>(wr, gq, gqf, t0, T, pitch, rsrc, line_bytes, dx, dy,
+                                             st, stt, sp);
 }
 
 // warped_patch_zncc (depth.rs:1772-1822) in two halves. The bounds half: every tap's bilinear footprint lies inside the plane
