@@ -275,10 +275,14 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
 // the per-view pointers at a's ray-group slot. Must be entered by the whole
 // block after every read of the previous view's context has finished. The
 // first ray group comes from patch_nodes unless first_group_ready is false
-// (a second pass over the views).
+// (a second pass over the views). With `live`, the block first checks
+// whether any node of n_begin .. n_end - 1 has its reference patch and the
+// view's global warp at its centre; if none has, *live is false, every ZNCC
+// of the view is None for these nodes and the tap context is not built (the
+// narrow modules of a rig see a small part of the reference).
 __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                                uint32_t a, bool first_group_ready = true, uint32_t n_begin = 0,
-                               uint32_t n_end = ~0u) {
+                               uint32_t n_end = ~0u, bool *live = nullptr) {
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const int r = (int)S.radius;
     const uint32_t width = 2u * S.radius + 1u;
@@ -294,6 +298,13 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
     Q.nv_ok = P.nv_ok + slot * nb;
     Q.gc_ok = P.gc_ok + slot * nb;
     Q.ray_ok = P.ray_ok + slot * nb;
+    if (live) {
+        int any = 0;
+        for (uint32_t n = n_begin + tid; n < n_end; n += nth)
+            any |= P.node_ok[n] && Q.nv_ok[n] ? 1 : 0;
+        *live = __syncthreads_or(any) != 0;
+        if (!*live) return Q;
+    }
     const DevWarp gw = view_warp(S.views[S.vlist[a]]);
     {  // the view's camera into LDS (the previous view's readers are done)
         constexpr uint32_t words = sizeof(chd_camera) / 4;
@@ -583,7 +594,18 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
     patch_nodes(S, P, nb, T, centre_value);
 
     for (uint32_t a = 0; a < V; a++) {
-        const PatchLds Q = patch_view(S, P, nb, T, a);
+        bool live;
+        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, ~0u, &live);
+        if (!live) {
+            // every score of the view is None: the top-K is unchanged
+            for (uint32_t i = tid; i < nb * Ls; i += nth) {
+                const uint32_t n = i / Ls, l = l0 + i % Ls;
+                if (!TopK) res[(size_t)a * nb * Ls + i] = nan_f();
+                if (J.view_scores && node0 + n < nodes)
+                    J.view_scores[((size_t)a * nodes + node0 + n) * L + l] = nan_f();
+            }
+            continue;
+        }
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
@@ -710,8 +732,15 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
         if (TopK)
             for (uint32_t i = tid; i < cn; i += nth) res[(size_t)i * KS] = __uint_as_float(0u);
         for (uint32_t a = 0; a < V; a++) {
+            bool live;
             const PatchLds Q = patch_view(S, P, nb, T, a, c0 == 0, item_node[c0],
-                                          item_node[c0 + cn - 1] + 1u);
+                                          item_node[c0 + cn - 1] + 1u, &live);
+            if (!live) {
+                // every score of the view is None: the top-K is unchanged
+                if (!TopK)
+                    for (uint32_t i = tid; i < cn; i += nth) res[(size_t)a * C + i] = nan_f();
+                continue;
+            }
             const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
             const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
             const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
