@@ -924,9 +924,13 @@ size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
     return bytes;
 }
 
-// Default shape: about 1024 (node, label) units per block, within 40 KB of
-// LDS (4 blocks of 256 threads per CU, the occupancy the tap loop's
-// registers allow), one slab.
+// Dynamic LDS per block for four blocks per CU (the occupancy the tap loop's
+// registers allow): a quarter of the CU's 160 KB less room for the kernels'
+// static variables, which share the block's allocation.
+constexpr size_t kQuarterLds = 40 * 1024 - 256;
+
+// Default shape: about 1024 (node, label) units per block, within a quarter
+// of the CU's LDS, one slab.
 void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
     const uint32_t V = J.S.view_count;
     J.T = (2 * J.S.radius + 1) * (2 * J.S.radius + 1);
@@ -937,19 +941,20 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
         // band windows hold 18 .. 66 labels: about 512 items per block (two
         // rounds of the 256 threads) at the widest window, 4 .. 14 nodes
         // (measured on config 4: 14 nodes best at +-8 and +-16, 6 at +-32 and
-        // for fallback tiles); within 40 KB of LDS (four blocks per CU) with
-        // room for +-16 windows in one pass, the rest of the 40 KB holding
-        // more items per pass (fallback tiles take several passes)
+        // for fallback tiles); within a quarter of the CU's LDS (four blocks
+        // per CU) with room for +-16 windows in one pass, the rest of the
+        // quarter holding more items per pass (fallback tiles take several
+        // passes)
         const uint32_t KS = J.topk ? topk_slots(J.best_view_count, V) : V;
         if (nb == 0) {
             nb = J.window_max ? 512 / J.window_max : 14;
             nb = nb < 4 ? 4 : nb > 14 ? 14 : nb;
             while (nb > 1 &&
-                   cost_lds_bytes(nb, V, J.T, nb * 34, J.best_view_count, true, J.L) > 40 * 1024)
+                   cost_lds_bytes(nb, V, J.T, nb * 34, J.best_view_count, true, J.L) > kQuarterLds)
                 nb--;
         }
         const size_t fixed = cost_lds_bytes(nb, V, J.T, 0, J.best_view_count, true, J.L);
-        size_t cap = fixed < 40 * 1024 ? (40 * 1024 - fixed) / ((size_t)KS * 4) : 0;
+        size_t cap = fixed < kQuarterLds ? (kQuarterLds - fixed) / ((size_t)KS * 4) : 0;
         if (cap > (size_t)nb * J.L) cap = (size_t)nb * J.L;
         J.slab = cap > 64 ? (uint32_t)cap : 64u;
         J.nb = nb;
@@ -960,7 +965,7 @@ void cost_job_shape(CostJob &J, uint32_t nodes_per_block, uint32_t label_slab) {
         nb = 1024 / J.slab;
         if (nb < 1) nb = 1;
         if (nb > 16) nb = 16;
-        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab, J.best_view_count) > 40 * 1024) nb--;
+        while (nb > 1 && cost_lds_bytes(nb, V, J.T, J.slab, J.best_view_count) > kQuarterLds) nb--;
     }
     J.nb = nb;
     const uint32_t nodes = J.G.columns * J.G.rows;
@@ -1284,7 +1289,7 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
     // 16 nodes give 224 (seeded) to 528 (unseeded) items per view for 256
     // threads; 40 KB of LDS keeps 4 blocks per CU
     uint32_t nb = 16;
-    while (nb > 1 && direct_lds_bytes(nb, V, T) > 40 * 1024) nb--;
+    while (nb > 1 && direct_lds_bytes(nb, V, T) > kQuarterLds) nb--;
     const size_t lds = direct_lds_bytes(nb, V, T);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
