@@ -1462,6 +1462,7 @@ struct chd_batch {
     SgmJob *sgm_jobs = nullptr;
     uint32_t *sgm_first = nullptr;
     uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
+    bool sgm_same_grids = true;  // every capture on the same coarse grid
     bool persistent = false;  // one queue launch for every cost volume
     int queue_mode = 4;       // the jobs' common kind for the queue kernel (4: mixed)
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
@@ -1518,7 +1519,7 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
     }
     if (stages & CHD_STAGE_SGM) {
         CHD_HIP(launch_sgm_batch(st, b->sgm_jobs, b->sgm_first, n, b->sgm_total, b->labels,
-                                 b->sgm_paths_per_block));
+                                 b->sgm_paths_per_block, b->sgm_same_grids));
         for (chd_session *s : b->sessions) CHD_CHECK(run_sgm_rest(s, keep));
     }
     if (stages & CHD_STAGE_DIRECT)
@@ -1568,6 +1569,8 @@ int batch_prepare(chd_batch *b) {
         chd_session *s = b->sessions[k];
         sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
         sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
+        if (s->grid.coarse_columns != sgm[0].columns || s->grid.coarse_rows != sgm[0].rows)
+            b->sgm_same_grids = false;
     }
     b->sgm_total = sgm_first[count];
     const uint32_t wpb = b->sessions[0]->tuning.sgm_paths_per_block;

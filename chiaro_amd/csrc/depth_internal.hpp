@@ -106,10 +106,13 @@ struct SgmJob {
     uint32_t columns, rows;
     float *paths;
 };
+// Paths of one capture's sweep (2R + 2C + 4(C + R - 1)).
 uint32_t sgm_waves(uint32_t columns, uint32_t rows);
+// first[]: prefix of the jobs' path counts; same_grids: every job has the
+// same columns and rows (their paths are then interleaved by length).
 hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
-                            uint32_t njobs, uint32_t total_waves, uint32_t labels,
-                            uint32_t paths_per_block);
+                            uint32_t njobs, uint32_t total_paths, uint32_t labels,
+                            uint32_t paths_per_block, bool same_grids);
 
 hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float *scores,
                              const uint8_t *tested, const double *labels, uint32_t columns,

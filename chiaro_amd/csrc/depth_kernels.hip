@@ -52,8 +52,8 @@ __device__ __forceinline__ float wave_min(float v) {
     return v;
 }
 
-// One wavefront walks one path of one of the 8 families, lanes over labels
-// (label = lane + 64 j), writing that family's L_r volume. Family order is
+// A wavefront walks one path (d, p) of one of the 8 families at a time,
+// lanes over labels (label = lane + 64 j), writing that family's L_r volume. Family order is
 // H->, H<-, V-down, V-up, DR->, DR<-, DL->, DL<- (depth.rs:1287-1309).
 // The path geometry is wave-uniform (scalar registers); the costs of the
 // next kSgmPrefetch nodes are loaded ahead so that the recurrence waits on
@@ -78,28 +78,65 @@ __device__ __forceinline__ LabelSet label_set(const uint16_t *lo, const uint16_t
     return w;
 }
 
+__host__ __device__ __forceinline__ uint32_t sgm_path_count(uint32_t columns, uint32_t rows) {
+    return 2 * rows + 2 * columns + 4 * (columns + rows - 1);
+}
+
+// Path of rank `rank` in the order of decreasing length: the horizontal
+// paths (length C), the vertical ones (R), then the diagonals of the four
+// diagonal families from the longest (min(C, R), |C - R| + 1 of them per
+// family) down to length 1 (two per family and length). Returns the family
+// d (0..7, the reference's order) and the path index p within it, as
+// diagonal_paths numbers them (depth.rs:1313-1338). A bijection onto the
+// 2R + 2C + 4(C + R - 1) paths.
+__device__ __forceinline__ void sgm_path_by_rank(uint32_t rank, uint32_t C, uint32_t R,
+                                                 uint32_t &d, uint32_t &p) {
+    if (rank < 2 * R) {
+        d = rank / R;
+        p = rank % R;
+        return;
+    }
+    rank -= 2 * R;
+    if (rank < 2 * C) {
+        d = 2 + rank / C;
+        p = rank % C;
+        return;
+    }
+    rank -= 2 * C;
+    const uint32_t m = C < R ? C : R, M = C < R ? R : C, full = M - m + 1;
+    uint32_t f, q;  // family among the diagonal four, rank within it
+    if (rank < 4 * full) {
+        f = rank % 4;
+        q = rank / 4;
+    } else {
+        const uint32_t g = rank - 4 * full;
+        f = (g % 8) / 2;
+        q = full + 2 * (g / 8) + g % 2;
+    }
+    d = 4 + f;
+    const bool mirrored = f >= 2;
+    if (q < full) {  // full-length diagonals
+        if (C >= R)
+            p = mirrored ? R - 1 + q : q;
+        else
+            p = q == 0 ? (mirrored ? C - 1 : 0) : C + q - 1;
+        return;
+    }
+    // the two diagonals of length l: one starting on the first row, one on
+    // the first (mirrored: last) column
+    const uint32_t k = q - full, l = m - 1 - k / 2;
+    if (k % 2 == 0)
+        p = mirrored ? l - 1 : C - l;
+    else
+        p = C + R - 1 - l;
+}
+
 template <uint32_t J>
 __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
                                          const float *__restrict__ guidance, uint32_t C,
                                          uint32_t R, uint32_t L, float *__restrict__ paths,
-                                         uint32_t wave) {
+                                         uint32_t d, uint32_t p) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t nH = R, nV = C, nD = C + R - 1;
-    uint32_t d, p;
-    if (wave < 2 * nH) {
-        d = wave / nH;
-        p = wave % nH;
-    } else if (wave < 2 * nH + 2 * nV) {
-        uint32_t w = wave - 2 * nH;
-        d = 2 + w / nV;
-        p = w % nV;
-    } else if (wave < 2 * nH + 2 * nV + 4 * nD) {
-        uint32_t w = wave - 2 * nH - 2 * nV;
-        d = 4 + w / nD;
-        p = w % nD;
-    } else {
-        return;
-    }
     int64_t start, stride;
     uint32_t len;
     switch (d) {
@@ -214,46 +251,96 @@ __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
     }
 }
 
+// Waves take paths by rank in a snake order over the rounds (wave w takes
+// rank w in even rounds and W - 1 - w in odd ones), so that with the ranks
+// sorted by decreasing length every wave, and so every SIMD, gets about the
+// same number of steps. (One wave per path in launch order gave the most
+// loaded SIMD a third more steps than the average.)
+__device__ __forceinline__ uint32_t snake_rank(uint32_t round, uint32_t wave, uint32_t waves) {
+    return round * waves + (round % 2 == 0 ? wave : waves - 1 - wave);
+}
+
 template <uint32_t J>
 __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ costs,
                                                    const float *__restrict__ guidance, uint32_t C,
                                                    uint32_t R, uint32_t L,
-                                                   float *__restrict__ paths) {
+                                                   float *__restrict__ paths, uint32_t waves) {
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    sgm_path<J>(costs, guidance, C, R, L, paths, wave);
+    if (wave >= waves) return;
+    const uint32_t total = sgm_path_count(C, R);
+    for (uint32_t round = 0; round * waves < total; round++) {
+        const uint32_t rank = snake_rank(round, wave, waves);
+        if (rank >= total) continue;
+        uint32_t d, p;
+        sgm_path_by_rank(rank, C, R, d, p);
+        sgm_path<J>(costs, guidance, C, R, L, paths, d, p);
+    }
 }
 
 // All path sweeps of several captures in one launch; first[] is the prefix
-// of each job's wavefront count (njobs + 1 entries).
+// of each job's path count (njobs + 1 entries). With `interleave` (every job
+// on the same grid) global rank g is rank g / njobs of job g % njobs, so the
+// longest paths of every capture come first.
 template <uint32_t J>
 __global__ void __launch_bounds__(256) k_sgm_paths_batch(const SgmJob *__restrict__ jobs,
                                                          const uint32_t *__restrict__ first,
-                                                         uint32_t njobs, uint32_t L) {
+                                                         uint32_t njobs, uint32_t L,
+                                                         uint32_t waves, uint32_t interleave) {
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (wave >= first[njobs]) return;
-    uint32_t j = 0;
-    while (j + 1 < njobs && first[j + 1] <= wave) j++;
-    const SgmJob job = jobs[j];
-    sgm_path<J>(job.costs, job.guidance, job.columns, job.rows, L, job.paths, wave - first[j]);
+    if (wave >= waves) return;
+    const uint32_t total = first[njobs];
+    for (uint32_t round = 0; round * waves < total; round++) {
+        const uint32_t g = snake_rank(round, wave, waves);
+        if (g >= total) continue;
+        uint32_t j = 0, local;
+        if (interleave) {
+            j = g % njobs;
+            local = g / njobs;
+        } else {
+            while (j + 1 < njobs && first[j + 1] <= g) j++;
+            local = g - first[j];
+        }
+        const SgmJob job = jobs[j];
+        uint32_t d, p;
+        sgm_path_by_rank(local, job.columns, job.rows, d, p);
+        sgm_path<J>(job.costs, job.guidance, job.columns, job.rows, L, job.paths, d, p);
+    }
 }
 
-uint32_t sgm_waves(uint32_t columns, uint32_t rows) {
-    return 2 * rows + 2 * columns + 4 * (columns + rows - 1);
+uint32_t sgm_waves(uint32_t columns, uint32_t rows) { return sgm_path_count(columns, rows); }
+
+// Path waves of one launch: four per SIMD of the device (CUs read once per
+// device), at most one per path.
+static uint32_t sgm_launch_waves(uint32_t paths) {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int n = dev >= 0 && dev < 64 ? cus[dev] : 0;
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        if (dev >= 0 && dev < 64) cus[dev] = n;
+    }
+    const uint32_t w = (uint32_t)n * 16u;
+    return paths < w ? paths : w;
 }
 
 hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
-                            uint32_t njobs, uint32_t total_waves, uint32_t labels,
-                            uint32_t paths_per_block) {
+                            uint32_t njobs, uint32_t total_paths, uint32_t labels,
+                            uint32_t paths_per_block, bool same_grids) {
     const uint32_t J = (labels + 63) / 64;
     const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
-    const dim3 blocks((total_waves + wpb - 1) / wpb), threads(64 * wpb);
+    const uint32_t waves = sgm_launch_waves(total_paths);
+    const dim3 blocks((waves + wpb - 1) / wpb), threads(64 * wpb);
+    const uint32_t interleave = same_grids ? 1u : 0u;
     switch (J) {
 #define CHD_SGM_CASE(n)                                                                        \
     case n:                                                                                    \
         hipLaunchKernelGGL(k_sgm_paths_batch<n>, blocks, threads, 0, stream, jobs, first,      \
-                           njobs, labels);                                                     \
+                           njobs, labels, waves, interleave);                                  \
         break;
         CHD_SGM_CASE(1)
         CHD_SGM_CASE(2)
@@ -274,14 +361,14 @@ hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guida
                       uint32_t columns, uint32_t rows, uint32_t labels, float *paths,
                       uint32_t paths_per_block) {
     const uint32_t J = (labels + 63) / 64;
-    const uint32_t waves = sgm_waves(columns, rows);
+    const uint32_t waves = sgm_launch_waves(sgm_path_count(columns, rows));
     const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
     const dim3 blocks((waves + wpb - 1) / wpb), threads(64 * wpb);
     switch (J) {
 #define CHD_SGM_CASE(n)                                                                        \
     case n:                                                                                    \
         hipLaunchKernelGGL(k_sgm_paths<n>, blocks, threads, 0, stream, costs, guidance,        \
-                           columns, rows, labels, paths);                                      \
+                           columns, rows, labels, paths, waves);                               \
         break;
         CHD_SGM_CASE(1)
         CHD_SGM_CASE(2)
@@ -298,22 +385,35 @@ hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guida
     return hipGetLastError();
 }
 
-// Broadcast value of label `l` held as chunk l/64 of lane l%64.
-__device__ __forceinline__ float label_value(const float *v, uint32_t J, uint32_t l) {
+// Value of label `l` (wave-uniform) held as chunk l/64 of lane l%64.
+template <uint32_t J>
+__device__ __forceinline__ float label_value(const float (&v)[J], uint32_t l) {
     float out = 0.0f;
 #pragma unroll
-    for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
-        if (j < J) {
-            float x = __shfl(v[j], (int)(l & 63u), 64);
-            if (j == (l >> 6)) out = x;
-        }
-    }
+    for (uint32_t j = 0; j < J; j++)
+        if (j == (l >> 6))
+            out = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[j]), (int)(l & 63u)));
     return out;
+}
+
+// Minimum of an int over the wavefront, in a scalar register (the order of
+// an integer minimum does not matter).
+__device__ __forceinline__ int32_t wave_min_i32_uniform(int32_t v) {
+    const int32_t big = 0x7fffffff;
+    auto step = [&](int32_t shifted) { v = shifted < v ? shifted : v; };
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowShr1, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowShr2, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowShr4, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowShr8, 0xf, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowBcast15, 0xa, 0xf, false));
+    step(__builtin_amdgcn_update_dpp(big, v, kDppRowBcast31, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // select_depths (depth.rs:1385-1467) for one node whose regularised costs
 // are held as chunks in the wave; lane 0 writes the node.
-__device__ void select_node(const float *s, uint32_t J, uint32_t L, uint32_t lane, size_t node,
+template <uint32_t J>
+__device__ void select_node(const float (&s)[J], uint32_t L, uint32_t lane, size_t node,
                             const float *scores, const uint8_t *tested, const double *labels,
                             const chd_options &o, chd_node_depth *field, uint8_t *fillable,
                             uint16_t *best_label_out) {
@@ -327,62 +427,43 @@ __device__ void select_node(const float *s, uint32_t J, uint32_t L, uint32_t lan
     uint8_t fill = 0;
     uint16_t best_out = 0;
     if (tested[node] && L >= 2) {
-        // WTA: first minimum over labels 1..L-1 by total_cmp
+        // WTA: first minimum over labels 1..L-1 by total_cmp: the smallest
+        // key over the wave, then the lowest label holding it (chunks in
+        // label order, lanes by the ballot's lowest bit)
+        int32_t key[J];
         int32_t bk = 0x7fffffff;
-        uint32_t bl = 0xffffffffu;
 #pragma unroll
-        for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
-            if (j < J) {
-                uint32_t l = lane + 64 * j;
-                if (l >= 1 && l < L) {
-                    int32_t key = tkey32(s[j]);
-                    if (key < bk) {
-                        bk = key;
-                        bl = l;
-                    }
-                }
-            }
+        for (uint32_t j = 0; j < J; j++) {
+            const uint32_t l = lane + 64 * j;
+            key[j] = tkey32(s[j]);
+            if (l >= 1 && l < L && key[j] < bk) bk = key[j];
         }
+        bk = wave_min_i32_uniform(bk);
+        uint32_t best = 0;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            int32_t ok = __shfl_xor(bk, off, 64);
-            uint32_t ol = __shfl_xor(bl, off, 64);
-            if (ok < bk || (ok == bk && ol < bl)) {
-                bk = ok;
-                bl = ol;
-            }
+        for (uint32_t j = J; j-- > 0;) {
+            const uint32_t l = lane + 64 * j;
+            const uint64_t hit = __ballot(l >= 1 && l < L && key[j] == bk);
+            if (hit) best = 64 * j + (uint32_t)__builtin_ctzll(hit);
         }
-        const uint32_t best = bl;
         best_out = (uint16_t)best;
-        const float best_cost = label_value(s, J, best);
-        const float left = label_value(s, J, best >= 1 ? best - 1 : 0);
-        const float right = label_value(s, J, best + 1 < L ? best + 1 : best);
+        const float best_cost = label_value(s, best);
+        const float left = label_value(s, best >= 1 ? best - 1 : 0);
+        const float right = label_value(s, best + 1 < L ? best + 1 : best);
         // competing minimum over labels more than two away
         int32_t ck = 0x7fffffff;
-        int have = 0;
+        bool candidate = false;
 #pragma unroll
-        for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
-            if (j < J) {
-                uint32_t l = lane + 64 * j;
-                uint32_t diff = l > best ? l - best : best - l;
-                if (l >= 1 && l < L && diff > 2) {
-                    int32_t key = tkey32(s[j]);
-                    if (!have || key < ck) {
-                        ck = key;
-                        have = 1;
-                    }
-                }
+        for (uint32_t j = 0; j < J; j++) {
+            const uint32_t l = lane + 64 * j;
+            const uint32_t diff = l > best ? l - best : best - l;
+            if (l >= 1 && l < L && diff > 2) {
+                candidate = true;
+                if (key[j] < ck) ck = key[j];
             }
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            int32_t ok = __shfl_xor(ck, off, 64);
-            int oh = __shfl_xor(have, off, 64);
-            if (oh && (!have || ok < ck)) {
-                ck = ok;
-                have = 1;
-            }
-        }
+        const int have = __ballot(candidate) != 0;
+        ck = wave_min_i32_uniform(ck);
         if (lane == 0) {
             // sublabel_depth (depth.rs:1469-1488)
             double depth = labels[best];
@@ -447,11 +528,12 @@ __device__ void select_node(const float *s, uint32_t J, uint32_t L, uint32_t lan
 
 // Sum of the eight L_r volumes in the reference's order, fused with
 // select_depths. One wavefront per node.
+template <uint32_t J>
 __global__ void __launch_bounds__(256) k_sgm_select(const float *__restrict__ paths,
                                                     const float *__restrict__ scores,
                                                     const uint8_t *__restrict__ tested,
                                                     const double *__restrict__ labels,
-                                                    uint32_t nodes, uint32_t L, uint32_t J,
+                                                    uint32_t nodes, uint32_t L,
                                                     chd_options o, float *aggregated,
                                                     chd_node_depth *field, uint8_t *fillable,
                                                     uint16_t *best_label,
@@ -462,26 +544,24 @@ __global__ void __launch_bounds__(256) k_sgm_select(const float *__restrict__ pa
     if (node >= nodes) return;
     const size_t volume = (size_t)nodes * L;
     const LabelSet w = label_set(win_lo, win_hi, node, L);
-    float s[kSgmMaxChunks];
+    float s[J];
 #pragma unroll
-    for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
+    for (uint32_t j = 0; j < J; j++) {
         s[j] = 0.0f;
-        if (j < J) {
-            const uint32_t l = lane + 64 * j;
-            if (l < L) {
-                const size_t at = (size_t)node * L + l;
-                float acc = __int_as_float(0x7f800000);  // outside the set: 8 x +inf
-                if (w.has(l)) {
-                    acc = 0.0f;
+        const uint32_t l = lane + 64 * j;
+        if (l < L) {
+            const size_t at = (size_t)node * L + l;
+            float acc = __int_as_float(0x7f800000);  // outside the set: 8 x +inf
+            if (w.has(l)) {
+                acc = 0.0f;
 #pragma unroll
-                    for (uint32_t d = 0; d < 8; d++) acc += paths[d * volume + at];
-                }
-                s[j] = acc;
-                if (aggregated) aggregated[at] = acc;
+                for (uint32_t d = 0; d < 8; d++) acc += paths[d * volume + at];
             }
+            s[j] = acc;
+            if (aggregated) aggregated[at] = acc;
         }
     }
-    select_node(s, J, L, lane, node, scores, tested, labels, o, field, fillable, best_label);
+    select_node<J>(s, L, lane, node, scores, tested, labels, o, field, fillable, best_label);
 }
 
 hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float *scores,
@@ -494,33 +574,47 @@ hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float
     if (J > kSgmMaxChunks) return hipErrorInvalidValue;
     const uint32_t nodes = columns * rows;
     dim3 blocks((nodes + 3) / 4);
-    hipLaunchKernelGGL(k_sgm_select, blocks, dim3(256), 0, stream, paths, scores, tested, labels,
-                       nodes, label_count, J, options, aggregated, field, fillable, best_label,
-                       window_lo, window_hi);
+    switch (J) {
+#define CHD_SELECT_CASE(n)                                                                     \
+    case n:                                                                                    \
+        hipLaunchKernelGGL(k_sgm_select<n>, blocks, dim3(256), 0, stream, paths, scores,       \
+                           tested, labels, nodes, label_count, options, aggregated, field,     \
+                           fillable, best_label, window_lo, window_hi);                        \
+        break;
+        CHD_SELECT_CASE(1)
+        CHD_SELECT_CASE(2)
+        CHD_SELECT_CASE(3)
+        CHD_SELECT_CASE(4)
+        CHD_SELECT_CASE(5)
+        CHD_SELECT_CASE(6)
+        CHD_SELECT_CASE(7)
+        CHD_SELECT_CASE(8)
+#undef CHD_SELECT_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 // select_depths alone, from a given regularised volume.
+template <uint32_t J>
 __global__ void __launch_bounds__(256) k_select_depths(const float *__restrict__ regularised,
                                                        const float *__restrict__ scores,
                                                        const uint8_t *__restrict__ tested,
                                                        const double *__restrict__ labels,
-                                                       uint32_t nodes, uint32_t L, uint32_t J,
+                                                       uint32_t nodes, uint32_t L,
                                                        chd_options o, chd_node_depth *field,
                                                        uint8_t *fillable, uint16_t *best_label) {
     const uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63u;
     if (node >= nodes) return;
-    float s[kSgmMaxChunks];
+    float s[J];
 #pragma unroll
-    for (uint32_t j = 0; j < kSgmMaxChunks; j++) {
-        s[j] = 0.0f;
-        if (j < J) {
-            const uint32_t l = lane + 64 * j;
-            if (l < L) s[j] = regularised[(size_t)node * L + l];
-        }
+    for (uint32_t j = 0; j < J; j++) {
+        const uint32_t l = lane + 64 * j;
+        s[j] = l < L ? regularised[(size_t)node * L + l] : 0.0f;
     }
-    select_node(s, J, L, lane, node, scores, tested, labels, o, field, fillable, best_label);
+    select_node<J>(s, L, lane, node, scores, tested, labels, o, field, fillable, best_label);
 }
 
 hipError_t launch_select_depths(hipStream_t stream, const float *regularised,
@@ -531,8 +625,25 @@ hipError_t launch_select_depths(hipStream_t stream, const float *regularised,
     const uint32_t J = (label_count + 63) / 64;
     if (J > kSgmMaxChunks) return hipErrorInvalidValue;
     dim3 blocks((nodes + 3) / 4);
-    hipLaunchKernelGGL(k_select_depths, blocks, dim3(256), 0, stream, regularised, scores, tested,
-                       labels, nodes, label_count, J, options, field, fillable, best_label);
+    switch (J) {
+#define CHD_SELECT_CASE(n)                                                                     \
+    case n:                                                                                    \
+        hipLaunchKernelGGL(k_select_depths<n>, blocks, dim3(256), 0, stream, regularised,      \
+                           scores, tested, labels, nodes, label_count, options, field,         \
+                           fillable, best_label);                                              \
+        break;
+        CHD_SELECT_CASE(1)
+        CHD_SELECT_CASE(2)
+        CHD_SELECT_CASE(3)
+        CHD_SELECT_CASE(4)
+        CHD_SELECT_CASE(5)
+        CHD_SELECT_CASE(6)
+        CHD_SELECT_CASE(7)
+        CHD_SELECT_CASE(8)
+#undef CHD_SELECT_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
