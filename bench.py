@@ -154,7 +154,30 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB of device memory once (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--copy-bandwidth", type=float, default=0.0, metavar="GIB",
+                    help="only time device-to-device copies of GIB (the practical ceiling of "
+                         "kernels that read and write HBM in equal parts)")
     args = ap.parse_args()
+
+    if args.copy_bandwidth > 0:
+        import torch
+        n = int(args.copy_bandwidth * (1 << 30)) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        start.record()
+        for _ in range(reps):
+            y.copy_(x)
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / reps
+        print(json.dumps({"copy_bytes_each_way": n * 4, "ms_per_copy": round(ms, 4),
+                          "read_plus_write_GBs": round(2 * n * 4 / (ms / 1000.0) / 1e9, 1)}))
+        return 0
 
     if args.copy_probe > 0:
         import torch
