@@ -97,6 +97,16 @@ def cpu_baseline(cap, rows: int, threads: int) -> dict:
     scale = g.coarse_rows / rows
     total = (t2 - t0) * scale
     pixels = ref.width * ref.height
+    # the same cost volume on one thread over a few of those rows
+    rows1 = min(8, rows)
+    first1 = (g.coarse_rows - rows1) // 2
+    t3 = time.perf_counter()
+    lib.or_cost_volume_rows(views.views, len(cap.modules), cap.reference_index,
+                            C.byref(cap.options.to_c()), first1, first1 + rows1, 1,
+                            scores.ctypes.data, costs.ctypes.data, guidance.ctypes.data,
+                            tested.ctypes.data)
+    t4 = time.perf_counter()
+    one_thread_cost = (t4 - t3) * g.coarse_rows / rows1
     return {
         "value": pixels * cap.options.planes / total / 1e6,
         "unit": "Mpix·hyp/s",
@@ -110,6 +120,12 @@ def cpu_baseline(cap, rows: int, threads: int) -> dict:
         "seconds_cost_sample": t1 - t0,
         "seconds_sgm_sample": t2 - t1,
         "seconds_estimated_full": total,
+        "one_thread": {
+            "sample": f"oracle cost volume over coarse rows {first1}..{first1 + rows1}, 1 thread",
+            "seconds_sample": t4 - t3,
+            "cost_value": pixels * cap.options.planes / one_thread_cost / 1e6,
+            "unit": "Mpix·hyp/s (cost volume alone)",
+        },
     }
 
 
