@@ -35,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("Mpix·hyp/s cost+SGM on 1 GPU; learned-band speedup at ≥98% "
           "±1-hyp agreement")
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # Algorithmic work per (node, label, valid view) ZNCC of the cost volume:
 # 81 taps x (3 bilinear samples' worth of f32 arithmetic + 2 weights) in
@@ -313,6 +314,21 @@ def main() -> int:
         "peak_GBs": PEAK_HBM_GBS,
     }
     hbm["frac"] = round(hbm["achieved_GBs"] / PEAK_HBM_GBS, 4)
+    # composite lower bound of the step: per stage the larger of its HBM
+    # time and its FP32 + FP64 issue time at the published peaks
+    t_cost = max(units * 8 / (PEAK_HBM_GBS * 1e9),
+                 work["cost_zncc"] * (FP32_PER_ZNCC / (PEAK_FP32_TFLOPS * 1e12)
+                                      + FP64_PER_ZNCC / (PEAK_FP64_TFLOPS * 1e12)))
+    t_sgm = units * 96 / (PEAK_HBM_GBS * 1e9)
+    t_select = units * 4 / (PEAK_HBM_GBS * 1e9)
+    t_lb = (t_cost + t_sgm + t_select) * 1000.0
+    composite = {
+        "definition": "sum over cost, SGM paths, select of max(bytes / 8 TB/s, "
+                      "FP32 / 157.3 TFLOP/s + FP64 / 78.6 TFLOP/s); frac = lower bound / step",
+        "lower_bound_ms": round(t_lb, 4),
+        "step_ms": round(ms_per_step, 4),
+        "frac": round(t_lb / ms_per_step, 4),
+    }
 
     cpu = None
     if args.cpu_rows > 0:
@@ -347,6 +363,7 @@ def main() -> int:
         "kernels_ms": {k: round(v[1] / args.steps, 4) for k, v in kernels.items()},
         "roofline": roofline,
         "sgm_hbm": hbm,
+        "composite_roofline": composite,
         "full_path_ms": None if full_ms is None else round(full_ms, 2),
         "input_prep_ms_incl_pcie": None if prep_ms is None else round(prep_ms, 2),
         "full_path_kernels_ms": full_kernels,
