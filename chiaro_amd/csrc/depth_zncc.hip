@@ -416,8 +416,9 @@ __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t
                                              st, stt, sp);
 }
 
-// warped_patch_zncc (depth.rs:1772-1822) in two halves. The bounds half: every tap's bilinear footprint lies inside the plane
-// (the extremes of the warped taps decide it, since to_plane is monotone).
+// warped_patch_zncc (depth.rs:1772-1822) in two halves. The bounds half:
+// every tap's bilinear footprint lies inside the plane (the extremes of the
+// warped taps decide it, since to_plane is monotone).
 __device__ __forceinline__ bool zncc_inside(const PatchLds &P, uint32_t n, const DevPlane &tp,
                                             double dx, double dy) {
     const float4 e = P.ext[n];
