@@ -126,7 +126,10 @@ def test_completion_and_options_bit_exact(gpu_context):
 def test_sgm_stage_on_oracle_costs(gpu_context, oracle_lib):
     _say("sgm stage on random costs")
     rng = np.random.default_rng(11)
-    for columns, rows, labels in [(5, 1, 4), (17, 9, 65), (31, 23, 129), (7, 40, 257)]:
+    # wide, tall, square and single-column grids: every path order and the
+    # diagonal families' two length classes
+    for columns, rows, labels in [(5, 1, 4), (17, 9, 65), (31, 23, 129), (7, 40, 257),
+                                  (1, 6, 4), (12, 12, 64), (3, 50, 129)]:
         costs = rng.random((rows * columns, labels), dtype=np.float32) * 2.25
         guidance = rng.random(rows * columns, dtype=np.float32) * 3
         got = np.empty_like(costs)
