@@ -153,6 +153,9 @@ def main() -> int:
                     default="config1")
     ap.add_argument("--train-scenes", type=int, default=512,
                     help="config3: training scenes for the learned band proposal")
+    ap.add_argument("--train-wide-only", action="store_true",
+                    help="config4-learned: training captures render only the five wide modules "
+                         "(labels from their full sweep; a third of the rendering per capture)")
     ap.add_argument("--eval-scenes", type=int, default=24, help="config3: held-out scenes")
     ap.add_argument("--epochs", type=int, default=10, help="config3: training epochs")
     ap.add_argument("--learned-scenes", type=int, default=1000,
@@ -644,7 +647,7 @@ def bench_learned4(args) -> int:
     made = [0]
 
     def train_capture(seed):
-        cap = scene.config4(seed=seed, scale=0.5)
+        cap = scene.config4(seed=seed, scale=0.5, wide_only=args.train_wide_only)
         cap.options.grid_step = 4
         cap.raws = None
         made[0] += 1
@@ -702,7 +705,9 @@ def bench_learned4(args) -> int:
         "config": {
             "workload": (f"config4 learned band: {count} captures x 16 modules 4160x3120, 256 "
                          f"hypotheses, batched; network trained on {args.train_scenes} half-scale "
-                         f"config4 captures x {args.epochs} epochs"),
+                         f"config4 captures"
+                         f"{' (five wide modules)' if args.train_wide_only else ''} x "
+                         f"{args.epochs} epochs"),
             "parallelism": "1 GPU",
         },
         "full_sweep_ms": round(full_ms, 3),

@@ -255,7 +255,7 @@ def config3(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float
 
 
 def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
-            plane_count: int = 24) -> Capture:
+            plane_count: int = 24, wide_only: bool = False) -> Capture:
     """One capture of the full-capture workload: sixteen 4160x3120 modules
     viewing nested fields of one 24-plane scene - five wide (f = 3300 px: the
     reference at the origin and four on a 30 mm ring at 0/90/180/270 deg),
@@ -265,13 +265,16 @@ def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
     config1, 256 hypotheses, near 800 mm, far 50000 mm. `scale` shrinks every
     raster, focal length and the texture's cell size together (small test
     captures; at 0.5 with a 4 px grid the coarse grid matches the full
-    capture's at 8 px)."""
+    capture's at 8 px). `wide_only` keeps the five wide modules (the same
+    scene and reference, a third of the rendering)."""
     def size(n):
         return max(16, int(round(n * scale)) // 8 * 8)
 
     w, h = size(4160), size(3120)
     modules = [_module(3300.0 * scale, w, h, (0.0, 0.0, 0.0))]
     rings = [(3300.0, 30.0, 0.0, 4), (8250.0, 20.0, 36.0, 5), (17700.0, 45.0, 30.0, 6)]
+    if wide_only:
+        rings = rings[:1]
     for f, radius, first, count in rings:
         step = 90.0 if count == 4 else 360.0 / count
         for k in range(count):
@@ -284,7 +287,8 @@ def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
     raws = _render(desc, modules, 0, 0.0, bayer)
     planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
     options = DepthOptions(near_depth=800.0, far_depth=50000.0, planes=planes)
-    names = [f"W{k}" for k in range(5)] + [f"M{k}" for k in range(5)] + [f"T{k}" for k in range(6)]
+    names = ([f"W{k}" for k in range(5)] + [f"M{k}" for k in range(5)]
+             + [f"T{k}" for k in range(6)])[:len(modules)]
     cap = _assemble(modules, planes_, 0, options,
                     {"workload": "config4", "seed": seed, "scale": scale, "views": len(modules),
                      "planes": planes}, desc, names)
