@@ -677,7 +677,7 @@ def bench_learned4(args) -> int:
         caps.append(scene.config4(seed=100000 + k))
         caps[-1].raws = None
     settings = [(thr, tile, mh) for mh in (8, 16, 24, 32) for tile in (4, 8)
-                for thr in (0.0, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)]
+                for thr in (0.0, 0.2, 0.25, 0.3, 0.35, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)]
     rows = batch_agreement(ctx, caps, net, settings, steps=args.steps)
     for r in rows:
         log(f"[bench] max half {r['max_half_width']} tile {r['tile']} threshold {r['threshold']}: "
