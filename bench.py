@@ -676,13 +676,13 @@ def bench_learned4(args) -> int:
     for k in range(count):
         caps.append(scene.config4(seed=100000 + k))
         caps[-1].raws = None
-    settings = [(thr, tile, mh) for mh in (8, 16, 24, 32) for tile in (4, 8)
-                for thr in (0.0, 0.2, 0.25, 0.3, 0.35, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9)]
+    settings = [(thr, tile, mh, half) for mh in (8, 16, 24, 32) for half in (8, 12) if half <= mh
+                for tile in (4, 8) for thr in (0.0, 0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
     rows = batch_agreement(ctx, caps, net, settings, steps=args.steps)
     for r in rows:
-        log(f"[bench] max half {r['max_half_width']} tile {r['tile']} threshold {r['threshold']}: "
-            f"agreement {r['agreement']:.4f} fallback {r['fallback_fraction']:.3f} "
-            f"speedup {r['speedup']:.3f}")
+        log(f"[bench] max half {r['max_half_width']} min half {r['half_width']} tile {r['tile']} "
+            f"threshold {r['threshold']}: agreement {r['agreement']:.4f} "
+            f"fallback {r['fallback_fraction']:.3f} speedup {r['speedup']:.3f}")
     ok = [r for r in rows if r["agreement"] >= 0.98]
     best = max(ok, key=lambda r: r["speedup"]) if ok else None
     chosen = best or max(rows, key=lambda r: r["agreement"])
@@ -717,6 +717,7 @@ def bench_learned4(args) -> int:
             "threshold": chosen["threshold"],
             "tile": chosen["tile"],
             "max_half_width": chosen["max_half_width"],
+            "half_width": chosen["half_width"],
             "speedup": round(chosen["speedup"], 3),
             "agreement": round(chosen["agreement"], 5),
             "mask_agreement": round(chosen["mask_agreement"], 5),

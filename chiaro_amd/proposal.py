@@ -477,8 +477,8 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
 
 def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int = 3):
     """The captures run as one batch (chd_batch): the full sweep against the
-    band sweep for each (threshold, tile, max_half_width) in `settings`
-    (half_width 8). Per setting: the fraction of nodes tested by the full
+    band sweep for each (threshold, tile, max_half_width[, half_width]) in
+    `settings` (half_width 8 unless given). Per setting: the fraction of nodes tested by the full
     sweep whose band label is within +-1, mask agreement, fallback-tile
     fraction and both times per batch (the band time includes the network
     and the windows of every capture)."""
@@ -497,10 +497,12 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
         full_ms = (time.perf_counter() - t0) * 1000 / steps
         full_best = [s.coarse()[2] for s in sessions]
         full_tested = [s.tested() for s in sessions]
-        for threshold, tile, max_half in settings:
+        for setting in settings:
+            threshold, tile, max_half = setting[:3]
+            half = setting[3] if len(setting) > 3 else 8
             fallback = tiles = 0
             for s in sessions:
-                f, t = proposal.apply(s, threshold, 8, tile, max_half_width=max_half)
+                f, t = proposal.apply(s, threshold, half, tile, max_half_width=max_half)
                 fallback += f
                 tiles += t
             batch.run(stages)
@@ -508,7 +510,8 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
             t0 = time.perf_counter()
             for _ in range(steps):
                 for s in sessions:
-                    proposal.apply(s, threshold, 8, tile, report=False, max_half_width=max_half)
+                    proposal.apply(s, threshold, half, tile, report=False,
+                                   max_half_width=max_half)
                 batch.run(stages)
             ctx.synchronize()
             band_ms = (time.perf_counter() - t0) * 1000 / steps
@@ -521,6 +524,7 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
                 valid += int(v.sum())
                 mask += float((bt == ft).mean()) / len(sessions)
             rows.append({"threshold": threshold, "tile": tile, "max_half_width": max_half,
+                         "half_width": half,
                          "agreement": close / max(valid, 1), "mask_agreement": mask,
                          "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
                          "band_ms": band_ms, "speedup": full_ms / band_ms})
