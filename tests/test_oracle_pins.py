@@ -118,6 +118,21 @@ def test_scene_global_warp_matches_oracle(oracle_lib):
     assert np.all(np.abs(got[..., 0] - xs) < 1e-3) and np.all(np.abs(got[..., 1] - ys) < 1e-3)
 
 
+def test_config4_wide_only_keeps_the_scene():
+    """The five-wide-module config 4 capture (the learned band's training
+    captures) is the full capture's first five modules: same cameras, planes,
+    warps and truth, byte for byte."""
+    wide = scene.config4(seed=10007, scale=0.25, wide_only=True)
+    full = scene.config4(seed=10007, scale=0.25)
+    assert len(wide.modules) == 5 and len(full.modules) == 16
+    for a, b in zip(wide.modules, full.modules[:5]):
+        assert a.name == b.name
+        assert bytes(a.camera) == bytes(b.camera)
+        assert np.array_equal(a.luminance.view(np.uint32), b.luminance.view(np.uint32))
+        assert np.array_equal(a.warp_points.view(np.uint32), b.warp_points.view(np.uint32))
+    assert np.array_equal(wide.truth_inverse_distance, full.truth_inverse_distance)
+
+
 # ------------------------------------------------------------------- ZNCC
 
 def _identity_warp(w, h, step=32):
