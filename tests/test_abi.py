@@ -85,3 +85,16 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         for m in members:
             field = "global_" if m == "global" else m
             assert int(out[f"{cname}.{m}"]) == getattr(cls, field).offset, f"{cname}.{m}"
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits non-zero with a message,
+    before touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    proc = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                          capture_output=True, text=True, timeout=120, cwd=root)
+    assert proc.returncode != 0
+    assert "one GPU" in proc.stdout + proc.stderr
