@@ -23,8 +23,9 @@ VP = C.c_void_p
 _lib = None
 
 
-def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+def build(force: bool = False) -> Path:
+    """Compile the oracle library (with `force`, even when it is up to date)."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", str(HERE)], check=True)
     return LIB_PATH
 
 
