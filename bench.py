@@ -62,6 +62,61 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def host_cpu() -> dict:
+    """The host's processor: model string, family/model/stepping numbers,
+    logical CPUs, physical cores, the affinity mask and the cgroup CPU
+    quota, and std::thread::available_parallelism's answer (the smaller of
+    the affinity mask and the quota), which sizes the reference's row bands
+    (depth.rs:729-731, 1204-1206)."""
+    info = {"model": cpu_model(), "logical_cpus": os.cpu_count()}
+    ids, cores, fields = set(), set(), {}
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key in ("cpu family", "model", "stepping", "vendor_id") and key not in fields:
+                fields[key] = val
+            if key == "physical id":
+                phys = val
+                ids.add(val)
+            elif key == "core id":
+                core = val
+            elif not key and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    names = {"vendor_id": "vendor", "cpu family": "family", "model": "model_number",
+             "stepping": "stepping"}
+    info.update({names[k]: v for k, v in fields.items()})
+    info["sockets"] = len(ids) or None
+    info["physical_cores"] = len(cores) or None
+    affinity = len(os.sched_getaffinity(0))
+    info["affinity_cpus"] = affinity
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = -(-int(q) // int(period))
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    info["available_parallelism"] = min(affinity, quota) if quota else affinity
+    return info
+
+
+def cpu_threads(host: dict) -> int:
+    """Threads of the CPU baseline: available_parallelism, as the reference
+    sizes its row bands, within the host's CPU share for this GPU when the
+    environment states one (OMP_NUM_THREADS on a shared multi-GPU host)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    n = host["available_parallelism"]
+    return max(1, min(n, share) if share > 0 else n)
+
+
 def cpu_baseline(cap, rows: int, threads: int) -> dict:
     """The oracle's cost volume over a band of coarse rows with the
     reference's row-band threading, plus its single-threaded SGM over the
@@ -126,6 +181,41 @@ def cpu_baseline(cap, rows: int, threads: int) -> dict:
             "seconds_sample": t4 - t3,
             "cost_value": pixels * cap.options.planes / one_thread_cost / 1e6,
             "unit": "Mpix·hyp/s (cost volume alone)",
+        },
+    }
+
+
+def valu_roofline(zncc: int, ms: float, kernel: str, traffic) -> dict:
+    """Roofline block of a ZNCC kernel. Its bound is the vector ALU: the
+    published FP32 peak (157.3 TFLOP/s) counts a fused multiply-add as two
+    flops and packed FP32 as two lanes, and bit-exact ZNCC arithmetic may use
+    neither, so the kernel's own ceiling is one unfused, unpacked lane
+    operation per lane per clock: 157.3 / 4 = 39.3 T lane-ops/s, which FP32
+    and FP64 operations share (78.6 / 2)."""
+    s = ms / 1000.0
+    fp32 = zncc * FP32_PER_ZNCC / s / 1e12
+    fp64 = zncc * FP64_PER_ZNCC / s / 1e12
+    ceiling = PEAK_FP32_TFLOPS / 4.0
+    return {
+        "bound": "valu",
+        "achieved": round(fp32, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(fp32 / PEAK_FP32_TFLOPS, 4),
+        "traffic": traffic,
+        "kernel": kernel,
+        "compute": "vector FP32 + FP64 ALU (ZNCC gathers have no matrix form)",
+        "avg_launch_ms": round(ms, 4),
+        "algorithmic_fp32_per_zncc": FP32_PER_ZNCC,
+        "algorithmic_fp64_per_zncc": FP64_PER_ZNCC,
+        "zncc_per_launch": zncc,
+        "fp64_achieved_tflops": round(fp64, 3),
+        "issue_ceiling": {
+            "definition": "unfused, unpacked lane operations: FP32 157.3/4 = FP64 78.6/2 "
+                          "T ops/s, shared by both; achieved = (FP32 + FP64 algorithmic ops) / time",
+            "peak_tops": round(ceiling, 2),
+            "achieved_tops": round(fp32 + fp64, 3),
+            "frac": round((fp32 + fp64) / ceiling, 4),
         },
     }
 
@@ -292,22 +382,8 @@ def main() -> int:
 
     launches, total_ms = kernels["cost_volume"]
     cost_ms = total_ms / launches
-    achieved = work["cost_zncc"] * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
-    roofline = {
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": PEAK_FP32_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": load_traffic("cost_volume", args.workload),
-        "kernel": "k_cost_volume",
-        "compute": "vector FP32 + FP64 ALU (ZNCC gathers have no matrix form)",
-        "avg_launch_ms": round(cost_ms, 4),
-        "algorithmic_fp32_per_zncc": FP32_PER_ZNCC,
-        "algorithmic_fp64_per_zncc": FP64_PER_ZNCC,
-        "zncc_per_launch": work["cost_zncc"],
-        "fp64_achieved_tflops": round(work["cost_zncc"] * FP64_PER_ZNCC / (cost_ms / 1000.0) / 1e12, 3),
-    }
+    roofline = valu_roofline(work["cost_zncc"], cost_ms, "k_cost_volume",
+                             load_traffic("cost_volume", args.workload))
     sgm_ms = sum(v[1] for k, v in kernels.items() if k.startswith("sgm")) / args.steps
     hbm = {
         "kernels": "sgm_paths + sgm_select",
@@ -335,9 +411,11 @@ def main() -> int:
 
     cpu = None
     if args.cpu_rows > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = args.cpu_threads or cpu_threads(host)
         log(f"[bench] CPU baseline: {args.cpu_rows} coarse rows on {threads} threads")
         cpu = cpu_baseline(cap, args.cpu_rows, threads)
+        cpu["host"] = host
 
     line = {
         "metric": METRIC,
@@ -454,7 +532,6 @@ def bench_batch(args) -> int:
                     f"8-path SGM, seeds {args.seed}..{args.seed + len(caps) - 1}, batched")
     value = pixels * planes / (ms / 1000.0) / 1e6
     g = sessions[0].grid
-    achieved = zncc * FP32_PER_ZNCC / (cost_ms / 1000.0) / 1e12
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -479,16 +556,7 @@ def bench_batch(args) -> int:
         "sequential_ms_per_step": round(seq_ms, 3),
         "batched_over_sequential": round(seq_ms / ms, 3),
         "cost_ms_per_step": round(cost_ms, 3),
-        "roofline": {
-            "bound": "mfma",
-            "achieved": round(achieved, 3),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
-            "kernel": "k_cost_volume (one launch per capture)",
-            "zncc_per_step": zncc,
-        },
+        "roofline": valu_roofline(zncc, cost_ms, "k_cost_volume (one launch per capture)", None),
         "cpu_baseline": None,
     }
     batch.close()

@@ -294,6 +294,7 @@ DEPTH_SIGNATURES = {
     "chd_proposal_create": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P(VP)),
     "chd_proposal_destroy": (None, VP),
     "chd_proposal_parameters": (C.c_int, VP, VP, VP, P(C.c_size_t)),
+    "chd_proposal_optimizer_state": (C.c_int, VP, VP, VP, P(C.c_size_t), P(C.c_uint64)),
     "chd_proposal_set_dataset": (C.c_int, VP, VP, VP, C.c_uint32),
     "chd_proposal_train": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, VP),
     "chd_proposal_predict": (C.c_int, VP, VP, C.c_uint32, VP),
@@ -305,6 +306,7 @@ DEPTH_SIGNATURES = {
     "chd_session_set_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
                                  C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
     "chd_session_clear_proposal": (C.c_int, VP),
+    "chd_session_read_band": (C.c_int, VP, VP, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),
     "chd_select_depths": (C.c_int, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32, P(Options), VP, VP,
                           VP),

@@ -99,7 +99,40 @@ struct chd_context {
     // luminance_half table for the last (black, white) pair
     float *luminance_table = nullptr;
     float luminance_black = 0.0f, luminance_white = 0.0f;
+    // Live objects made on this context. chd_context_destroy with any of
+    // them alive only marks the context released; the last child's destroy
+    // frees it, so objects may be destroyed in any order.
+    std::vector<chd_session *> sessions;
+    std::vector<chd_batch *> batches;
+    std::vector<chd_proposal *> proposals;
+    bool released = false;
 };
+
+namespace {
+
+template <class T>
+void unregister(std::vector<T *> &list, T *item) {
+    list.erase(std::remove(list.begin(), list.end(), item), list.end());
+}
+
+void context_free(chd_context *c) {
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    if (c->luminance_table) hipFree(c->luminance_table);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Frees a released context once its last child is gone.
+void context_release_if_unused(chd_context *c) {
+    if (c->released && c->sessions.empty() && c->batches.empty() && c->proposals.empty())
+        context_free(c);
+}
+
+// Drops a destroyed session from a batch and marks the batch unusable.
+void batch_forget_session(chd_batch *b, chd_session *s);
+
+}  // namespace
 
 struct chd_session {
     chd_context *context = nullptr;
@@ -147,6 +180,7 @@ struct chd_session {
     bool band = false;
     uint32_t band_window_max = 0;  // labels of the widest band window (2 max_half + 2)
     uint32_t version = 0;  // bumped when launch parameters change (batches recapture)
+    std::vector<chd_batch *> batches;  // live batches that hold this session
     chd_tuning tuning{};
     uint32_t stages_done = 0;
     bool timing = false;
@@ -506,12 +540,11 @@ int chd_context_create(int device, chd_context **out) {
 }
 
 void chd_context_destroy(chd_context *c) {
-    if (!c) return;
+    if (!c || c->released) return;
+    c->released = true;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    if (c->luminance_table) hipFree(c->luminance_table);
-    hipStreamDestroy(c->stream);
-    delete c;
+    context_release_if_unused(c);
 }
 
 int chd_context_synchronize(chd_context *c) {
@@ -525,6 +558,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
                        chd_session **out) {
     if (!context || !views || !options || !out) return fail(CHD_ERR_ARGUMENT, "session: null argument");
     *out = nullptr;
+    if (context->released) return fail(CHD_ERR_STATE, "session: the context was destroyed");
     CHD_HIP(hipSetDevice(context->device));
     auto *s = new chd_session;
     s->context = context;
@@ -535,6 +569,7 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     if (!options_valid(*options) || reference_index >= count ||
         views[reference_index].camera == nullptr) {
         s->none = true;
+        context->sessions.push_back(s);
         *out = s;
         return CHD_OK;
     }
@@ -718,17 +753,22 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
 #undef CHD_UPLOAD
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("upload sync: ") + hipGetErrorString(e)));
+    context->sessions.push_back(s);
     *out = s;
     return CHD_OK;
 }
 
 void chd_session_destroy(chd_session *s) {
     if (!s) return;
-    if (s->context) {
-        hipSetDevice(s->context->device);
-        hipStreamSynchronize(s->context->stream);
-    }
+    chd_context *c = s->context;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    // a batch that holds this session can no longer run (chd_batch_run
+    // returns CHD_ERR_STATE); it stays valid for chd_batch_destroy
+    for (chd_batch *b : s->batches) batch_forget_session(b, s);
+    unregister(c->sessions, s);
     delete s;
+    context_release_if_unused(c);
 }
 
 int chd_session_grid(const chd_session *s, chd_grid_sizes *out) {
@@ -1397,6 +1437,11 @@ int chd_autotune(chd_session *s, const char *cache_path, chd_tuning *out) {
         *out = best;
         return CHD_OK;
     }
+    // every trial shape is set on the session directly; the session ends
+    // on its original shape (on failure) or on `best` through
+    // chd_session_set_tuning, which bumps the version so that batches
+    // holding the session rebuild their job tables
+    const chd_tuning original = s->tuning;
     const bool timing = s->timing;
     s->timing = false;
     const uint32_t L = s->grid.labels;
@@ -1441,8 +1486,9 @@ int chd_autotune(chd_session *s, const char *cache_path, chd_tuning *out) {
         }
     }
     s->timing = timing;
+    s->tuning = original;
     if (r != CHD_OK) return r;
-    s->tuning = best;
+    CHD_CHECK(chd_session_set_tuning(s, &best));
     append_tuning_cache(cache_path, key, best);
     *out = best;
     return CHD_OK;
@@ -1464,6 +1510,7 @@ struct chd_batch {
     uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
     bool sgm_same_grids = true;  // every capture on the same coarse grid
     bool persistent = false;  // one queue launch for every cost volume
+    bool broken = false;      // a session was destroyed under the batch
     int queue_mode = 4;       // the jobs' common kind for the queue kernel (4: mixed)
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
     std::vector<hipGraph_t> templates;
@@ -1482,6 +1529,17 @@ struct chd_batch {
         for (void *p : allocations) hipFree(p);
     }
 };
+
+namespace {
+
+void batch_forget_session(chd_batch *b, chd_session *s) {
+    for (chd_session *&m : b->sessions)
+        if (m == s) m = nullptr;
+    b->broken = true;
+    b->drop_graphs();  // their nodes point into the session's buffers
+}
+
+}  // namespace
 
 extern "C++" {
 namespace {
@@ -1592,6 +1650,7 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
     if (!c || !sessions || !out || count == 0) return fail(CHD_ERR_ARGUMENT, "batch: bad argument");
     if (flags & ~CHD_BATCH_PERSISTENT) return fail(CHD_ERR_ARGUMENT, "batch: unknown flags");
     *out = nullptr;
+    if (c->released) return fail(CHD_ERR_STATE, "batch: the context was destroyed");
     CHD_HIP(hipSetDevice(c->device));
     auto *b = new chd_batch;
     b->context = c;
@@ -1619,12 +1678,17 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
     if (r == CHD_OK) r = balloc(b, &b->sgm_first, count + 1);
     if (r == CHD_OK) r = batch_prepare(b);
     if (r != CHD_OK) return bail(r);
+    for (chd_session *s : b->sessions)
+        if (std::find(s->batches.begin(), s->batches.end(), b) == s->batches.end())
+            s->batches.push_back(b);
+    c->batches.push_back(b);
     *out = b;
     return CHD_OK;
 }
 
 int chd_batch_run(chd_batch *b, uint32_t stages) {
     if (!b) return fail(CHD_ERR_ARGUMENT, "null batch");
+    if (b->broken) return fail(CHD_ERR_STATE, "batch: one of its sessions was destroyed");
     CHD_HIP(hipSetDevice(b->context->device));
     for (chd_session *s : b->sessions) {
         const uint32_t done = s->stages_done | (stages & (CHD_STAGE_COST | CHD_STAGE_SGM | CHD_STAGE_DIRECT));
@@ -1695,9 +1759,14 @@ int chd_batch_run(chd_batch *b, uint32_t stages) {
 
 void chd_batch_destroy(chd_batch *b) {
     if (!b) return;
-    hipSetDevice(b->context->device);
-    hipStreamSynchronize(b->context->stream);
+    chd_context *c = b->context;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (chd_session *s : b->sessions)
+        if (s) unregister(s->batches, b);
+    unregister(c->batches, b);
     delete b;
+    context_release_if_unused(c);
 }
 
 // ------------------------------------------------------------ proposal
@@ -1844,6 +1913,7 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
     if ((size_t)width * height * max_batch > ((size_t)1 << 26))
         return fail(CHD_ERR_LIMIT, "proposal: batch x width x height above 2^26 pixels");
     *out = nullptr;
+    if (c->released) return fail(CHD_ERR_STATE, "proposal: the context was destroyed");
     CHD_HIP(hipSetDevice(c->device));
     auto *p = new chd_proposal;
     p->context = c;
@@ -1905,18 +1975,23 @@ int chd_proposal_create(chd_context *c, uint32_t width, uint32_t height, uint32_
     if (r == CHD_OK) r = proposal_refresh_weights(p);
     if (r != CHD_OK) return bail(r);
     // act[0]'s padding channels must read zero
-    CHD_HIP(launch_fill_u32(c->stream, (uint32_t *)p->act[0], 0u,
-                            (size_t)max_batch * width * height * 16 / 2));
-    CHD_HIP(hipStreamSynchronize(c->stream));
+    hipError_t e = launch_fill_u32(c->stream, (uint32_t *)p->act[0], 0u,
+                                   (size_t)max_batch * width * height * 16 / 2);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("proposal init: ") + hipGetErrorString(e)));
+    c->proposals.push_back(p);
     *out = p;
     return CHD_OK;
 }
 
 void chd_proposal_destroy(chd_proposal *p) {
     if (!p) return;
-    hipSetDevice(p->context->device);
-    hipStreamSynchronize(p->context->stream);
+    chd_context *c = p->context;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    unregister(c->proposals, p);
     delete p;
+    context_release_if_unused(c);
 }
 
 int chd_proposal_parameters(chd_proposal *p, float *get, const float *set, size_t *count) {
@@ -1943,6 +2018,42 @@ int chd_proposal_parameters(chd_proposal *p, float *get, const float *set, size_
     }
     if (set) CHD_CHECK(proposal_refresh_weights(p));
     CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_proposal_optimizer_state(chd_proposal *p, float *get, const float *set, size_t *count,
+                                 uint64_t *step) {
+    if (!p || !count || !step) return fail(CHD_ERR_ARGUMENT, "proposal_optimizer_state: bad argument");
+    const size_t n = 2 * proposal_parameter_count(p);
+    if (!get && !set) {
+        *count = n;
+        *step = p->step;
+        return CHD_OK;
+    }
+    if (*count != n) return fail(CHD_ERR_ARGUMENT, "proposal_optimizer_state: wrong count");
+    CHD_HIP(hipSetDevice(p->context->device));
+    hipStream_t st = p->context->stream;
+    const size_t half = n / 2;
+    size_t at = 0;
+    for (int i = 0; i < 7; i++) {
+        const ConvLayer &L = p->layers[i];
+        const size_t nw = (size_t)L.k * L.cout_p;
+        auto &q = p->params[i];
+        // first moments in parameter order, then second moments
+        float *const dev[4] = {q.mw, q.mb, q.vw, q.vb};
+        const size_t off[4] = {at, at + nw, half + at, half + at + nw};
+        const size_t len[4] = {nw, L.cout_p, nw, L.cout_p};
+        for (int k = 0; k < 4; k++) {
+            if (get)
+                CHD_HIP(hipMemcpyAsync(get + off[k], dev[k], len[k] * 4, hipMemcpyDeviceToHost, st));
+            else
+                CHD_HIP(hipMemcpyAsync(dev[k], set + off[k], len[k] * 4, hipMemcpyHostToDevice, st));
+        }
+        at += nw + L.cout_p;
+    }
+    CHD_HIP(hipStreamSynchronize(st));
+    if (get) *step = p->step;
+    else p->step = *step;
     return CHD_OK;
 }
 
@@ -2147,6 +2258,18 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
         CHD_HIP(hipMemcpyAsync(full_tiles, p->full_tiles, 4, hipMemcpyDeviceToHost, st));
         CHD_HIP(hipStreamSynchronize(st));
     }
+    return CHD_OK;
+}
+
+int chd_session_read_band(chd_session *s, uint16_t *lo, uint16_t *hi) {
+    if (!s || !lo || !hi) return fail(CHD_ERR_ARGUMENT, "read_band: bad argument");
+    if (s->none) return CHD_NONE;
+    if (!s->band) return fail(CHD_ERR_STATE, "read_band: the session sweeps every label");
+    CHD_HIP(hipSetDevice(s->context->device));
+    const size_t nodes = (size_t)s->grid.coarse_columns * s->grid.coarse_rows;
+    CHD_CHECK(download(s, lo, s->band_lo, nodes * 2));
+    CHD_CHECK(download(s, hi, s->band_hi, nodes * 2));
+    CHD_HIP(hipStreamSynchronize(s->context->stream));
     return CHD_OK;
 }
 

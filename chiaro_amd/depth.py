@@ -85,9 +85,11 @@ class Context:
         handle = C.c_void_p()
         _abi.check(self.lib.chd_context_create(device, C.byref(handle)), "context create")
         self.handle = handle
-        # objects living on this context, closed before it (batches first)
+        # objects living on this context, closed with it (batches first); the
+        # library also accepts any destroy order
         self._batches = weakref.WeakSet()
         self._sessions = weakref.WeakSet()
+        self._proposals = weakref.WeakSet()
 
     def synchronize(self):
         _abi.check(self.lib.chd_context_synchronize(self.handle), "synchronize")
@@ -98,6 +100,8 @@ class Context:
                 b.close()
             for s in list(self._sessions):
                 s.close()
+            for p in list(self._proposals):
+                p.close()
             self.lib.chd_context_destroy(self.handle)
             self.handle = None
 
@@ -166,6 +170,7 @@ class Session:
         self.grid = g
 
     def close(self):
+        """Frees the session; a Batch that holds it can no longer run."""
         if self.handle:
             self.lib.chd_session_destroy(self.handle)
             self.handle = None
@@ -225,6 +230,17 @@ class Session:
         _abi.check(self.lib.chd_session_read_view_scores(self.handle, _abi.ptr(out)),
                    "read view scores")
         return out
+
+    def band(self):
+        """The label window (lo, hi) of every coarse node set by the last
+        proposal (labels 0 and lo..hi are swept)."""
+        g = self.grid
+        n = g.coarse_columns * g.coarse_rows
+        lo = np.empty(n, np.uint16)
+        hi = np.empty(n, np.uint16)
+        _abi.check(self.lib.chd_session_read_band(self.handle, _abi.ptr(lo), _abi.ptr(hi)),
+                   "read band")
+        return lo, hi
 
     def cost_volume(self):
         g = self.grid

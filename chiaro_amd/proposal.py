@@ -244,6 +244,7 @@ class Proposal:
         self.parameter_count = n.value
         self.input_offset, self.input_scale = 0.0, 0.125
         self.label_count = 129
+        context._proposals.add(self)
 
     def set_label_count(self, label_count: int):
         """Labels of the sessions served (centre = (L-1)/2 + (L-1)/8 o0)."""
@@ -283,6 +284,43 @@ class Proposal:
         n = C.c_size_t(self.parameter_count)
         _abi.check(self.lib.chd_proposal_parameters(self.handle, None, v.ctypes.data, C.byref(n)),
                    "set parameters")
+
+    def optimizer_state(self):
+        """Adam moments (first, then second, in parameter order) and the step."""
+        n, step = C.c_size_t(), C.c_uint64()
+        _abi.check(self.lib.chd_proposal_optimizer_state(self.handle, None, None, C.byref(n),
+                                                         C.byref(step)), "optimizer state")
+        out = np.empty(n.value, np.float32)
+        _abi.check(self.lib.chd_proposal_optimizer_state(self.handle, out.ctypes.data, None,
+                                                         C.byref(n), C.byref(step)),
+                   "optimizer state")
+        return out, int(step.value)
+
+    def set_optimizer_state(self, moments: np.ndarray, step: int):
+        v = np.ascontiguousarray(moments, np.float32)
+        n, s = C.c_size_t(v.size), C.c_uint64(step)
+        _abi.check(self.lib.chd_proposal_optimizer_state(self.handle, None, v.ctypes.data,
+                                                         C.byref(n), C.byref(s)),
+                   "set optimizer state")
+
+    def save(self, path):
+        """Checkpoint: parameters, Adam state, step, input transform and
+        label count, as an uncompressed .npz (no pickled objects)."""
+        moments, step = self.optimizer_state()
+        np.savez(path, parameters=self.parameters(), moments=moments,
+                 step=np.array(step, np.uint64), shape=np.array([self.width, self.height]),
+                 transform=np.array([self.input_offset, self.input_scale], np.float32),
+                 labels=np.array(self.label_count, np.uint32))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as f:
+            if tuple(f["shape"]) != (self.width, self.height):
+                raise ValueError("checkpoint is for another network size")
+            self.set_parameters(f["parameters"])
+            self.set_optimizer_state(f["moments"], int(f["step"]))
+            offset, scale = (float(v) for v in f["transform"])
+            self.set_input_transform(offset, scale)
+            self.set_label_count(int(f["labels"]))
 
     def set_dataset(self, inputs: np.ndarray, labels: np.ndarray):
         x = np.ascontiguousarray(inputs, np.uint16)

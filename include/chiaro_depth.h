@@ -226,7 +226,15 @@ int chd_camera_canonical(const double k[9], const double rotation_wc[9],
 typedef struct chd_context chd_context;
 typedef struct chd_session chd_session;
 
-/* One context per thread of use; it owns a HIP stream on `device`. */
+/* One context per thread of use; it owns a HIP stream on `device`.
+ *
+ * Lifetimes: sessions, batches and proposals made on a context may be
+ * destroyed in any order, before or after the context. chd_context_destroy
+ * with objects still alive waits for the stream and marks the context
+ * released (no new object can be made on it); its memory is freed with the
+ * last of those objects. Destroying a session that a batch holds leaves
+ * the batch valid for chd_batch_destroy only: chd_batch_run then returns
+ * CHD_ERR_STATE. Every handle is destroyed at most once. */
 int chd_context_create(int device, chd_context **out);
 void chd_context_destroy(chd_context *context);
 int chd_context_synchronize(chd_context *context);
@@ -363,6 +371,15 @@ void chd_proposal_destroy(chd_proposal *proposal);
  * *count must equal it and get receives or set replaces them. */
 int chd_proposal_parameters(chd_proposal *proposal, float *get, const float *set,
                             size_t *count);
+/* Adam state for checkpoints: 2 x the parameter count f32 (the first
+ * moments in chd_proposal_parameters order, then the second moments) and
+ * the number of steps taken (bias corrections). With get and set NULL,
+ * *count and *step receive the sizes and the step; else *count must equal
+ * it, get receives the state and *step, or set and *step replace them.
+ * Parameters plus this state restore the optimiser exactly: the next Adam
+ * step after a restore equals the one the saved network would take. */
+int chd_proposal_optimizer_state(chd_proposal *proposal, float *get, const float *set,
+                                 size_t *count, uint64_t *step);
 /* The input transform: each input value is (cell mean of a plane - offset) *
  * scale in f32, rounded to BF16 (default offset 0, scale 1/8). Inputs given
  * to set_dataset / predict / gradients must have been made with the same
@@ -410,6 +427,10 @@ int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float
                              uint32_t half_width, uint32_t max_half_width, uint32_t tile,
                              uint32_t *full_tiles, uint32_t *tiles);
 int chd_session_clear_proposal(chd_session *session);
+/* The label window of every coarse node set by the last
+ * chd_session_set_proposal: labels 0 and lo..hi are swept (lo = 1 and
+ * hi = L - 1 on fallback tiles). CHD_ERR_STATE without a proposal. */
+int chd_session_read_band(chd_session *session, uint16_t *lo, uint16_t *hi);
 
 /* Stage entry points on host arrays, for parity checks of one stage. */
 

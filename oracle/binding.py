@@ -89,6 +89,8 @@ SIGNATURES = {
                                   C.c_int),
     "or_cost_volume_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options),
                             C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP, VP),
+    "or_direct_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options), VP,
+                       C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP),
 }
 
 

@@ -1805,6 +1805,51 @@ static float fraction(uint64_t count, uint64_t total) {
     return total == 0 ? 0.0f : (float)count / (float)total;
 }
 
+int or_direct_rows(const chd_view *views, uint32_t count, uint32_t reference_index,
+                   const chd_options *o, const chd_node_depth *coarse, uint32_t row_begin,
+                   uint32_t row_end, uint32_t threads, chd_node_depth *field, float *guidance,
+                   uint8_t *tested) {
+    /* measure_direct_depths (depth.rs:715-876) over fine rows
+     * [row_begin, row_end), row-banded over `threads` workers as the
+     * reference bands all rows (depth.rs:729-731) */
+    ctx_t c = {views, count, reference_index, o};
+    const chd_view *reference = &views[reference_index];
+    uint32_t coarse_step = o->grid_step > 4 ? o->grid_step : 4;
+    uint32_t coarse_columns = (uint32_t)div_ceil(reference->width, coarse_step) + 1;
+    uint32_t coarse_rows = (uint32_t)div_ceil(reference->height, coarse_step) + 1;
+    uint32_t step = coarse_step / 2 > 4 ? coarse_step / 2 : 4;
+    uint32_t columns = (uint32_t)div_ceil(reference->width, step) + 1;
+    chd_options direct_options = *o, wide_options = *o;
+    direct_options.patch_radius = o->patch_radius > 3 ? o->patch_radius : 3;
+    wide_options.patch_radius = o->patch_radius * 2 > 8 ? o->patch_radius * 2 : 8;
+    double inverse_step = (1.0 / o->near_depth - 1.0 / o->far_depth) / (double)(o->planes - 1);
+    if (row_end <= row_begin) return 0;
+    uint32_t workers = worker_count(threads, row_end - row_begin);
+    uint32_t per = (uint32_t)div_ceil(row_end - row_begin, workers);
+    direct_job jobs[256];
+    uint32_t n = 0;
+    for (uint32_t first = row_begin; first < row_end && n < 256; first += per) {
+        direct_job *j = &jobs[n++];
+        j->c = &c;
+        j->coarse = coarse;
+        j->coarse_columns = coarse_columns;
+        j->coarse_rows = coarse_rows;
+        j->coarse_step = coarse_step;
+        j->columns = columns;
+        j->step = step;
+        j->row_begin = first;
+        j->row_end = first + per < row_end ? first + per : row_end;
+        j->inverse_step = inverse_step;
+        j->direct = &direct_options;
+        j->wide = &wide_options;
+        j->nodes = field;
+        j->guidance = guidance;
+        j->tested = tested;
+    }
+    run_bands(direct_worker, jobs, sizeof(direct_job), n);
+    return 0;
+}
+
 int or_refine_multiview_depth(const chd_view *views, uint32_t count,
                               uint32_t reference_index, const chd_options *o,
                               chd_depth_map *map, chd_target_result *targets,
@@ -1813,7 +1858,6 @@ int or_refine_multiview_depth(const chd_view *views, uint32_t count,
     if (!or_options_valid(o) || reference_index >= count || count > CHD_MAX_VIEWS ||
         views[reference_index].camera == NULL)
         return CHD_NONE;
-    ctx_t c = {views, count, reference_index, o};
     const chd_view *reference = &views[reference_index];
     uint32_t coarse_step = o->grid_step > 4 ? o->grid_step : 4;
     uint32_t coarse_columns = (uint32_t)div_ceil(reference->width, coarse_step) + 1;
@@ -1871,37 +1915,11 @@ int or_refine_multiview_depth(const chd_view *views, uint32_t count,
     uint32_t columns = (uint32_t)div_ceil(reference->width, step) + 1;
     uint32_t rows = (uint32_t)div_ceil(reference->height, step) + 1;
     size_t fine = (size_t)columns * rows;
-    chd_options direct_options = *o, wide_options = *o;
-    direct_options.patch_radius = o->patch_radius > 3 ? o->patch_radius : 3;
-    wide_options.patch_radius = o->patch_radius * 2 > 8 ? o->patch_radius * 2 : 8;
     chd_node_depth *field = (chd_node_depth *)malloc(fine * sizeof(chd_node_depth));
     float *fguidance = (float *)malloc(fine * sizeof(float));
     uint8_t *ftested = (uint8_t *)malloc(fine);
-    {
-        uint32_t workers = worker_count(threads, rows);
-        uint32_t per = (uint32_t)div_ceil(rows, workers);
-        direct_job jobs[256];
-        uint32_t n = 0;
-        for (uint32_t first = 0; first < rows && n < 256; first += per) {
-            direct_job *j = &jobs[n++];
-            j->c = &c;
-            j->coarse = coarse;
-            j->coarse_columns = coarse_columns;
-            j->coarse_rows = coarse_rows;
-            j->coarse_step = coarse_step;
-            j->columns = columns;
-            j->step = step;
-            j->row_begin = first;
-            j->row_end = first + per < rows ? first + per : rows;
-            j->inverse_step = inverse_step;
-            j->direct = &direct_options;
-            j->wide = &wide_options;
-            j->nodes = field;
-            j->guidance = fguidance;
-            j->tested = ftested;
-        }
-        run_bands(direct_worker, jobs, sizeof(direct_job), n);
-    }
+    or_direct_rows(views, count, reference_index, o, coarse, 0, rows, threads, field, fguidance,
+                   ftested);
     free(coarse);
     double t3 = now_seconds();
     if (debug && debug->direct[0]) memcpy(debug->direct[0], field, fine * sizeof(chd_node_depth));

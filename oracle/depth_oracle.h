@@ -152,6 +152,14 @@ int or_cost_volume_rows(const chd_view *views, uint32_t count, uint32_t referenc
                         uint32_t threads, float *scores, float *costs, float *guidance,
                         uint8_t *tested);
 
+/* measure_direct_depths (depth.rs:715-876) from a given coarse field over
+ * fine rows [row_begin, row_end); field, guidance and tested are whole
+ * fine-grid arrays of which only those rows are written. */
+int or_direct_rows(const chd_view *views, uint32_t count, uint32_t reference_index,
+                   const chd_options *o, const chd_node_depth *coarse, uint32_t row_begin,
+                   uint32_t row_end, uint32_t threads, chd_node_depth *field, float *guidance,
+                   uint8_t *tested);
+
 #ifdef __cplusplus
 }
 #endif

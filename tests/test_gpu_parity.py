@@ -28,7 +28,7 @@ def _small_config0(seed, scale=0.25, **kw):
     return scene.config0(seed=seed, width=w, height=h, focal_px=1600.0 * scale, **kw)
 
 
-def _full_parity(ctx, cap, label):
+def _full_parity(ctx, cap, label, threads=8):
     _say(f"parity case {label}: {cap.description}")
     sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
     try:
@@ -41,7 +41,7 @@ def _full_parity(ctx, cap, label):
         direct = [sess.direct(k) for k in range(4)]
     finally:
         sess.close()
-    omap, otargets, dbg = binding.refine(cap, threads=8)
+    omap, otargets, dbg = binding.refine(cap, threads=threads)
     assert_bitwise("scores", scores, dbg["scores"])
     assert_bitwise("costs", costs, dbg["costs"])
     assert_bitwise("guidance", guidance, dbg["guidance"])

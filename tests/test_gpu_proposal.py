@@ -36,6 +36,20 @@ def _params(width, height, rng, scale=0.2):
     return np.concatenate(out).astype(np.float32)
 
 
+def assert_band_equals_full(name, session, scores, costs, want_scores, want_costs):
+    """Inside each node's window (label 0 and lo..hi) the band sweep's scores
+    and costs equal the full sweep's bit for bit; outside they are NaN and
+    +inf (depth.rs:1224-1247 evaluated on the window only)."""
+    lo, hi = session.band()
+    labels = np.arange(scores.shape[1])[None, :]
+    inside = (labels == 0) | ((labels >= lo[:, None]) & (labels <= hi[:, None]))
+    assert_bitwise(name + " scores in window", scores[inside], want_scores[inside])
+    assert_bitwise(name + " costs in window", costs[inside], want_costs[inside])
+    assert np.isnan(scores[~inside]).all(), name + ": a score outside the window"
+    assert np.all(costs[~inside] == np.inf), name + ": a cost outside the window"
+    return int(inside.sum())
+
+
 def _inputs(rng, n, h, w):
     x = np.zeros((n, h, w, 16), np.float32)
     x[..., :3] = rng.uniform(0, 1, size=(n, h, w, 3))
@@ -147,7 +161,10 @@ def test_training_and_band_sweep(gpu_context, oracle_lib):
             fb, tiles = net.apply(full, threshold=0.0, max_half_width=max_half)
             assert fb == 0
             full.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
-            _, costs, guidance, _ = full.cost_volume()
+            band_scores, costs, guidance, _ = full.cost_volume()
+            swept = assert_band_equals_full(f"band (max half {max_half})", full, band_scores,
+                                            costs, want_scores, want_costs)
+            _say(f"max half {max_half}: {swept} (node, label) pairs equal the full sweep")
             swept = np.isfinite(costs).sum(axis=1)
             assert swept.max() <= most and swept.min() >= 1
             if max_half == 0:
@@ -219,6 +236,8 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
     cap = scene.config4(seed=3, scale=0.125)
     sess = Session(gpu_context, cap.modules, 0, cap.options)
     g = sess.grid
+    sess.run(_abi.CHD_STAGE_COST)
+    want_scores, want_costs, _, _ = sess.cost_volume()
     ref = cap.modules[0].luminance
     h, w = ref.shape[0] // 4, ref.shape[1] // 4
     net = Proposal(gpu_context, w, h, max_batch=1, seed=5)
@@ -231,7 +250,9 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
         fb, tiles = net.apply(sess, threshold=0.0)
         assert fb == 0 and tiles > 0
         sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
-        _, costs, guidance, _ = sess.cost_volume()
+        band_scores, costs, guidance, _ = sess.cost_volume()
+        assert_band_equals_full("band at 257 labels", sess, band_scores, costs, want_scores,
+                                want_costs)
         pred = net.predict(capture_inputs(cap, offset, scale)[None])[0]
         r = np.minimum(np.arange(g.coarse_rows), h - 1)
         c = np.minimum(np.arange(g.coarse_columns), w - 1)
