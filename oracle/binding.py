@@ -44,6 +44,7 @@ SIGNATURES = {
     "or_libm_exp": (C.c_double, C.c_double),
     "or_set_libm_mode": (None, C.c_int),
     "or_expf_mismatches": (C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32),
+    "or_expf_compare": (C.c_uint64, C.c_uint32, C.c_uint64, VP, C.c_uint32),
     "or_inverse3": (C.c_int, VP, VP),
     "or_determinant3": (C.c_double, VP),
     "or_camera_canonical": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, P(_abi.Distortion),

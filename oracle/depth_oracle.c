@@ -350,6 +350,45 @@ static void *expf_worker(void *arg) {
     return NULL;
 }
 
+typedef struct {
+    uint64_t lo, begin, end;
+    const float *got;
+    uint64_t mismatches;
+} expf_compare_job;
+
+static void *expf_compare_worker(void *arg) {
+    expf_compare_job *job = (expf_compare_job *)arg;
+    uint64_t bad = 0;
+    for (uint64_t i = job->begin; i < job->end; i++) {
+        float a = or_expf(u2f((uint32_t)(job->lo + i))), b = job->got[i];
+        if (f2u(a) != f2u(b) && !(a != a && b != b)) bad++;
+    }
+    job->mismatches = bad;
+    return NULL;
+}
+
+uint64_t or_expf_compare(uint32_t lo_bits, uint64_t n, const float *got, uint32_t threads) {
+    if (threads == 0) threads = 1;
+    if (threads > 64) threads = 64;
+    pthread_t tid[64];
+    expf_compare_job jobs[64];
+    uint64_t per = (n + threads - 1) / threads;
+    for (uint32_t i = 0; i < threads; i++) {
+        jobs[i].lo = lo_bits;
+        jobs[i].begin = per * i < n ? per * i : n;
+        jobs[i].end = per * (i + 1) < n ? per * (i + 1) : n;
+        jobs[i].got = got;
+        jobs[i].mismatches = 0;
+        pthread_create(&tid[i], NULL, expf_compare_worker, &jobs[i]);
+    }
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < threads; i++) {
+        pthread_join(tid[i], NULL);
+        total += jobs[i].mismatches;
+    }
+    return total;
+}
+
 uint64_t or_expf_mismatches(uint32_t lo_bits, uint32_t hi_bits, uint32_t stride,
                             uint32_t threads) {
     if (threads == 0) threads = 1;

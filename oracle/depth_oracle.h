@@ -36,6 +36,9 @@ double or_libm_exp(double x);
 void or_set_libm_mode(int use_libm);
 /* count of inputs in [lo_bits, hi_bits) (f32 bit patterns, stepping by
  * `stride`) where or_expf differs from the platform expf. */
+/* Mismatches of got[i] against or_expf of the float with bits lo_bits + i,
+ * i < n (NaN equals NaN), on `threads` workers. */
+uint64_t or_expf_compare(uint32_t lo_bits, uint64_t n, const float *got, uint32_t threads);
 uint64_t or_expf_mismatches(uint32_t lo_bits, uint32_t hi_bits, uint32_t stride,
                             uint32_t threads);
 

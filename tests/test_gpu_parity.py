@@ -73,19 +73,31 @@ def _full_parity(ctx, cap, label, threads=8):
     return dbg
 
 
-def test_device_expf_matches_oracle_and_libm(gpu_context, oracle_lib):
-    _say("expf parity: 2^22 strided inputs")
-    bits_ = np.arange(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
-    x = bits_.view(np.float32).copy()
-    y = np.empty_like(x)
-    _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(x), _abi.ptr(y), x.size), "expf")
-    want = np.array([oracle_lib.or_libm_expf(float(v)) for v in x[::97]], dtype=np.float32)
-    assert_bitwise("expf vs libm", y[::97], want)
-    dom = np.linspace(-12.0, 0.0, 200001, dtype=np.float32)
-    yd = np.empty_like(dom)
-    _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(dom), _abi.ptr(yd), dom.size), "expf")
-    want = np.array([oracle_lib.or_libm_expf(float(v)) for v in dom], dtype=np.float32)
-    assert_bitwise("expf domain", yd, want)
+def test_device_expf_exhaustive_on_the_used_domain(gpu_context, oracle_lib):
+    """Every argument the kernels give expf is -d^2 / (2 sigma^2), -1.2 |dv|
+    or -3 e: zero or negative, so the domain is +0 and every float with the
+    sign bit set (2^31 + 1 values, negative NaNs and -inf included). The
+    device expf equals the oracle's or_expf on all of them, and or_expf equals
+    the platform expf on all 2^32 floats (test_expf_exhaustive_against_libm)."""
+    import os
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    chunk = 1 << 27
+    zero = np.zeros(1, np.float32)
+    y0 = np.empty(1, np.float32)
+    _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(zero), _abi.ptr(y0), 1),
+               "expf")
+    assert oracle_lib.or_expf_compare(0, 1, _abi.ptr(y0), 1) == 0
+    bits_ = np.empty(chunk, np.uint32)
+    y = np.empty(chunk, np.float32)
+    total = 0
+    for lo in range(0x80000000, 1 << 32, chunk):
+        np.add(np.arange(chunk, dtype=np.uint32), np.uint32(lo), out=bits_)
+        _abi.check(gpu_context.lib.chd_device_expf(gpu_context.handle, _abi.ptr(bits_),
+                                                   _abi.ptr(y), chunk), "expf")
+        bad = oracle_lib.or_expf_compare(lo, chunk, _abi.ptr(y), threads)
+        assert bad == 0, f"{bad} mismatches in bits {lo:#x}..{lo + chunk - 1:#x}"
+        total += chunk
+    _say(f"expf: {total + 1} inputs, all equal to the oracle")
 
 
 def test_device_exp_matches_libm(gpu_context, oracle_lib):
