@@ -25,6 +25,7 @@ CHD_ERR_NO_DEVICE = -3
 CHD_ERR_LIMIT = -4
 CHD_ERR_STATE = -5
 CHD_MAX_VIEWS = 32
+CHD_NOALLOC_MAX_NODES = 2097152
 
 CHD_STAGE_COST = 1
 CHD_STAGE_SGM = 2
@@ -314,6 +315,8 @@ DEPTH_SIGNATURES = {
                                           C.c_double),
     "chd_reject_small_direct_components": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32,
                                            C.c_double, C.c_uint32),
+    "chd_reject_small_direct_components_noalloc": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32,
+                                                   C.c_double, C.c_uint32, VP),
     "chd_fit_local_depth_planes": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_double),
     "chd_enforce_warp_consensus": (C.c_int, VP, VP, C.c_uint32, C.c_uint32),
     "chd_suppress_warp_boundaries": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32),

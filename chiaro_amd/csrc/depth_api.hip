@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,41 @@ int32_t tkey32_host(float f) {
 
 // MINIMUM_DIRECT_COMPONENT_NODES (depth.rs:37)
 constexpr uint32_t kMinimumComponentNodes = 24;
+
+}  // namespace
+
+// Device scratch of the allocation-free entry point: the union-find parents
+// and component sizes of CHD_NOALLOC_MAX_NODES fine nodes (16 MiB), one copy
+// per GPU for the life of the process. Host code never passes `scratch`
+// itself to a kernel: its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device when the first context on
+// that device is created.
+static __device__ __align__(256) uint32_t scratch[2 * (size_t)CHD_NOALLOC_MAX_NODES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch[kMaxDevices] = {};
+
+// Records `scratch`'s address on the current device (`device`).
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(CHD_ERR_LIMIT, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return CHD_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(CHD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (uint32_t *)ptr;
+    return CHD_OK;
+}
+
+uint32_t *scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch[device];
+}
 
 struct TimedLaunch {
     const char *name;
@@ -528,6 +564,7 @@ int chd_context_create(int device, chd_context **out) {
         return fail(CHD_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(CHD_ERR_ARGUMENT, "device index out of range");
     CHD_HIP(hipSetDevice(device));
+    CHD_CHECK(scratch_setup(device));
     auto *c = new chd_context;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1264,6 +1301,25 @@ int chd_reject_small_direct_components(chd_context *c, chd_node_depth *field,
                            return launch_reject_components(st, src, dst, g, columns, rows, tol,
                                                            minimum_nodes, parent, sizes);
                        });
+}
+
+int chd_reject_small_direct_components_noalloc(chd_context *c, const chd_node_depth *field,
+                                               const float *guidance, uint32_t columns,
+                                               uint32_t rows, double tol, uint32_t minimum_nodes,
+                                               chd_node_depth *out) {
+    // the size limit comes first: it needs no device
+    if ((uint64_t)columns * rows > CHD_NOALLOC_MAX_NODES)
+        return fail(CHD_ERR_LIMIT, "reject_small_direct_components_noalloc: more than "
+                                   "CHD_NOALLOC_MAX_NODES nodes");
+    if (!c || !field || !guidance || !out || columns == 0 || rows == 0)
+        return fail(CHD_ERR_ARGUMENT, "reject_small_direct_components_noalloc: bad argument");
+    CHD_HIP(hipSetDevice(c->device));
+    uint32_t *ptr = scratch_of(c->device);
+    if (!ptr) return fail(CHD_ERR_STATE, "reject_small_direct_components_noalloc: no scratch on this device");
+    uint32_t *parent = ptr, *sizes = ptr + CHD_NOALLOC_MAX_NODES;
+    CHD_HIP(launch_reject_components(c->stream, field, out, guidance, columns, rows, tol,
+                                     minimum_nodes, parent, sizes));
+    return CHD_OK;
 }
 
 int chd_fit_local_depth_planes(chd_context *c, chd_node_depth *field, const float *guidance,

@@ -457,6 +457,26 @@ int chd_reject_small_direct_components(chd_context *context, chd_node_depth *fie
                                        const float *guidance, uint32_t columns,
                                        uint32_t rows, double inverse_tolerance,
                                        uint32_t minimum_nodes);
+/* Largest grid of the allocation-free entry point below (2^21 nodes; a
+ * 4160x3120 raster's fine grid has 813,021). */
+#define CHD_NOALLOC_MAX_NODES 2097152u
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): reject_small_direct_components (depth.rs:933-989) on
+ * device buffers, with the union-find parents and sizes in a static device
+ * scratch of CHD_NOALLOC_MAX_NODES nodes (16 MiB per GPU, held until the
+ * process ends). field, guidance and out are device buffers of
+ * columns x rows (out may equal field); the work is enqueued on the
+ * context's stream without synchronising, and nothing is allocated.
+ * Grids above CHD_NOALLOC_MAX_NODES return CHD_ERR_LIMIT. Every call on one
+ * GPU shares its scratch: calls that overlap on one GPU (other streams or
+ * contexts) race. */
+int chd_reject_small_direct_components_noalloc(chd_context *context,
+                                               const chd_node_depth *field,
+                                               const float *guidance, uint32_t columns,
+                                               uint32_t rows, double inverse_tolerance,
+                                               uint32_t minimum_nodes, chd_node_depth *out);
 int chd_fit_local_depth_planes(chd_context *context, chd_node_depth *field,
                                const float *guidance, uint32_t columns, uint32_t rows,
                                double inverse_tolerance);
