@@ -186,3 +186,24 @@ def test_config4_full_size(gpu_context, oracle_lib):
     cap = scene.config4(seed=0)
     g = _staged_parity(gpu_context, oracle_lib, cap, "config4", coarse_band=3, fine_band=4)
     assert g.labels == 257
+
+
+def test_config1_wta_labels_follow_the_ground_truth(gpu_context):
+    """At config 1's size the GPU's WTA labels of interior nodes (one scene
+    plane over the 5 x 5 nodes around them) sit within one label of the
+    rendered scene's true inverse distance: a size-independent property."""
+    from test_oracle_nversion import true_labels
+    cap = scene.config1(seed=1)
+    sess = Session(gpu_context, cap.modules, cap.reference_index, cap.options)
+    try:
+        sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+        field, _, best = sess.coarse()
+    finally:
+        sess.close()
+    label, interior = true_labels(cap)
+    sel = interior & field["valid"].astype(bool)
+    within = (np.abs(best.astype(int) - label) <= 1)[sel]
+    _say(f"config1 ground truth: {within.mean():.4f} of {sel.sum()} interior nodes within one label "
+         f"(exact {(best.astype(int) == label)[sel].mean():.4f})")
+    assert sel.sum() > 0.6 * sel.size
+    assert within.mean() >= 0.9
