@@ -13,8 +13,10 @@ value = reference raster pixels x planes / step time / 1e6. The full path
 reported beside it.
 
 With --learned-scenes > 0 (the default) the line also carries
-"learned_band": the learned band proposal trained and measured on config 3
-at a reduced size (--workload config3 runs the full grid).
+"learned_band": the learned band proposal trained on GPU-rendered half-scale
+config 4 captures, its setting chosen on full config 4 captures and reported
+on other, held-out full captures batched, for cost + SGM and end to end
+(--workload config3 measures it on config 3).
 
 Prints one JSON line. --gpus above 1 is rejected: one GPU is measured (the
 path shards by capture, whole captures per GPU).
@@ -242,16 +244,21 @@ def main() -> int:
                                            "config4-learned"],
                     default="config1")
     ap.add_argument("--train-scenes", type=int, default=512,
-                    help="config3: training scenes for the learned band proposal")
-    ap.add_argument("--train-wide-only", action="store_true",
-                    help="config4-learned: training captures render only the five wide modules "
-                         "(labels from their full sweep; a third of the rendering per capture)")
+                    help="config3 / config4-learned: training scenes of the learned band")
+    ap.add_argument("--train-all-modules", action="store_true",
+                    help="config4-learned: training labels from all sixteen modules' full sweep "
+                         "(default: the five wide modules)")
     ap.add_argument("--eval-scenes", type=int, default=24, help="config3: held-out scenes")
-    ap.add_argument("--epochs", type=int, default=10, help="config3: training epochs")
-    ap.add_argument("--learned-scenes", type=int, default=1000,
-                    help="config1: training scenes of the learned-band line beside it (0 skips)")
+    ap.add_argument("--epochs", type=int, default=10,
+                    help="config3 / config4-learned: training epochs")
+    ap.add_argument("--learned-scenes", type=int, default=1600,
+                    help="config1: training captures of the config 4 learned-band measurement "
+                         "in the same line (0 skips it)")
     ap.add_argument("--learned-epochs", type=int, default=60)
-    ap.add_argument("--learned-eval", type=int, default=4)
+    ap.add_argument("--learned-validation", type=int, default=4,
+                    help="full config 4 captures that choose the band setting")
+    ap.add_argument("--learned-test", type=int, default=4,
+                    help="held-out full config 4 captures the band is reported on")
     ap.add_argument("--captures", type=int, default=0,
                     help="config2/config4: captures per batch (default 8 and 4)")
     ap.add_argument("--tuning", choices=["default", "auto"], default="default",
@@ -455,15 +462,11 @@ def main() -> int:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     sess.close()
     if args.learned_scenes > 0 and args.workload == "config1":
-        # the metric's second half, at a reduced size: the learned band on
-        # config 3 (the full grid and sizes: --workload config3)
-        grid = [(mh, 8, thr) for mh in (16, 24, 32) for thr in (0.2, 0.3, 0.4, 0.5)]
-        res = learned_config3(ctx, args.learned_scenes, args.learned_epochs, args.learned_eval,
-                              args.seed, grid, ablation=False)
-        line["learned_band"] = learned_summary(
-            res, f"config3 (1040x780, 3 views, 128 hypotheses): {args.learned_scenes} training "
-                 f"scenes x {args.learned_epochs} epochs, {args.learned_eval} held-out scenes, "
-                 f"8x8 tiles")
+        # the metric's second half: the learned band on config 4 (4 captures
+        # of 16 modules batched, 256 hypotheses) against the plain full sweep
+        line["learned_band"] = learned_config4(ctx, args.learned_scenes, args.learned_epochs,
+                                               args.learned_validation, args.learned_test,
+                                               args.seed)
     ctx.close()
     print(json.dumps(line))
     return 0
@@ -698,109 +701,176 @@ def bench_learned(args) -> int:
     return 0
 
 
-def bench_learned4(args) -> int:
-    """Config 4 with the learned band: the network is trained on the full
-    sweep's label maps of half-scale config 4 captures swept on a 4 px grid
-    (the coarse grid and the network's grid of the full-size capture at 8 px,
-    at a quarter of the rendering cost), then `captures` full-size captures
-    run as one batch, full sweep against band sweep, over a grid of
-    confidence thresholds, tile sizes and width caps; the fastest setting
-    with at least 98 % +-1-hypothesis agreement is reported."""
+TRAIN_SEED, VALIDATION_SEED, TEST_SEED = 10_000, 200_000, 300_000
+
+
+def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: int,
+                    test_captures: int, seed: int, steps: int = 2, settings=None,
+                    wide_only: bool = True) -> dict:
+    """The learned band on config 4, every capture rendered on the GPU.
+
+    Training: the full sweep's WTA label maps of `train_captures` half-scale
+    config 4 captures of the five wide modules on a 4 px grid (the coarse
+    grid, angles, disparities and 257 labels of the full capture at 8 px),
+    rendered, converted and swept on the device (one session fed new planes
+    per capture); the network inputs are built on the device from the same
+    planes. Selection: the settings (confidence threshold, tile, width cap)
+    are timed and scored on `validation_captures` full 16-module captures
+    batched; the fastest with >= 98 % +-1-hypothesis agreement is chosen.
+    Report: on `test_captures` other full captures batched, that setting and
+    the specification's (+-8 labels, tiles below confidence 0.5 fall back)
+    against the plain full sweep, for cost + SGM and end to end (cost, SGM,
+    WTA, direct pass, filters, per-target consistency), with the agreement
+    of the u8 consistency mask (provenance) and the per-target decision
+    codes."""
     import numpy as np
 
-    from chiaro_amd import scene
-    from chiaro_amd.depth import Context
-    from chiaro_amd.proposal import (Proposal, batch_agreement, build_dataset, input_transform)
+    from chiaro_amd import _abi, scene
+    from chiaro_amd.depth import Session
+    from chiaro_amd.proposal import Proposal, batch_agreement_sessions, input_transform
 
-    made = [0]
-
-    def train_capture(seed):
-        cap = scene.config4(seed=seed, scale=0.5, wide_only=args.train_wide_only)
-        cap.options.grid_step = 4
-        cap.raws = None
-        made[0] += 1
-        if made[0] % 40 == 0:
-            log(f"[bench] {made[0]} training captures")
-        return cap
-
-    ctx = Context(0)
-    t = time.perf_counter()
-    offset, scale = input_transform([train_capture(10000 + k) for k in range(4)])
-    x, y = build_dataset(ctx, range(10000, 10000 + args.train_scenes), train_capture, offset, scale)
-    log(f"[bench] {args.train_scenes} half-scale config4 captures and labels in "
-        f"{time.perf_counter() - t:.1f} s, grid {x.shape[2]}x{x.shape[1]}")
-    h, w = x.shape[1:3]
-    net = Proposal(ctx, w, h, max_batch=16, seed=args.seed)
-    net.set_label_count(257)
+    t_all = time.perf_counter()
+    rig = scene.rig_config4(seed=TRAIN_SEED, scale=0.5, wide_only=wide_only)
+    rig.options.grid_step = 4
+    frames = scene.Frames(ctx, rig)
+    # input standardisation from four captures' input planes
+    probes = []
+    for k in range(4):
+        frames.render(TRAIN_SEED + k)
+        mods = [scene.ModuleInput(name=str(i), luminance=frames.read(i)[1], width=0, height=0,
+                                  camera=None, warp_points=np.zeros((1, 1, 2), np.float32),
+                                  warp_step=32) for i in range(3)]
+        probes.append(scene.Capture(modules=mods, reference_index=0, options=rig.options))
+    offset, scale = input_transform(probes)
+    frames.render(TRAIN_SEED)
+    sess = Session(ctx, frames.capture().modules, 0, rig.options)
+    g = sess.grid
+    ph, pw = frames.planes()[0].shape
+    h, w = ph // 2, pw // 2
+    net = Proposal(ctx, w, h, max_batch=16, seed=seed)
+    net.set_label_count(g.labels)
     net.set_input_transform(offset, scale)
+    x = np.empty((train_captures, h, w, 16), np.uint16)
+    y = np.empty((train_captures, h, w), np.uint16)
+    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    t = time.perf_counter()
+    for k in range(train_captures):
+        frames.render(TRAIN_SEED + k)
+        sess.set_planes(frames.planes())
+        sess.run(stages)
+        _, _, best = sess.coarse()
+        tested = sess.tested()
+        labels = np.where(tested > 0, best, 0).astype(np.uint16)
+        y[k] = labels.reshape(g.coarse_rows, g.coarse_columns)[:h, :w]
+        x[k] = net.session_input(sess)
+        if (k + 1) % 200 == 0:
+            log(f"[bench] {k + 1} training captures on the device")
+    data_s = time.perf_counter() - t
+    sess.close()
+    frames.close()
+    log(f"[bench] {train_captures} half-scale config4 training captures in {data_s:.1f} s")
     net.set_dataset(x, y)
     del x, y
     t = time.perf_counter()
-    losses = net.train(epochs=args.epochs, batch=16, lr=1e-3, seed=args.seed)
+    losses = net.train(epochs=epochs, batch=16, lr=1e-3, seed=seed)
     train_s = time.perf_counter() - t
-    log(f"[bench] trained {args.epochs} epochs in {train_s:.1f} s, last losses {losses[-3:]}")
-    count = args.captures or 4
-    caps = []
-    for k in range(count):
-        caps.append(scene.config4(seed=100000 + k))
-        caps[-1].raws = None
-    settings = [(thr, tile, mh, half) for mh in (8, 16, 24, 32) for half in (8, 12) if half <= mh
-                for tile in (4, 8) for thr in (0.0, 0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
-    rows = batch_agreement(ctx, caps, net, settings, steps=args.steps)
-    for r in rows:
-        log(f"[bench] max half {r['max_half_width']} min half {r['half_width']} tile {r['tile']} "
-            f"threshold {r['threshold']}: agreement {r['agreement']:.4f} "
-            f"fallback {r['fallback_fraction']:.3f} speedup {r['speedup']:.3f}")
-    ok = [r for r in rows if r["agreement"] >= 0.98]
-    best = max(ok, key=lambda r: r["speedup"]) if ok else None
-    chosen = best or max(rows, key=lambda r: r["agreement"])
-    ref = caps[0].modules[0]
-    pixels_hyp = ref.width * ref.height * caps[0].options.planes * count
-    full_ms = float(np.mean([r["full_ms"] for r in rows]))
+    log(f"[bench] trained {epochs} epochs in {train_s:.1f} s, losses {losses[0]:.3f} -> "
+        f"{losses[-1]:.3f}")
+
+    full_rig = scene.rig_config4(seed=TEST_SEED)
+    frames = scene.Frames(ctx, full_rig)
+
+    def sessions_for(first, count):
+        out = []
+        for k in range(count):
+            frames.render(first + k)
+            out.append(Session(ctx, frames.capture().modules, 0, full_rig.options))
+        return out
+
+    if settings is None:
+        settings = [(thr, tile, mh) for mh in (16, 24, 32) for tile in (4, 8)
+                    for thr in (0.2, 0.3, 0.4, 0.5)]
+    validation = sessions_for(VALIDATION_SEED, validation_captures)
+    val_rows = batch_agreement_sessions(ctx, validation, net, settings, steps)
+    for s in validation:
+        s.close()
+    for r in val_rows:
+        log(f"[bench] validation: max half {r['max_half_width']} tile {r['tile']} threshold "
+            f"{r['threshold']}: agreement {r['agreement']:.4f} fallback "
+            f"{r['fallback_fraction']:.3f} speedup {r['speedup']:.3f}")
+    ok = [r for r in val_rows if r["agreement"] >= 0.98]
+    chosen = max(ok, key=lambda r: r["speedup"]) if ok else max(val_rows, key=lambda r: r["agreement"])
+    chosen_setting = (chosen["threshold"], chosen["tile"], chosen["max_half_width"])
+    spec_setting = (0.5, 8, 8)
+    test = sessions_for(TEST_SEED, test_captures)
+    cost_rows = batch_agreement_sessions(ctx, test, net, [chosen_setting, spec_setting], steps)
+    e2e_rows = batch_agreement_sessions(ctx, test, net, [chosen_setting, spec_setting], steps,
+                                        stages=_abi.CHD_STAGE_ALL)
+    for s in test:
+        s.close()
+    frames.close()
+    net.close()
+
+    def pick(row, keys):
+        return {k: (round(row[k], 5) if isinstance(row[k], float) else row[k]) for k in keys}
+
+    keys = ("threshold", "tile", "max_half_width", "half_width", "agreement", "mask_agreement",
+            "fallback_fraction", "full_ms", "band_ms", "speedup")
+    e2e_keys = keys + ("consistency_mask_agreement", "decision_agreement")
+    return {
+        "workload": (f"config4: {test_captures} held-out captures x 16 modules 4160x3120, 256 "
+                     f"hypotheses, batched; network trained on {train_captures} half-scale "
+                     f"config4 captures ({'five wide' if wide_only else 'all sixteen'} modules, "
+                     f"4 px grid) x {epochs} epochs; setting chosen on {validation_captures} "
+                     f"other full captures"),
+        "met_98_percent": cost_rows[0]["agreement"] >= 0.98,
+        "speedup": round(cost_rows[0]["speedup"], 3),
+        "agreement": round(cost_rows[0]["agreement"], 5),
+        "end_to_end_speedup": round(e2e_rows[0]["speedup"], 3),
+        "chosen": {"cost_sgm": pick(cost_rows[0], keys), "end_to_end": pick(e2e_rows[0], e2e_keys)},
+        "specification_setting": {"cost_sgm": pick(cost_rows[1], keys),
+                                  "end_to_end": pick(e2e_rows[1], e2e_keys)},
+        "validation": [pick(r, keys) for r in val_rows],
+        "seeds": {"train": [TRAIN_SEED, TRAIN_SEED + train_captures - 1],
+                  "validation": [VALIDATION_SEED, VALIDATION_SEED + validation_captures - 1],
+                  "test": [TEST_SEED, TEST_SEED + test_captures - 1]},
+        "input_transform": {"offset": offset, "scale": scale},
+        "training_data_seconds": round(data_s, 2),
+        "train_seconds": round(train_s, 2),
+        "total_seconds": round(time.perf_counter() - t_all, 2),
+        "losses": [round(v, 4) for v in losses[:: max(1, len(losses) // 10)]],
+    }
+
+
+def bench_learned4(args) -> int:
+    """Config 4 with the learned band as its own line (learned_config4); the
+    value is the band's cost + SGM throughput on the held-out batch."""
+    from chiaro_amd.depth import Context
+    ctx = Context(0)
+    res = learned_config4(ctx, args.train_scenes, args.epochs, args.learned_validation,
+                          args.captures or args.learned_test, args.seed, steps=args.steps,
+                          wide_only=not args.train_all_modules)
+    ctx.close()
+    row = res["chosen"]["cost_sgm"]
+    pixels_hyp = 4160 * 3120 * 256 * (args.captures or args.learned_test)
     line = {
         "metric": METRIC,
-        "value": round(pixels_hyp / (chosen["band_ms"] / 1000.0) / 1e6, 2),
+        "value": round(pixels_hyp / (row["band_ms"] / 1000.0) / 1e6, 2),
         "unit": "Mpix·hyp/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": 1,
-        "ms_per_step": round(chosen["band_ms"], 3),
+        "ms_per_step": round(row["band_ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 sweep, bf16 network",
         "data": "synthetic",
-        "config": {
-            "workload": (f"config4 learned band: {count} captures x 16 modules 4160x3120, 256 "
-                         f"hypotheses, batched; network trained on {args.train_scenes} half-scale "
-                         f"config4 captures"
-                         f"{' (five wide modules)' if args.train_wide_only else ''} x "
-                         f"{args.epochs} epochs"),
-            "parallelism": "1 GPU",
-        },
-        "full_sweep_ms": round(full_ms, 3),
-        "full_sweep_value": round(pixels_hyp / (full_ms / 1000.0) / 1e6, 2),
-        "learned_band": {
-            "met_98_percent": best is not None,
-            "threshold": chosen["threshold"],
-            "tile": chosen["tile"],
-            "max_half_width": chosen["max_half_width"],
-            "half_width": chosen["half_width"],
-            "speedup": round(chosen["speedup"], 3),
-            "agreement": round(chosen["agreement"], 5),
-            "mask_agreement": round(chosen["mask_agreement"], 5),
-            "fallback_fraction": round(chosen["fallback_fraction"], 4),
-            "settings": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
-                         for r in rows],
-            "input_transform": {"offset": offset, "scale": scale},
-            "train_seconds": round(train_s, 2),
-            "losses": [round(l, 4) for l in losses],
-        },
+        "config": {"workload": "config4 learned band: " + res["workload"], "parallelism": "1 GPU"},
+        "learned_band": res,
         "roofline": None,
         "cpu_baseline": None,
     }
-    net.close()
-    ctx.close()
     print(json.dumps(line))
     return 0
 

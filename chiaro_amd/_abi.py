@@ -324,6 +324,16 @@ DEPTH_SIGNATURES = {
                            C.c_int, VP),
     "chd_luminance_half_device": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, C.c_float,
                                   C.c_float, C.c_int, VP),
+    "chd_frames_create": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_float,
+                          C.c_float, P(VP)),
+    "chd_frames_render": (C.c_int, VP, VP),
+    "chd_frames_plane": (C.c_int, VP, C.c_uint32, P(VP), P(C.c_uint32), P(C.c_uint32)),
+    "chd_frames_read": (C.c_int, VP, C.c_uint32, VP, VP),
+    "chd_frames_destroy": (None, VP),
+    "chd_session_create_device": (C.c_int, VP, P(View), C.c_uint32, C.c_uint32, P(Options),
+                                  P(VP)),
+    "chd_session_set_planes": (C.c_int, VP, P(VP), C.c_uint32, C.c_int),
+    "chd_session_network_input": (C.c_int, VP, VP, VP),
     "chd_device_expf": (C.c_int, VP, VP, VP, C.c_size_t),
     "chd_device_exp": (C.c_int, VP, VP, VP, C.c_size_t),
 }

@@ -141,6 +141,7 @@ struct chd_context {
     std::vector<chd_session *> sessions;
     std::vector<chd_batch *> batches;
     std::vector<chd_proposal *> proposals;
+    std::vector<chd_frames *> frames;
     bool released = false;
 };
 
@@ -161,7 +162,8 @@ void context_free(chd_context *c) {
 
 // Frees a released context once its last child is gone.
 void context_release_if_unused(chd_context *c) {
-    if (c->released && c->sessions.empty() && c->batches.empty() && c->proposals.empty())
+    if (c->released && c->sessions.empty() && c->batches.empty() && c->proposals.empty() &&
+        c->frames.empty())
         context_free(c);
 }
 
@@ -590,9 +592,13 @@ int chd_context_synchronize(chd_context *c) {
     return CHD_OK;
 }
 
-int chd_session_create(chd_context *context, const chd_view *views, uint32_t count,
-                       uint32_t reference_index, const chd_options *options,
-                       chd_session **out) {
+}  // extern "C"
+
+namespace {
+
+int session_create(chd_context *context, const chd_view *views, uint32_t count,
+                   uint32_t reference_index, const chd_options *options, bool device_planes,
+                   chd_session **out) {
     if (!context || !views || !options || !out) return fail(CHD_ERR_ARGUMENT, "session: null argument");
     *out = nullptr;
     if (context->released) return fail(CHD_ERR_STATE, "session: the context was destroyed");
@@ -668,7 +674,10 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
             hipError_t e2 = hipMemcpy2DAsync(plane, pitch * sizeof(float), v.luminance,
                                              (size_t)v.plane_width * sizeof(float),
                                              (size_t)v.plane_width * sizeof(float),
-                                             v.plane_height, hipMemcpyHostToDevice, st);
+                                             v.plane_height,
+                                             device_planes ? hipMemcpyDeviceToDevice
+                                                           : hipMemcpyHostToDevice,
+                                             st);
             if (e2 == hipSuccess) e2 = launch_pad_plane(st, plane, v.plane_width, v.plane_height);
             if (e2 != hipSuccess)
                 return bail(fail(CHD_ERR_HIP, std::string("plane upload: ") + hipGetErrorString(e2)));
@@ -792,6 +801,48 @@ int chd_session_create(chd_context *context, const chd_view *views, uint32_t cou
     if (e != hipSuccess) return bail(fail(CHD_ERR_HIP, std::string("upload sync: ") + hipGetErrorString(e)));
     context->sessions.push_back(s);
     *out = s;
+    return CHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chd_session_create(chd_context *context, const chd_view *views, uint32_t count,
+                       uint32_t reference_index, const chd_options *options,
+                       chd_session **out) {
+    return session_create(context, views, count, reference_index, options, false, out);
+}
+
+int chd_session_create_device(chd_context *context, const chd_view *views, uint32_t count,
+                              uint32_t reference_index, const chd_options *options,
+                              chd_session **out) {
+    return session_create(context, views, count, reference_index, options, true, out);
+}
+
+int chd_session_set_planes(chd_session *s, const float *const *planes, uint32_t count,
+                           int device) {
+    if (!s || !planes) return fail(CHD_ERR_ARGUMENT, "set_planes: bad argument");
+    if (s->none) return CHD_NONE;
+    if (count != s->count) return fail(CHD_ERR_ARGUMENT, "set_planes: one plane per view");
+    for (uint32_t i = 0; i < count; i++)
+        if (!planes[i]) return fail(CHD_ERR_ARGUMENT, "set_planes: null plane");
+    CHD_HIP(hipSetDevice(s->context->device));
+    hipStream_t st = s->context->stream;
+    for (uint32_t i = 0; i < count; i++) {
+        const ViewDev &d = s->host_views[i];
+        float *plane = const_cast<float *>(d.plane);
+        CHD_HIP(hipMemcpy2DAsync(plane, (size_t)d.plane_pitch * sizeof(float), planes[i],
+                                 (size_t)d.plane_width * sizeof(float),
+                                 (size_t)d.plane_width * sizeof(float), d.plane_height,
+                                 device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        CHD_HIP(launch_pad_plane(st, plane, d.plane_width, d.plane_height));
+        if (d.columns)
+            CHD_HIP(launch_transpose_plane(st, d.plane, const_cast<float *>(d.columns),
+                                           d.plane_width, d.plane_height));
+    }
+    if (!device) CHD_HIP(hipStreamSynchronize(st));  // the host planes may go away
+    s->stages_done = 0;
     return CHD_OK;
 }
 
@@ -1379,6 +1430,218 @@ int chd_luminance_half(chd_context *c, const uint16_t *samples, uint32_t width, 
     CHD_HIP(hipMemcpyAsync(plane, d_plane, m * 4, hipMemcpyDeviceToHost, c->stream));
     CHD_HIP(hipStreamSynchronize(c->stream));
     return CHD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ synthetic frames
+
+struct chd_frames {
+    chd_context *context = nullptr;
+    std::vector<RenderModule> modules;
+    uint32_t reference = 0;
+    BayerSpec bayer{};
+    uint32_t shift = 0;
+    float black = 0.0f, white = 0.0f;
+    std::vector<uint16_t *> mosaics;
+    std::vector<float *> planes;
+    std::vector<void *> allocations;
+    SceneDev host_scene{};          // source of the last upload (kept alive)
+    SceneDev *scene = nullptr;      // device copy
+    float *canvas = nullptr;
+    uint8_t *cells = nullptr;
+    size_t canvas_points = 0;
+
+    ~chd_frames() {
+        if (canvas) hipFree(canvas);
+        if (cells) hipFree(cells);
+        for (void *p : allocations) hipFree(p);
+    }
+};
+
+namespace {
+
+uint64_t splitmix64_host(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+uint64_t hash3_host(uint64_t seed, uint64_t stream, uint64_t index) {
+    return splitmix64_host(splitmix64_host(seed ^ splitmix64_host(stream * 0x632be59bd9b4e019ULL)) ^
+                           index);
+}
+double unit_host(uint64_t h) { return (double)(h >> 11) * 0x1p-53; }
+
+// The scene of `sc` for a rig: Voronoi sites over the reference raster,
+// inverse depths, nearest-first order, and a canvas with a margin of the
+// largest parallax of the rig (plus 64 px) around the reference.
+bool scene_state(const chd_synthetic_scene &sc, const std::vector<RenderModule> &modules,
+                 uint32_t reference, SceneDev &s) {
+    if (sc.plane_count == 0 || sc.plane_count > kMaxScenePlanes) return false;
+    const RenderModule &ref = modules[reference];
+    std::memset(&s, 0, sizeof s);
+    s.seed = sc.seed;
+    s.n = sc.plane_count;
+    s.texture_cell_px = sc.texture_cell_px;
+    for (uint32_t i = 0; i < s.n; i++) {
+        s.site_x[i] = unit_host(hash3_host(sc.seed, 1, i)) * (double)ref.width;
+        s.site_y[i] = unit_host(hash3_host(sc.seed, 2, i)) * (double)ref.height;
+        s.inverse[i] = sc.inverse_depth_min +
+                       (sc.inverse_depth_max - sc.inverse_depth_min) * unit_host(hash3_host(sc.seed, 3, i));
+        s.order[i] = i;
+    }
+    for (uint32_t i = 1; i < s.n; i++) {  // nearest first, ties by index
+        uint32_t v = s.order[i], j = i;
+        while (j > 0 && s.inverse[s.order[j - 1]] < s.inverse[v]) {
+            s.order[j] = s.order[j - 1];
+            j--;
+        }
+        s.order[j] = v;
+    }
+    double worst = 0.0;
+    for (const RenderModule &m : modules) {
+        const double b = std::sqrt(m.center[0] * m.center[0] + m.center[1] * m.center[1]);
+        const double d = ref.focal_px * b * sc.inverse_depth_max;
+        if (d > worst) worst = d;
+    }
+    const uint32_t margin = (uint32_t)std::ceil(worst) + 64;
+    s.canvas_x0 = -(double)margin;
+    s.canvas_y0 = -(double)margin;
+    s.canvas_w = ref.width + 2 * margin;
+    s.canvas_h = ref.height + 2 * margin;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chd_frames_create(chd_context *c, const chd_synthetic_module *modules, uint32_t count,
+                      uint32_t reference_index, const chd_bayer *bayer, uint32_t shift,
+                      float black_q6, float white_q6, chd_frames **out) {
+    if (!c || !modules || !bayer || !out || count == 0 || reference_index >= count || shift > 4)
+        return fail(CHD_ERR_ARGUMENT, "frames_create: bad argument");
+    *out = nullptr;
+    if (c->released) return fail(CHD_ERR_STATE, "frames_create: the context was destroyed");
+    for (uint32_t i = 0; i < count; i++)
+        if (modules[i].width < 2 || modules[i].height < 2 || modules[i].width % 2 ||
+            modules[i].height % 2 || !(modules[i].focal_px > 0.0))
+            return fail(CHD_ERR_ARGUMENT, "frames_create: module sizes must be even and focal lengths positive");
+    CHD_HIP(hipSetDevice(c->device));
+    auto *f = new chd_frames;
+    f->context = c;
+    f->reference = reference_index;
+    f->bayer = BayerSpec{bayer->black_level, bayer->white_level, bayer->exposure,
+                         bayer->electrons_per_dn, bayer->read_noise_dn};
+    f->shift = shift;
+    f->black = black_q6;
+    f->white = white_q6;
+    for (uint32_t i = 0; i < count; i++) {
+        RenderModule m{};
+        m.focal_px = modules[i].focal_px;
+        std::memcpy(m.center, modules[i].center, sizeof m.center);
+        std::memcpy(m.rotation_wc, modules[i].rotation_wc, sizeof m.rotation_wc);
+        m.width = modules[i].width;
+        m.height = modules[i].height;
+        f->modules.push_back(m);
+    }
+    auto alloc = [&](size_t bytes, void **p) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+        if (e != hipSuccess) return fail(CHD_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        f->allocations.push_back(*p);
+        return CHD_OK;
+    };
+    int r = CHD_OK;
+    void *p = nullptr;
+    if (r == CHD_OK) r = alloc(sizeof(SceneDev), &p);
+    f->scene = (SceneDev *)p;
+    for (uint32_t i = 0; i < count && r == CHD_OK; i++) {
+        const RenderModule &m = f->modules[i];
+        r = alloc((size_t)m.width * m.height * sizeof(uint16_t), &p);
+        f->mosaics.push_back((uint16_t *)p);
+        if (r == CHD_OK) r = alloc((size_t)(m.width / 2) * (m.height / 2) * sizeof(float), &p);
+        f->planes.push_back((float *)p);
+    }
+    if (r != CHD_OK) {
+        delete f;
+        return r;
+    }
+    c->frames.push_back(f);
+    *out = f;
+    return CHD_OK;
+}
+
+int chd_frames_render(chd_frames *f, const chd_synthetic_scene *scene) {
+    if (!f || !scene) return fail(CHD_ERR_ARGUMENT, "frames_render: bad argument");
+    chd_context *c = f->context;
+    CHD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    SceneDev next;
+    if (!scene_state(*scene, f->modules, f->reference, next))
+        return fail(CHD_ERR_ARGUMENT, "frames_render: plane_count must be 1..64");
+    if (scene->texture_cell_px == 0) return fail(CHD_ERR_ARGUMENT, "frames_render: zero texture cell");
+    const size_t points = (size_t)next.canvas_w * next.canvas_h;
+    // the previous render may still read the canvas and the scene copy
+    CHD_HIP(hipStreamSynchronize(st));
+    if (points > f->canvas_points) {
+        if (f->canvas) hipFree(f->canvas);
+        if (f->cells) hipFree(f->cells);
+        f->canvas = nullptr;
+        f->cells = nullptr;
+        f->canvas_points = 0;
+        CHD_HIP(hipMalloc(&f->canvas, points * sizeof(float)));
+        CHD_HIP(hipMalloc(&f->cells, points));
+        f->canvas_points = points;
+    }
+    f->host_scene = next;
+    CHD_HIP(hipMemcpyAsync(f->scene, &f->host_scene, sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_scene_canvas(st, f->scene, next.canvas_w, next.canvas_h, f->canvas, f->cells));
+    CHD_CHECK(luminance_table(c, f->black, f->white, false));
+    const RenderModule &ref = f->modules[f->reference];
+    for (uint32_t i = 0; i < f->modules.size(); i++) {
+        const RenderModule &m = f->modules[i];
+        CHD_HIP(launch_render_bayer(st, f->scene, f->canvas, f->cells, m, ref, i, i == f->reference,
+                                    f->bayer, f->shift, f->mosaics[i]));
+        CHD_HIP(launch_luminance_half(st, f->mosaics[i], m.width, m.height, c->luminance_table,
+                                      0, f->planes[i]));
+    }
+    return CHD_OK;
+}
+
+int chd_frames_plane(const chd_frames *f, uint32_t index, const float **plane,
+                     uint32_t *plane_width, uint32_t *plane_height) {
+    if (!f || !plane || index >= f->modules.size())
+        return fail(CHD_ERR_ARGUMENT, "frames_plane: bad argument");
+    *plane = f->planes[index];
+    if (plane_width) *plane_width = f->modules[index].width / 2;
+    if (plane_height) *plane_height = f->modules[index].height / 2;
+    return CHD_OK;
+}
+
+int chd_frames_read(chd_frames *f, uint32_t index, uint16_t *mosaic, float *plane) {
+    if (!f || index >= f->modules.size()) return fail(CHD_ERR_ARGUMENT, "frames_read: bad argument");
+    chd_context *c = f->context;
+    CHD_HIP(hipSetDevice(c->device));
+    const RenderModule &m = f->modules[index];
+    if (mosaic)
+        CHD_HIP(hipMemcpyAsync(mosaic, f->mosaics[index], (size_t)m.width * m.height * 2,
+                               hipMemcpyDeviceToHost, c->stream));
+    if (plane)
+        CHD_HIP(hipMemcpyAsync(plane, f->planes[index], (size_t)(m.width / 2) * (m.height / 2) * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+    CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
+}
+
+void chd_frames_destroy(chd_frames *f) {
+    if (!f) return;
+    chd_context *c = f->context;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    unregister(c->frames, f);
+    delete f;
+    context_release_if_unused(c);
 }
 
 int chd_device_expf(chd_context *c, const float *x, float *y, size_t n) {
@@ -2260,13 +2523,15 @@ int chd_proposal_adam_step(chd_proposal *p, const float *gradients, float learni
     return CHD_OK;
 }
 
-int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
-                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
-                             uint32_t *full_tiles, uint32_t *tiles) {
-    if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
-        return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
-    if (s->none) return CHD_NONE;
-    if (s->context != p->context) return fail(CHD_ERR_ARGUMENT, "set_proposal: another context");
+}  // extern "C"
+
+namespace {
+
+// The network input of the session's planes into the proposal's first
+// batch slot: the reference and the first two targets, means over cells of
+// (grid step / 2)^2 plane pixels through the input transform.
+int proposal_input(chd_session *s, chd_proposal *p) {
+    if (s->context != p->context) return fail(CHD_ERR_ARGUMENT, "proposal: another context");
     if (s->active.empty()) return fail(CHD_ERR_STATE, "set_proposal: no target view");
     const auto ref = s->plane_sizes[s->reference];
     // one network pixel per coarse node: cells of step / 2 plane pixels
@@ -2296,6 +2561,32 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
     }
     CHD_HIP(launch_quarter_input(st, planes, pitches, ref.first, ref.second, p->width, p->height,
                                  factor, p->input_offset, p->input_scale, p->act[0]));
+    return CHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chd_session_network_input(chd_session *s, chd_proposal *p, uint16_t *out) {
+    if (!s || !p || !out) return fail(CHD_ERR_ARGUMENT, "network_input: bad argument");
+    if (s->none) return CHD_NONE;
+    CHD_CHECK(proposal_input(s, p));
+    hipStream_t st = s->context->stream;
+    CHD_HIP(hipMemcpyAsync(out, p->act[0], (size_t)p->width * p->height * 16 * sizeof(uint16_t),
+                           hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
+                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
+                             uint32_t *full_tiles, uint32_t *tiles) {
+    if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
+        return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
+    if (s->none) return CHD_NONE;
+    CHD_CHECK(proposal_input(s, p));
+    hipStream_t st = s->context->stream;
     CHD_CHECK(proposal_forward(p, 1));
     CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
     CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,

@@ -218,4 +218,43 @@ hipError_t launch_exp(hipStream_t stream, const double *x, double *y, size_t n);
 
 hipError_t launch_fill_u8(hipStream_t stream, uint8_t *data, uint8_t value, size_t n);
 
+// ------------------------------------------------ synthetic capture generator
+
+// The scene as the render kernels see it: Voronoi sites, plane inverse
+// depths (1/mm) and their nearest-first order, and the texture canvas
+// placement (reference raster coordinates of its first point).
+constexpr uint32_t kMaxScenePlanes = 64;
+struct SceneDev {
+    uint64_t seed;
+    uint32_t n;
+    uint32_t texture_cell_px;
+    double site_x[kMaxScenePlanes], site_y[kMaxScenePlanes];
+    double inverse[kMaxScenePlanes];
+    uint32_t order[kMaxScenePlanes];
+    double canvas_x0, canvas_y0;
+    uint32_t canvas_w, canvas_h;
+};
+
+// A pinhole module: principal point at the raster centre, optical centre in
+// mm in the reference frame, world-to-camera rotation (row-major).
+struct RenderModule {
+    double focal_px;
+    double center[3];
+    double rotation_wc[9];
+    uint32_t width, height;
+};
+
+// Sensor model (f32 fields: the noise arithmetic promotes them as written).
+struct BayerSpec {
+    float black_level, white_level, exposure, electrons_per_dn, read_noise_dn;
+};
+
+hipError_t launch_scene_canvas(hipStream_t stream, const SceneDev *scene, uint32_t canvas_w,
+                               uint32_t canvas_h, float *canvas, uint8_t *cells);
+// Bayer codes of one module, shifted left by `shift`, into out (width x height).
+hipError_t launch_render_bayer(hipStream_t stream, const SceneDev *scene, const float *canvas,
+                               const uint8_t *cells, const RenderModule &module,
+                               const RenderModule &reference, uint32_t view, bool is_reference,
+                               const BayerSpec &bayer, uint32_t shift, uint16_t *out);
+
 }  // namespace chd

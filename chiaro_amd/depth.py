@@ -90,6 +90,7 @@ class Context:
         self._batches = weakref.WeakSet()
         self._sessions = weakref.WeakSet()
         self._proposals = weakref.WeakSet()
+        self._frames = weakref.WeakSet()
 
     def synchronize(self):
         _abi.check(self.lib.chd_context_synchronize(self.handle), "synchronize")
@@ -102,6 +103,8 @@ class Context:
                 s.close()
             for p in list(self._proposals):
                 p.close()
+            for f in list(self._frames):
+                f.close()
             self.lib.chd_context_destroy(self.handle)
             self.handle = None
 
@@ -110,6 +113,25 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class DevicePlane:
+    """A luminance plane in device memory (row pitch = width), owned by
+    whoever made it (e.g. chiaro_amd.scene.Frames)."""
+
+    address: int
+    shape: tuple  # (height, width)
+    owner: object = None  # kept alive with the plane
+
+
+def device_planes(modules) -> bool:
+    """True when every module's luminance is a DevicePlane (False when none
+    is; a mix is an error)."""
+    kinds = {isinstance(m.luminance, DevicePlane) for m in modules}
+    if len(kinds) > 1:
+        raise ValueError("modules mix host and device planes")
+    return kinds == {True}
 
 
 class _ViewArrays:
@@ -121,13 +143,19 @@ class _ViewArrays:
         n = len(modules)
         self.views = (_abi.View * n)()
         for i, m in enumerate(modules):
-            plane = np.ascontiguousarray(m.luminance, dtype=np.float32)
             warp = np.ascontiguousarray(m.warp_points, dtype=np.float32)
             conf = np.where(np.isnan(warp[..., 0]), 0.0, 1.0).astype(np.float32)
-            self.keep += [plane, warp, conf]
+            self.keep += [warp, conf]
             v = self.views[i]
-            v.luminance = _abi.fptr(plane)
-            v.plane_height, v.plane_width = plane.shape
+            if isinstance(m.luminance, DevicePlane):
+                v.luminance = C.cast(C.c_void_p(m.luminance.address), C.POINTER(C.c_float))
+                v.plane_height, v.plane_width = m.luminance.shape
+                self.keep.append(m.luminance)
+            else:
+                plane = np.ascontiguousarray(m.luminance, dtype=np.float32)
+                self.keep.append(plane)
+                v.luminance = _abi.fptr(plane)
+                v.plane_height, v.plane_width = plane.shape
             v.width, v.height = m.width, m.height
             if m.camera is not None:
                 cam = _abi.Camera()
@@ -159,9 +187,10 @@ class Session:
         self.options = options
         self._views = _ViewArrays(modules)
         handle = C.c_void_p()
-        _abi.check(self.lib.chd_session_create(context.handle, self._views.views, len(modules),
-                                               reference_index, C.byref(options.to_c()),
-                                               C.byref(handle)), "session create")
+        create = (self.lib.chd_session_create_device if device_planes(modules)
+                  else self.lib.chd_session_create)
+        _abi.check(create(context.handle, self._views.views, len(modules), reference_index,
+                          C.byref(options.to_c()), C.byref(handle)), "session create")
         self.handle = handle
         context._sessions.add(self)
         self._reference = reference_index
@@ -180,6 +209,25 @@ class Session:
             self.close()
         except Exception:
             pass
+
+    def set_planes(self, planes):
+        """New luminance for every view (a new capture of the same rig):
+        numpy arrays or DevicePlanes, one per module."""
+        device = isinstance(planes[0], DevicePlane)
+        keep = []
+        arr = (C.c_void_p * len(planes))()
+        for i, p in enumerate(planes):
+            if isinstance(p, DevicePlane) != device:
+                raise ValueError("planes mix host and device memory")
+            if device:
+                arr[i] = p.address
+            else:
+                a = np.ascontiguousarray(p, np.float32)
+                keep.append(a)
+                arr[i] = a.ctypes.data
+        _abi.check(self.lib.chd_session_set_planes(self.handle, arr, len(planes), int(device)),
+                   "set planes")
+        self._planes = planes
 
     def run(self, stages: int = _abi.CHD_STAGE_ALL) -> int:
         return _abi.check(self.lib.chd_session_run(self.handle, stages), "session run")
@@ -343,6 +391,25 @@ class Batch:
                                              C.byref(handle)), "batch create")
         self.handle = handle
         context._batches.add(self)
+
+    def set_planes(self, planes):
+        """New luminance for every view (a new capture of the same rig):
+        numpy arrays or DevicePlanes, one per module."""
+        device = isinstance(planes[0], DevicePlane)
+        keep = []
+        arr = (C.c_void_p * len(planes))()
+        for i, p in enumerate(planes):
+            if isinstance(p, DevicePlane) != device:
+                raise ValueError("planes mix host and device memory")
+            if device:
+                arr[i] = p.address
+            else:
+                a = np.ascontiguousarray(p, np.float32)
+                keep.append(a)
+                arr[i] = a.ctypes.data
+        _abi.check(self.lib.chd_session_set_planes(self.handle, arr, len(planes), int(device)),
+                   "set planes")
+        self._planes = planes
 
     def run(self, stages: int = _abi.CHD_STAGE_ALL) -> int:
         return _abi.check(self.lib.chd_batch_run(self.handle, stages), "batch run")

@@ -285,6 +285,14 @@ class Proposal:
         _abi.check(self.lib.chd_proposal_parameters(self.handle, None, v.ctypes.data, C.byref(n)),
                    "set parameters")
 
+    def session_input(self, session) -> np.ndarray:
+        """The network input the device builds from a session's planes
+        (what apply() feeds the network): (height, width, 16) BF16 bits."""
+        out = np.empty((self.height, self.width, 16), np.uint16)
+        _abi.check(self.lib.chd_session_network_input(session.handle, self.handle,
+                                                      out.ctypes.data), "network input")
+        return out
+
     def optimizer_state(self):
         """Adam moments (first, then second, in parameter order) and the step."""
         n, step = C.c_size_t(), C.c_uint64()
@@ -514,18 +522,43 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
 
 
 def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int = 3):
-    """The captures run as one batch (chd_batch): the full sweep against the
-    band sweep for each (threshold, tile, max_half_width[, half_width]) in
-    `settings` (half_width 8 unless given). Per setting: the fraction of nodes tested by the full
-    sweep whose band label is within +-1, mask agreement, fallback-tile
-    fraction and both times per batch (the band time includes the network
-    and the windows of every capture)."""
-    from .depth import Batch
-    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    """batch_agreement_sessions over fresh sessions of `caps`."""
     sessions = [Session(ctx, c.modules, c.reference_index, c.options) for c in caps]
+    try:
+        return batch_agreement_sessions(ctx, sessions, proposal, settings, steps)
+    finally:
+        for s in sessions:
+            s.close()
+
+
+def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, settings,
+                             steps: int = 3, stages: int = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM):
+    """The sessions run as one batch (chd_batch): the full sweep against the
+    band sweep for each (threshold, tile, max_half_width[, half_width]) in
+    `settings` (half_width 8 unless given). Per setting: the fraction of
+    nodes tested by the full sweep whose band label is within +-1, mask
+    agreement, fallback-tile fraction and both times per batch (the band
+    time includes the network and the windows of every capture). With the
+    direct and target stages in `stages` the rows also compare the end
+    products: the fraction of fine nodes whose u8 provenance (the
+    consistency mask, depth.rs:548-579) and whose per-target decision codes
+    (depth.rs:420-487) equal the full sweep's (consistency_mask_agreement,
+    decision_agreement)."""
+    from .depth import Batch
+    end_to_end = bool(stages & _abi.CHD_STAGE_TARGETS)
     batch = Batch(ctx, sessions)
     rows = []
+
+    def products():
+        out = []
+        for s in sessions:
+            dmap, targets = s.results()
+            out.append((dmap.provenance.copy(), {t: r.decisions.copy() for t, r in targets.items()}))
+        return out
+
     try:
+        for s in sessions:
+            clear(s)
         batch.run(stages)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -535,6 +568,7 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
         full_ms = (time.perf_counter() - t0) * 1000 / steps
         full_best = [s.coarse()[2] for s in sessions]
         full_tested = [s.tested() for s in sessions]
+        full_products = products() if end_to_end else None
         for setting in settings:
             threshold, tile, max_half = setting[:3]
             half = setting[3] if len(setting) > 3 else 8
@@ -561,15 +595,24 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
                 close += int((np.abs(bb.astype(np.int32) - fb.astype(np.int32)) <= 1)[v].sum())
                 valid += int(v.sum())
                 mask += float((bt == ft).mean()) / len(sessions)
-            rows.append({"threshold": threshold, "tile": tile, "max_half_width": max_half,
-                         "half_width": half,
-                         "agreement": close / max(valid, 1), "mask_agreement": mask,
-                         "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
-                         "band_ms": band_ms, "speedup": full_ms / band_ms})
+            row = {"threshold": threshold, "tile": tile, "max_half_width": max_half,
+                   "half_width": half,
+                   "agreement": close / max(valid, 1), "mask_agreement": mask,
+                   "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
+                   "band_ms": band_ms, "speedup": full_ms / band_ms}
+            if end_to_end:
+                same_prov = total_prov = same_dec = total_dec = 0
+                for (fp, fd), (bp, bd) in zip(full_products, products()):
+                    same_prov += int((fp == bp).sum())
+                    total_prov += fp.size
+                    for t in fd:
+                        same_dec += int((fd[t] == bd[t]).sum())
+                        total_dec += fd[t].size
+                row["consistency_mask_agreement"] = same_prov / max(total_prov, 1)
+                row["decision_agreement"] = same_dec / max(total_dec, 1)
+            rows.append(row)
     finally:
         for s in sessions:
             clear(s)
         batch.close()
-        for s in sessions:
-            s.close()
     return rows

@@ -168,8 +168,8 @@ BAYER_12BIT = dict(black_level=64.0, white_level=4095.0, exposure=0.85, electron
                    read_noise_dn=2.0)
 
 
-def config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: float = 3300.0,
-            planes: int = 128, plane_count: int = 16, ring_mm: float = 30.0) -> Capture:
+def rig_config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: float = 3300.0,
+                planes: int = 128, plane_count: int = 16, ring_mm: float = 30.0) -> "Rig":
     """Five modules: the reference at the centre and four on a 30 mm ring at
     0/90/180/270 deg; f = 3300 px; 12-bit RGGB Bayer codes (black 64, Poisson
     shot noise at 0.5 e-/DN, read noise 2 DN) converted by luminance_half with
@@ -181,15 +181,15 @@ def config1(seed: int = 0, width: int = 4160, height: int = 3120, focal_px: floa
         modules.append(_module(focal_px, width, height, (ring_mm * math.cos(a), ring_mm * math.sin(a), 0.0)))
     desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.05e-3,
                           inverse_depth_max=1.25e-3, texture_cell_px=32)
-    bayer = _abi.Bayer(**BAYER_12BIT)
-    raws = _render(desc, modules, 0, 0.0, bayer)
-    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
     options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
-    cap = _assemble(modules, planes_, 0, options,
-                    {"workload": "config1", "seed": seed, "width": width, "height": height,
-                     "views": 5, "planes": planes}, desc)
-    cap.raws = raws
-    return cap
+    return Rig(modules, 0, desc, _abi.Bayer(**BAYER_12BIT), options,
+               {"workload": "config1", "seed": seed, "width": width, "height": height,
+                "views": 5, "planes": planes})
+
+
+def config1(seed: int = 0, **kw) -> Capture:
+    """Config 1 rendered on the host (see rig_config1)."""
+    return render_host(rig_config1(seed, **kw))
 
 
 def _small_rotation(rng: np.random.Generator, degrees: float) -> np.ndarray:
@@ -201,8 +201,8 @@ def _small_rotation(rng: np.random.Generator, degrees: float) -> np.ndarray:
     return np.eye(3) + math.sin(a) * k + (1.0 - math.cos(a)) * (k @ k)
 
 
-def config2(seed: int = 0, scale: float = 1.0, planes: int = 128,
-            plane_count: int = 16) -> Capture:
+def rig_config2(seed: int = 0, scale: float = 1.0, planes: int = 128,
+                plane_count: int = 16) -> "Rig":
     """One capture of the batched workload: ten modules of uneven size, the
     reference (4160x3120, f = 3300 px) at the origin plus four more of that
     kind, three 2080x1560 at f = 4125 px and two 1040x780 at f = 4425 px;
@@ -231,16 +231,15 @@ def config2(seed: int = 0, scale: float = 1.0, planes: int = 128,
                                rotation))
     desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.05e-3,
                           inverse_depth_max=1.25e-3, texture_cell_px=32)
-    bayer = _abi.Bayer(**BAYER_12BIT)
-    raws = _render(desc, modules, 0, 0.0, bayer)
-    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
     options = DepthOptions(near_depth=800.0, far_depth=20000.0, planes=planes)
-    cap = _assemble(modules, planes_, 0, options,
-                    {"workload": "config2", "seed": seed, "scale": scale, "views": len(specs),
-                     "planes": planes}, desc)
-    cap.raws = raws
-    return cap
+    return Rig(modules, 0, desc, _abi.Bayer(**BAYER_12BIT), options,
+               {"workload": "config2", "seed": seed, "scale": scale, "views": len(specs),
+                "planes": planes})
 
+
+def config2(seed: int = 0, **kw) -> Capture:
+    """Config 2 rendered on the host (see rig_config2)."""
+    return render_host(rig_config2(seed, **kw))
 
 
 def config3(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float = 1600.0) -> Capture:
@@ -254,8 +253,8 @@ def config3(seed: int = 0, width: int = 1040, height: int = 780, focal_px: float
     return cap
 
 
-def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
-            plane_count: int = 24, wide_only: bool = False) -> Capture:
+def rig_config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
+                plane_count: int = 24, wide_only: bool = False) -> "Rig":
     """One capture of the full-capture workload: sixteen 4160x3120 modules
     viewing nested fields of one 24-plane scene - five wide (f = 3300 px: the
     reference at the origin and four on a 30 mm ring at 0/90/180/270 deg),
@@ -283,14 +282,130 @@ def config4(seed: int = 0, scale: float = 1.0, planes: int = 256,
     desc = _abi.SceneDesc(seed=seed, plane_count=plane_count, inverse_depth_min=0.02e-3,
                           inverse_depth_max=1.25e-3,
                           texture_cell_px=max(4, int(round(32 * scale))))
-    bayer = _abi.Bayer(**BAYER_12BIT)
-    raws = _render(desc, modules, 0, 0.0, bayer)
-    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
     options = DepthOptions(near_depth=800.0, far_depth=50000.0, planes=planes)
     names = ([f"W{k}" for k in range(5)] + [f"M{k}" for k in range(5)]
              + [f"T{k}" for k in range(6)])[:len(modules)]
-    cap = _assemble(modules, planes_, 0, options,
-                    {"workload": "config4", "seed": seed, "scale": scale, "views": len(modules),
-                     "planes": planes}, desc, names)
+    return Rig(modules, 0, desc, _abi.Bayer(**BAYER_12BIT), options,
+               {"workload": "config4", "seed": seed, "scale": scale, "views": len(modules),
+                "planes": planes}, names)
+
+
+def config4(seed: int = 0, **kw) -> Capture:
+    """Config 4 rendered on the host (see rig_config4)."""
+    return render_host(rig_config4(seed, **kw))
+
+
+# ------------------------------------------------------------ rigs
+
+@dataclass
+class Rig:
+    """What a Bayer capture is made of: modules, reference, the scene
+    description (its seed is the capture's), the sensor model and the
+    depth options."""
+
+    modules: list
+    reference: int
+    scene: _abi.SceneDesc
+    bayer: _abi.Bayer
+    options: DepthOptions
+    description: dict
+    names: list | None = None
+
+    def with_seed(self, seed: int) -> "Rig":
+        d = _abi.SceneDesc()
+        C.pointer(d)[0] = self.scene
+        d.seed = seed
+        desc = dict(self.description, seed=seed)
+        return Rig(self.modules, self.reference, d, self.bayer, DepthOptions(**vars(self.options)),
+                   desc, self.names)
+
+
+def render_host(rig: Rig) -> Capture:
+    """The capture rendered by the host generator (scene.c): Bayer codes,
+    luminance_half of the codes shifted left by 4 (black 64*16, white
+    4095*16)."""
+    raws = _render(rig.scene, rig.modules, rig.reference, 0.0, rig.bayer)
+    planes_ = [luminance_half_u16(r, 4, 64.0 * 16, 4095.0 * 16) for r in raws]
+    cap = _assemble(rig.modules, planes_, rig.reference, DepthOptions(**vars(rig.options)),
+                    dict(rig.description), rig.scene, rig.names)
     cap.raws = raws
     return cap
+
+
+class Frames:
+    """A rig's frames rendered on the GPU (chd_frames): the same codes as the
+    host generator, shifted left by 4 and converted by luminance_half on the
+    device, planes kept in HBM for sessions (chiaro_amd.depth.DevicePlane)."""
+
+    SHIFT, BLACK, WHITE = 4, 64.0 * 16, 4095.0 * 16
+
+    def __init__(self, context, rig: Rig):
+        from .depth import DevicePlane  # noqa: F401  (the planes' type)
+        self.context = context
+        self.lib = context.lib
+        self.rig = rig
+        n = len(rig.modules)
+        arr = (_abi.Module * n)(*rig.modules)
+        handle = C.c_void_p()
+        _abi.check(self.lib.chd_frames_create(context.handle, arr, n, rig.reference,
+                                              C.byref(rig.bayer), self.SHIFT, self.BLACK,
+                                              self.WHITE, C.byref(handle)), "frames create")
+        self.handle = handle
+        context._frames.add(self)
+        self.seed = None
+        self._cameras = [camera_for(m) for m in rig.modules]
+        self._warps = None
+
+    def close(self):
+        if self.handle:
+            self.lib.chd_frames_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, seed: int):
+        """Draws the rig's scene with this seed (asynchronous on the
+        context's stream)."""
+        d = _abi.SceneDesc()
+        C.pointer(d)[0] = self.rig.scene
+        d.seed = seed
+        _abi.check(self.lib.chd_frames_render(self.handle, C.byref(d)), "frames render")
+        self.seed = seed
+
+    def planes(self):
+        from .depth import DevicePlane
+        out = []
+        for i in range(len(self.rig.modules)):
+            p, w, h = C.c_void_p(), C.c_uint32(), C.c_uint32()
+            _abi.check(self.lib.chd_frames_plane(self.handle, i, C.byref(p), C.byref(w),
+                                                 C.byref(h)), "frames plane")
+            out.append(DevicePlane(p.value, (h.value, w.value), self))
+        return out
+
+    def read(self, index: int):
+        """(mosaic u16 of shifted codes, plane f32) of one module."""
+        m = self.rig.modules[index]
+        mosaic = np.empty((m.height, m.width), np.uint16)
+        plane = np.empty((m.height // 2, m.width // 2), np.float32)
+        _abi.check(self.lib.chd_frames_read(self.handle, index, mosaic.ctypes.data,
+                                            plane.ctypes.data), "frames read")
+        return mosaic, plane
+
+    def capture(self) -> Capture:
+        """The last render as a Capture whose planes stay on the device."""
+        rig = self.rig.with_seed(self.seed)
+        ref = rig.modules[rig.reference]
+        if self._warps is None:
+            self._warps = [identity_warp(ref.width, ref.height) if i == rig.reference
+                           else global_warp(self._cameras[rig.reference], cam, ref.width, ref.height)
+                           for i, cam in enumerate(self._cameras)]
+        inputs = [ModuleInput(name=(rig.names[i] if rig.names else f"M{i}"), luminance=plane,
+                              width=m.width, height=m.height, camera=self._cameras[i],
+                              warp_points=self._warps[i], warp_step=32)
+                  for i, (m, plane) in enumerate(zip(rig.modules, self.planes()))]
+        return Capture(modules=inputs, reference_index=rig.reference, options=rig.options,
+                       description=rig.description)

@@ -514,6 +514,71 @@ int chd_luminance_half_device(chd_context *context, const uint16_t *samples, uin
                               uint32_t height, float black_q6, float white_q6,
                               int reserve_headroom, float *plane);
 
+/* Synthetic captures rendered on the GPU. No counterpart in the reference
+ * (its only fakes are gradient mock LRIs, crates/chiaro/src/mock.rs:62):
+ * a seeded scene of fronto-parallel planes over a Voronoi partition of the
+ * reference raster, textured by 4 octaves of value noise, seen by pinhole
+ * modules (principal point at the raster centre) and recorded as 12-bit
+ * Bayer codes with black level, Poisson shot noise and Gaussian read noise;
+ * the codes are shifted left by `shift` bits (the pipeline's Q6 step,
+ * pipeline.rs:1026-1030) and converted by luminance_half (image.rs:401-426)
+ * into the planes a session takes. */
+typedef struct chd_synthetic_scene {
+    uint64_t seed;
+    uint32_t plane_count;     /* 1..64 */
+    double inverse_depth_min; /* 1/mm */
+    double inverse_depth_max;
+    uint32_t texture_cell_px; /* lattice spacing of the coarsest octave */
+} chd_synthetic_scene;
+
+typedef struct chd_synthetic_module {
+    double focal_px;
+    uint32_t width; /* raster size (even) */
+    uint32_t height;
+    double center[3];      /* optical centre in the reference frame, mm */
+    double rotation_wc[9]; /* world-to-camera rotation, row-major */
+} chd_synthetic_module;
+
+typedef struct chd_bayer {
+    float black_level; /* DN */
+    float white_level;
+    float exposure;    /* fraction of (white - black) reached by radiance 1 */
+    float electrons_per_dn;
+    float read_noise_dn;
+} chd_bayer;
+
+/* Device buffers of one rig's frames: a u16 mosaic and an f32 plane per
+ * module. create allocates them; render draws a scene into them on the
+ * context's stream (asynchronous; reuse for many scenes of the same rig);
+ * plane gives a module's device plane (plane_width x plane_height, row
+ * pitch plane_width) for chd_session_create_device / chd_session_set_planes;
+ * read copies a module's mosaic and/or plane to the host. */
+typedef struct chd_frames chd_frames;
+int chd_frames_create(chd_context *context, const chd_synthetic_module *modules, uint32_t count,
+                      uint32_t reference_index, const chd_bayer *bayer, uint32_t shift,
+                      float black_q6, float white_q6, chd_frames **out);
+int chd_frames_render(chd_frames *frames, const chd_synthetic_scene *scene);
+int chd_frames_plane(const chd_frames *frames, uint32_t index, const float **plane,
+                     uint32_t *plane_width, uint32_t *plane_height);
+int chd_frames_read(chd_frames *frames, uint32_t index, uint16_t *mosaic, float *plane);
+void chd_frames_destroy(chd_frames *frames);
+
+/* chd_session_create with every chd_view.luminance a device pointer (row
+ * pitch plane_width), e.g. from chd_frames_plane: the planes never leave
+ * HBM. */
+int chd_session_create_device(chd_context *context, const chd_view *views, uint32_t count,
+                              uint32_t reference_index, const chd_options *options,
+                              chd_session **out);
+/* Replaces the luminance of every view of a session (same sizes; planes[i]
+ * on the device when `device`, else on the host) for the next run: a new
+ * capture of the same rig. Earlier results are invalidated. */
+int chd_session_set_planes(chd_session *session, const float *const *planes, uint32_t count,
+                           int device);
+/* The learned proposal's input for the session's current planes (what
+ * chd_session_set_proposal feeds the network): proposal width x height x 16
+ * BF16 bits into `out` (host). */
+int chd_session_network_input(chd_session *session, chd_proposal *proposal, uint16_t *out);
+
 /* Device f32 exp / f64 exp used by the kernels, evaluated on the GPU for
  * n inputs (their parity with the platform libm is checked by the tests). */
 int chd_device_expf(chd_context *context, const float *x, float *y, size_t n);
