@@ -259,6 +259,9 @@ def main() -> int:
                     help="full config 4 captures that choose the band setting")
     ap.add_argument("--learned-test", type=int, default=4,
                     help="held-out full config 4 captures the band is reported on")
+    ap.add_argument("--full-path", action="store_true",
+                    help="config2/config4: time the whole path (direct pass, filters and "
+                         "per-target decisions too) instead of cost + SGM")
     ap.add_argument("--captures", type=int, default=0,
                     help="config2/config4: captures per batch (default 8 and 4)")
     ap.add_argument("--tuning", choices=["default", "auto"], default="default",
@@ -482,22 +485,26 @@ def bench_batch(args) -> int:
     from chiaro_amd.depth import Batch, Context, Session
 
     count = args.captures or (8 if args.workload == "config2" else 4)
-    make = scene.config2 if args.workload == "config2" else scene.config4
-    t = time.perf_counter()
-    caps = []
-    for k in range(count):
-        caps.append(make(seed=args.seed + k))
-        caps[-1].raws = None          # only the luminance planes are used here
-    log(f"[bench] {count} {args.workload} captures generated in {time.perf_counter() - t:.1f} s")
+    rig_of = scene.rig_config2 if args.workload == "config2" else scene.rig_config4
     ctx = Context(0)
-    sessions = [Session(ctx, c.modules, c.reference_index, c.options) for c in caps]
+    t = time.perf_counter()
+    caps, sessions = [], []
+    for k in range(count):
+        # rendered on the GPU; the sessions copy the device planes
+        frames = scene.Frames(ctx, rig_of(seed=args.seed + k))
+        frames.render(args.seed + k)
+        caps.append(frames.capture())
+        sessions.append(Session(ctx, caps[-1].modules, 0, caps[-1].options))
+        frames.close()
+    log(f"[bench] {count} {args.workload} captures rendered on the GPU in "
+        f"{time.perf_counter() - t:.1f} s")
     tuning = sessions[0].autotune() if args.tuning == "auto" else sessions[0].tuning()
     if args.tuning == "auto":
         for s in sessions[1:]:
             s.set_tuning(**tuning)
     batch = Batch(ctx, sessions)
     queued = Batch(ctx, sessions, persistent=True)
-    stages = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
+    stages = _abi.CHD_STAGE_ALL if args.full_path else _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
 
     def timed(fn, steps):
         ctx.synchronize()

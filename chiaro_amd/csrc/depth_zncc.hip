@@ -279,10 +279,12 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
 // whether any node of n_begin .. n_end - 1 has its reference patch and the
 // view's global warp at its centre; if none has, *live is false, every ZNCC
 // of the view is None for these nodes and the tap context is not built (the
-// narrow modules of a rig see a small part of the reference).
+// narrow modules of a rig see a small part of the reference). `wanted`
+// (optional, per node) narrows the check to the nodes a caller will score.
 __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                                uint32_t a, bool first_group_ready = true, uint32_t n_begin = 0,
-                               uint32_t n_end = ~0u, bool *live = nullptr) {
+                               uint32_t n_end = ~0u, bool *live = nullptr,
+                               const int *wanted = nullptr) {
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const int r = (int)S.radius;
     const uint32_t width = 2u * S.radius + 1u;
@@ -301,7 +303,7 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
     if (live) {
         int any = 0;
         for (uint32_t n = n_begin + tid; n < n_end; n += nth)
-            any |= P.node_ok[n] && Q.nv_ok[n] ? 1 : 0;
+            any |= P.node_ok[n] && Q.nv_ok[n] && (!wanted || wanted[n]) ? 1 : 0;
         *live = __syncthreads_or(any) != 0;
         if (!*live) return Q;
     }
@@ -1207,7 +1209,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (uint32_t k = 0; node0 + n < nodes && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
     __syncthreads();
     for (uint32_t a = 0; a < V; a++) {
-        const PatchLds Q = patch_view(S, P, nb, T, a);
+        // a view no node of the tile sees builds no tap context: its items
+        // fail the node / warp test in score_slots and record None
+        bool live;
+        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, ~0u, &live);
         score_slots(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
                     &evaluation_count);
         __syncthreads();
@@ -1449,9 +1454,11 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
     const double min_inverse = 1.0 / o.far_depth, max_inverse = 1.0 / o.near_depth;
 
+    int *measured = state + nb;                    // [nb] the node has a depth
     for (uint32_t n = tid; n < nb; n += blockDim.x) {
         P.pos[n] = node_position(F, node0 + n);
         P.node_ok[n] = node0 + n < nodes ? 1 : 0;
+        measured[n] = node0 + n < nodes && field[node0 + n].valid ? 1 : 0;
     }
     if (tid == 0) list_count = 0;
     unsigned int evaluations = 0;  // thread 0's tally of ZNCCs past the bounds check
@@ -1459,11 +1466,32 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     patch_nodes(S, P, nb, T, centre_value);
 
     for (uint32_t a = 0; a < V; a++) {
-        const PatchLds Q = patch_view(S, P, nb, T, a);
+        bool live;
+        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, ~0u, &live, measured);
+        chd_node_warp *out = decisions + (size_t)a * nodes;
+        if (!live) {
+            // no node with a depth has its reference patch and this view's
+            // global warp: every ZNCC of the view is None, so every node is
+            // Global where the warp is defined at its centre and Undefined
+            // elsewhere (depth.rs:433-440, 458-485)
+            for (uint32_t n = tid; n < nb; n += blockDim.x) {
+                const uint32_t g = node0 + n;
+                if (g >= nodes) continue;
+                chd_node_warp d;
+                d.code = Q.gc_ok[n] ? CHD_WARP_GLOBAL : CHD_WARP_UNDEFINED;
+                d.global[0] = Q.gc_ok[n] ? Q.gc[n].x : 0.0f;
+                d.global[1] = Q.gc_ok[n] ? Q.gc[n].y : 0.0f;
+                d.point[0] = d.point[1] = 0.0f;
+                d.confidence = 0.0f;
+                d.measured = 0;
+                out[g] = d;
+            }
+            __syncthreads();
+            continue;
+        }
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
         const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
-        chd_node_warp *out = decisions + (size_t)a * nodes;
         // Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
@@ -1660,7 +1688,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
 
 size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
     const PatchLayout l = patch_layout(nb, V, T, false);
-    return (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 4;
+    return (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 5;
 }
 
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
