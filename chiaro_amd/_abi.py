@@ -258,6 +258,50 @@ def _fn(lib, name, restype, *argtypes):
     return f
 
 
+class AlignOptions(C.Structure):
+    _fields_ = [
+        ("grid_step", C.c_uint32),
+        ("patch", C.c_uint32),
+        ("coarse_radius", C.c_uint32),
+        ("min_score", C.c_float),
+        ("inlier_px", C.c_float),
+        ("min_inlier_ratio", C.c_float),
+        ("refine", C.c_int32),
+    ]
+
+
+class LevelReport(C.Structure):
+    _fields_ = [
+        ("scale", C.c_uint32),
+        ("patches", C.c_uint32),
+        ("inliers", C.c_uint32),
+        ("median_residual_px", C.c_float),
+    ]
+
+
+CHD_MAX_ALIGN_LEVELS = 16
+ALIGN_STATUS = ["reference", "rejected", "factory model only", "refined", "weak refinement",
+                "no correlation support; factory model kept"]
+
+
+class AlignmentReport(C.Structure):
+    _fields_ = [
+        ("initialised_from", C.c_uint32),
+        ("coverage", C.c_float),
+        ("correction_median_px", C.c_float * 2),
+        ("level_count", C.c_uint32),
+        ("levels", LevelReport * CHD_MAX_ALIGN_LEVELS),
+        ("inliers", C.c_uint32),
+        ("patches", C.c_uint32),
+        ("residual_median_px", C.c_float),
+        ("residual_p90_px", C.c_float),
+        ("inlier_ratio", C.c_float),
+        ("accepted", C.c_int32),
+        ("status", C.c_uint32),
+        ("gain", C.c_float),
+    ]
+
+
 # Every function declared in include/chiaro_depth.h, with its signature.
 DEPTH_SIGNATURES = {
     "chd_last_error": (C.c_char_p,),
@@ -324,6 +368,11 @@ DEPTH_SIGNATURES = {
                            C.c_int, VP),
     "chd_luminance_half_device": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, C.c_float,
                                   C.c_float, C.c_int, VP),
+    "chd_default_align_options": (None, P(AlignOptions)),
+    "chd_align_module": (C.c_int, VP, P(View), P(View), P(AlignOptions), P(Warp),
+                         P(AlignmentReport)),
+    "chd_match_patches": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, VP, VP, C.c_uint32,
+                          C.c_uint32, C.c_uint32, VP, VP, VP),
     "chd_frames_create": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_float,
                           C.c_float, P(VP)),
     "chd_frames_render": (C.c_int, VP, VP),

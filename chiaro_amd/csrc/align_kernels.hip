@@ -1,0 +1,257 @@
+// align_kernels.hip - the device half of align_module's refinement
+// (crates/chiaro-fusion/src/align.rs:270-519) on gfx950.
+//
+//   k_downsample       Plane::downsample, one pyramid level (image.rs:469-484)
+//   k_initial_grid     the factory model: map_from(.., 1e8) on an 8 px grid
+//                      (align.rs:317-324)
+//   k_render_through   the target through initial(correction(p)) at one
+//                      level (align.rs:523-547)
+//   k_match_patches    match_patch (image.rs:531-640) of many windows, one
+//                      block per window: the reference window and its
+//                      statistics in LDS, one lane per integer shift
+//                      summing the window in the reference's row-major order
+//                      (neighbouring lanes read neighbouring target texels),
+//                      then the peak and its sub-pixel parabola
+//
+// The RANSAC and least-squares fits run on the host (a few thousand pairs).
+// Planes of the pyramids are stored with one replicated padding column and
+// row (pitch width + 1), so Plane::sample's clamp of x0 + 1 at the border is
+// a plain read.
+#include <hip/hip_runtime.h>
+
+#include "depth_device.hpp"
+#include "depth_internal.hpp"
+
+namespace chd {
+
+namespace {
+
+__global__ void k_downsample(const float *src, uint32_t sw, uint32_t sh, uint32_t spitch,
+                             float *dst, uint32_t dw, uint32_t dh, uint32_t dpitch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= dw || j >= dh) return;
+    const uint32_t x = 2 * i, y = 2 * j;
+    const uint32_t x1 = min(x + 1, sw - 1), y1 = min(y + 1, sh - 1);
+    const float a = src[(size_t)y * spitch + x], b = src[(size_t)y * spitch + x1];
+    const float c = src[(size_t)y1 * spitch + x], d = src[(size_t)y1 * spitch + x1];
+    dst[(size_t)j * dpitch + i] = 0.25f * (a + b + c + d);
+}
+
+__global__ void k_initial_grid(const chd_camera *reference, const chd_camera *target,
+                               uint32_t columns, uint32_t rows, uint32_t step, float2 *out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c >= columns || r >= rows) return;
+    // target.map_from(reference, p, 1e8) filtered to finite points
+    // (geometry.rs:249-252)
+    d3 o, d;
+    pixel_to_ray(*reference, (double)((size_t)c * step), (double)((size_t)r * step), o, d);
+    const d3 far{o.x + d.x * kInfinityDepth, o.y + d.y * kInfinityDepth, o.z + d.z * kInfinityDepth};
+    double qx, qy;
+    float2 q = make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000));
+    if (project(*target, far, qx, qy) && finite_d(qx) && finite_d(qy))
+        q = make_float2((float)qx, (float)qy);
+    out[(size_t)r * columns + c] = q;
+}
+
+struct Homography {
+    double h[9];
+};
+
+// apply_homography (math.rs:131-138)
+__device__ __forceinline__ bool apply_homography(const Homography &H, double x, double y,
+                                                 double &ox, double &oy) {
+    const double *h = H.h;
+    const double v0 = h[0] * x + h[1] * y + h[2] * 1.0;
+    const double v1 = h[3] * x + h[4] * y + h[5] * 1.0;
+    const double v2 = h[6] * x + h[7] * y + h[8] * 1.0;
+    if (fabs(v2) < 1e-12) return false;
+    ox = v0 / v2;
+    oy = v1 / v2;
+    return true;
+}
+
+__global__ void k_render_through(uint32_t width, uint32_t height, uint32_t reference_scale,
+                                 DevPlane target, double target_scale, DevWarp initial,
+                                 Homography correction, float *out) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= width || y >= height) return;
+    const double rs = (double)reference_scale;
+    const double px = ((double)(float)x * rs * 2.0) + rs - 0.5;
+    const double py = ((double)(float)y * rs * 2.0) + rs - 0.5;
+    float value = __int_as_float(0x7fc00000);
+    double cx, cy;
+    float2 q;
+    float v;
+    if (apply_homography(correction, px, py, cx, cy) && warp_map(initial, (float)cx, (float)cy, q) &&
+        plane_sample(target, (float)(((double)q.x + 0.5) / (2.0 * target_scale) - 0.5),
+                     (float)(((double)q.y + 0.5) / (2.0 * target_scale) - 0.5), v))
+        value = v;
+    out[(size_t)y * width + x] = value;
+}
+
+__device__ __forceinline__ int32_t total_key(float f) {
+    const int32_t k = __float_as_int(f);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+
+__device__ __forceinline__ float refine_peak(float minus, float centre, float plus) {
+    const float denominator = minus - 2.0f * centre + plus;
+    if (fabsf(denominator) < 1e-9f) return 0.0f;
+    const float v = 0.5f * (minus - plus) / denominator;
+    return v < -0.5f ? -0.5f : v > 0.5f ? 0.5f : v;  // f32::clamp keeps NaN
+}
+
+// One block per window k. LDS: the centred reference window (size^2 f32)
+// and the shift scores ((2 radius + 1)^2 f32).
+__global__ void __launch_bounds__(256) k_match_patches(const float *reference, const float *target,
+                                                       uint32_t width, uint32_t height,
+                                                       const uint32_t *rxs, const uint32_t *rys,
+                                                       uint32_t size, uint32_t radius,
+                                                       int require_covered, float *shift_out,
+                                                       float *score_out, uint8_t *valid_out) {
+    extern __shared__ __align__(16) float lds_f[];
+    __shared__ float stats[2];  // mean, std
+    __shared__ int usable;
+    const uint32_t k = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t rx = rxs[k], ry = rys[k];
+    const uint32_t area = size * size, span = 2 * radius + 1, shifts = span * span;
+    float *window = lds_f;           // [size * size]
+    float *scores = lds_f + area;    // [span * span]
+    // match_patch's bounds (image.rs:540-550), both planes width x height
+    const bool inside = rx + size <= width && ry + size <= height && rx >= radius && ry >= radius &&
+                        rx + size + radius <= width && ry + size + radius <= height;
+    if (!inside) {
+        if (tid == 0) valid_out[k] = 0;
+        return;
+    }
+    int nan_seen = 0;
+    for (uint32_t i = tid; i < area; i += nth) {
+        const uint32_t y = i / size, x = i % size;
+        window[i] = reference[(size_t)(ry + y) * width + rx + x];
+        if (require_covered) {
+            const float t = target[(size_t)(ry + y) * width + rx + x];
+            nan_seen |= t != t;
+        }
+    }
+    const int uncovered = __syncthreads_or(nan_seen);
+    if (tid == 0) {
+        // Iterator::sum in order from -0.0, then the centred second moment
+        float sum = -0.0f;
+        for (uint32_t i = 0; i < area; i++) sum += window[i];
+        const float n = (float)area;
+        const float mean = sum / n;
+        float sq = -0.0f;
+        for (uint32_t i = 0; i < area; i++) {
+            const float dv = window[i] - mean;
+            sq += dv * dv;
+        }
+        stats[0] = mean;
+        stats[1] = sqrtf(sq / n);
+        usable = !uncovered && !(stats[1] < 1e-4f);
+    }
+    __syncthreads();
+    if (!usable) {
+        if (tid == 0) valid_out[k] = 0;
+        return;
+    }
+    const float mean = stats[0], ref_std = stats[1], n = (float)area;
+    for (uint32_t i = tid; i < area; i += nth) window[i] = window[i] - mean;
+    __syncthreads();
+    for (uint32_t s = tid; s < shifts; s += nth) {
+        const uint32_t dy = s / span, dx = s % span;
+        const uint32_t tx = rx + dx - radius, ty = ry + dy - radius;
+        float sum = 0.0f, sum_sq = 0.0f, cross = 0.0f;
+        for (uint32_t y = 0; y < size; y++) {
+            const float *row = target + (size_t)(ty + y) * width + tx;
+            const float *rrow = window + y * size;
+            for (uint32_t x = 0; x < size; x++) {
+                const float t = row[x];
+                sum += t;
+                sum_sq += t * t;
+                cross += t * rrow[x];
+            }
+        }
+        const float m = sum / n;
+        const float var = fmaxf(sum_sq / n - m * m, 0.0f);
+        float score = -__int_as_float(0x7f800000);
+        if (!(var < 1e-8f)) score = (cross / n) / (sqrtf(var) * ref_std);
+        scores[s] = score;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t best = 0;  // max_by(total_cmp): the last maximum
+        int32_t best_key = total_key(scores[0]);
+        for (uint32_t s = 1; s < shifts; s++) {
+            const int32_t key = total_key(scores[s]);
+            if (key >= best_key) {
+                best = s;
+                best_key = key;
+            }
+        }
+        const float score = scores[best];
+        if (!(score - score == 0.0f)) {
+            valid_out[k] = 0;
+            return;
+        }
+        const uint32_t bx = best % span, by = best / span;
+        const float sub_x = bx > 0 && bx + 1 < span
+                                ? refine_peak(scores[by * span + bx - 1], score, scores[by * span + bx + 1])
+                                : 0.0f;
+        const float sub_y = by > 0 && by + 1 < span
+                                ? refine_peak(scores[(by - 1) * span + bx], score, scores[(by + 1) * span + bx])
+                                : 0.0f;
+        shift_out[2 * k] = ((float)bx - (float)radius) + sub_x;
+        shift_out[2 * k + 1] = ((float)by - (float)radius) + sub_y;
+        score_out[k] = score;
+        valid_out[k] = 1;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_downsample(hipStream_t stream, const float *src, uint32_t sw, uint32_t sh,
+                             uint32_t spitch, float *dst, uint32_t dw, uint32_t dh,
+                             uint32_t dpitch) {
+    const dim3 block(256), grid((dw + 255) / 256, dh);
+    hipLaunchKernelGGL(k_downsample, grid, block, 0, stream, src, sw, sh, spitch, dst, dw, dh,
+                       dpitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_initial_grid(hipStream_t stream, const chd_camera *reference,
+                               const chd_camera *target, uint32_t columns, uint32_t rows,
+                               uint32_t step, float2 *out) {
+    const dim3 block(64), grid((columns + 63) / 64, rows);
+    hipLaunchKernelGGL(k_initial_grid, grid, block, 0, stream, reference, target, columns, rows,
+                       step, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_through(hipStream_t stream, uint32_t width, uint32_t height,
+                                 uint32_t reference_scale, const DevPlane &target,
+                                 double target_scale, const DevWarp &initial,
+                                 const double correction[9], float *out) {
+    Homography h;
+    for (int i = 0; i < 9; i++) h.h[i] = correction[i];
+    const dim3 block(256), grid((width + 255) / 256, height);
+    hipLaunchKernelGGL(k_render_through, grid, block, 0, stream, width, height, reference_scale,
+                       target, target_scale, initial, h, out);
+    return hipGetLastError();
+}
+
+size_t match_patches_lds(uint32_t size, uint32_t radius) {
+    return ((size_t)size * size + (size_t)(2 * radius + 1) * (2 * radius + 1)) * sizeof(float);
+}
+
+hipError_t launch_match_patches(hipStream_t stream, const float *reference, const float *target,
+                                uint32_t width, uint32_t height, const uint32_t *rx,
+                                const uint32_t *ry, uint32_t count, uint32_t size, uint32_t radius,
+                                bool require_covered, float *shift, float *score, uint8_t *valid) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_match_patches, dim3(count), dim3(256), match_patches_lds(size, radius),
+                       stream, reference, target, width, height, rx, ry, size, radius,
+                       require_covered ? 1 : 0, shift, score, valid);
+    return hipGetLastError();
+}
+
+}  // namespace chd

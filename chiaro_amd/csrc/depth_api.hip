@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/chiaro_depth.h"
+#include "depth_device.hpp"
 #include "depth_internal.hpp"
 #include "proposal.hpp"
 
@@ -1429,6 +1430,565 @@ int chd_luminance_half(chd_context *c, const uint16_t *samples, uint32_t width, 
                                         reserve_headroom, d_plane));
     CHD_HIP(hipMemcpyAsync(plane, d_plane, m * 4, hipMemcpyDeviceToHost, c->stream));
     CHD_HIP(hipStreamSynchronize(c->stream));
+    return CHD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ alignment
+
+extern "C++" {
+namespace {
+
+// Warp::map (align.rs:84-111) on host points (x, y interleaved).
+bool host_warp_map(const float *points, uint32_t step, uint32_t columns, uint32_t rows, float x,
+                   float y, float out[2]) {
+    const float fx = x / (float)step, fy = y / (float)step;
+    if (fx < 0.0f || fy < 0.0f) return false;
+    auto index = [](float f, uint32_t n) -> uint32_t {
+        const float fl = std::floor(f);
+        uint64_t v = !(fl > 0.0f) ? 0 : fl >= 18446744073709551616.0f ? UINT64_MAX : (uint64_t)fl;
+        return (uint32_t)std::min<uint64_t>(v, n - 1);
+    };
+    const uint32_t c0 = index(fx, columns), r0 = index(fy, rows);
+    const uint32_t c1 = std::min(c0 + 1, columns - 1), r1 = std::min(r0 + 1, rows - 1);
+    const float tx = fx - (float)c0, ty = fy - (float)r0;
+    const float *a = points + 2 * ((size_t)r0 * columns + c0), *b = points + 2 * ((size_t)r0 * columns + c1);
+    const float *c = points + 2 * ((size_t)r1 * columns + c0), *d = points + 2 * ((size_t)r1 * columns + c1);
+    for (int k = 0; k < 2; k++) {
+        const float top = a[k] * (1.0f - tx) + b[k] * tx;
+        const float bottom = c[k] * (1.0f - tx) + d[k] * tx;
+        out[k] = top * (1.0f - ty) + bottom * ty;
+    }
+    return !(out[0] != out[0] || out[1] != out[1]);
+}
+
+// Plane::sample (image.rs:452-466) on a host plane.
+bool host_sample(const float *data, uint32_t width, uint32_t height, float x, float y, float &out) {
+    if (!(x >= 0.0f && y >= 0.0f && x <= (float)(width - 1) && y <= (float)(height - 1)))
+        return false;
+    const uint32_t x0 = (uint32_t)std::floor(x), y0 = (uint32_t)std::floor(y);
+    const uint32_t x1 = std::min(x0 + 1, width - 1), y1 = std::min(y0 + 1, height - 1);
+    const float tx = x - (float)x0, ty = y - (float)y0;
+    auto at = [&](uint32_t i, uint32_t j) { return data[(size_t)j * width + i]; };
+    const float top = at(x0, y0) * (1.0f - tx) + at(x1, y0) * tx;
+    const float bottom = at(x0, y1) * (1.0f - tx) + at(x1, y1) * tx;
+    out = top * (1.0f - ty) + bottom * ty;
+    return true;
+}
+
+// apply_homography (math.rs:131-138) and Mat3 products (math.rs:48-56)
+bool homography_apply(const double *h, double x, double y, double &ox, double &oy) {
+    const double v0 = h[0] * x + h[1] * y + h[2] * 1.0;
+    const double v1 = h[3] * x + h[4] * y + h[5] * 1.0;
+    const double v2 = h[6] * x + h[7] * y + h[8] * 1.0;
+    if (std::fabs(v2) < 1e-12) return false;
+    ox = v0 / v2;
+    oy = v1 / v2;
+    return true;
+}
+
+void mat3_mul(const double *a, const double *b, double *out) {
+    double r[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r[i * 3 + j] = a[i * 3 + 0] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+    std::memcpy(out, r, sizeof r);
+}
+
+int64_t total_key64(double d) {
+    int64_t k;
+    std::memcpy(&k, &d, 8);
+    return k ^ (int64_t)(((uint64_t)(k >> 63)) >> 1);
+}
+int32_t total_key32(float f) {
+    int32_t k;
+    std::memcpy(&k, &f, 4);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+
+// solve_linear (align.rs:765-793): Gaussian elimination, partial pivoting
+// on the last maximal |a[i][column]|
+bool solve8(double a[8][8], double b[8], double x[8]) {
+    for (int column = 0; column < 8; column++) {
+        int pivot = column;
+        for (int i = column + 1; i < 8; i++)
+            if (total_key64(std::fabs(a[i][column])) >= total_key64(std::fabs(a[pivot][column])))
+                pivot = i;
+        if (std::fabs(a[pivot][column]) < 1e-12) return false;
+        for (int k = 0; k < 8; k++) std::swap(a[column][k], a[pivot][k]);
+        std::swap(b[column], b[pivot]);
+        double pivot_row[8];
+        std::memcpy(pivot_row, a[column], sizeof pivot_row);
+        for (int row = column + 1; row < 8; row++) {
+            const double factor = a[row][column] / pivot_row[column];
+            for (int k = column; k < 8; k++) a[row][k] -= factor * pivot_row[k];
+            b[row] -= factor * b[column];
+        }
+    }
+    for (int row = 7; row >= 0; row--) {
+        double sum = b[row];
+        for (int k = row + 1; k < 8; k++) sum -= a[row][k] * x[k];
+        x[row] = sum / a[row][row];
+    }
+    return true;
+}
+
+struct Pair {
+    double a[2], b[2];
+    float score;
+};
+
+// fit_homography_least_squares (align.rs:713-762): weighted DLT on
+// centroid-normalised coordinates, h33 = 1
+bool homography_least_squares(const Pair *pairs, size_t count, double h[9]) {
+    if (count < 4) return false;
+    const double n = (double)count;
+    double s[4] = {-0.0, -0.0, -0.0, -0.0};  // Iterator::sum starts from -0.0
+    for (int k = 0; k < 4; k++)
+        for (size_t i = 0; i < count; i++) s[k] += k < 2 ? pairs[i].a[k] : pairs[i].b[k - 2];
+    const double ax = s[0] / n, ay = s[1] / n, bx = s[2] / n, by = s[3] / n;
+    double sa = -0.0, sb = -0.0;
+    for (size_t i = 0; i < count; i++) {
+        const double dx = pairs[i].a[0] - ax, dy = pairs[i].a[1] - ay;
+        sa += std::sqrt(dx * dx + dy * dy);
+    }
+    for (size_t i = 0; i < count; i++) {
+        const double dx = pairs[i].b[0] - bx, dy = pairs[i].b[1] - by;
+        sb += std::sqrt(dx * dx + dy * dy);
+    }
+    sa = std::fmax(sa / n, 1e-9);
+    sb = std::fmax(sb / n, 1e-9);
+    double ata[8][8] = {}, atb[8] = {};
+    for (size_t i = 0; i < count; i++) {
+        const double w = std::fmax((double)pairs[i].score, 0.05);
+        const double x = (pairs[i].a[0] - ax) / sa, y = (pairs[i].a[1] - ay) / sa;
+        const double u = (pairs[i].b[0] - bx) / sb, v = (pairs[i].b[1] - by) / sb;
+        const double rows[2][8] = {{x, y, 1.0, 0.0, 0.0, 0.0, -u * x, -u * y},
+                                   {0.0, 0.0, 0.0, x, y, 1.0, -v * x, -v * y}};
+        const double rhs[2] = {u, v};
+        for (int r = 0; r < 2; r++)
+            for (int p = 0; p < 8; p++) {
+                atb[p] += w * rows[r][p] * rhs[r];
+                for (int q = 0; q < 8; q++) ata[p][q] += w * rows[r][p] * rows[r][q];
+            }
+    }
+    double x[8];
+    if (!solve8(ata, atb, x)) return false;
+    const double normalized[9] = {x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], 1.0};
+    const double t_a[9] = {1.0 / sa, 0.0, -ax / sa, 0.0, 1.0 / sa, -ay / sa, 0.0, 0.0, 1.0};
+    const double t_b_inv[9] = {sb, 0.0, bx, 0.0, sb, by, 0.0, 0.0, 1.0};
+    double tmp[9], hh[9];
+    mat3_mul(t_b_inv, normalized, tmp);
+    mat3_mul(tmp, t_a, hh);
+    const double scale = hh[8];
+    if (std::fabs(scale) < 1e-12 || !std::isfinite(scale)) return false;
+    for (int k = 0; k < 9; k++) h[k] = hh[k] / scale;
+    return true;
+}
+
+// fit_homography_ransac (align.rs:644-710): 300 four-point samples from a
+// seeded LCG, then a refit on the best consensus; residuals of the final
+// inliers sorted by total_cmp
+bool homography_ransac(const std::vector<Pair> &pairs, float threshold, double h[9],
+                       std::vector<float> &residuals) {
+    const size_t count = pairs.size();
+    if (count < 6) return false;
+    uint64_t state = 0x9E3779B97F4A7C15ull ^ (uint64_t)count;
+    std::vector<uint32_t> best, current;
+    for (int iteration = 0; iteration < 300; iteration++) {
+        uint32_t sample[4];
+        for (int slot = 0; slot < 4; slot++)
+            for (;;) {
+                state = state * 6364136223846793005ull + 1442695040888963407ull;
+                const uint32_t candidate = (uint32_t)((state >> 33) % (uint64_t)count);
+                bool seen = false;
+                for (int k = 0; k < slot; k++) seen = seen || sample[k] == candidate;
+                if (!seen) {
+                    sample[slot] = candidate;
+                    break;
+                }
+            }
+        Pair subset[4];
+        for (int k = 0; k < 4; k++) subset[k] = pairs[sample[k]];
+        double hs[9];
+        if (!homography_least_squares(subset, 4, hs)) continue;
+        current.clear();
+        for (size_t i = 0; i < count; i++) {
+            double px, py;
+            if (homography_apply(hs, pairs[i].a[0], pairs[i].a[1], px, py)) {
+                const double dx = px - pairs[i].b[0], dy = py - pairs[i].b[1];
+                if (std::sqrt(dx * dx + dy * dy) <= (double)threshold) current.push_back((uint32_t)i);
+            }
+        }
+        if (current.size() > best.size()) best = current;
+    }
+    if (best.size() < 6) return false;
+    std::vector<Pair> inliers;
+    for (uint32_t i : best) inliers.push_back(pairs[i]);
+    double hf[9];
+    if (!homography_least_squares(inliers.data(), inliers.size(), hf)) return false;
+    residuals.clear();
+    for (const Pair &p : pairs) {
+        double px, py;
+        if (homography_apply(hf, p.a[0], p.a[1], px, py)) {
+            const double dx = px - p.b[0], dy = py - p.b[1];
+            const float r = (float)std::sqrt(dx * dx + dy * dy);
+            if (r <= threshold) residuals.push_back(r);
+        }
+    }
+    if (residuals.size() < 6) return false;
+    std::stable_sort(residuals.begin(), residuals.end(),
+                     [](float x, float y) { return total_key32(x) < total_key32(y); });
+    std::memcpy(h, hf, sizeof hf);
+    return true;
+}
+
+// photometric_gain (align.rs:795-826): median linear-luminance ratio of
+// reference / target at every 16th plane pixel, the platform expf
+bool photometric_gain_host(const chd_view &reference, const chd_view &target, const float *points,
+                           uint32_t step, uint32_t columns, uint32_t rows, float &gain) {
+    std::vector<float> ratios;
+    const uint32_t w = reference.plane_width, h = reference.plane_height;
+    for (uint32_t y = 16; y + 16 < h; y += 16)
+        for (uint32_t x = 16; x + 16 < w; x += 16) {
+            const float rx = (float)x * 2.0f + 0.5f, ry = (float)y * 2.0f + 0.5f;
+            float q[2], t;
+            if (!host_warp_map(points, step, columns, rows, rx, ry, q)) continue;
+            if (!host_sample(target.luminance, target.plane_width, target.plane_height,
+                             (q[0] - 0.5f) / 2.0f, (q[1] - 0.5f) / 2.0f, t))
+                continue;
+            const float r = reference.luminance[(size_t)y * w + x];
+            const float lr = (expf(r) - 1.0f) / 1000.0f, lt = (expf(t) - 1.0f) / 1000.0f;
+            if (lr > 0.01f && lt > 0.01f && lr < 0.95f && lt < 0.95f) ratios.push_back(lr / lt);
+        }
+    if (ratios.size() < 16) return false;
+    std::stable_sort(ratios.begin(), ratios.end(),
+                     [](float x, float y) { return total_key32(x) < total_key32(y); });
+    gain = std::min(std::max(ratios[ratios.size() / 2], 0.25f), 4.0f);
+    return true;
+}
+
+// A padded device plane (pitch width + 1, height + 1 rows).
+struct DevicePlane {
+    float *data = nullptr;
+    uint32_t width = 0, height = 0;
+    DevPlane view() const {
+        DevPlane p;
+        p.data = data;
+        p.width = width;
+        p.height = height;
+        p.pitch = width + 1;
+        p.lines = height + 1;
+        p.transposed = 0;
+        return p;
+    }
+};
+
+// Plane::pyramid (image.rs:487-498) on the device from a host plane.
+int device_pyramid(chd_context *c, Scratch &sc, const chd_view &v, uint32_t min_size,
+                   std::vector<DevicePlane> &levels) {
+    hipStream_t st = c->stream;
+    DevicePlane base;
+    base.width = v.plane_width;
+    base.height = v.plane_height;
+    CHD_CHECK(sc.alloc(&base.data, (size_t)(base.width + 1) * (base.height + 1)));
+    CHD_HIP(hipMemcpy2DAsync(base.data, (size_t)(base.width + 1) * 4, v.luminance,
+                             (size_t)base.width * 4, (size_t)base.width * 4, base.height,
+                             hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_pad_plane(st, base.data, base.width, base.height));
+    levels.push_back(base);
+    const uint32_t floor_size = std::max(min_size, 8u);
+    while (std::min(levels.back().width, levels.back().height) / 2 >= floor_size) {
+        const DevicePlane &last = levels.back();
+        DevicePlane next;
+        next.width = std::max(last.width / 2, 1u);
+        next.height = std::max(last.height / 2, 1u);
+        CHD_CHECK(sc.alloc(&next.data, (size_t)(next.width + 1) * (next.height + 1)));
+        CHD_HIP(launch_downsample(st, last.data, last.width, last.height, last.width + 1, next.data,
+                                  next.width, next.height, next.width + 1));
+        CHD_HIP(launch_pad_plane(st, next.data, next.width, next.height));
+        levels.push_back(next);
+    }
+    return CHD_OK;
+}
+
+// Warp::from_fn (align.rs:50-80) of initial(correction(p)), or the identity.
+void fill_warp(uint32_t width, uint32_t height, uint32_t step, const std::vector<float> *grid,
+               uint32_t gc, uint32_t gr, const double *correction, chd_warp &w) {
+    w.step = step;
+    w.columns = (uint32_t)div_ceil(width, step) + 1;
+    w.rows = (uint32_t)div_ceil(height, step) + 1;
+    for (uint32_t r = 0; r < w.rows; r++)
+        for (uint32_t col = 0; col < w.columns; col++) {
+            const size_t i = (size_t)r * w.columns + col;
+            const double px = (double)((size_t)col * step), py = (double)((size_t)r * step);
+            double q[2] = {px, py};
+            bool ok = true;
+            if (grid) {
+                double cx, cy;
+                float g[2];
+                ok = homography_apply(correction, px, py, cx, cy) &&
+                     host_warp_map(grid->data(), 8, gc, gr, (float)cx, (float)cy, g);
+                q[0] = g[0];
+                q[1] = g[1];
+            }
+            w.points[2 * i] = ok ? (float)q[0] : NAN;
+            w.points[2 * i + 1] = ok ? (float)q[1] : NAN;
+            if (w.confidence) w.confidence[i] = ok ? 1.0f : 0.0f;
+        }
+}
+
+}  // namespace
+}  // extern "C++"
+
+extern "C" {
+
+void chd_default_align_options(chd_align_options *o) {  // align.rs:232-244
+    if (!o) return;
+    o->grid_step = 32;
+    o->patch = 32;
+    o->coarse_radius = 12;
+    o->min_score = 0.5f;
+    o->inlier_px = 3.0f;
+    o->min_inlier_ratio = 0.45f;
+    o->refine = 1;
+}
+
+int chd_match_patches(chd_context *c, const float *reference, const float *target, uint32_t width,
+                      uint32_t height, const uint32_t *rx, const uint32_t *ry, uint32_t count,
+                      uint32_t size, uint32_t radius, float *shift, float *score, uint8_t *valid) {
+    if (!c || !reference || !target || !rx || !ry || !shift || !score || !valid || size == 0)
+        return fail(CHD_ERR_ARGUMENT, "match_patches: bad argument");
+    if (size > 64 || radius > 32) return fail(CHD_ERR_LIMIT, "match_patches: size above 64 or radius above 32");
+    if (count == 0) return CHD_OK;
+    CHD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t n = (size_t)width * height;
+    Scratch sc;
+    float *d_ref, *d_tgt, *d_shift, *d_score;
+    uint32_t *d_rx, *d_ry;
+    uint8_t *d_valid;
+    CHD_CHECK(sc.alloc(&d_ref, n));
+    CHD_CHECK(sc.alloc(&d_tgt, n));
+    CHD_CHECK(sc.alloc(&d_rx, count));
+    CHD_CHECK(sc.alloc(&d_ry, count));
+    CHD_CHECK(sc.alloc(&d_shift, 2 * (size_t)count));
+    CHD_CHECK(sc.alloc(&d_score, count));
+    CHD_CHECK(sc.alloc(&d_valid, count));
+    CHD_HIP(hipMemcpyAsync(d_ref, reference, n * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_tgt, target, n * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_rx, rx, count * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_ry, ry, count * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_match_patches(st, d_ref, d_tgt, width, height, d_rx, d_ry, count, size, radius,
+                                 false, d_shift, d_score, d_valid));
+    CHD_HIP(hipMemcpyAsync(shift, d_shift, 8 * (size_t)count, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipMemcpyAsync(score, d_score, 4 * (size_t)count, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipMemcpyAsync(valid, d_valid, count, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    return CHD_OK;
+}
+
+int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *target,
+                     const chd_align_options *o, chd_warp *warp, chd_alignment_report *report) {
+    if (!c || !reference || !target || !o || !warp || !report || !warp->points)
+        return fail(CHD_ERR_ARGUMENT, "align_module: bad argument");
+    if (o->grid_step == 0 || !reference->luminance || !target->luminance)
+        return fail(CHD_ERR_ARGUMENT, "align_module: zero grid step or missing plane");
+    if (o->patch > 64 || o->coarse_radius > 32)
+        return fail(CHD_ERR_LIMIT, "align_module: patch above 64 or coarse radius above 32");
+    std::memset(report, 0, sizeof *report);
+    const uint32_t width = reference->width, height = reference->height;
+    if (reference == target) {  // align.rs:291-309
+        fill_warp(width, height, o->grid_step, nullptr, 0, 0, nullptr, *warp);
+        report->initialised_from = CHD_ALIGN_FROM_REFERENCE;
+        report->coverage = 1.0f;
+        report->inlier_ratio = 1.0f;
+        report->accepted = 1;
+        report->status = CHD_ALIGN_STATUS_REFERENCE;
+        report->gain = 1.0f;
+        return CHD_OK;
+    }
+    if (!reference->camera || !target->camera)
+        return fail(CHD_ERR_ARGUMENT, "align_module: both modules need cameras");
+    CHD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    report->initialised_from = CHD_ALIGN_FROM_CALIBRATION;
+    Scratch sc;
+    // the factory model on an 8 px grid (align.rs:317-324)
+    const uint32_t gc = (uint32_t)div_ceil(width, 8) + 1, gr = (uint32_t)div_ceil(height, 8) + 1;
+    chd_camera *cams;
+    float2 *d_grid;
+    CHD_CHECK(sc.alloc(&cams, 2));
+    CHD_CHECK(sc.alloc(&d_grid, (size_t)gc * gr));
+    CHD_HIP(hipMemcpyAsync(cams, reference->camera, sizeof(chd_camera), hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(cams + 1, target->camera, sizeof(chd_camera), hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_initial_grid(st, cams, cams + 1, gc, gr, 8, d_grid));
+    std::vector<float> grid((size_t)gc * gr * 2);
+    CHD_HIP(hipMemcpyAsync(grid.data(), d_grid, grid.size() * 4, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    auto initial = [&](double x, double y, double out[2]) {
+        float q[2];
+        if (!host_warp_map(grid.data(), 8, gc, gr, (float)x, (float)y, q)) return false;
+        out[0] = q[0];
+        out[1] = q[1];
+        return true;
+    };
+    double correction[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::vector<float> finest;
+    if (o->refine) {
+        std::vector<DevicePlane> rp, tp;
+        CHD_CHECK(device_pyramid(c, sc, *reference, 96, rp));
+        CHD_CHECK(device_pyramid(c, sc, *target, 48, tp));
+        double magnification;
+        {
+            const double cx = (double)width / 2.0, cy = (double)height / 2.0, h = 64.0;
+            double a[2], b[2], d[2];
+            if (initial(cx, cy, a) && initial(cx + h, cy, b) && initial(cx, cy + h, d)) {
+                const double dx = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1])) / h;
+                const double dy = std::sqrt((d[0] - a[0]) * (d[0] - a[0]) + (d[1] - a[1]) * (d[1] - a[1])) / h;
+                magnification = std::sqrt(dx * dy);
+            } else {
+                magnification = target->camera->focal_px / reference->camera->focal_px;
+            }
+        }
+        DevWarp dw;
+        dw.points = d_grid;
+        dw.step = 8;
+        dw.columns = gc;
+        dw.rows = gr;
+        dw.inverse_step = 0.125f;
+        float *rendered, *d_shift, *d_score;
+        uint32_t *d_rx, *d_ry;
+        uint8_t *d_valid;
+        const size_t finest_points = (size_t)rp[0].width * rp[0].height;
+        CHD_CHECK(sc.alloc(&rendered, finest_points));
+        // window capacity: every level has at most the finest level's windows
+        const size_t capacity = finest_points / 16 + 64;
+        CHD_CHECK(sc.alloc(&d_rx, capacity));
+        CHD_CHECK(sc.alloc(&d_ry, capacity));
+        CHD_CHECK(sc.alloc(&d_shift, 2 * capacity));
+        CHD_CHECK(sc.alloc(&d_score, capacity));
+        CHD_CHECK(sc.alloc(&d_valid, capacity));
+        float *unpadded;
+        CHD_CHECK(sc.alloc(&unpadded, finest_points));
+        for (int level = (int)rp.size() - 1; level >= 0; level--) {
+            const uint32_t scale = 1u << level;
+            const DevicePlane &ref_plane = rp[(size_t)level];
+            const double l2 = std::fmax(std::round(std::log2((double)scale * magnification)), 0.0);
+            const uint32_t last = (uint32_t)tp.size() - 1;
+            const uint32_t target_level = l2 >= (double)last ? last : (uint32_t)l2;
+            const double target_scale = (double)(1u << target_level);
+            const uint32_t W = ref_plane.width, H = ref_plane.height;
+            CHD_HIP(launch_render_through(st, W, H, scale, tp[target_level].view(), target_scale, dw,
+                                          correction, rendered));
+            // the reference level without its padding column
+            CHD_HIP(hipMemcpy2DAsync(unpadded, (size_t)W * 4, ref_plane.data, (size_t)(W + 1) * 4,
+                                     (size_t)W * 4, H, hipMemcpyDeviceToDevice, st));
+            const uint32_t radius = level + 1 == (int)rp.size() ? o->coarse_radius : 3;
+            const uint32_t patch = std::max(std::min(o->patch, W / 4), 8u);
+            const uint32_t stride = patch / 2;
+            std::vector<uint32_t> xs, ys;
+            for (uint32_t y = radius; y + patch + radius <= H; y += stride)
+                for (uint32_t x = radius; x + patch + radius <= W; x += stride) {
+                    xs.push_back(x);
+                    ys.push_back(y);
+                }
+            const uint32_t count = (uint32_t)xs.size();
+            if (count > capacity) return fail(CHD_ERR_LIMIT, "align_module: too many windows");
+            std::vector<float> shifts(2 * (size_t)count), scores(count);
+            std::vector<uint8_t> valid(count);
+            if (count > 0) {
+                CHD_HIP(hipMemcpyAsync(d_rx, xs.data(), count * 4, hipMemcpyHostToDevice, st));
+                CHD_HIP(hipMemcpyAsync(d_ry, ys.data(), count * 4, hipMemcpyHostToDevice, st));
+                CHD_HIP(launch_match_patches(st, unpadded, rendered, W, H, d_rx, d_ry, count, patch,
+                                             radius, true, d_shift, d_score, d_valid));
+                CHD_HIP(hipMemcpyAsync(shifts.data(), d_shift, 8 * (size_t)count,
+                                       hipMemcpyDeviceToHost, st));
+                CHD_HIP(hipMemcpyAsync(scores.data(), d_score, 4 * (size_t)count,
+                                       hipMemcpyDeviceToHost, st));
+                CHD_HIP(hipMemcpyAsync(valid.data(), d_valid, count, hipMemcpyDeviceToHost, st));
+            }
+            CHD_HIP(hipStreamSynchronize(st));
+            std::vector<Pair> pairs;
+            const double s = (double)scale;
+            for (uint32_t k = 0; k < count; k++) {
+                if (!valid[k] || !(scores[k] >= o->min_score)) continue;
+                const float cx = (float)xs[k] + (float)patch / 2.0f, cy = (float)ys[k] + (float)patch / 2.0f;
+                const float sx = cx + shifts[2 * k], sy = cy + shifts[2 * k + 1];
+                Pair p;
+                p.a[0] = (double)cx * s * 2.0 + s - 0.5;
+                p.a[1] = (double)cy * s * 2.0 + s - 0.5;
+                p.b[0] = (double)sx * s * 2.0 + s - 0.5;
+                p.b[1] = (double)sy * s * 2.0 + s - 0.5;
+                p.score = scores[k];
+                pairs.push_back(p);
+            }
+            float threshold = o->inlier_px * (float)scale * 2.0f;
+            if (!(threshold >= o->inlier_px)) threshold = o->inlier_px;
+            double update[9];
+            std::vector<float> residuals;
+            chd_level_report *lr = report->level_count < CHD_MAX_ALIGN_LEVELS
+                                       ? &report->levels[report->level_count++]
+                                       : nullptr;
+            if (homography_ransac(pairs, threshold, update, residuals)) {
+                mat3_mul(correction, update, correction);
+                if (lr) {
+                    lr->scale = scale * 2;
+                    lr->patches = (uint32_t)pairs.size();
+                    lr->inliers = (uint32_t)residuals.size();
+                    lr->median_residual_px = residuals[residuals.size() / 2];
+                }
+                if (level == 0) {
+                    finest = residuals;
+                    report->patches = (uint32_t)pairs.size();
+                    report->inliers = (uint32_t)residuals.size();
+                }
+            } else if (lr) {
+                lr->scale = scale * 2;
+                lr->patches = (uint32_t)pairs.size();
+                lr->inliers = 0;
+                lr->median_residual_px = NAN;
+            }
+        }
+    }
+    fill_warp(width, height, o->grid_step, &grid, gc, gr, correction, *warp);
+    size_t covered = 0, total = (size_t)warp->columns * warp->rows;
+    for (size_t i = 0; i < total; i++) {
+        const float x = warp->points[2 * i], y = warp->points[2 * i + 1];
+        if (std::isfinite(x) && x >= 0.0f && y >= 0.0f && x <= (float)(target->width - 1) &&
+            y <= (float)(target->height - 1))
+            covered++;
+    }
+    report->coverage = (float)covered / (float)total;
+    if (!finest.empty()) {
+        report->residual_median_px = finest[finest.size() / 2];
+        report->residual_p90_px = finest[std::min(finest.size() * 9 / 10, finest.size() - 1)];
+    }
+    {
+        const double cx = (double)(width / 2), cy = (double)(height / 2);
+        double mx, my;
+        if (!homography_apply(correction, cx, cy, mx, my)) {
+            mx = cx;
+            my = cy;
+        }
+        report->correction_median_px[0] = (float)(mx - cx);
+        report->correction_median_px[1] = (float)(my - cy);
+    }
+    report->inlier_ratio =
+        report->patches == 0 ? 0.0f : (float)report->inliers / (float)report->patches;
+    report->accepted = !o->refine || report->inlier_ratio >= o->min_inlier_ratio;
+    if (!report->accepted) report->status = CHD_ALIGN_STATUS_REJECTED;
+    else if (!o->refine) report->status = CHD_ALIGN_STATUS_FACTORY_MODEL;
+    else if (report->inliers >= 12 && report->residual_median_px < o->inlier_px)
+        report->status = CHD_ALIGN_STATUS_REFINED;
+    else if (report->inliers > 0) report->status = CHD_ALIGN_STATUS_WEAK;
+    else report->status = CHD_ALIGN_STATUS_NO_SUPPORT;
+    float gain;
+    report->gain = photometric_gain_host(*reference, *target, warp->points, warp->step,
+                                         warp->columns, warp->rows, gain)
+                       ? gain
+                       : 1.0f;
     return CHD_OK;
 }
 

@@ -218,6 +218,31 @@ hipError_t launch_exp(hipStream_t stream, const double *x, double *y, size_t n);
 
 hipError_t launch_fill_u8(hipStream_t stream, uint8_t *data, uint8_t value, size_t n);
 
+// ------------------------------------------------ alignment (align_kernels.hip)
+
+// One pyramid level: dst (dw x dh, row pitch dpitch) from src (sw x sh,
+// pitch spitch) by Plane::downsample.
+hipError_t launch_downsample(hipStream_t stream, const float *src, uint32_t sw, uint32_t sh,
+                             uint32_t spitch, float *dst, uint32_t dw, uint32_t dh, uint32_t dpitch);
+// target.map_from(reference, grid point, 1e8) on a columns x rows grid of
+// `step` (NaN where undefined); cameras in device memory.
+hipError_t launch_initial_grid(hipStream_t stream, const chd_camera *reference,
+                               const chd_camera *target, uint32_t columns, uint32_t rows,
+                               uint32_t step, float2 *out);
+// render_through of a width x height level into out (row pitch width).
+hipError_t launch_render_through(hipStream_t stream, uint32_t width, uint32_t height,
+                                 uint32_t reference_scale, const struct DevPlane &target,
+                                 double target_scale, const struct DevWarp &initial,
+                                 const double correction[9], float *out);
+size_t match_patches_lds(uint32_t size, uint32_t radius);
+// match_patch of `count` windows (device arrays rx, ry) between two planes of
+// width x height (row pitch width); with require_covered a window whose
+// target pixels hold a NaN is None.
+hipError_t launch_match_patches(hipStream_t stream, const float *reference, const float *target,
+                                uint32_t width, uint32_t height, const uint32_t *rx,
+                                const uint32_t *ry, uint32_t count, uint32_t size, uint32_t radius,
+                                bool require_covered, float *shift, float *score, uint8_t *valid);
+
 // ------------------------------------------------ synthetic capture generator
 
 // The scene as the render kernels see it: Voronoi sites, plane inverse

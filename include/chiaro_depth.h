@@ -514,6 +514,86 @@ int chd_luminance_half_device(chd_context *context, const uint16_t *samples, uin
                               uint32_t height, float black_q6, float white_q6,
                               int reserve_headroom, float *plane);
 
+/* ---------------------------------------------------------- alignment */
+
+/* AlignOptions (align.rs:211-244) without the nested DepthOptions. */
+typedef struct chd_align_options {
+    uint32_t grid_step;     /* output warp grid, reference raster pixels (32) */
+    uint32_t patch;         /* correlation window, plane pixels (32) */
+    uint32_t coarse_radius; /* search radius at the coarsest level (12) */
+    float min_score;        /* NCC peak for a patch to vote (0.5) */
+    float inlier_px;        /* RANSAC inlier threshold, reference pixels (3) */
+    float min_inlier_ratio; /* finest-level consensus to accept (0.45) */
+    int32_t refine;         /* 0: keep the factory model (align.rs:317-324) */
+} chd_align_options;
+
+/* LevelReport (align.rs:202-208). */
+typedef struct chd_level_report {
+    uint32_t scale;
+    uint32_t patches;
+    uint32_t inliers;
+    float median_residual_px; /* NaN when the level's fit failed */
+} chd_level_report;
+
+#define CHD_ALIGN_FROM_REFERENCE 0
+#define CHD_ALIGN_FROM_CALIBRATION 1
+
+#define CHD_ALIGN_STATUS_REFERENCE 0
+#define CHD_ALIGN_STATUS_REJECTED 1
+#define CHD_ALIGN_STATUS_FACTORY_MODEL 2
+#define CHD_ALIGN_STATUS_REFINED 3
+#define CHD_ALIGN_STATUS_WEAK 4
+#define CHD_ALIGN_STATUS_NO_SUPPORT 5
+#define CHD_MAX_ALIGN_LEVELS 16
+
+/* The numeric fields of AlignmentReport (align.rs:160-200) that
+ * align_module fills, plus ModuleAlignment::gain (align.rs:148-157); the
+ * status string's case is a CHD_ALIGN_STATUS_* code (align.rs:488-501). */
+typedef struct chd_alignment_report {
+    uint32_t initialised_from; /* CHD_ALIGN_FROM_* */
+    float coverage;
+    float correction_median_px[2];
+    uint32_t level_count;
+    chd_level_report levels[CHD_MAX_ALIGN_LEVELS];
+    uint32_t inliers;
+    uint32_t patches;
+    float residual_median_px;
+    float residual_p90_px;
+    float inlier_ratio;
+    int32_t accepted;
+    uint32_t status;
+    float gain;
+} chd_alignment_report;
+
+/* AlignOptions::default (align.rs:232-244). */
+void chd_default_align_options(chd_align_options *out);
+
+/* align_module (align.rs:270-519) for calibrated modules: the factory
+ * model (an 8 px infinity-map grid, align.rs:317-324) refined
+ * coarse-to-fine by a reference-space homography fitted by RANSAC to
+ * match_patch correlations (image.rs:531-640) between the reference
+ * pyramid and the target rendered through the current mapping; writes the
+ * warp (grid_step spacing over the reference raster; caller-allocated
+ * points and confidence of ceil(w / step) + 1 by ceil(h / step) + 1 nodes)
+ * and the report. reference == target (the same view) is the reference's
+ * own identity alignment. Both views need cameras (the uncalibrated
+ * nominal-focal start is not supported: CHD_ERR_ARGUMENT). The pyramids,
+ * rendering and correlations run on the device; the RANSAC and least
+ * squares fits (a few hundred to a few thousand pairs) on the host. */
+int chd_align_module(chd_context *context, const chd_view *reference, const chd_view *target,
+                     const chd_align_options *options, chd_warp *warp,
+                     chd_alignment_report *report);
+
+/* match_patch (image.rs:531-640) on the device for a batch of windows of
+ * one reference / target plane pair (same sizes, host arrays): window k at
+ * (rx[k], ry[k]) of `size` with search radius `radius`; valid[k] = 0 is
+ * None, else shift[2k..2k+1] and score[k]. A stage entry for parity
+ * checks. */
+int chd_match_patches(chd_context *context, const float *reference, const float *target,
+                      uint32_t width, uint32_t height, const uint32_t *rx, const uint32_t *ry,
+                      uint32_t count, uint32_t size, uint32_t radius, float *shift, float *score,
+                      uint8_t *valid);
+
 /* Synthetic captures rendered on the GPU. No counterpart in the reference
  * (its only fakes are gradient mock LRIs, crates/chiaro/src/mock.rs:62):
  * a seeded scene of fronto-parallel planes over a Voronoi partition of the

@@ -90,6 +90,15 @@ SIGNATURES = {
                                   C.c_int),
     "or_cost_volume_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options),
                             C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP, VP),
+    "or_match_patch": (C.c_int, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_uint32,
+                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, P(C.c_float)),
+    "or_apply_homography": (C.c_int, VP, VP, VP),
+    "or_fit_homography_least_squares": (C.c_int, VP, VP, C.c_uint32, VP),
+    "or_fit_homography_ransac": (C.c_int, VP, VP, C.c_uint32, C.c_float, VP, P(C.c_uint32), VP),
+    "or_pyramid_levels": (C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32),
+    "or_default_align_options": (None, P(_abi.AlignOptions)),
+    "or_align_module": (C.c_int, P(_abi.View), P(_abi.View), P(_abi.AlignOptions), P(_abi.Warp),
+                        P(_abi.AlignmentReport)),
     "or_direct_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options), VP,
                        C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP),
 }

@@ -163,6 +163,21 @@ int or_direct_rows(const chd_view *views, uint32_t count, uint32_t reference_ind
                    uint32_t row_end, uint32_t threads, chd_node_depth *field, float *guidance,
                    uint8_t *tested);
 
+/* align_module's refinement (align_oracle.c; align.rs:270-519, 644-793,
+ * image.rs:469-498, 531-640, math.rs:131-138). */
+int or_match_patch(const float *reference, uint32_t rw, uint32_t rh, const float *target,
+                   uint32_t tw, uint32_t th, uint32_t rx, uint32_t ry, uint32_t size,
+                   uint32_t radius, float shift[2], float *score);
+int or_apply_homography(const double *h, const double p[2], double out[2]);
+int or_fit_homography_least_squares(const double *pairs, const float *scores, uint32_t count,
+                                    double h[9]);
+int or_fit_homography_ransac(const double *pairs, const float *scores, uint32_t count,
+                             float threshold, double h[9], uint32_t *inliers, float *residuals);
+uint32_t or_pyramid_levels(uint32_t width, uint32_t height, uint32_t min_size);
+void or_default_align_options(chd_align_options *o);
+int or_align_module(const chd_view *reference, const chd_view *target,
+                    const chd_align_options *o, chd_warp *warp, chd_alignment_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,14 @@ LAYOUT = {
     "chd_node_warp": (_abi.NodeWarp, ["point", "measured"]),
     "chd_kernel_time": (_abi.KernelTime, ["launches", "total_ms"]),
     "chd_work": (_abi.Work, ["direct_zncc", "wide_nodes"]),
+    "chd_align_options": (_abi.AlignOptions, ["min_score", "min_inlier_ratio", "refine"]),
+    "chd_level_report": (_abi.LevelReport, ["inliers", "median_residual_px"]),
+    "chd_alignment_report": (_abi.AlignmentReport, ["coverage", "levels", "inliers",
+                                                    "residual_p90_px", "accepted", "gain"]),
+    "chd_synthetic_scene": (_abi.SceneDesc, ["plane_count", "inverse_depth_max",
+                                             "texture_cell_px"]),
+    "chd_synthetic_module": (_abi.Module, ["width", "center", "rotation_wc"]),
+    "chd_bayer": (_abi.Bayer, ["exposure", "read_noise_dn"]),
 }
 
 
