@@ -302,6 +302,15 @@ class AlignmentReport(C.Structure):
     ]
 
 
+class ResolutionReport(C.Structure):
+    _fields_ = [
+        ("verified_fraction", C.c_float),
+        ("supported_fraction", C.c_float),
+        ("median_correction_px", C.c_float),
+        ("mean_confidence", C.c_float),
+    ]
+
+
 # Every function declared in include/chiaro_depth.h, with its signature.
 DEPTH_SIGNATURES = {
     "chd_last_error": (C.c_char_p,),
@@ -373,6 +382,9 @@ DEPTH_SIGNATURES = {
                          P(AlignmentReport)),
     "chd_match_patches": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, VP, VP, C.c_uint32,
                           C.c_uint32, C.c_uint32, VP, VP, VP),
+    "chd_refine_resolution_warp": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32,
+                                   C.c_uint32, P(Warp), C.c_uint32, C.c_uint32, P(Warp),
+                                   P(ResolutionReport)),
     "chd_frames_create": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_float,
                           C.c_float, P(VP)),
     "chd_frames_render": (C.c_int, VP, VP),

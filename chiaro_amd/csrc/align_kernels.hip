@@ -12,6 +12,9 @@
 //                      summing the window in the reference's row-major order
 //                      (neighbouring lanes read neighbouring target texels),
 //                      then the peak and its sub-pixel parabola
+//   k_render_bandwidth the target through a warp at the reference plane's
+//                      sampling, blurred by a Gaussian footprint when the
+//                      target is magnified (resolution.rs:254-270, 441-465)
 //
 // The RANSAC and least-squares fits run on the host (a few thousand pairs).
 // Planes of the pyramids are stored with one replicated padding column and
@@ -107,7 +110,7 @@ __global__ void __launch_bounds__(256) k_match_patches(const float *reference, c
                                                        uint32_t width, uint32_t height,
                                                        const uint32_t *rxs, const uint32_t *rys,
                                                        uint32_t size, uint32_t radius,
-                                                       int require_covered, float *shift_out,
+                                                       int coverage, float *shift_out,
                                                        float *score_out, uint8_t *valid_out) {
     extern __shared__ __align__(16) float lds_f[];
     __shared__ float stats[2];  // mean, std
@@ -128,9 +131,19 @@ __global__ void __launch_bounds__(256) k_match_patches(const float *reference, c
     for (uint32_t i = tid; i < area; i += nth) {
         const uint32_t y = i / size, x = i % size;
         window[i] = reference[(size_t)(ry + y) * width + rx + x];
-        if (require_covered) {
+        if (coverage == kCoverWindowNan) {
             const float t = target[(size_t)(ry + y) * width + rx + x];
             nan_seen |= t != t;
+        }
+    }
+    if (coverage == kCoverSearchFinite) {
+        // every target pixel the search can read is finite; the bounds
+        // above keep [rx - radius, rx + size + radius) inside the plane
+        const uint32_t side = size + 2 * radius;
+        for (uint32_t i = tid; i < side * side; i += nth) {
+            const uint32_t y = i / side, x = i % side;
+            const float t = target[(size_t)(ry - radius + y) * width + rx - radius + x];
+            nan_seen |= !(t - t == 0.0f);
         }
     }
     const int uncovered = __syncthreads_or(nan_seen);
@@ -207,7 +220,45 @@ __global__ void __launch_bounds__(256) k_match_patches(const float *reference, c
     }
 }
 
+__global__ void k_render_bandwidth(uint32_t width, uint32_t height, DevPlane target, DevWarp base,
+                                   BandwidthKernel kernel, float *out) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= width || y >= height) return;
+    float value = __int_as_float(0x7fc00000);
+    float2 q;
+    if (warp_map(base, (float)x * 2.0f + 0.5f, (float)y * 2.0f + 0.5f, q)) {
+        const float u = (q.x + 0.5f) * 0.5f - 0.5f, v = (q.y + 0.5f) * 0.5f - 0.5f;
+        float s;
+        if (kernel.taps == 0) {
+            if (plane_sample(target, u, v, s)) value = s;
+        } else {
+            // the weighted mean over the footprint's defined samples, summed
+            // row by row from (-taps, -taps)
+            const int r = (int)kernel.taps;
+            float sum = 0.0f, weight = 0.0f;
+            int k = 0;
+            for (int dy = -r; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++, k++)
+                    if (plane_sample(target, u + (float)dx, v + (float)dy, s)) {
+                        sum += kernel.weight[k] * s;
+                        weight += kernel.weight[k];
+                    }
+            if (weight > 1.0e-8f) value = sum / weight;
+        }
+    }
+    out[(size_t)y * width + x] = value;
+}
+
 }  // namespace
+
+hipError_t launch_render_bandwidth(hipStream_t stream, uint32_t width, uint32_t height,
+                                   const DevPlane &target, const DevWarp &base,
+                                   const BandwidthKernel &kernel, float *out) {
+    const dim3 block(256), grid((width + 255) / 256, height);
+    hipLaunchKernelGGL(k_render_bandwidth, grid, block, 0, stream, width, height, target, base,
+                       kernel, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_downsample(hipStream_t stream, const float *src, uint32_t sw, uint32_t sh,
                              uint32_t spitch, float *dst, uint32_t dw, uint32_t dh,
@@ -246,11 +297,11 @@ size_t match_patches_lds(uint32_t size, uint32_t radius) {
 hipError_t launch_match_patches(hipStream_t stream, const float *reference, const float *target,
                                 uint32_t width, uint32_t height, const uint32_t *rx,
                                 const uint32_t *ry, uint32_t count, uint32_t size, uint32_t radius,
-                                bool require_covered, float *shift, float *score, uint8_t *valid) {
+                                int coverage, float *shift, float *score, uint8_t *valid) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_match_patches, dim3(count), dim3(256), match_patches_lds(size, radius),
-                       stream, reference, target, width, height, rx, ry, size, radius,
-                       require_covered ? 1 : 0, shift, score, valid);
+                       stream, reference, target, width, height, rx, ry, size, radius, coverage,
+                       shift, score, valid);
     return hipGetLastError();
 }
 

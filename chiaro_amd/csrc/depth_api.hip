@@ -1781,7 +1781,7 @@ int chd_match_patches(chd_context *c, const float *reference, const float *targe
     CHD_HIP(hipMemcpyAsync(d_rx, rx, count * 4, hipMemcpyHostToDevice, st));
     CHD_HIP(hipMemcpyAsync(d_ry, ry, count * 4, hipMemcpyHostToDevice, st));
     CHD_HIP(launch_match_patches(st, d_ref, d_tgt, width, height, d_rx, d_ry, count, size, radius,
-                                 false, d_shift, d_score, d_valid));
+                                 kCoverNone, d_shift, d_score, d_valid));
     CHD_HIP(hipMemcpyAsync(shift, d_shift, 8 * (size_t)count, hipMemcpyDeviceToHost, st));
     CHD_HIP(hipMemcpyAsync(score, d_score, 4 * (size_t)count, hipMemcpyDeviceToHost, st));
     CHD_HIP(hipMemcpyAsync(valid, d_valid, count, hipMemcpyDeviceToHost, st));
@@ -1902,7 +1902,7 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
                 CHD_HIP(hipMemcpyAsync(d_rx, xs.data(), count * 4, hipMemcpyHostToDevice, st));
                 CHD_HIP(hipMemcpyAsync(d_ry, ys.data(), count * 4, hipMemcpyHostToDevice, st));
                 CHD_HIP(launch_match_patches(st, unpadded, rendered, W, H, d_rx, d_ry, count, patch,
-                                             radius, true, d_shift, d_score, d_valid));
+                                             radius, kCoverWindowNan, d_shift, d_score, d_valid));
                 CHD_HIP(hipMemcpyAsync(shifts.data(), d_shift, 8 * (size_t)count,
                                        hipMemcpyDeviceToHost, st));
                 CHD_HIP(hipMemcpyAsync(scores.data(), d_score, 4 * (size_t)count,
@@ -1989,6 +1989,288 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
                                          warp->columns, warp->rows, gain)
                        ? gain
                        : 1.0f;
+    return CHD_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------- refine_resolution_warp
+
+extern "C++" {
+namespace {
+
+constexpr uint32_t kLocalWarpStep = 32;     // LOCAL_WARP_STEP
+constexpr uint32_t kLocalPatch = 12;        // LOCAL_MATCH_PATCH
+constexpr uint32_t kLocalRadius = 2;        // LOCAL_MATCH_RADIUS
+constexpr uint32_t kMagnifiedRadius = 5;    // MAGNIFIED_MATCH_RADIUS
+constexpr float kLocalMinimumScore = 0.72f; // LOCAL_MINIMUM_SCORE
+
+// Warp::confidence (align.rs:112-131); 0 without a confidence array.
+float host_warp_confidence(const chd_warp &w, float x, float y) {
+    const float fx = x / (float)w.step, fy = y / (float)w.step;
+    if (fx < 0.0f || fy < 0.0f || !w.confidence) return 0.0f;
+    auto index = [](float f, uint32_t n) -> uint32_t {
+        const float fl = std::floor(f);
+        uint64_t v = !(fl > 0.0f) ? 0 : fl >= 18446744073709551616.0f ? UINT64_MAX : (uint64_t)fl;
+        return (uint32_t)std::min<uint64_t>(v, n - 1);
+    };
+    const uint32_t c0 = index(fx, w.columns), r0 = index(fy, w.rows);
+    const uint32_t c1 = std::min(c0 + 1, w.columns - 1), r1 = std::min(r0 + 1, w.rows - 1);
+    const float tx = fx - (float)c0, ty = fy - (float)r0;
+    auto at = [&](uint32_t col, uint32_t row) { return w.confidence[(size_t)row * w.columns + col]; };
+    const float top = at(c0, r0) * (1.0f - tx) + at(c1, r0) * tx;
+    const float bottom = at(c0, r1) * (1.0f - tx) + at(c1, r1) * tx;
+    const float v = top * (1.0f - ty) + bottom * ty;
+    return v < 0.0f ? 0.0f : v > 1.0f ? 1.0f : v;  // f32::clamp keeps NaN
+}
+
+float clamp_keep_nan(float v, float lo, float hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+// smoothstep (resolution.rs:486-489)
+float smoothstep01(float v) {
+    v = clamp_keep_nan(v, 0.0f, 1.0f);
+    return v * v * (3.0f - 2.0f * v);
+}
+
+// The medians of the x and y components of a 3x3 neighbourhood's entries
+// (sort_by(total_cmp), element len / 2), sorted lists kept for callers.
+struct Neighbourhood {
+    float xs[9], ys[9];
+    uint32_t n = 0;
+    void push(float x, float y) {
+        xs[n] = x;
+        ys[n] = y;
+        n++;
+    }
+    void sort() {
+        auto by_key = [](float a, float b) { return total_key32(a) < total_key32(b); };
+        std::sort(xs, xs + n, by_key);
+        std::sort(ys, ys + n, by_key);
+    }
+};
+
+}  // namespace
+}  // extern "C++"
+
+extern "C" {
+
+int chd_refine_resolution_warp(chd_context *c, const float *reference, uint32_t rw, uint32_t rh,
+                               const float *target, uint32_t tw, uint32_t th, const chd_warp *base,
+                               uint32_t width, uint32_t height, chd_warp *out,
+                               chd_resolution_report *report) {
+    if (!c || !reference || !target || !base || !base->points || !out || !out->points ||
+        !out->confidence || !report)
+        return fail(CHD_ERR_ARGUMENT, "refine_resolution_warp: bad argument");
+    if (base->step == 0 || base->columns == 0 || base->rows == 0 || rw == 0 || rh == 0 ||
+        tw == 0 || th == 0)
+        return fail(CHD_ERR_ARGUMENT, "refine_resolution_warp: empty plane or base warp");
+    CHD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const chd_warp &bw = *base;
+    auto base_map = [&](float x, float y, float q[2]) {
+        return host_warp_map(bw.points, bw.step, bw.columns, bw.rows, x, y, q);
+    };
+    // Warp::magnification at the raster centre (align.rs:135-143), default 1
+    float magnification = 1.0f;
+    {
+        const float cx = (float)width * 0.5f, cy = (float)height * 0.5f, h = (float)bw.step;
+        float a[2], b[2], d[2];
+        if (base_map(cx, cy, a) && base_map(cx + h, cy, b) && base_map(cx, cy + h, d)) {
+            const float dx = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1])) / h;
+            const float dy = std::sqrt((d[0] - a[0]) * (d[0] - a[0]) + (d[1] - a[1]) * (d[1] - a[1])) / h;
+            magnification = std::fmax(std::sqrt(dx * dy), 1e-6f);
+        }
+    }
+    magnification = clamp_keep_nan(magnification, 0.5f, 4.0f);
+    const uint32_t radius = magnification > 1.35f ? kMagnifiedRadius : kLocalRadius;
+    // the render footprint (resolution.rs:441-465): bilinear up to 1.15x,
+    // else a Gaussian of sigma max(0.42 m, 0.55) over ceil(2 sigma) taps
+    BandwidthKernel footprint{};
+    if (magnification > 1.15f) {
+        const float sigma = std::fmax(0.42f * magnification, 0.55f);
+        const uint32_t taps = (uint32_t)clamp_keep_nan(std::ceil(2.0f * sigma), 1.0f, 4.0f);
+        footprint.taps = taps;
+        uint32_t k = 0;
+        for (int dy = -(int)taps; dy <= (int)taps; dy++)
+            for (int dx = -(int)taps; dx <= (int)taps; dx++)
+                footprint.weight[k++] = std::exp(-0.5f * (float)(dx * dx + dy * dy) / (sigma * sigma));
+    }
+
+    Scratch sc;
+    float *d_ref, *d_rendered, *d_shift, *d_score;
+    float2 *d_base;
+    uint32_t *d_rx, *d_ry;
+    uint8_t *d_valid;
+    DevicePlane tp;
+    tp.width = tw;
+    tp.height = th;
+    const size_t plane_points = (size_t)rw * rh, base_nodes = (size_t)bw.columns * bw.rows;
+    CHD_CHECK(sc.alloc(&d_ref, plane_points));
+    CHD_CHECK(sc.alloc(&d_rendered, plane_points));
+    CHD_CHECK(sc.alloc(&tp.data, (size_t)(tw + 1) * (th + 1)));
+    CHD_CHECK(sc.alloc(&d_base, base_nodes));
+    CHD_HIP(hipMemcpyAsync(d_ref, reference, plane_points * 4, hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpy2DAsync(tp.data, (size_t)(tw + 1) * 4, target, (size_t)tw * 4, (size_t)tw * 4,
+                             th, hipMemcpyHostToDevice, st));
+    CHD_HIP(launch_pad_plane(st, tp.data, tw, th));
+    CHD_HIP(hipMemcpyAsync(d_base, bw.points, base_nodes * 8, hipMemcpyHostToDevice, st));
+    DevWarp dw;
+    dw.points = d_base;
+    dw.step = bw.step;
+    dw.columns = bw.columns;
+    dw.rows = bw.rows;
+    dw.inverse_step = (bw.step & (bw.step - 1)) == 0 ? 1.0f / (float)bw.step : 0.0f;
+    CHD_HIP(launch_render_bandwidth(st, rw, rh, tp.view(), dw, footprint, d_rendered));
+
+    // one 12 x 12 window per node of the 32 px grid whose search stays
+    // inside the plane (resolution.rs:272-301)
+    const uint32_t columns = (uint32_t)div_ceil(width, kLocalWarpStep) + 1;
+    const uint32_t rows = (uint32_t)div_ceil(height, kLocalWarpStep) + 1;
+    const size_t nodes = (size_t)columns * rows;
+    std::vector<uint32_t> xs, ys, node_of;
+    for (uint32_t row = 0; row < rows; row++)
+        for (uint32_t column = 0; column < columns; column++) {
+            const float rx = (float)(column * kLocalWarpStep), ry = (float)(row * kLocalWarpStep);
+            if (rx >= (float)width || ry >= (float)height) continue;
+            const int64_t x = (int64_t)std::round((rx - 0.5f) * 0.5f) - (int64_t)kLocalPatch / 2;
+            const int64_t y = (int64_t)std::round((ry - 0.5f) * 0.5f) - (int64_t)kLocalPatch / 2;
+            if (x < (int64_t)radius || y < (int64_t)radius ||
+                x + kLocalPatch + radius > (int64_t)rw || y + kLocalPatch + radius > (int64_t)rh)
+                continue;
+            xs.push_back((uint32_t)x);
+            ys.push_back((uint32_t)y);
+            node_of.push_back(row * columns + column);
+        }
+    const uint32_t count = (uint32_t)xs.size();
+    std::vector<float> shifts(2 * (size_t)count), scores(count);
+    std::vector<uint8_t> valid(count);
+    CHD_CHECK(sc.alloc(&d_rx, count));
+    CHD_CHECK(sc.alloc(&d_ry, count));
+    CHD_CHECK(sc.alloc(&d_shift, 2 * (size_t)count));
+    CHD_CHECK(sc.alloc(&d_score, count));
+    CHD_CHECK(sc.alloc(&d_valid, count));
+    if (count > 0) {
+        CHD_HIP(hipMemcpyAsync(d_rx, xs.data(), count * 4, hipMemcpyHostToDevice, st));
+        CHD_HIP(hipMemcpyAsync(d_ry, ys.data(), count * 4, hipMemcpyHostToDevice, st));
+        CHD_HIP(launch_match_patches(st, d_ref, d_rendered, rw, rh, d_rx, d_ry, count, kLocalPatch,
+                                     radius, kCoverSearchFinite, d_shift, d_score, d_valid));
+        CHD_HIP(hipMemcpyAsync(shifts.data(), d_shift, 8 * (size_t)count, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipMemcpyAsync(scores.data(), d_score, 4 * (size_t)count, hipMemcpyDeviceToHost, st));
+        CHD_HIP(hipMemcpyAsync(valid.data(), d_valid, count, hipMemcpyDeviceToHost, st));
+    }
+    CHD_HIP(hipStreamSynchronize(st));
+
+    // measured shifts in raster px (resolution.rs:295-300)
+    std::vector<uint8_t> measured(nodes, 0);
+    std::vector<float> mshift(2 * nodes, 0.0f), mscore(nodes, 0.0f);
+    for (uint32_t k = 0; k < count; k++)
+        if (valid[k] && scores[k] >= kLocalMinimumScore) {
+            const uint32_t i = node_of[k];
+            measured[i] = 1;
+            mshift[2 * i] = shifts[2 * k] * 2.0f;
+            mshift[2 * i + 1] = shifts[2 * k + 1] * 2.0f;
+            mscore[i] = scores[k];
+        }
+    auto gather = [&](uint32_t row, uint32_t column, const std::vector<uint8_t> &have,
+                      const std::vector<float> &vec, Neighbourhood &nb,
+                      const std::vector<float> *conf, float *conf_sum) {
+        const uint32_t r0 = row > 0 ? row - 1 : 0, r1 = std::min(row + 1, rows - 1);
+        const uint32_t c0 = column > 0 ? column - 1 : 0, c1 = std::min(column + 1, columns - 1);
+        for (uint32_t r = r0; r <= r1; r++)
+            for (uint32_t cc = c0; cc <= c1; cc++) {
+                const size_t j = (size_t)r * columns + cc;
+                if (!have[j]) continue;
+                nb.push(vec[2 * j], vec[2 * j + 1]);
+                if (conf) *conf_sum += (*conf)[j];
+            }
+    };
+
+    // neighbourhood-median verification (resolution.rs:303-360)
+    out->step = kLocalWarpStep;
+    out->columns = columns;
+    out->rows = rows;
+    std::vector<uint8_t> corrected(nodes, 0);
+    std::vector<float> correction(2 * nodes, 0.0f), corrections;
+    float confidence_sum = 0.0f;
+    for (uint32_t row = 0; row < rows; row++)
+        for (uint32_t column = 0; column < columns; column++) {
+            const size_t i = (size_t)row * columns + column;
+            const float rx = (float)(column * kLocalWarpStep), ry = (float)(row * kLocalWarpStep);
+            bool refined = false;
+            float point[2], local = 0.0f, median[2];
+            if (measured[i]) {
+                Neighbourhood nb;
+                gather(row, column, measured, mshift, nb, nullptr, nullptr);
+                if (nb.n >= 3) {
+                    nb.sort();
+                    median[0] = nb.xs[nb.n / 2];
+                    median[1] = nb.ys[nb.n / 2];
+                    if (!(std::hypot(mshift[2 * i] - median[0], mshift[2 * i + 1] - median[1]) > 1.5f) &&
+                        base_map(rx + median[0], ry + median[1], point)) {
+                        local = smoothstep01((mscore[i] - kLocalMinimumScore) / 0.20f) *
+                                host_warp_confidence(bw, rx, ry);
+                        refined = true;
+                    }
+                }
+            }
+            if (refined) {
+                out->points[2 * i] = point[0];
+                out->points[2 * i + 1] = point[1];
+                out->confidence[i] = local;
+                corrected[i] = 1;
+                correction[2 * i] = median[0];
+                correction[2 * i + 1] = median[1];
+                corrections.push_back(std::hypot(median[0], median[1]));
+                confidence_sum += local;
+            } else {
+                float q[2];
+                if (!base_map(rx, ry, q)) q[0] = q[1] = NAN;
+                out->points[2 * i] = q[0];
+                out->points[2 * i + 1] = q[1];
+                out->confidence[i] = 0.0f;
+            }
+        }
+    // two completion passes over snapshots (resolution.rs:362-412)
+    for (int pass = 0; pass < 2; pass++) {
+        const std::vector<uint8_t> have = corrected;
+        const std::vector<float> snapshot = correction;
+        const std::vector<float> conf(out->confidence, out->confidence + nodes);
+        for (uint32_t row = 0; row < rows; row++)
+            for (uint32_t column = 0; column < columns; column++) {
+                const size_t i = (size_t)row * columns + column;
+                if (have[i]) continue;
+                Neighbourhood nb;
+                float csum = 0.0f;
+                gather(row, column, have, snapshot, nb, &conf, &csum);
+                if (nb.n < 3) continue;
+                nb.sort();
+                const float cx = nb.xs[nb.n / 2], cy = nb.ys[nb.n / 2];
+                // the reference pairs the two independently sorted lists
+                bool coherent = true;
+                for (uint32_t k = 0; k < nb.n; k++)
+                    coherent &= std::hypot(nb.xs[k] - cx, nb.ys[k] - cy) <= 1.5f;
+                if (!coherent) continue;
+                float p[2];
+                if (!base_map((float)(column * kLocalWarpStep) + cx,
+                              (float)(row * kLocalWarpStep) + cy, p))
+                    continue;
+                out->points[2 * i] = p[0];
+                out->points[2 * i + 1] = p[1];
+                out->confidence[i] = 0.45f * csum / (float)nb.n;
+                corrected[i] = 1;
+                correction[2 * i] = cx;
+                correction[2 * i + 1] = cy;
+            }
+    }
+    std::sort(corrections.begin(), corrections.end(),
+              [](float a, float b) { return total_key32(a) < total_key32(b); });
+    size_t supported = 0;
+    for (size_t i = 0; i < nodes; i++) supported += out->confidence[i] > 0.0f;
+    const float denominator = (float)std::max<size_t>(nodes, 1);
+    report->verified_fraction = (float)corrections.size() / denominator;
+    report->supported_fraction = (float)supported / denominator;
+    report->median_correction_px = corrections.empty() ? 0.0f : corrections[corrections.size() / 2];
+    report->mean_confidence = confidence_sum / (float)std::max<size_t>(corrections.size(), 1);
     return CHD_OK;
 }
 

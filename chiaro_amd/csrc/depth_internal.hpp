@@ -236,12 +236,27 @@ hipError_t launch_render_through(hipStream_t stream, uint32_t width, uint32_t he
                                  const double correction[9], float *out);
 size_t match_patches_lds(uint32_t size, uint32_t radius);
 // match_patch of `count` windows (device arrays rx, ry) between two planes of
-// width x height (row pitch width); with require_covered a window whose
-// target pixels hold a NaN is None.
+// width x height (row pitch width). `coverage` adds a precondition on the
+// target: kCoverWindowNan makes a window whose target pixels hold a NaN None
+// (align.rs:550-561), kCoverSearchFinite one with a non-finite target pixel
+// anywhere in the window grown by `radius` (resolution.rs:467-483).
+enum { kCoverNone = 0, kCoverWindowNan = 1, kCoverSearchFinite = 2 };
 hipError_t launch_match_patches(hipStream_t stream, const float *reference, const float *target,
                                 uint32_t width, uint32_t height, const uint32_t *rx,
                                 const uint32_t *ry, uint32_t count, uint32_t size, uint32_t radius,
-                                bool require_covered, float *shift, float *score, uint8_t *valid);
+                                int coverage, float *shift, float *score, uint8_t *valid);
+// sample_at_reference_bandwidth of the target through `base` at every pixel
+// of a width x height reference plane (resolution.rs:254-270): out[y][x] is
+// the sample at ((q + 0.5) / 2 - 0.5) for q = base.map(2x + 0.5, 2y + 0.5),
+// or NaN. taps = 0 samples bilinearly; otherwise `kernel` holds the
+// (2 taps + 1)^2 Gaussian weights, row-major from (-taps, -taps).
+struct BandwidthKernel {
+    uint32_t taps;
+    float weight[81];
+};
+hipError_t launch_render_bandwidth(hipStream_t stream, uint32_t width, uint32_t height,
+                                   const struct DevPlane &target, const struct DevWarp &base,
+                                   const BandwidthKernel &kernel, float *out);
 
 // ------------------------------------------------ synthetic capture generator
 

@@ -594,6 +594,29 @@ int chd_match_patches(chd_context *context, const float *reference, const float 
                       uint32_t count, uint32_t size, uint32_t radius, float *shift, float *score,
                       uint8_t *valid);
 
+/* ResolutionAlignmentReport (crates/chiaro-fusion/src/resolution.rs:87-98). */
+typedef struct chd_resolution_report {
+    float verified_fraction;    /* nodes with a coherent correlation peak */
+    float supported_fraction;   /* nodes with confidence > 0 after completion */
+    float median_correction_px; /* median |correction| of verified nodes, raster px */
+    float mean_confidence;      /* mean confidence of verified nodes */
+} chd_resolution_report;
+
+/* refine_resolution_warp (crates/chiaro-fusion/src/resolution.rs:232-432),
+ * called per non-reference module at fusion.rs:384 and pipeline.rs:1185:
+ * local 12x12 NCC on a 32 px grid of the reference raster (width x height)
+ * between the reference plane and the target plane rendered through `base`
+ * at the reference's bandwidth, neighbourhood-median checks and two
+ * completion passes. Planes are host arrays (row-major f32); `base` is a
+ * host warp. `out` gets step 32, columns = ceil(width / 32) + 1,
+ * rows = ceil(height / 32) + 1; its points (2 * columns * rows f32) and
+ * confidence (columns * rows f32) arrays are the caller's. */
+int chd_refine_resolution_warp(chd_context *context, const float *reference,
+                               uint32_t reference_width, uint32_t reference_height,
+                               const float *target, uint32_t target_width,
+                               uint32_t target_height, const chd_warp *base, uint32_t width,
+                               uint32_t height, chd_warp *out, chd_resolution_report *report);
+
 /* Synthetic captures rendered on the GPU. No counterpart in the reference
  * (its only fakes are gradient mock LRIs, crates/chiaro/src/mock.rs:62):
  * a seeded scene of fronto-parallel planes over a Voronoi partition of the

@@ -630,3 +630,264 @@ int or_align_module(const chd_view *reference, const chd_view *target,
     free(grid.confidence);
     return CHD_OK;
 }
+
+/* -------------------------------------------- refine_resolution_warp */
+
+#define LOCAL_WARP_STEP 32
+#define LOCAL_MATCH_PATCH 12
+#define LOCAL_MATCH_RADIUS 2
+#define MAGNIFIED_MATCH_RADIUS 5
+#define LOCAL_MINIMUM_SCORE 0.72f
+
+static float clamp01(float v) { return v < 0.0f ? 0.0f : v > 1.0f ? 1.0f : v; }
+
+/* smoothstep (resolution.rs:485-489) */
+static float smoothstep(float v) {
+    v = clamp01(v);
+    return v * v * (3.0f - 2.0f * v);
+}
+
+/* Warp::magnification (align.rs:135-143) */
+static int warp_magnification(const chd_warp *w, float x, float y, float *out) {
+    float h = (float)w->step, a[2], b[2], c[2];
+    if (!or_warp_map(w, x, y, a) || !or_warp_map(w, x + h, y, b) || !or_warp_map(w, x, y + h, c))
+        return 0;
+    float dx = sqrtf((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1])) / h;
+    float dy = sqrtf((c[0] - a[0]) * (c[0] - a[0]) + (c[1] - a[1]) * (c[1] - a[1])) / h;
+    *out = fmaxf(sqrtf(dx * dy), 1e-6f);
+    return 1;
+}
+
+/* Warp::confidence (align.rs:112-131) */
+static float warp_confidence(const chd_warp *w, float x, float y) {
+    float fx = x / (float)w->step, fy = y / (float)w->step;
+    if (fx < 0.0f || fy < 0.0f || !w->confidence) return 0.0f;
+    float flx = floorf(fx), fly = floorf(fy);
+    uint32_t c0 = !(flx > 0.0f) ? 0 : (flx >= 4294967296.0f ? UINT32_MAX : (uint32_t)flx);
+    uint32_t r0 = !(fly > 0.0f) ? 0 : (fly >= 4294967296.0f ? UINT32_MAX : (uint32_t)fly);
+    if (c0 > w->columns - 1) c0 = w->columns - 1;
+    if (r0 > w->rows - 1) r0 = w->rows - 1;
+    uint32_t c1 = c0 + 1 < w->columns - 1 ? c0 + 1 : w->columns - 1;
+    uint32_t r1 = r0 + 1 < w->rows - 1 ? r0 + 1 : w->rows - 1;
+    float tx = fx - (float)c0, ty = fy - (float)r0;
+#define V(c, r) w->confidence[(size_t)(r) * w->columns + (c)]
+    float top = V(c0, r0) * (1.0f - tx) + V(c1, r0) * tx;
+    float bottom = V(c0, r1) * (1.0f - tx) + V(c1, r1) * tx;
+#undef V
+    float v = top * (1.0f - ty) + bottom * ty;
+    return v < 0.0f ? 0.0f : v > 1.0f ? 1.0f : v;
+}
+
+/* sample_at_reference_bandwidth (resolution.rs:441-465) */
+static int sample_at_bandwidth(const plane_t *t, float x, float y, float magnification, float *out) {
+    if (magnification <= 1.15f) return sample(t, x, y, out);
+    float sigma = fmaxf(0.42f * magnification, 0.55f);
+    float rr = ceilf(2.0f * sigma);
+    rr = rr < 1.0f ? 1.0f : rr > 4.0f ? 4.0f : rr;
+    int radius = (int)rr;
+    float sum = 0.0f, weight = 0.0f;
+    for (int dy = -radius; dy <= radius; dy++)
+        for (int dx = -radius; dx <= radius; dx++) {
+            float distance_squared = (float)(dx * dx + dy * dy);
+            float kernel = expf(-0.5f * distance_squared / (sigma * sigma));
+            float v;
+            if (sample(t, x + (float)dx, y + (float)dy, &v)) {
+                sum += kernel * v;
+                weight += kernel;
+            }
+        }
+    if (!(weight > 1.0e-8f)) return 0;
+    *out = sum / weight;
+    return 1;
+}
+
+static int window_finite(const float *r, uint32_t w, size_t x0, size_t y0, size_t x1, size_t y1) {
+    for (size_t row = y0; row < y1; row++)
+        for (size_t col = x0; col < x1; col++)
+            if (!isfinite(r[row * w + col])) return 0;
+    return 1;
+}
+
+/* f32::round: half away from zero */
+static float round_half_away(float v) { return roundf(v); }
+
+/* refine_resolution_warp (resolution.rs:232-432) */
+int or_refine_resolution_warp(const float *reference, uint32_t rw, uint32_t rh,
+                              const float *target, uint32_t tw, uint32_t th, const chd_warp *base,
+                              uint32_t width, uint32_t height, chd_warp *out,
+                              chd_resolution_report *report) {
+    plane_t tp = {(float *)target, tw, th};
+    float centre[2] = {(float)width * 0.5f, (float)height * 0.5f}, mag;
+    if (!warp_magnification(base, centre[0], centre[1], &mag)) mag = 1.0f;
+    mag = mag < 0.5f ? 0.5f : mag > 4.0f ? 4.0f : mag;
+    uint32_t radius = mag > 1.35f ? MAGNIFIED_MATCH_RADIUS : LOCAL_MATCH_RADIUS;
+    float *rendered = (float *)malloc((size_t)rw * rh * sizeof(float));
+    for (uint32_t y = 0; y < rh; y++)
+        for (uint32_t x = 0; x < rw; x++) {
+            float rx = (float)x * 2.0f + 0.5f, ry = (float)y * 2.0f + 0.5f, q[2], v = NAN;
+            if (or_warp_map(base, rx, ry, q)) {
+                float s;
+                if (sample_at_bandwidth(&tp, (q[0] + 0.5f) * 0.5f - 0.5f, (q[1] + 0.5f) * 0.5f - 0.5f,
+                                        mag, &s))
+                    v = s;
+            }
+            rendered[(size_t)y * rw + x] = v;
+        }
+    uint32_t columns = (width + LOCAL_WARP_STEP - 1) / LOCAL_WARP_STEP + 1;
+    uint32_t rows = (height + LOCAL_WARP_STEP - 1) / LOCAL_WARP_STEP + 1;
+    size_t nodes = (size_t)columns * rows;
+    uint8_t *has = (uint8_t *)calloc(nodes, 1);
+    float *mshift = (float *)calloc(nodes * 2, sizeof(float));
+    float *mscore = (float *)calloc(nodes, sizeof(float));
+    for (uint32_t row = 0; row < rows; row++)
+        for (uint32_t column = 0; column < columns; column++) {
+            float rx = (float)(column * LOCAL_WARP_STEP), ry = (float)(row * LOCAL_WARP_STEP);
+            if (rx >= (float)width || ry >= (float)height) continue;
+            long plane_x = (long)round_half_away((rx - 0.5f) * 0.5f);
+            long plane_y = (long)round_half_away((ry - 0.5f) * 0.5f);
+            long x = plane_x - LOCAL_MATCH_PATCH / 2, y = plane_y - LOCAL_MATCH_PATCH / 2;
+            if (x < (long)radius || y < (long)radius ||
+                x + LOCAL_MATCH_PATCH + (long)radius > (long)rw ||
+                y + LOCAL_MATCH_PATCH + (long)radius > (long)rh)
+                continue;
+            if (!window_finite(rendered, rw, (size_t)(x - radius), (size_t)(y - radius),
+                               (size_t)(x + LOCAL_MATCH_PATCH + radius),
+                               (size_t)(y + LOCAL_MATCH_PATCH + radius)))
+                continue;
+            float shift[2], score;
+            if (!or_match_patch(reference, rw, rh, rendered, rw, rh, (uint32_t)x, (uint32_t)y,
+                                LOCAL_MATCH_PATCH, radius, shift, &score))
+                continue;
+            if (!(score >= LOCAL_MINIMUM_SCORE)) continue;
+            size_t i = (size_t)row * columns + column;
+            has[i] = 1;
+            mshift[2 * i] = shift[0] * 2.0f;
+            mshift[2 * i + 1] = shift[1] * 2.0f;
+            mscore[i] = score;
+        }
+    free(rendered);
+    uint8_t *cgrid = (uint8_t *)calloc(nodes, 1);
+    float *correction = (float *)calloc(nodes * 2, sizeof(float));
+    float *corrections = (float *)malloc((nodes + 1) * sizeof(float));
+    size_t ncorr = 0;
+    float confidence_sum = 0.0f;
+    out->step = LOCAL_WARP_STEP;
+    out->columns = columns;
+    out->rows = rows;
+    for (uint32_t row = 0; row < rows; row++)
+        for (uint32_t column = 0; column < columns; column++) {
+            float rx = (float)(column * LOCAL_WARP_STEP), ry = (float)(row * LOCAL_WARP_STEP);
+            size_t index = (size_t)row * columns + column;
+            int refined = 0;
+            float point[2], local_conf = 0.0f, med[2];
+            if (has[index]) {
+                float xs[9], ys[9];
+                int n = 0;
+                uint32_t r0 = row > 0 ? row - 1 : 0, r1 = row + 1 < rows - 1 ? row + 1 : rows - 1;
+                uint32_t c0 = column > 0 ? column - 1 : 0, c1 = column + 1 < columns - 1 ? column + 1 : columns - 1;
+                for (uint32_t nr = r0; nr <= r1; nr++)
+                    for (uint32_t nc = c0; nc <= c1; nc++) {
+                        size_t j = (size_t)nr * columns + nc;
+                        if (has[j]) {
+                            xs[n] = mshift[2 * j];
+                            ys[n] = mshift[2 * j + 1];
+                            n++;
+                        }
+                    }
+                if (n >= 3) {
+                    qsort(xs, n, sizeof(float), cmp_f32_total);
+                    qsort(ys, n, sizeof(float), cmp_f32_total);
+                    med[0] = xs[n / 2];
+                    med[1] = ys[n / 2];
+                    if (!(hypotf(mshift[2 * index] - med[0], mshift[2 * index + 1] - med[1]) > 1.5f) &&
+                        or_warp_map(base, rx + med[0], ry + med[1], point)) {
+                        float sc = smoothstep((mscore[index] - LOCAL_MINIMUM_SCORE) / 0.20f);
+                        local_conf = sc * warp_confidence(base, rx, ry);
+                        refined = 1;
+                    }
+                }
+            }
+            if (refined) {
+                out->points[2 * index] = point[0];
+                out->points[2 * index + 1] = point[1];
+                out->confidence[index] = local_conf;
+                cgrid[index] = 1;
+                correction[2 * index] = med[0];
+                correction[2 * index + 1] = med[1];
+                corrections[ncorr++] = hypotf(med[0], med[1]);
+                confidence_sum += local_conf;
+            } else {
+                float q[2];
+                if (!or_warp_map(base, rx, ry, q)) q[0] = q[1] = NAN;
+                out->points[2 * index] = q[0];
+                out->points[2 * index + 1] = q[1];
+                out->confidence[index] = 0.0f;
+            }
+        }
+    /* two completion passes (resolution.rs:362-412) */
+    uint8_t *csnap = (uint8_t *)malloc(nodes);
+    float *corr_snap = (float *)malloc(nodes * 2 * sizeof(float));
+    float *conf_snap = (float *)malloc(nodes * sizeof(float));
+    for (int pass = 0; pass < 2; pass++) {
+        memcpy(csnap, cgrid, nodes);
+        memcpy(corr_snap, correction, nodes * 2 * sizeof(float));
+        memcpy(conf_snap, out->confidence, nodes * sizeof(float));
+        for (uint32_t row = 0; row < rows; row++)
+            for (uint32_t column = 0; column < columns; column++) {
+                size_t index = (size_t)row * columns + column;
+                if (csnap[index]) continue;
+                float xs[9], ys[9], csum = 0.0f;
+                int n = 0;
+                uint32_t r0 = row > 0 ? row - 1 : 0, r1 = row + 1 < rows - 1 ? row + 1 : rows - 1;
+                uint32_t c0 = column > 0 ? column - 1 : 0, c1 = column + 1 < columns - 1 ? column + 1 : columns - 1;
+                for (uint32_t nr = r0; nr <= r1; nr++)
+                    for (uint32_t nc = c0; nc <= c1; nc++) {
+                        size_t j = (size_t)nr * columns + nc;
+                        if (csnap[j]) {
+                            xs[n] = corr_snap[2 * j];
+                            ys[n] = corr_snap[2 * j + 1];
+                            csum += conf_snap[j];
+                            n++;
+                        }
+                    }
+                if (n < 3) continue;
+                float sx[9], sy[9];
+                memcpy(sx, xs, n * sizeof(float));
+                memcpy(sy, ys, n * sizeof(float));
+                qsort(sx, n, sizeof(float), cmp_f32_total);
+                qsort(sy, n, sizeof(float), cmp_f32_total);
+                float c[2] = {sx[n / 2], sy[n / 2]};
+                int coherent = 1;
+                for (int k = 0; k < n; k++)
+                    if (!(hypotf(sx[k] - c[0], sy[k] - c[1]) <= 1.5f)) coherent = 0;
+                if (!coherent) continue;
+                float rx = (float)(column * LOCAL_WARP_STEP), ry = (float)(row * LOCAL_WARP_STEP), p[2];
+                if (or_warp_map(base, rx + c[0], ry + c[1], p)) {
+                    out->points[2 * index] = p[0];
+                    out->points[2 * index + 1] = p[1];
+                    out->confidence[index] = 0.45f * csum / (float)n;
+                    cgrid[index] = 1;
+                    correction[2 * index] = c[0];
+                    correction[2 * index + 1] = c[1];
+                }
+            }
+    }
+    qsort(corrections, ncorr, sizeof(float), cmp_f32_total);
+    size_t supported = 0;
+    for (size_t i = 0; i < nodes; i++) supported += out->confidence[i] > 0.0f;
+    float denom = (float)(nodes > 0 ? nodes : 1);
+    report->verified_fraction = (float)ncorr / denom;
+    report->supported_fraction = (float)supported / denom;
+    report->median_correction_px = ncorr > 0 ? corrections[ncorr / 2] : 0.0f;
+    report->mean_confidence = confidence_sum / (float)(ncorr > 0 ? ncorr : 1);
+    free(has);
+    free(mshift);
+    free(mscore);
+    free(cgrid);
+    free(correction);
+    free(corrections);
+    free(csnap);
+    free(corr_snap);
+    free(conf_snap);
+    return CHD_OK;
+}

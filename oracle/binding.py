@@ -99,6 +99,9 @@ SIGNATURES = {
     "or_default_align_options": (None, P(_abi.AlignOptions)),
     "or_align_module": (C.c_int, P(_abi.View), P(_abi.View), P(_abi.AlignOptions), P(_abi.Warp),
                         P(_abi.AlignmentReport)),
+    "or_refine_resolution_warp": (C.c_int, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32,
+                                  C.c_uint32, P(_abi.Warp), C.c_uint32, C.c_uint32, P(_abi.Warp),
+                                  P(_abi.ResolutionReport)),
     "or_direct_rows": (C.c_int, P(_abi.View), C.c_uint32, C.c_uint32, P(_abi.Options), VP,
                        C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP),
 }

@@ -177,6 +177,10 @@ uint32_t or_pyramid_levels(uint32_t width, uint32_t height, uint32_t min_size);
 void or_default_align_options(chd_align_options *o);
 int or_align_module(const chd_view *reference, const chd_view *target,
                     const chd_align_options *o, chd_warp *warp, chd_alignment_report *report);
+int or_refine_resolution_warp(const float *reference, uint32_t rw, uint32_t rh,
+                              const float *target, uint32_t tw, uint32_t th, const chd_warp *base,
+                              uint32_t width, uint32_t height, chd_warp *out,
+                              chd_resolution_report *report);
 
 #ifdef __cplusplus
 }
