@@ -241,7 +241,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["config1", "config0", "config2", "config3", "config4",
-                                           "config4-learned"],
+                                           "config4-learned", "config4-align"],
                     default="config1")
     ap.add_argument("--train-scenes", type=int, default=512,
                     help="config3 / config4-learned: training scenes of the learned band")
@@ -322,6 +322,8 @@ def main() -> int:
         return bench_learned(args)
     if args.workload == "config4-learned":
         return bench_learned4(args)
+    if args.workload == "config4-align":
+        return bench_align(args)
 
     from chiaro_amd import _abi, scene
     from chiaro_amd.depth import Context, Session
@@ -847,6 +849,147 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         "total_seconds": round(time.perf_counter() - t_all, 2),
         "losses": [round(v, 4) for v in losses[:: max(1, len(losses) // 10)]],
     }
+
+
+def _turn(degrees_x: float, degrees_y: float):
+    """Rotation about x then y, degrees."""
+    import numpy as np
+    ax, ay = np.radians(degrees_x), np.radians(degrees_y)
+    rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    return ry @ rx
+
+
+def bench_align(args) -> int:
+    """config4-align: align_module (align.rs:270-519) and then
+    refine_resolution_warp (resolution.rs:232-432) for the fifteen
+    non-reference modules of one full config 4 capture rendered on the GPU,
+    each module's factory model turned by up to 0.3 deg about x and y from
+    its true orientation. The planes stay in HBM (the value); the same pass
+    from host planes, as the reference's signature hands them over, is
+    reported beside it. The kernel times are in the rocprof trace."""
+    import ctypes as C
+
+    import numpy as np
+
+    from chiaro_amd import _abi, align, scene
+    from chiaro_amd.depth import Context
+
+    rig = scene.rig_config4(seed=args.seed)
+    ctx = Context(0)
+    frames = scene.Frames(ctx, rig)
+    frames.render(args.seed)
+    device = frames.planes()                                  # resident in HBM
+    planes = [frames.read(i)[1] for i in range(len(rig.modules))]
+    rng = np.random.default_rng(args.seed)
+    modules, host_modules = [], []
+    for i, m in enumerate(rig.modules):
+        if i == rig.reference:
+            factory = m
+        else:
+            turn = _turn(*rng.uniform(-0.3, 0.3, 2))
+            rot = turn @ np.array(m.rotation_wc[:]).reshape(3, 3)
+            factory = scene._module(m.focal_px, m.width, m.height, tuple(m.center[:]), rot)
+        for out, lum in ((modules, device[i]), (host_modules, planes[i])):
+            out.append(scene.ModuleInput(
+                name=rig.names[i], luminance=lum, width=m.width, height=m.height,
+                camera=scene.camera_for(factory),
+                warp_points=scene.identity_warp(m.width, m.height), warp_step=32))
+    ref = rig.reference
+    w, h = rig.modules[ref].width, rig.modules[ref].height
+    targets = [i for i in range(len(modules)) if i != ref]
+
+    def one_pass(mods, lum):
+        t0 = time.perf_counter()
+        results = [align.align_module(ctx, mods, ref, t) for t in targets]
+        t1 = time.perf_counter()
+        warps = [align.refine_resolution_warp(ctx, lum[ref], lum[t], results[k].warp, w, h)
+                 for k, t in enumerate(targets)]
+        t2 = time.perf_counter()
+        return results, warps, (t1 - t0) * 1000.0, (t2 - t1) * 1000.0
+
+    align.align_module(ctx, modules, ref, targets[0])        # warm-up
+    log(f"[bench] aligning {len(targets)} modules")
+    for _ in range(args.warmup):
+        one_pass(modules, device)
+    align_ms, res_ms = [], []
+    for _ in range(args.steps):
+        results, warps, a, r = one_pass(modules, device)
+        align_ms.append(a)
+        res_ms.append(r)
+    a_ms, r_ms = float(np.median(align_ms)), float(np.median(res_ms))
+    per_module = (a_ms + r_ms) / len(targets)
+    # the same pass from host planes (PCIe copies of both planes per call)
+    host_results, host_warps, ha, hr = one_pass(host_modules, planes)
+    same = all(np.array_equal(x.warp.points, y.warp.points, equal_nan=True)
+               for x, y in zip(results, host_results)) and \
+        all(np.array_equal(x.warp.points, y.warp.points, equal_nan=True)
+            for x, y in zip(warps, host_warps))
+    frames.close()
+    statuses = {}
+    for r in results:
+        key = r.report["status"].split(":")[0]
+        statuses[key] = statuses.get(key, 0) + 1
+    modules_report = [{
+        "module": modules[t].name, "status": results[k].report["status"],
+        "inliers": results[k].report["inliers"], "patches": results[k].report["patches"],
+        "residual_median_px": round(results[k].report["residual_median_px"], 3),
+        "verified_fraction": round(warps[k].report["verified_fraction"], 4),
+        "supported_fraction": round(warps[k].report["supported_fraction"], 4),
+    } for k, t in enumerate(targets)]
+
+    cpu = None
+    if args.cpu_rows > 0:
+        # the oracle on two modules (a wide and a tele), one thread
+        from chiaro_amd.depth import _ViewArrays
+        from oracle import binding
+        lib = binding.lib()
+        sample = [targets[0], targets[-1]]
+        c0 = time.perf_counter()
+        for t in sample:
+            views = _ViewArrays([host_modules[ref], host_modules[t]])
+            o = align.AlignOptions().ffi()
+            wp = align.Warp.empty(w, h, 32)
+            rep = _abi.AlignmentReport()
+            lib.or_align_module(C.byref(views.views[0]), C.byref(views.views[1]), C.byref(o),
+                                C.byref(wp.ffi()), C.byref(rep))
+            out = align.Warp.empty(w, h, 32)
+            rr = _abi.ResolutionReport()
+            lib.or_refine_resolution_warp(planes[ref].ctypes.data, planes[ref].shape[1],
+                                          planes[ref].shape[0], planes[t].ctypes.data,
+                                          planes[t].shape[1], planes[t].shape[0],
+                                          C.byref(wp.ffi()), w, h, C.byref(out.ffi()),
+                                          C.byref(rr))
+        cpu_ms = (time.perf_counter() - c0) * 1000.0 / len(sample)
+        cpu = {"value": round(1000.0 / cpu_ms, 3), "unit": "modules/s", "cores": 1,
+               "kind": "port", "sample": f"align_module + refine_resolution_warp of modules "
+                                         f"{modules[sample[0]].name} and {modules[sample[1]].name}",
+               "ms_per_module": round(cpu_ms, 1), "host": host_cpu()}
+    line = {
+        "metric": "modules/s: align_module + refine_resolution_warp on 1 GPU",
+        "value": round(1000.0 / per_module, 3),
+        "unit": "modules/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(a_ms + r_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 geometry and fits)", "data": "synthetic",
+        "config": {"workload": "config4-align: one 16-module config 4 capture (4160x3120 rasters, "
+                               "2080x1560 planes), 15 modules aligned to the reference from factory "
+                               "models turned by up to 0.3 deg, then refined at 32 px",
+                   "seed": args.seed},
+        "align_ms_per_step": round(a_ms, 3),
+        "resolution_ms_per_step": round(r_ms, 3),
+        "ms_per_module": round(per_module, 3),
+        "host_planes": {"align_ms_per_step": round(ha, 3), "resolution_ms_per_step": round(hr, 3),
+                        "ms_per_module": round((ha + hr) / len(targets), 3),
+                        "same_warps_as_device_planes": same},
+        "status_counts": statuses,
+        "modules": modules_report,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+    return 0
 
 
 def bench_learned4(args) -> int:

@@ -302,6 +302,30 @@ class AlignmentReport(C.Structure):
     ]
 
 
+class SynthSource(C.Structure):
+    _fields_ = [
+        ("warp", Warp),
+        ("mosaic_width", C.c_uint32),
+        ("mosaic_height", C.c_uint32),
+        ("magnification", C.c_float),
+        ("confidence", C.c_float),
+        ("reference", C.c_int32),
+        ("has_focus_distance", C.c_int32),
+        ("focus_distance", C.c_double),
+    ]
+
+
+class Canvas(C.Structure):
+    _fields_ = [
+        ("crop_x", C.c_float),
+        ("crop_y", C.c_float),
+        ("crop_width", C.c_float),
+        ("crop_height", C.c_float),
+        ("scale", C.c_float),
+        ("feather_px", C.c_float),
+    ]
+
+
 class ResolutionReport(C.Structure):
     _fields_ = [
         ("verified_fraction", C.c_float),
@@ -385,6 +409,9 @@ DEPTH_SIGNATURES = {
     "chd_refine_resolution_warp": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32,
                                    C.c_uint32, P(Warp), C.c_uint32, C.c_uint32, P(Warp),
                                    P(ResolutionReport)),
+    "chd_canvas_size": (C.c_int, P(Canvas), P(C.c_uint32), P(C.c_uint32)),
+    "chd_synthesis_geometry": (C.c_int, VP, P(DepthMap), P(SynthSource), C.c_uint32, P(Canvas),
+                               VP, VP, VP, VP, VP),
     "chd_frames_create": (C.c_int, VP, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32, C.c_float,
                           C.c_float, P(VP)),
     "chd_frames_render": (C.c_int, VP, VP),

@@ -44,7 +44,7 @@ def build_depth() -> Path:
     tmp = LIB / "_obj"
     tmp.mkdir(exist_ok=True)
     for src in ("depth_kernels.hip", "depth_zncc.hip", "proposal.hip", "scene_render.hip",
-                "align_kernels.hip", "depth_api.hip"):
+                "align_kernels.hip", "synth_kernels.hip", "depth_api.hip"):
         obj = tmp / (src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))

@@ -12,11 +12,16 @@
 //                      summing the window in the reference's row-major order
 //                      (neighbouring lanes read neighbouring target texels),
 //                      then the peak and its sub-pixel parabola
+//   k_gain_samples     the texel pairs of the photometric gain
+//   k_ransac_count     the consensus count of every RANSAC hypothesis
+//                      (align.rs:655-686), one block per hypothesis
 //   k_render_bandwidth the target through a warp at the reference plane's
 //                      sampling, blurred by a Gaussian footprint when the
 //                      target is magnified (resolution.rs:254-270, 441-465)
 //
-// The RANSAC and least-squares fits run on the host (a few thousand pairs).
+// The RANSAC samples, the 4-point and final least-squares fits and the
+// residual medians run on the host; the 300 consensus counts over a few
+// thousand pairs, which dominate RANSAC, run on the device.
 // Planes of the pyramids are stored with one replicated padding column and
 // row (pitch width + 1), so Plane::sample's clamp of x0 + 1 at the border is
 // a plain read.
@@ -220,6 +225,50 @@ __global__ void __launch_bounds__(256) k_match_patches(const float *reference, c
     }
 }
 
+// The samples of photometric_gain (align.rs:795-824): at every 16th
+// reference plane pixel (x, y) = (16 (i + 1), 16 (j + 1)) the reference
+// texel and the target sampled through the warp at ((q - 0.5) / 2); the
+// ratios, their sort and median stay on the host (the platform expf).
+__global__ void k_gain_samples(DevPlane reference, DevPlane target, DevWarp warp, uint32_t gx,
+                               uint32_t gy, float2 *samples, uint8_t *ok) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= gx || j >= gy) return;
+    const uint32_t x = 16 * (i + 1), y = 16 * (j + 1);
+    const size_t k = (size_t)j * gx + i;
+    float2 q;
+    float t;
+    const bool defined = warp_map(warp, (float)x * 2.0f + 0.5f, (float)y * 2.0f + 0.5f, q) &&
+                         plane_sample(target, (q.x - 0.5f) / 2.0f, (q.y - 0.5f) / 2.0f, t);
+    ok[k] = defined ? 1 : 0;
+    samples[k] = make_float2(reference.data[(size_t)y * reference.pitch + x], defined ? t : 0.0f);
+}
+
+// RANSAC's consensus counts (align.rs:672-683): one block per hypothesis
+// (9 f64 each, row-major), threads over the pairs (a.x, a.y, b.x, b.y); a
+// pair counts when its residual under the hypothesis is at most the
+// threshold. Counts are integers, so the order of the block's sum is free.
+__global__ void __launch_bounds__(256) k_ransac_count(const double4 *pairs, uint32_t count,
+                                                      const double *hypotheses, double threshold,
+                                                      uint32_t *counts) {
+    __shared__ uint32_t total;
+    Homography H;
+    for (int i = 0; i < 9; i++) H.h[i] = hypotheses[(size_t)blockIdx.x * 9 + i];
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    uint32_t n = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        const double4 p = pairs[i];
+        double px, py;
+        if (apply_homography(H, p.x, p.y, px, py)) {
+            const double dx = px - p.z, dy = py - p.w;
+            n += sqrt(dx * dx + dy * dy) <= threshold ? 1u : 0u;
+        }
+    }
+    atomicAdd(&total, n);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
 __global__ void k_render_bandwidth(uint32_t width, uint32_t height, DevPlane target, DevWarp base,
                                    BandwidthKernel kernel, float *out) {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -250,6 +299,24 @@ __global__ void k_render_bandwidth(uint32_t width, uint32_t height, DevPlane tar
 }
 
 }  // namespace
+
+hipError_t launch_gain_samples(hipStream_t stream, const DevPlane &reference,
+                              const DevPlane &target, const DevWarp &warp, uint32_t gx,
+                              uint32_t gy, float2 *samples, uint8_t *ok) {
+    if (gx == 0 || gy == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gain_samples, dim3((gx + 63) / 64, gy), dim3(64), 0, stream, reference,
+                       target, warp, gx, gy, samples, ok);
+    return hipGetLastError();
+}
+
+hipError_t launch_ransac_count(hipStream_t stream, const double *pairs, uint32_t count,
+                              const double *hypotheses, uint32_t hypothesis_count,
+                              double threshold, uint32_t *counts) {
+    if (hypothesis_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ransac_count, dim3(hypothesis_count), dim3(256), 0, stream,
+                       (const double4 *)pairs, count, hypotheses, threshold, counts);
+    return hipGetLastError();
+}
 
 hipError_t launch_render_bandwidth(hipStream_t stream, uint32_t width, uint32_t height,
                                    const DevPlane &target, const DevWarp &base,

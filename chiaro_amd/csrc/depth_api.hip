@@ -1587,16 +1587,40 @@ bool homography_least_squares(const Pair *pairs, size_t count, double h[9]) {
     return true;
 }
 
-// fit_homography_ransac (align.rs:644-710): 300 four-point samples from a
-// seeded LCG, then a refit on the best consensus; residuals of the final
-// inliers sorted by total_cmp
-bool homography_ransac(const std::vector<Pair> &pairs, float threshold, double h[9],
-                       std::vector<float> &residuals) {
+// Device buffers of the RANSAC consensus counts: `capacity` pairs and the
+// 300 hypotheses.
+constexpr int kRansacIterations = 300;
+struct RansacDevice {
+    hipStream_t stream = nullptr;
+    double *pairs = nullptr;       // capacity x {a.x, a.y, b.x, b.y}
+    double *hypotheses = nullptr;  // kRansacIterations x 9
+    uint32_t *counts = nullptr;    // kRansacIterations
+    size_t capacity = 0;
+};
+
+int ransac_device_alloc(Scratch &sc, hipStream_t stream, size_t capacity, RansacDevice &dev) {
+    dev.stream = stream;
+    dev.capacity = capacity;
+    CHD_CHECK(sc.alloc(&dev.pairs, 4 * std::max<size_t>(capacity, 1)));
+    CHD_CHECK(sc.alloc(&dev.hypotheses, 9 * (size_t)kRansacIterations));
+    CHD_CHECK(sc.alloc(&dev.counts, (size_t)kRansacIterations));
+    return CHD_OK;
+}
+
+// fit_homography_ransac (align.rs:644-711): 300 LCG-sampled 4-point fits
+// (host), their consensus counts over all pairs (device), the first
+// hypothesis with the most inliers, its inliers refitted by least squares
+// and re-selected under the refit; *found = false is the reference's None.
+int homography_ransac(const RansacDevice &dev, const std::vector<Pair> &pairs, float threshold,
+                      double h[9], std::vector<float> &residuals, bool &found) {
+    found = false;
     const size_t count = pairs.size();
-    if (count < 6) return false;
+    if (count < 6) return CHD_OK;
+    if (count > dev.capacity) return fail(CHD_ERR_LIMIT, "RANSAC: more pairs than the buffers hold");
     uint64_t state = 0x9E3779B97F4A7C15ull ^ (uint64_t)count;
-    std::vector<uint32_t> best, current;
-    for (int iteration = 0; iteration < 300; iteration++) {
+    std::vector<double> hypotheses(9 * (size_t)kRansacIterations, 0.0);
+    std::vector<uint8_t> fitted(kRansacIterations, 0);
+    for (int iteration = 0; iteration < kRansacIterations; iteration++) {
         uint32_t sample[4];
         for (int slot = 0; slot < 4; slot++)
             for (;;) {
@@ -1611,23 +1635,48 @@ bool homography_ransac(const std::vector<Pair> &pairs, float threshold, double h
             }
         Pair subset[4];
         for (int k = 0; k < 4; k++) subset[k] = pairs[sample[k]];
-        double hs[9];
-        if (!homography_least_squares(subset, 4, hs)) continue;
-        current.clear();
-        for (size_t i = 0; i < count; i++) {
-            double px, py;
-            if (homography_apply(hs, pairs[i].a[0], pairs[i].a[1], px, py)) {
-                const double dx = px - pairs[i].b[0], dy = py - pairs[i].b[1];
-                if (std::sqrt(dx * dx + dy * dy) <= (double)threshold) current.push_back((uint32_t)i);
-            }
-        }
-        if (current.size() > best.size()) best = current;
+        fitted[iteration] = homography_least_squares(subset, 4, &hypotheses[9 * (size_t)iteration]);
     }
-    if (best.size() < 6) return false;
+    std::vector<double> packed(4 * count);
+    for (size_t i = 0; i < count; i++) {
+        packed[4 * i] = pairs[i].a[0];
+        packed[4 * i + 1] = pairs[i].a[1];
+        packed[4 * i + 2] = pairs[i].b[0];
+        packed[4 * i + 3] = pairs[i].b[1];
+    }
+    std::vector<uint32_t> counts(kRansacIterations);
+    CHD_HIP(hipMemcpyAsync(dev.pairs, packed.data(), packed.size() * 8, hipMemcpyHostToDevice,
+                           dev.stream));
+    CHD_HIP(hipMemcpyAsync(dev.hypotheses, hypotheses.data(), hypotheses.size() * 8,
+                           hipMemcpyHostToDevice, dev.stream));
+    CHD_HIP(launch_ransac_count(dev.stream, dev.pairs, (uint32_t)count, dev.hypotheses,
+                                kRansacIterations, (double)threshold, dev.counts));
+    CHD_HIP(hipMemcpyAsync(counts.data(), dev.counts, counts.size() * 4, hipMemcpyDeviceToHost,
+                           dev.stream));
+    CHD_HIP(hipStreamSynchronize(dev.stream));
+    // the first hypothesis whose inlier count beats every earlier one
+    int best = -1;
+    uint32_t best_count = 0;
+    for (int k = 0; k < kRansacIterations; k++)
+        if (fitted[k] && counts[k] > best_count) {
+            best = k;
+            best_count = counts[k];
+        }
+    if (best < 0 || best_count < 6) return CHD_OK;
+    const double *hb = &hypotheses[9 * (size_t)best];
     std::vector<Pair> inliers;
-    for (uint32_t i : best) inliers.push_back(pairs[i]);
+    inliers.reserve(best_count);
+    for (const Pair &p : pairs) {
+        double px, py;
+        if (homography_apply(hb, p.a[0], p.a[1], px, py)) {
+            const double dx = px - p.b[0], dy = py - p.b[1];
+            if (std::sqrt(dx * dx + dy * dy) <= (double)threshold) inliers.push_back(p);
+        }
+    }
+    if (inliers.size() != best_count)
+        return fail(CHD_ERR_STATE, "RANSAC: device and host consensus counts differ");
     double hf[9];
-    if (!homography_least_squares(inliers.data(), inliers.size(), hf)) return false;
+    if (!homography_least_squares(inliers.data(), inliers.size(), hf)) return CHD_OK;
     residuals.clear();
     for (const Pair &p : pairs) {
         double px, py;
@@ -1637,37 +1686,20 @@ bool homography_ransac(const std::vector<Pair> &pairs, float threshold, double h
             if (r <= threshold) residuals.push_back(r);
         }
     }
-    if (residuals.size() < 6) return false;
+    if (residuals.size() < 6) return CHD_OK;
     std::stable_sort(residuals.begin(), residuals.end(),
                      [](float x, float y) { return total_key32(x) < total_key32(y); });
     std::memcpy(h, hf, sizeof hf);
-    return true;
+    found = true;
+    return CHD_OK;
 }
 
-// photometric_gain (align.rs:795-826): median linear-luminance ratio of
-// reference / target at every 16th plane pixel, the platform expf
-bool photometric_gain_host(const chd_view &reference, const chd_view &target, const float *points,
-                           uint32_t step, uint32_t columns, uint32_t rows, float &gain) {
-    std::vector<float> ratios;
-    const uint32_t w = reference.plane_width, h = reference.plane_height;
-    for (uint32_t y = 16; y + 16 < h; y += 16)
-        for (uint32_t x = 16; x + 16 < w; x += 16) {
-            const float rx = (float)x * 2.0f + 0.5f, ry = (float)y * 2.0f + 0.5f;
-            float q[2], t;
-            if (!host_warp_map(points, step, columns, rows, rx, ry, q)) continue;
-            if (!host_sample(target.luminance, target.plane_width, target.plane_height,
-                             (q[0] - 0.5f) / 2.0f, (q[1] - 0.5f) / 2.0f, t))
-                continue;
-            const float r = reference.luminance[(size_t)y * w + x];
-            const float lr = (expf(r) - 1.0f) / 1000.0f, lt = (expf(t) - 1.0f) / 1000.0f;
-            if (lr > 0.01f && lt > 0.01f && lr < 0.95f && lt < 0.95f) ratios.push_back(lr / lt);
-        }
-    if (ratios.size() < 16) return false;
-    std::stable_sort(ratios.begin(), ratios.end(),
-                     [](float x, float y) { return total_key32(x) < total_key32(y); });
-    gain = std::min(std::max(ratios[ratios.size() / 2], 0.25f), 4.0f);
-    return true;
-}
+// photometric_gain (align.rs:795-824): median linear-luminance ratio of
+// reference / target at every 16th plane pixel. The texel pairs are sampled
+// on the device from the level-0 planes; the ratios use the platform expf.
+struct DevicePlane;
+int photometric_gain(chd_context *c, Scratch &sc, const DevicePlane &reference,
+                     const DevicePlane &target, const chd_warp &warp, float &gain);
 
 // A padded device plane (pitch width + 1, height + 1 rows).
 struct DevicePlane {
@@ -1685,18 +1717,24 @@ struct DevicePlane {
     }
 };
 
-// Plane::pyramid (image.rs:487-498) on the device from a host plane.
-int device_pyramid(chd_context *c, Scratch &sc, const chd_view &v, uint32_t min_size,
+// A view's luminance as a padded device plane; the source may be host or
+// device memory (the copy's direction comes from the pointer).
+int device_plane(chd_context *c, Scratch &sc, const chd_view &v, DevicePlane &out) {
+    hipStream_t st = c->stream;
+    out.width = v.plane_width;
+    out.height = v.plane_height;
+    CHD_CHECK(sc.alloc(&out.data, (size_t)(out.width + 1) * (out.height + 1)));
+    CHD_HIP(hipMemcpy2DAsync(out.data, (size_t)(out.width + 1) * 4, v.luminance,
+                             (size_t)out.width * 4, (size_t)out.width * 4, out.height,
+                             hipMemcpyDefault, st));
+    CHD_HIP(launch_pad_plane(st, out.data, out.width, out.height));
+    return CHD_OK;
+}
+
+// Plane::pyramid (image.rs:487-498) on the device from a padded base plane.
+int device_pyramid(chd_context *c, Scratch &sc, const DevicePlane &base, uint32_t min_size,
                    std::vector<DevicePlane> &levels) {
     hipStream_t st = c->stream;
-    DevicePlane base;
-    base.width = v.plane_width;
-    base.height = v.plane_height;
-    CHD_CHECK(sc.alloc(&base.data, (size_t)(base.width + 1) * (base.height + 1)));
-    CHD_HIP(hipMemcpy2DAsync(base.data, (size_t)(base.width + 1) * 4, v.luminance,
-                             (size_t)base.width * 4, (size_t)base.width * 4, base.height,
-                             hipMemcpyHostToDevice, st));
-    CHD_HIP(launch_pad_plane(st, base.data, base.width, base.height));
     levels.push_back(base);
     const uint32_t floor_size = std::max(min_size, 8u);
     while (std::min(levels.back().width, levels.back().height) / 2 >= floor_size) {
@@ -1710,6 +1748,47 @@ int device_pyramid(chd_context *c, Scratch &sc, const chd_view &v, uint32_t min_
         CHD_HIP(launch_pad_plane(st, next.data, next.width, next.height));
         levels.push_back(next);
     }
+    return CHD_OK;
+}
+
+int photometric_gain(chd_context *c, Scratch &sc, const DevicePlane &reference,
+                     const DevicePlane &target, const chd_warp &warp, float &gain) {
+    hipStream_t st = c->stream;
+    gain = 1.0f;
+    uint32_t gx = 0, gy = 0;
+    while (16 * (gx + 1) + 16 < reference.width) gx++;
+    while (16 * (gy + 1) + 16 < reference.height) gy++;
+    const size_t n = (size_t)gx * gy, nodes = (size_t)warp.columns * warp.rows;
+    if (n == 0) return CHD_OK;
+    float2 *d_samples, *d_points;
+    uint8_t *d_ok;
+    CHD_CHECK(sc.alloc(&d_samples, n));
+    CHD_CHECK(sc.alloc(&d_ok, n));
+    CHD_CHECK(sc.alloc(&d_points, nodes));
+    CHD_HIP(hipMemcpyAsync(d_points, warp.points, nodes * 8, hipMemcpyHostToDevice, st));
+    DevWarp dw;
+    dw.points = d_points;
+    dw.step = warp.step;
+    dw.columns = warp.columns;
+    dw.rows = warp.rows;
+    dw.inverse_step = (warp.step & (warp.step - 1)) == 0 ? 1.0f / (float)warp.step : 0.0f;
+    CHD_HIP(launch_gain_samples(st, reference.view(), target.view(), dw, gx, gy, d_samples, d_ok));
+    std::vector<float> samples(2 * n);
+    std::vector<uint8_t> ok(n);
+    CHD_HIP(hipMemcpyAsync(samples.data(), d_samples, n * 8, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipMemcpyAsync(ok.data(), d_ok, n, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
+    std::vector<float> ratios;
+    for (size_t k = 0; k < n; k++) {  // rows of x, as the reference walks them
+        if (!ok[k]) continue;
+        const float r = samples[2 * k], t = samples[2 * k + 1];
+        const float lr = (expf(r) - 1.0f) / 1000.0f, lt = (expf(t) - 1.0f) / 1000.0f;
+        if (lr > 0.01f && lt > 0.01f && lr < 0.95f && lt < 0.95f) ratios.push_back(lr / lt);
+    }
+    if (ratios.size() < 16) return CHD_OK;
+    std::stable_sort(ratios.begin(), ratios.end(),
+                     [](float x, float y) { return total_key32(x) < total_key32(y); });
+    gain = std::min(std::max(ratios[ratios.size() / 2], 0.25f), 4.0f);
     return CHD_OK;
 }
 
@@ -1836,10 +1915,13 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
     };
     double correction[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     std::vector<float> finest;
+    DevicePlane reference0, target0;
+    CHD_CHECK(device_plane(c, sc, *reference, reference0));
+    CHD_CHECK(device_plane(c, sc, *target, target0));
     if (o->refine) {
         std::vector<DevicePlane> rp, tp;
-        CHD_CHECK(device_pyramid(c, sc, *reference, 96, rp));
-        CHD_CHECK(device_pyramid(c, sc, *target, 48, tp));
+        CHD_CHECK(device_pyramid(c, sc, reference0, 96, rp));
+        CHD_CHECK(device_pyramid(c, sc, target0, 48, tp));
         double magnification;
         {
             const double cx = (double)width / 2.0, cy = (double)height / 2.0, h = 64.0;
@@ -1870,6 +1952,8 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
         CHD_CHECK(sc.alloc(&d_shift, 2 * capacity));
         CHD_CHECK(sc.alloc(&d_score, capacity));
         CHD_CHECK(sc.alloc(&d_valid, capacity));
+        RansacDevice ransac;
+        CHD_CHECK(ransac_device_alloc(sc, st, capacity, ransac));
         float *unpadded;
         CHD_CHECK(sc.alloc(&unpadded, finest_points));
         for (int level = (int)rp.size() - 1; level >= 0; level--) {
@@ -1931,7 +2015,9 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
             chd_level_report *lr = report->level_count < CHD_MAX_ALIGN_LEVELS
                                        ? &report->levels[report->level_count++]
                                        : nullptr;
-            if (homography_ransac(pairs, threshold, update, residuals)) {
+            bool fitted = false;
+            CHD_CHECK(homography_ransac(ransac, pairs, threshold, update, residuals, fitted));
+            if (fitted) {
                 mat3_mul(correction, update, correction);
                 if (lr) {
                     lr->scale = scale * 2;
@@ -1984,11 +2070,7 @@ int chd_align_module(chd_context *c, const chd_view *reference, const chd_view *
         report->status = CHD_ALIGN_STATUS_REFINED;
     else if (report->inliers > 0) report->status = CHD_ALIGN_STATUS_WEAK;
     else report->status = CHD_ALIGN_STATUS_NO_SUPPORT;
-    float gain;
-    report->gain = photometric_gain_host(*reference, *target, warp->points, warp->step,
-                                         warp->columns, warp->rows, gain)
-                       ? gain
-                       : 1.0f;
+    CHD_CHECK(photometric_gain(c, sc, reference0, target0, *warp, report->gain));
     return CHD_OK;
 }
 
@@ -2109,9 +2191,11 @@ int chd_refine_resolution_warp(chd_context *c, const float *reference, uint32_t 
     CHD_CHECK(sc.alloc(&d_rendered, plane_points));
     CHD_CHECK(sc.alloc(&tp.data, (size_t)(tw + 1) * (th + 1)));
     CHD_CHECK(sc.alloc(&d_base, base_nodes));
-    CHD_HIP(hipMemcpyAsync(d_ref, reference, plane_points * 4, hipMemcpyHostToDevice, st));
+    // the planes may be host or device memory: the copies' direction comes
+    // from the pointers
+    CHD_HIP(hipMemcpyAsync(d_ref, reference, plane_points * 4, hipMemcpyDefault, st));
     CHD_HIP(hipMemcpy2DAsync(tp.data, (size_t)(tw + 1) * 4, target, (size_t)tw * 4, (size_t)tw * 4,
-                             th, hipMemcpyHostToDevice, st));
+                             th, hipMemcpyDefault, st));
     CHD_HIP(launch_pad_plane(st, tp.data, tw, th));
     CHD_HIP(hipMemcpyAsync(d_base, bw.points, base_nodes * 8, hipMemcpyHostToDevice, st));
     DevWarp dw;
@@ -2271,6 +2355,164 @@ int chd_refine_resolution_warp(chd_context *c, const float *reference, uint32_t 
     report->supported_fraction = (float)supported / denominator;
     report->median_correction_px = corrections.empty() ? 0.0f : corrections[corrections.size() / 2];
     report->mean_confidence = confidence_sum / (float)std::max<size_t>(corrections.size(), 1);
+    return CHD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ synthesis depth/warp stage
+
+extern "C++" {
+namespace {
+
+// `as usize` of an f32 (NaN and negatives to 0, saturating)
+uint64_t usize_of(float f) {
+    if (!(f > 0.0f)) return 0;
+    if (f >= 18446744073709551616.0f) return UINT64_MAX;
+    return (uint64_t)f;
+}
+
+// band_touches_module (synth.rs:1688-1721) for canvas rows start..end
+bool band_touches(const chd_synth_source &s, uint64_t start, uint64_t end, const chd_canvas &cv) {
+    const chd_warp &w = s.warp;
+    const float margin = (float)((uint64_t)w.step * 2) * std::fmax(s.magnification, 1.0f);
+    const uint64_t rows[3] = {start, (start + end) / 2, end > 0 ? end - 1 : 0};
+    for (uint64_t v : rows) {
+        const float ry = cv.crop_y + ((float)v + 0.5f) / cv.scale - 0.5f;
+        const uint64_t samples = std::max<uint64_t>(usize_of(cv.crop_width / (float)w.step), 8) + 1;
+        for (uint64_t i = 0; i <= samples; i++) {
+            const float rx = cv.crop_x + ((float)i / (float)samples) * (cv.crop_width - 1.0f);
+            float q[2];
+            if (host_warp_map(w.points, w.step, w.columns, w.rows, rx, ry, q) && q[0] >= -margin &&
+                q[1] >= -margin && q[0] <= (float)s.mosaic_width + margin &&
+                q[1] <= (float)s.mosaic_height + margin)
+                return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace
+}  // extern "C++"
+
+extern "C" {
+
+int chd_canvas_size(const chd_canvas *cv, uint32_t *width, uint32_t *height) {
+    if (!cv || !width || !height) return fail(CHD_ERR_ARGUMENT, "canvas_size: null argument");
+    // synth.rs:611-620
+    if (cv->scale != cv->scale || cv->scale <= 0.0f || cv->crop_width < 1.0f ||
+        cv->crop_height < 1.0f)
+        return fail(CHD_ERR_ARGUMENT, "invalid canvas geometry");
+    const uint64_t w = usize_of(std::fmax(std::round(cv->crop_width * cv->scale), 1.0f));
+    const uint64_t h = usize_of(std::fmax(std::round(cv->crop_height * cv->scale), 1.0f));
+    if (w > (1u << 20) || h > (1u << 20)) return fail(CHD_ERR_LIMIT, "canvas above 2^20 pixels a side");
+    *width = (uint32_t)w;
+    *height = (uint32_t)h;
+    return CHD_OK;
+}
+
+int chd_synthesis_geometry(chd_context *c, const chd_depth_map *map,
+                           const chd_synth_source *sources, uint32_t count,
+                           const chd_canvas *cv, float *points, float *feather,
+                           float *local_confidence, float *base_weight, float *focus_weight) {
+    if (!c || !sources || !cv) return fail(CHD_ERR_ARGUMENT, "synthesis_geometry: null argument");
+    if (count == 0) return fail(CHD_ERR_ARGUMENT, "no modules to synthesise from");
+    for (uint32_t s = 0; s < count; s++) {
+        const chd_warp &w = sources[s].warp;
+        if (!w.points || w.step == 0 || w.columns == 0 || w.rows == 0 ||
+            sources[s].mosaic_width == 0 || sources[s].mosaic_height == 0)
+            return fail(CHD_ERR_ARGUMENT, "synthesis_geometry: empty warp or mosaic");
+    }
+    if (map && (!map->depth || !map->confidence))
+        return fail(CHD_ERR_ARGUMENT, "synthesis_geometry: depth map without arrays");
+    uint32_t width, height;
+    CHD_CHECK(chd_canvas_size(cv, &width, &height));
+    CHD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t plane = (size_t)width * height, total = plane * count;
+    const uint32_t bands = (height + kSynthBandRows - 1) / kSynthBandRows;
+    std::vector<uint8_t> touch((size_t)bands * count);
+    for (uint32_t b = 0; b < bands; b++) {
+        const uint64_t start = (uint64_t)b * kSynthBandRows;
+        const uint64_t end = std::min<uint64_t>(start + kSynthBandRows, height);
+        for (uint32_t s = 0; s < count; s++)
+            touch[(size_t)b * count + s] = band_touches(sources[s], start, end, *cv) ? 1 : 0;
+    }
+    Scratch sc;
+    std::vector<SynthSourceDev> dev(count);
+    for (uint32_t s = 0; s < count; s++) {
+        const chd_synth_source &src = sources[s];
+        const size_t nodes = (size_t)src.warp.columns * src.warp.rows;
+        float2 *pts;
+        CHD_CHECK(sc.alloc(&pts, nodes));
+        CHD_HIP(hipMemcpyAsync(pts, src.warp.points, nodes * 8, hipMemcpyHostToDevice, st));
+        float *conf = nullptr;
+        if (src.warp.confidence) {
+            CHD_CHECK(sc.alloc(&conf, nodes));
+            CHD_HIP(hipMemcpyAsync(conf, src.warp.confidence, nodes * 4, hipMemcpyHostToDevice, st));
+        }
+        SynthSourceDev &d = dev[s];
+        d.warp.points = pts;
+        d.warp.step = src.warp.step;
+        d.warp.columns = src.warp.columns;
+        d.warp.rows = src.warp.rows;
+        d.warp.inverse_step =
+            (src.warp.step & (src.warp.step - 1)) == 0 ? 1.0f / (float)src.warp.step : 0.0f;
+        d.confidence = conf;
+        d.mosaic_width = src.mosaic_width;
+        d.mosaic_height = src.mosaic_height;
+        d.magnification = src.magnification;
+        d.source_confidence = src.confidence;
+        d.reference = src.reference;
+        d.has_focus_distance = src.has_focus_distance;
+        d.focus_distance = src.focus_distance;
+    }
+    SynthSourceDev *d_sources;
+    uint8_t *d_touch;
+    CHD_CHECK(sc.alloc(&d_sources, count));
+    CHD_CHECK(sc.alloc(&d_touch, touch.size()));
+    CHD_HIP(hipMemcpyAsync(d_sources, dev.data(), count * sizeof(SynthSourceDev),
+                           hipMemcpyHostToDevice, st));
+    CHD_HIP(hipMemcpyAsync(d_touch, touch.data(), touch.size(), hipMemcpyHostToDevice, st));
+    DepthMapDev dm{};
+    if (map) {
+        const size_t nodes = (size_t)map->columns * map->rows;
+        double *depth;
+        float *conf;
+        CHD_CHECK(sc.alloc(&depth, nodes));
+        CHD_CHECK(sc.alloc(&conf, nodes));
+        CHD_HIP(hipMemcpyAsync(depth, map->depth, nodes * 8, hipMemcpyHostToDevice, st));
+        CHD_HIP(hipMemcpyAsync(conf, map->confidence, nodes * 4, hipMemcpyHostToDevice, st));
+        dm.depth = depth;
+        dm.confidence = conf;
+        dm.columns = map->columns;
+        dm.rows = map->rows;
+        dm.step = map->step;
+    }
+    SynthCanvasDev canvas;
+    canvas.crop_x = cv->crop_x;
+    canvas.crop_y = cv->crop_y;
+    canvas.scale = cv->scale;
+    canvas.feather_scale = std::fmax(cv->feather_px, 1.0f);
+    canvas.width = width;
+    canvas.height = height;
+    float2 *d_points = nullptr;
+    float *d_feather = nullptr, *d_local = nullptr, *d_base = nullptr, *d_focus = nullptr;
+    if (points) CHD_CHECK(sc.alloc(&d_points, total));
+    if (feather) CHD_CHECK(sc.alloc(&d_feather, total));
+    if (local_confidence) CHD_CHECK(sc.alloc(&d_local, total));
+    if (base_weight) CHD_CHECK(sc.alloc(&d_base, total));
+    if (focus_weight) CHD_CHECK(sc.alloc(&d_focus, total));
+    CHD_HIP(launch_synthesis_geometry(st, canvas, dm, d_sources, count, d_touch, d_points,
+                                      d_feather, d_local, d_base, d_focus));
+    if (points) CHD_HIP(hipMemcpyAsync(points, d_points, total * 8, hipMemcpyDeviceToHost, st));
+    if (feather) CHD_HIP(hipMemcpyAsync(feather, d_feather, total * 4, hipMemcpyDeviceToHost, st));
+    if (local_confidence)
+        CHD_HIP(hipMemcpyAsync(local_confidence, d_local, total * 4, hipMemcpyDeviceToHost, st));
+    if (base_weight) CHD_HIP(hipMemcpyAsync(base_weight, d_base, total * 4, hipMemcpyDeviceToHost, st));
+    if (focus_weight)
+        CHD_HIP(hipMemcpyAsync(focus_weight, d_focus, total * 4, hipMemcpyDeviceToHost, st));
+    CHD_HIP(hipStreamSynchronize(st));
     return CHD_OK;
 }
 

@@ -359,6 +359,28 @@ struct DevWarp {
     float inverse_step;  // 1 / step when step is a power of two (then exact), else 0
 };
 
+// The synthesis stage's inputs on the device: one source (SynthSource's
+// fields that stage reads, the warp's confidence beside its points), the
+// depth map's node arrays and the canvas.
+constexpr uint32_t kSynthBandRows = 64;  // png16.rs:29: rows per output band
+struct SynthSourceDev {
+    DevWarp warp;
+    const float *confidence;  // warp confidence per node; null = none
+    uint32_t mosaic_width, mosaic_height;
+    float magnification, source_confidence;
+    int32_t reference, has_focus_distance;
+    double focus_distance;
+};
+struct DepthMapDev {
+    const double *depth;  // NaN = None; null = no map
+    const float *confidence;
+    uint32_t columns, rows, step;
+};
+struct SynthCanvasDev {
+    float crop_x, crop_y, scale, feather_scale;  // feather_scale = max(feather_px, 1)
+    uint32_t width, height;
+};
+
 // Plane::sample (image.rs:451-466) on a padded row-major plane.
 __device__ __forceinline__ bool plane_sample(const DevPlane &p, float x, float y, float &out) {
     if (!(x >= 0.0f && y >= 0.0f && x <= (float)(p.width - 1) && y <= (float)(p.height - 1)))

@@ -245,6 +245,18 @@ hipError_t launch_match_patches(hipStream_t stream, const float *reference, cons
                                 uint32_t width, uint32_t height, const uint32_t *rx,
                                 const uint32_t *ry, uint32_t count, uint32_t size, uint32_t radius,
                                 int coverage, float *shift, float *score, uint8_t *valid);
+// photometric_gain's samples: (reference texel, target sample) at the
+// gx x gy grid of every 16th reference plane pixel, ok = 0 where the warp or
+// the sample is undefined (planes padded, level 0).
+hipError_t launch_gain_samples(hipStream_t stream, const struct DevPlane &reference,
+                              const struct DevPlane &target, const struct DevWarp &warp,
+                              uint32_t gx, uint32_t gy, float2 *samples, uint8_t *ok);
+// RANSAC consensus: counts[k] = pairs (count x {a.x, a.y, b.x, b.y} f64,
+// device) whose residual under hypothesis k (9 f64, row-major, device) is
+// at most `threshold`.
+hipError_t launch_ransac_count(hipStream_t stream, const double *pairs, uint32_t count,
+                              const double *hypotheses, uint32_t hypothesis_count,
+                              double threshold, uint32_t *counts);
 // sample_at_reference_bandwidth of the target through `base` at every pixel
 // of a width x height reference plane (resolution.rs:254-270): out[y][x] is
 // the sample at ((q + 0.5) / 2 - 0.5) for q = base.map(2x + 0.5, 2y + 0.5),
@@ -257,6 +269,16 @@ struct BandwidthKernel {
 hipError_t launch_render_bandwidth(hipStream_t stream, uint32_t width, uint32_t height,
                                    const struct DevPlane &target, const struct DevWarp &base,
                                    const BandwidthKernel &kernel, float *out);
+
+// The synthesis stage: per canvas pixel and source the warp point, feather,
+// local confidence, base weight and focus weight (outputs [source][row][col];
+// any may be null). `touch` holds a flag per 64-row band and source.
+hipError_t launch_synthesis_geometry(hipStream_t stream, const struct SynthCanvasDev &canvas,
+                                     const struct DepthMapDev &map,
+                                     const struct SynthSourceDev *sources, uint32_t count,
+                                     const uint8_t *touch, float2 *points, float *feather,
+                                     float *local_confidence, float *base_weight,
+                                     float *focus_weight);
 
 // ------------------------------------------------ synthetic capture generator
 

@@ -577,9 +577,10 @@ void chd_default_align_options(chd_align_options *out);
  * points and confidence of ceil(w / step) + 1 by ceil(h / step) + 1 nodes)
  * and the report. reference == target (the same view) is the reference's
  * own identity alignment. Both views need cameras (the uncalibrated
- * nominal-focal start is not supported: CHD_ERR_ARGUMENT). The pyramids,
- * rendering and correlations run on the device; the RANSAC and least
- * squares fits (a few hundred to a few thousand pairs) on the host. */
+ * nominal-focal start is not supported: CHD_ERR_ARGUMENT). The luminance
+ * planes may be in host or device memory. The pyramids, rendering,
+ * correlations, RANSAC consensus counts and photometric samples run on the
+ * device; the RANSAC samples, least-squares fits and medians on the host. */
 int chd_align_module(chd_context *context, const chd_view *reference, const chd_view *target,
                      const chd_align_options *options, chd_warp *warp,
                      chd_alignment_report *report);
@@ -607,8 +608,8 @@ typedef struct chd_resolution_report {
  * local 12x12 NCC on a 32 px grid of the reference raster (width x height)
  * between the reference plane and the target plane rendered through `base`
  * at the reference's bandwidth, neighbourhood-median checks and two
- * completion passes. Planes are host arrays (row-major f32); `base` is a
- * host warp. `out` gets step 32, columns = ceil(width / 32) + 1,
+ * completion passes. Planes are row-major f32 (pitch = width) in host or
+ * device memory; `base` is a host warp. `out` gets step 32, columns = ceil(width / 32) + 1,
  * rows = ceil(height / 32) + 1; its points (2 * columns * rows f32) and
  * confidence (columns * rows f32) arrays are the caller's. */
 int chd_refine_resolution_warp(chd_context *context, const float *reference,
@@ -616,6 +617,58 @@ int chd_refine_resolution_warp(chd_context *context, const float *reference,
                                const float *target, uint32_t target_width,
                                uint32_t target_height, const chd_warp *base, uint32_t width,
                                uint32_t height, chd_warp *out, chd_resolution_report *report);
+
+/* The synthesis consumer's depth and warp stage. synthesize
+ * (crates/chiaro-fusion/src/synth.rs:591-725) walks the output canvas and,
+ * per pixel, reads the depth map (DenseDepthMap::sample_nearest,
+ * depth.rs:152-159) and every source's warp before it samples the mosaics;
+ * these entries compute that part (the demosaic, colour and blending that
+ * follow are outside this library). */
+
+/* A synthesis source (SynthSource, synth.rs:346-370): the fields this stage
+ * reads. Sources are the `usable` list of synth.rs:601-605, in order. */
+typedef struct chd_synth_source {
+    chd_warp warp;          /* alignment.warp (host arrays) */
+    uint32_t mosaic_width;  /* mosaic raster */
+    uint32_t mosaic_height;
+    float magnification;
+    float confidence;
+    int32_t reference;
+    int32_t has_focus_distance;
+    double focus_distance;  /* read when has_focus_distance */
+} chd_synth_source;
+
+/* CropWindow (synth.rs:106-112), the canvas scale and
+ * SynthOptions::feather_px. */
+typedef struct chd_canvas {
+    float crop_x;
+    float crop_y;
+    float crop_width;
+    float crop_height;
+    float scale;
+    float feather_px;
+} chd_canvas;
+
+/* The canvas size synthesize renders (synth.rs:619-620):
+ * round(crop * scale), at least 1. CHD_ERR_ARGUMENT on the reference's
+ * "invalid canvas geometry" (NaN or non-positive scale, crop below 1). */
+int chd_canvas_size(const chd_canvas *canvas, uint32_t *width, uint32_t *height);
+
+/* For every canvas pixel (u, v) and source s, in [s][v][u] order
+ * (width x height from chd_canvas_size): the target point q =
+ * warp.map(rx, ry) (points, 2 f32), the border feather, the warp's local
+ * confidence, base_weight = feather * local * m * m * confidence and the
+ * close-side focus weight from the depth map's nearest node
+ * (focus_consistency_weight, synth.rs:1469-1491), for the sources that
+ * contribute there (synth.rs:640-725: the row band of 64 canvas rows may
+ * reach the module, q is defined and inside the mosaic, feather > 0 and
+ * local confidence > 0). A source that does not contribute gets NaN points
+ * and zeros. `map` may be NULL (no depth map: every focus weight is 1).
+ * Host arrays; an output pointer may be NULL to skip it. */
+int chd_synthesis_geometry(chd_context *context, const chd_depth_map *map,
+                           const chd_synth_source *sources, uint32_t count,
+                           const chd_canvas *canvas, float *points, float *feather,
+                           float *local_confidence, float *base_weight, float *focus_weight);
 
 /* Synthetic captures rendered on the GPU. No counterpart in the reference
  * (its only fakes are gradient mock LRIs, crates/chiaro/src/mock.rs:62):

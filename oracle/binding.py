@@ -99,6 +99,11 @@ SIGNATURES = {
     "or_default_align_options": (None, P(_abi.AlignOptions)),
     "or_align_module": (C.c_int, P(_abi.View), P(_abi.View), P(_abi.AlignOptions), P(_abi.Warp),
                         P(_abi.AlignmentReport)),
+    "or_sample_nearest": (C.c_int, P(_abi.DepthMap), C.c_float, C.c_float, P(C.c_size_t)),
+    "or_band_touches": (C.c_int, P(_abi.SynthSource), C.c_size_t, C.c_size_t, P(_abi.Canvas)),
+    "or_canvas_size": (C.c_int, P(_abi.Canvas), P(C.c_uint32), P(C.c_uint32)),
+    "or_synthesis_geometry": (C.c_int, P(_abi.DepthMap), P(_abi.SynthSource), C.c_uint32,
+                              P(_abi.Canvas), VP, VP, VP, VP, VP),
     "or_refine_resolution_warp": (C.c_int, VP, C.c_uint32, C.c_uint32, VP, C.c_uint32,
                                   C.c_uint32, P(_abi.Warp), C.c_uint32, C.c_uint32, P(_abi.Warp),
                                   P(_abi.ResolutionReport)),

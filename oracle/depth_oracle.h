@@ -181,6 +181,13 @@ int or_refine_resolution_warp(const float *reference, uint32_t rw, uint32_t rh,
                               const float *target, uint32_t tw, uint32_t th, const chd_warp *base,
                               uint32_t width, uint32_t height, chd_warp *out,
                               chd_resolution_report *report);
+int or_sample_nearest(const chd_depth_map *map, float x, float y, size_t *index);
+int or_band_touches(const chd_synth_source *source, size_t start, size_t end,
+                    const chd_canvas *canvas);
+int or_canvas_size(const chd_canvas *canvas, uint32_t *width, uint32_t *height);
+int or_synthesis_geometry(const chd_depth_map *map, const chd_synth_source *sources,
+                          uint32_t count, const chd_canvas *canvas, float *points, float *feather,
+                          float *local_confidence, float *base_weight, float *focus_weight);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,9 @@ LAYOUT = {
     "chd_level_report": (_abi.LevelReport, ["inliers", "median_residual_px"]),
     "chd_alignment_report": (_abi.AlignmentReport, ["coverage", "levels", "inliers",
                                                     "residual_p90_px", "accepted", "gain"]),
+    "chd_synth_source": (_abi.SynthSource, ["mosaic_width", "magnification", "reference",
+                                            "focus_distance"]),
+    "chd_canvas": (_abi.Canvas, ["crop_height", "feather_px"]),
     "chd_resolution_report": (_abi.ResolutionReport, ["supported_fraction", "mean_confidence"]),
     "chd_synthetic_scene": (_abi.SceneDesc, ["plane_count", "inverse_depth_max",
                                              "texture_cell_px"]),

@@ -91,3 +91,73 @@ def test_full_size_config4_modules_bit_exact(gpu_context, oracle_lib):
             _parity(gpu_context, oracle_lib, f"config4 {rig.names[index]}", args)
     finally:
         frames.close()
+
+
+def test_align_then_refine_through_the_python_mirror(gpu_context, oracle_lib):
+    """The reference's chain (pipeline.rs:1185): align_module, then
+    refine_resolution_warp through the warp it produced, via chiaro_amd.align;
+    both steps equal the oracle's."""
+    from chiaro_amd import align
+    from test_align_oracle import rotated_pair
+    from test_gpu_align import _options, _run
+
+    cap, _ = rotated_pair(832, 624, 660.0)
+    ref, tgt = cap.modules[0], cap.modules[1]
+    got = align.align_module(gpu_context, cap.modules, 0, 1)
+    want = _run(oracle_lib.or_align_module, cap, 1, _options(gpu_context.lib))
+    assert_bitwise("aligned warp", got.warp.points, want[0])
+    assert got.report["status"] == "refined" and got.accepted
+    ident = align.align_module(gpu_context, cap.modules, 0, 0)
+    assert ident.report["status"] == "reference"
+    rr = align.refine_resolution_warp(gpu_context, ref.luminance, tgt.luminance, got.warp,
+                                      ref.width, ref.height)
+    w = oracle_refine(oracle_lib, ref.luminance, tgt.luminance, want[0], want[1], ref.width,
+                      ref.height)
+    assert_bitwise("resolution warp points", rr.warp.points, w[0])
+    assert_bitwise("resolution warp confidence", rr.warp.confidence, w[1])
+    assert rr.report["verified_fraction"] == w[2].verified_fraction
+    _say(f"align -> resolution: {got.report['status']}, verified "
+         f"{rr.report['verified_fraction']:.3f}")
+
+
+def test_device_planes_equal_host_planes(gpu_context, oracle_lib):
+    """Planes already in HBM (chd_frames) give the host planes' alignment and
+    resolution warp bit for bit, and both equal the oracle."""
+    from chiaro_amd import align
+    from test_align_oracle import rotation_y
+    from test_gpu_align import _options, _run
+
+    rig = scene.rig_config1(seed=9, width=832, height=624, focal_px=660.0, planes=32)
+    frames = scene.Frames(gpu_context, rig)
+    try:
+        frames.render(9)
+        device = frames.planes()
+        host = [frames.read(i)[1] for i in range(len(rig.modules))]
+        m = rig.modules[2]
+        turned = scene._module(m.focal_px, m.width, m.height, tuple(m.center[:]),
+                               rotation_y(0.3))
+        cams = [scene.camera_for(rig.modules[0]), scene.camera_for(turned)]
+        mods = {kind: [scene.ModuleInput(name=str(i), luminance=lum[k], width=m.width,
+                                         height=m.height, camera=cams[i],
+                                         warp_points=scene.identity_warp(m.width, m.height),
+                                         warp_step=32)
+                       for i, k in enumerate((0, 2))]
+                for kind, lum in (("device", device), ("host", host))}
+        got = align.align_module(gpu_context, mods["device"], 0, 1)
+        want = align.align_module(gpu_context, mods["host"], 0, 1)
+        assert_bitwise("device-plane alignment", got.warp.points, want.warp.points)
+        assert got.gain == want.gain and got.report["status"] == want.report["status"]
+        cap = scene.Capture(modules=mods["host"], reference_index=0, options=None)
+        oracle = _run(oracle_lib.or_align_module, cap, 1, _options(gpu_context.lib))
+        assert_bitwise("alignment vs oracle", got.warp.points, oracle[0])
+        rd = align.refine_resolution_warp(gpu_context, device[0], device[2], got.warp,
+                                          m.width, m.height)
+        rh = align.refine_resolution_warp(gpu_context, host[0], host[2], want.warp,
+                                          m.width, m.height)
+        assert_bitwise("device-plane resolution warp", rd.warp.points, rh.warp.points)
+        assert_bitwise("device-plane resolution confidence", rd.warp.confidence,
+                       rh.warp.confidence)
+        _say(f"device planes: {got.report['status']}, gain {got.gain:.4f}, verified "
+             f"{rd.report['verified_fraction']:.3f}")
+    finally:
+        frames.close()
