@@ -1717,17 +1717,27 @@ struct DevicePlane {
     }
 };
 
+// A plane of pitch `width` in host or device memory into a padded device
+// plane: one contiguous copy (its direction comes from the pointer; a
+// pitched copy from pageable host memory goes row by row and is many times
+// slower) and a device-side repack that writes the padding too.
+int upload_padded(hipStream_t st, Scratch &sc, const float *src, uint32_t width, uint32_t height,
+                  float *dst) {
+    float *staging;
+    CHD_CHECK(sc.alloc(&staging, (size_t)width * height));
+    CHD_HIP(hipMemcpyAsync(staging, src, (size_t)width * height * 4, hipMemcpyDefault, st));
+    CHD_HIP(launch_pitch_copy(st, staging, width, height, dst));
+    return CHD_OK;
+}
+
 // A view's luminance as a padded device plane; the source may be host or
-// device memory (the copy's direction comes from the pointer).
+// device memory.
 int device_plane(chd_context *c, Scratch &sc, const chd_view &v, DevicePlane &out) {
     hipStream_t st = c->stream;
     out.width = v.plane_width;
     out.height = v.plane_height;
     CHD_CHECK(sc.alloc(&out.data, (size_t)(out.width + 1) * (out.height + 1)));
-    CHD_HIP(hipMemcpy2DAsync(out.data, (size_t)(out.width + 1) * 4, v.luminance,
-                             (size_t)out.width * 4, (size_t)out.width * 4, out.height,
-                             hipMemcpyDefault, st));
-    CHD_HIP(launch_pad_plane(st, out.data, out.width, out.height));
+    CHD_CHECK(upload_padded(st, sc, v.luminance, out.width, out.height, out.data));
     return CHD_OK;
 }
 
@@ -2194,9 +2204,7 @@ int chd_refine_resolution_warp(chd_context *c, const float *reference, uint32_t 
     // the planes may be host or device memory: the copies' direction comes
     // from the pointers
     CHD_HIP(hipMemcpyAsync(d_ref, reference, plane_points * 4, hipMemcpyDefault, st));
-    CHD_HIP(hipMemcpy2DAsync(tp.data, (size_t)(tw + 1) * 4, target, (size_t)tw * 4, (size_t)tw * 4,
-                             th, hipMemcpyDefault, st));
-    CHD_HIP(launch_pad_plane(st, tp.data, tw, th));
+    CHD_CHECK(upload_padded(st, sc, target, tw, th, tp.data));
     CHD_HIP(hipMemcpyAsync(d_base, bw.points, base_nodes * 8, hipMemcpyHostToDevice, st));
     DevWarp dw;
     dw.points = d_base;

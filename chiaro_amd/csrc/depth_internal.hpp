@@ -188,6 +188,10 @@ hipError_t launch_final_map(hipStream_t stream, const chd_node_depth *field,
 // Replicates the last column and row of a plane stored with pitch width + 1
 // and height + 1 rows.
 hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height);
+// src (pitch width, device) into a padded plane dst (pitch width + 1,
+// height + 1 rows): the replicated column and row written with it.
+hipError_t launch_pitch_copy(hipStream_t stream, const float *src, uint32_t width, uint32_t height,
+                             float *dst);
 
 // Sums of a 2x2 cell of u16 samples index the luminance table.
 constexpr uint32_t kLuminanceTableSize = 4u * 65535u + 1u;

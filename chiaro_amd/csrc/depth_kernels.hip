@@ -1186,6 +1186,22 @@ __global__ void k_pad_plane(float *plane, uint32_t width, uint32_t height) {
     }
 }
 
+// A plane of pitch `width` into a padded one (pitch width + 1, height + 1
+// rows), padding included: one thread per stored element.
+__global__ void k_pitch_copy(const float *src, uint32_t width, uint32_t height, float *dst) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x > width || y > height) return;
+    const uint32_t sx = x < width ? x : width - 1, sy = y < height ? y : height - 1;
+    dst[(size_t)y * (width + 1) + x] = src[(size_t)sy * width + sx];
+}
+
+hipError_t launch_pitch_copy(hipStream_t stream, const float *src, uint32_t width, uint32_t height,
+                             float *dst) {
+    hipLaunchKernelGGL(k_pitch_copy, dim3((width + 1 + 255) / 256, height + 1), dim3(256), 0, stream,
+                       src, width, height, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_pad_plane(hipStream_t stream, float *plane, uint32_t width, uint32_t height) {
     const uint32_t n = (width + 1 > height ? width + 1 : height);
     hipLaunchKernelGGL(k_pad_plane, dim3((n + 255) / 256), dim3(256), 0, stream, plane, width,

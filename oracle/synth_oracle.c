@@ -57,7 +57,9 @@ static float warp_confidence_at(const chd_warp *w, float x, float y) {
 }
 
 /* focus_consistency_weight (synth.rs:1469-1491). relative.powi(4) is
- * evaluated as two squarings, the form LLVM gives a constant exponent. */
+ * evaluated as two squarings: LLVM lowers a constant exponent by repeated
+ * squaring, and compiler-rt's __powisf2 (binary exponentiation) rounds the
+ * same two products for exponent 4. */
 static float focus_consistency(int has_depth, double depth, float confidence,
                                const chd_synth_source *s) {
     if (s->reference || s->magnification <= 1.05f) return 1.0f;
