@@ -505,6 +505,7 @@ def bench_batch(args) -> int:
         for s in sessions[1:]:
             s.set_tuning(**tuning)
     batch = Batch(ctx, sessions)
+    one_sweep = Batch(ctx, sessions, one_sweep=True)
     queued = Batch(ctx, sessions, persistent=True)
     stages = _abi.CHD_STAGE_ALL if args.full_path else _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM
 
@@ -524,6 +525,8 @@ def bench_batch(args) -> int:
     zncc = sum(s.work()["cost_zncc"] for s in sessions)
     queued.run(stages)
     queued_ms = timed(lambda: queued.run(stages), max(1, args.steps // 2))
+    one_sweep.run(stages)
+    one_sweep_ms = timed(lambda: one_sweep.run(stages), max(1, args.steps // 2))
 
     def sequential():
         for s in sessions:
@@ -565,6 +568,7 @@ def bench_batch(args) -> int:
             "tuning": tuning,
         },
         "persistent_queue_ms_per_step": round(queued_ms, 3),
+        "one_sweep_ms_per_step": round(one_sweep_ms, 3),
         "sequential_ms_per_step": round(seq_ms, 3),
         "batched_over_sequential": round(seq_ms / ms, 3),
         "cost_ms_per_step": round(cost_ms, 3),
@@ -572,6 +576,7 @@ def bench_batch(args) -> int:
         "cpu_baseline": None,
     }
     batch.close()
+    one_sweep.close()
     queued.close()
     for s in sessions:
         s.close()

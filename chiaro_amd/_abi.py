@@ -35,6 +35,7 @@ CHD_STAGE_ALL = 15
 CHD_STAGE_KEEP_AGGREGATED = 256
 CHD_STAGE_KEEP_VIEW_SCORES = 512
 CHD_BATCH_PERSISTENT = 1
+CHD_BATCH_ONE_SWEEP = 2
 
 WARP_UNDEFINED, WARP_GLOBAL, WARP_REFINED, WARP_OCCLUDED, WARP_BOUNDARY = range(5)
 PROVENANCE_UNSUPPORTED, PROVENANCE_GLOBAL, PROVENANCE_MEASURED, PROVENANCE_REGULARIZED = range(4)

@@ -133,6 +133,7 @@ struct TimedLaunch {
 struct chd_context {
     int device;
     hipStream_t stream;
+    hipStream_t side = nullptr;  // second stream of pipelined batches (default priority)
     // luminance_half table for the last (black, white) pair
     float *luminance_table = nullptr;
     float luminance_black = 0.0f, luminance_white = 0.0f;
@@ -157,6 +158,10 @@ void context_free(chd_context *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     if (c->luminance_table) hipFree(c->luminance_table);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2921,6 +2926,9 @@ struct chd_batch {
     uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
     bool sgm_same_grids = true;  // every capture on the same coarse grid
     bool persistent = false;  // one queue launch for every cost volume
+    bool one_sweep = false;   // every capture's SGM sweep in one launch, no pipelining
+    std::vector<hipEvent_t> cost_done;  // per capture: its cost volume finished
+    hipEvent_t joined = nullptr;        // the side stream's work finished
     bool broken = false;      // a session was destroyed under the batch
     int queue_mode = 4;       // the jobs' common kind for the queue kernel (4: mixed)
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
@@ -2938,6 +2946,8 @@ struct chd_batch {
         for (auto &g : graphs) hipGraphExecDestroy(g.second);
         for (hipGraph_t g : templates) hipGraphDestroy(g);
         for (void *p : allocations) hipFree(p);
+        for (hipEvent_t e : cost_done) hipEventDestroy(e);
+        if (joined) hipEventDestroy(joined);
     }
 };
 
@@ -2965,6 +2975,17 @@ int balloc(chd_batch *b, T **out, size_t count) {
     return CHD_OK;
 }
 
+// Points the context's stream at another one for the stage functions that
+// launch on it, and back at the end of the scope.
+struct StreamSwap {
+    chd_context *c;
+    hipStream_t saved;
+    StreamSwap(chd_context *context, hipStream_t other) : c(context), saved(context->stream) {
+        c->stream = other;
+    }
+    ~StreamSwap() { c->stream = saved; }
+};
+
 // Enqueues the stages of every session; no host synchronisation, so that
 // the sequence can be captured into a graph.
 int batch_enqueue(chd_batch *b, uint32_t stages) {
@@ -2972,6 +2993,28 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
     const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
     const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
     const uint32_t n = (uint32_t)b->sessions.size();
+    if (!b->persistent && !b->one_sweep && (stages & CHD_STAGE_COST) && (stages & CHD_STAGE_SGM)) {
+        // pipelined: capture k's later stages on the side stream once its
+        // cost volume is done, beside capture k + 1's cost volume
+        hipStream_t side = b->context->side;
+        for (uint32_t k = 0; k < n; k++) {
+            chd_session *s = b->sessions[k];
+            CHD_CHECK(prepare_cost(s));
+            float *vs = nullptr;
+            CHD_CHECK(view_score_buffer(s, keep_views, &vs));
+            CHD_HIP(launch_cost_volume(st, cost_job(s, vs)));
+            CHD_HIP(hipEventRecord(b->cost_done[k], st));
+            CHD_HIP(hipStreamWaitEvent(side, b->cost_done[k], 0));
+            StreamSwap swap(b->context, side);
+            CHD_CHECK(run_sgm_paths(s));
+            CHD_CHECK(run_sgm_rest(s, keep));
+            if (stages & CHD_STAGE_DIRECT) CHD_CHECK(run_direct(s));
+            if (stages & CHD_STAGE_TARGETS) CHD_CHECK(run_targets(s));
+        }
+        CHD_HIP(hipEventRecord(b->joined, side));
+        CHD_HIP(hipStreamWaitEvent(st, b->joined, 0));
+        return CHD_OK;
+    }
     if (stages & CHD_STAGE_COST) {
         for (chd_session *s : b->sessions) CHD_CHECK(prepare_cost(s));
         if (b->persistent) {
@@ -3059,13 +3102,15 @@ int batch_prepare(chd_batch *b) {
 int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t count,
                      uint32_t flags, chd_batch **out) {
     if (!c || !sessions || !out || count == 0) return fail(CHD_ERR_ARGUMENT, "batch: bad argument");
-    if (flags & ~CHD_BATCH_PERSISTENT) return fail(CHD_ERR_ARGUMENT, "batch: unknown flags");
+    if (flags & ~(CHD_BATCH_PERSISTENT | CHD_BATCH_ONE_SWEEP))
+        return fail(CHD_ERR_ARGUMENT, "batch: unknown flags");
     *out = nullptr;
     if (c->released) return fail(CHD_ERR_STATE, "batch: the context was destroyed");
     CHD_HIP(hipSetDevice(c->device));
     auto *b = new chd_batch;
     b->context = c;
     b->persistent = (flags & CHD_BATCH_PERSISTENT) != 0;
+    b->one_sweep = (flags & CHD_BATCH_ONE_SWEEP) != 0;
     auto bail = [&](int code) {
         delete b;
         return code;
@@ -3087,6 +3132,24 @@ int chd_batch_create(chd_context *c, chd_session *const *sessions, uint32_t coun
     if (r == CHD_OK) r = balloc(b, &b->counter, kXcds);
     if (r == CHD_OK) r = balloc(b, &b->sgm_jobs, count);
     if (r == CHD_OK) r = balloc(b, &b->sgm_first, count + 1);
+    if (r == CHD_OK && !c->side) {
+        hipError_t e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            c->side = nullptr;
+            r = fail(CHD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+    }
+    for (uint32_t k = 0; r == CHD_OK && k <= count; k++) {
+        hipEvent_t e = nullptr;
+        hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (he != hipSuccess) {
+            r = fail(CHD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(he));
+        } else if (k < count) {
+            b->cost_done.push_back(e);
+        } else {
+            b->joined = e;
+        }
+    }
     if (r == CHD_OK) r = batch_prepare(b);
     if (r != CHD_OK) return bail(r);
     for (chd_session *s : b->sessions)

@@ -375,18 +375,22 @@ class Session:
 
 
 class Batch:
-    """Several sessions run together (chd_batch): one SGM sweep launch over
-    all captures, the other stages per capture (with persistent=True the cost
-    volumes as one persistent work-queue launch), captured once per stage mask
-    into a hipGraph."""
+    """Several sessions run together (chd_batch), captured once per stage
+    mask into a hipGraph: by default pipelined over two streams (capture k's
+    SGM and later stages beside capture k + 1's cost volume); one_sweep=True
+    runs every SGM path sweep in one launch after the cost volumes;
+    persistent=True runs the cost volumes as one persistent work-queue
+    launch."""
 
-    def __init__(self, context: Context, sessions, persistent: bool = False):
+    def __init__(self, context: Context, sessions, persistent: bool = False,
+                 one_sweep: bool = False):
         self.context = context
         self.lib = context.lib
         self.sessions = list(sessions)
         arr = (C.c_void_p * len(self.sessions))(*[s.handle.value for s in self.sessions])
         handle = C.c_void_p()
-        flags = _abi.CHD_BATCH_PERSISTENT if persistent else 0
+        flags = (_abi.CHD_BATCH_PERSISTENT if persistent else 0) | \
+            (_abi.CHD_BATCH_ONE_SWEEP if one_sweep else 0)
         _abi.check(self.lib.chd_batch_create(context.handle, arr, len(self.sessions), flags,
                                              C.byref(handle)), "batch create")
         self.handle = handle

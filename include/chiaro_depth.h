@@ -338,17 +338,22 @@ int chd_session_get_tuning(const chd_session *session, chd_tuning *out);
  * shape only reads it. */
 int chd_autotune(chd_session *session, const char *cache_path, chd_tuning *out);
 
-/* Batched execution of several sessions (captures) on one context: all
- * their SGM path sweeps in one launch, each capture's cost volume and
- * remaining stages in its own launches, the whole sequence for a given
- * `stages` mask captured once into a hipGraph and replayed by every later
- * run. With CHD_BATCH_PERSISTENT the cost volumes of all captures run
- * instead as one persistent launch fed by a (capture, node tile, label slab)
- * work queue. Results are read per session as after chd_session_run, and are
- * the same bit for bit. Sessions must share the label count and outlive the
- * batch; no counterpart in the reference (one capture at a time). */
+/* Batched execution of several sessions (captures) on one context, the
+ * whole sequence for a given `stages` mask captured once into a hipGraph and
+ * replayed by every later run. By default the captures are pipelined over
+ * two streams of the same priority: capture k's cost volume on the
+ * context's stream, and once it is done, capture k's SGM, select, direct
+ * pass and decisions on a second stream, beside capture k + 1's cost volume.
+ * CHD_BATCH_ONE_SWEEP runs every capture's SGM path sweep in one launch after
+ * all cost volumes instead (one stream). With CHD_BATCH_PERSISTENT the cost
+ * volumes of all captures run as one persistent launch fed by a (capture,
+ * node tile, label slab) work queue, followed by the one sweep. Results are
+ * read per session as after chd_session_run, and are the same bit for bit
+ * in every mode. Sessions must share the label count and outlive the batch;
+ * no counterpart in the reference (one capture at a time). */
 typedef struct chd_batch chd_batch;
 #define CHD_BATCH_PERSISTENT 1u
+#define CHD_BATCH_ONE_SWEEP 2u
 int chd_batch_create(chd_context *context, chd_session *const *sessions, uint32_t count,
                      uint32_t flags, chd_batch **out);
 int chd_batch_run(chd_batch *batch, uint32_t stages);

@@ -1,7 +1,7 @@
 """Schedules that must not change results: launch shapes (tuning), the
-autotuner and its cache, batched captures (persistent cost queue, batched SGM
-sweeps, hipGraph replay), and the per-view score slices against runs of one
-view pair alone. Every comparison is bit for bit; the uneven-module config 2
+autotuner and its cache, batched captures (two-stream pipelining, persistent
+cost queue, batched SGM sweeps, hipGraph replay), and the per-view score
+slices against runs of one view pair alone. Every comparison is bit for bit; the uneven-module config 2
 capture is also checked against the oracle."""
 import sys
 
@@ -115,20 +115,22 @@ def test_autotune_and_cache(gpu_context, tmp_path):
         sess.close()
 
 
-@pytest.mark.parametrize("persistent", [False, True])
-def test_batch_matches_single_sessions_on_replays(gpu_context, persistent):
-    """Three captures (two sizes) batched; the captured graph is replayed
+@pytest.mark.parametrize("mode", ["pipelined", "one_sweep", "persistent"])
+def test_batch_matches_single_sessions_on_replays(gpu_context, mode):
+    """Three captures (two sizes) batched in each schedule (two streams, one
+    SGM sweep launch, persistent cost queue); the captured graph is replayed
     three times with eager work of another session in between, and every
     replay equals the single-session results."""
     caps = [_small_config0(5), _small_config0(6, scale=0.3), _small_config0(7)]
-    _say(f"batch of 3 config0 captures, persistent cost queue {persistent}")
+    _say(f"batch of 3 config0 captures, {mode}")
     wants = [_eager(gpu_context, c) for c in caps]
     sessions = [Session(gpu_context, c.modules, c.reference_index, c.options) for c in caps]
     sessions[0].set_tuning(0, 0, 0, 1)  # the queue in XCD order
     sessions[1].set_tuning(6, 33, 2)    # a different shape inside the same batch
     other_cap = _small_config0(8)
     other = Session(gpu_context, other_cap.modules, 0, other_cap.options)
-    batch = Batch(gpu_context, sessions, persistent=persistent)
+    batch = Batch(gpu_context, sessions, persistent=mode == "persistent",
+                  one_sweep=mode == "one_sweep")
     try:
         for replay in range(3):
             _say(f"  replay {replay}")
