@@ -254,11 +254,12 @@ def main() -> int:
     ap.add_argument("--learned-scenes", type=int, default=1600,
                     help="config1: training captures of the config 4 learned-band measurement "
                          "in the same line (0 skips it)")
-    ap.add_argument("--learned-epochs", type=int, default=60)
-    ap.add_argument("--learned-validation", type=int, default=4,
+    ap.add_argument("--learned-epochs", type=int, default=100)
+    ap.add_argument("--learned-validation", type=int, default=8,
                     help="full config 4 captures that choose the band setting")
-    ap.add_argument("--learned-test", type=int, default=4,
-                    help="held-out full config 4 captures the band is reported on")
+    ap.add_argument("--learned-test", type=int, default=8,
+                    help="held-out full config 4 captures the band is reported on, in "
+                         "batches of 4")
     ap.add_argument("--full-path", action="store_true",
                     help="config2/config4: time the whole path (direct pass, filters and "
                          "per-target decisions too) instead of cost + SGM")
@@ -608,7 +609,8 @@ def learned_config3(ctx, train_scenes: int, epochs: int, eval_scenes: int, seed:
     net.set_input_transform(offset, scale)
     net.set_dataset(x, y)
     t = time.perf_counter()
-    losses = net.train(epochs=epochs, batch=32, lr=1e-3, seed=seed)
+    losses = net.train(epochs=epochs, batch=32, lr=1e-3, seed=seed, chunk=10,
+                       log=lambda done, l: log(f"[bench] epoch {done}/{epochs}: loss {l[-1]:.4f}"))
     train_s = time.perf_counter() - t
     log(f"[bench] trained {epochs} epochs in {train_s:.1f} s, last losses {losses[-3:]}")
     evals = [scene.config3(seed=100000 + k) for k in range(eval_scenes)]
@@ -731,12 +733,12 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
     planes. Selection: the settings (confidence threshold, tile, width cap)
     are timed and scored on `validation_captures` full 16-module captures
     batched; the fastest with >= 98 % +-1-hypothesis agreement is chosen.
-    Report: on `test_captures` other full captures batched, that setting and
-    the specification's (+-8 labels, tiles below confidence 0.5 fall back)
-    against the plain full sweep, for cost + SGM and end to end (cost, SGM,
-    WTA, direct pass, filters, per-target consistency), with the agreement
-    of the u8 consistency mask (provenance) and the per-target decision
-    codes."""
+    Report: on `test_captures` other full captures in batches of four, that
+    setting and the specification's (+-8 labels, tiles below confidence 0.5
+    fall back) against the plain full sweep, for cost + SGM and end to end
+    (cost, SGM, WTA, direct pass, filters, per-target consistency), with the
+    agreement of the u8 consistency mask (provenance) and the per-target
+    decision codes, averaged over the batches."""
     import numpy as np
 
     from chiaro_amd import _abi, scene
@@ -786,7 +788,8 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
     net.set_dataset(x, y)
     del x, y
     t = time.perf_counter()
-    losses = net.train(epochs=epochs, batch=16, lr=1e-3, seed=seed)
+    losses = net.train(epochs=epochs, batch=16, lr=1e-3, seed=seed, chunk=10,
+                       log=lambda done, l: log(f"[bench] epoch {done}/{epochs}: loss {l[-1]:.4f}"))
     train_s = time.perf_counter() - t
     log(f"[bench] trained {epochs} epochs in {train_s:.1f} s, losses {losses[0]:.3f} -> "
         f"{losses[-1]:.3f}")
@@ -802,8 +805,8 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         return out
 
     if settings is None:
-        settings = [(thr, tile, mh) for mh in (16, 24, 32) for tile in (4, 8)
-                    for thr in (0.2, 0.3, 0.4, 0.5)]
+        settings = [(thr, tile, mh) for mh in (16, 20, 24, 28, 32) for tile in (4, 8)
+                    for thr in (0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
     validation = sessions_for(VALIDATION_SEED, validation_captures)
     val_rows = batch_agreement_sessions(ctx, validation, net, settings, steps)
     for s in validation:
@@ -816,10 +819,28 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
     chosen = max(ok, key=lambda r: r["speedup"]) if ok else max(val_rows, key=lambda r: r["agreement"])
     chosen_setting = (chosen["threshold"], chosen["tile"], chosen["max_half_width"])
     spec_setting = (0.5, 8, 8)
+    # the held-out captures in batches of four (the configuration's captures
+    # per launch); agreements averaged over the batches, times per batch
     test = sessions_for(TEST_SEED, test_captures)
-    cost_rows = batch_agreement_sessions(ctx, test, net, [chosen_setting, spec_setting], steps)
-    e2e_rows = batch_agreement_sessions(ctx, test, net, [chosen_setting, spec_setting], steps,
-                                        stages=_abi.CHD_STAGE_ALL)
+    groups = [test[i:i + 4] for i in range(0, len(test), 4)]
+
+    def over_groups(stages):
+        per = [batch_agreement_sessions(ctx, g, net, [chosen_setting, spec_setting], steps,
+                                        stages=stages) for g in groups]
+        rows = []
+        for j in range(2):
+            r = dict(per[0][j])
+            for key in ("agreement", "mask_agreement", "window_coverage", "mean_window",
+                        "fallback_fraction", "full_ms", "band_ms",
+                        "consistency_mask_agreement", "decision_agreement"):
+                if key in r:
+                    r[key] = sum(p[j][key] for p in per) / len(per)
+            r["speedup"] = r["full_ms"] / r["band_ms"]
+            rows.append(r)
+        return rows
+
+    cost_rows = over_groups(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM)
+    e2e_rows = over_groups(_abi.CHD_STAGE_ALL)
     for s in test:
         s.close()
     frames.close()
@@ -829,11 +850,12 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         return {k: (round(row[k], 5) if isinstance(row[k], float) else row[k]) for k in keys}
 
     keys = ("threshold", "tile", "max_half_width", "half_width", "agreement", "mask_agreement",
-            "fallback_fraction", "full_ms", "band_ms", "speedup")
+            "window_coverage", "mean_window", "fallback_fraction", "full_ms", "band_ms",
+            "speedup")
     e2e_keys = keys + ("consistency_mask_agreement", "decision_agreement")
     return {
         "workload": (f"config4: {test_captures} held-out captures x 16 modules 4160x3120, 256 "
-                     f"hypotheses, batched; network trained on {train_captures} half-scale "
+                     f"hypotheses, in batches of 4; network trained on {train_captures} half-scale "
                      f"config4 captures ({'five wide' if wide_only else 'all sixteen'} modules, "
                      f"4 px grid) x {epochs} epochs; setting chosen on {validation_captures} "
                      f"other full captures"),

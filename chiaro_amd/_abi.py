@@ -376,6 +376,8 @@ DEPTH_SIGNATURES = {
     "chd_proposal_optimizer_state": (C.c_int, VP, VP, VP, P(C.c_size_t), P(C.c_uint64)),
     "chd_proposal_set_dataset": (C.c_int, VP, VP, VP, C.c_uint32),
     "chd_proposal_train": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, VP),
+    "chd_proposal_train_epochs": (C.c_int, VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_float, C.c_uint64, VP),
     "chd_proposal_predict": (C.c_int, VP, VP, C.c_uint32, VP),
     "chd_proposal_gradients": (C.c_int, VP, VP, VP, C.c_uint32, VP, P(C.c_float)),
     "chd_proposal_adam_step": (C.c_int, VP, VP, C.c_float),

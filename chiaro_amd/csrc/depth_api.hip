@@ -3572,7 +3572,14 @@ int chd_proposal_set_dataset(chd_proposal *p, const uint16_t *inputs, const uint
 
 int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float learning_rate,
                        uint64_t shuffle_seed, float *epoch_loss) {
-    if (!p || batch == 0 || batch > p->max_batch)
+    return chd_proposal_train_epochs(p, 0, epochs, epochs, batch, learning_rate, shuffle_seed,
+                                     epoch_loss);
+}
+
+int chd_proposal_train_epochs(chd_proposal *p, uint32_t first_epoch, uint32_t epochs,
+                              uint32_t total_epochs, uint32_t batch, float learning_rate,
+                              uint64_t shuffle_seed, float *epoch_loss) {
+    if (!p || batch == 0 || batch > p->max_batch || first_epoch + epochs > total_epochs)
         return fail(CHD_ERR_ARGUMENT, "proposal_train: bad argument");
     if (p->data_count == 0) return fail(CHD_ERR_STATE, "proposal_train: no training set");
     CHD_HIP(hipSetDevice(p->context->device));
@@ -3580,9 +3587,10 @@ int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float l
     const size_t pixels = (size_t)p->width * p->height;
     std::vector<uint32_t> order(p->data_count);
     // cosine decay from learning_rate to 5% of it over the whole run
-    const double total = (double)epochs * ((p->data_count + batch - 1) / batch);
-    double step = 0.0;
-    for (uint32_t e = 0; e < epochs; e++) {
+    const double per_epoch = (double)((p->data_count + batch - 1) / batch);
+    const double total = (double)total_epochs * per_epoch;
+    double step = (double)first_epoch * per_epoch;
+    for (uint32_t e = first_epoch; e < first_epoch + epochs; e++) {
         for (uint32_t k = 0; k < p->data_count; k++) order[k] = k;
         std::mt19937_64 rng(shuffle_seed + e);
         std::shuffle(order.begin(), order.end(), rng);
@@ -3605,7 +3613,7 @@ int chd_proposal_train(chd_proposal *p, uint32_t epochs, uint32_t batch, float l
         float totals[2] = {0.0f, 0.0f};
         CHD_HIP(hipMemcpyAsync(totals, p->totals, 8, hipMemcpyDeviceToHost, st));
         CHD_HIP(hipStreamSynchronize(st));
-        if (epoch_loss) epoch_loss[e] = totals[1] > 0.0f ? totals[0] / totals[1] : 0.0f;
+        if (epoch_loss) epoch_loss[e - first_epoch] = totals[1] > 0.0f ? totals[0] / totals[1] : 0.0f;
     }
     return CHD_OK;
 }

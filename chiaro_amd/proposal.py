@@ -336,10 +336,21 @@ class Proposal:
         _abi.check(self.lib.chd_proposal_set_dataset(self.handle, x.ctypes.data, y.ctypes.data,
                                                      x.shape[0]), "set dataset")
 
-    def train(self, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0) -> list:
+    def train(self, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0,
+              chunk: int = 0, log=None) -> list:
+        """`epochs` epochs of Adam; with `chunk` > 0 in calls of that many
+        epochs (the same run: shuffles and schedule of the whole), calling
+        log(epochs done, losses so far) after each."""
         losses = np.zeros(epochs, np.float32)
-        _abi.check(self.lib.chd_proposal_train(self.handle, epochs, batch, lr, seed,
-                                               losses.ctypes.data), "train")
+        step = chunk if chunk > 0 else epochs
+        for first in range(0, epochs, step):
+            n = min(step, epochs - first)
+            part = np.zeros(n, np.float32)
+            _abi.check(self.lib.chd_proposal_train_epochs(self.handle, first, n, epochs, batch,
+                                                          lr, seed, part.ctypes.data), "train")
+            losses[first:first + n] = part
+            if log is not None:
+                log(first + n, losses[:first + n])
         return [float(v) for v in losses]
 
     def predict(self, inputs: np.ndarray) -> np.ndarray:
@@ -543,7 +554,9 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
     products: the fraction of fine nodes whose u8 provenance (the
     consistency mask, depth.rs:548-579) and whose per-target decision codes
     (depth.rs:420-487) equal the full sweep's (consistency_mask_agreement,
-    decision_agreement)."""
+    decision_agreement). window_coverage is the fraction of the tested nodes
+    whose full-sweep label lies in their window (label 0 is always swept),
+    mean_window the mean label count of their windows."""
     from .depth import Batch
     end_to_end = bool(stages & _abi.CHD_STAGE_TARGETS)
     batch = Batch(ctx, sessions)
@@ -587,7 +600,7 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
                 batch.run(stages)
             ctx.synchronize()
             band_ms = (time.perf_counter() - t0) * 1000 / steps
-            close, valid, mask = 0, 0, 0.0
+            close, valid, mask, covered, width = 0, 0, 0.0, 0, 0
             for s, fb, ft in zip(sessions, full_best, full_tested):
                 bb = s.coarse()[2]
                 bt = s.tested()
@@ -595,9 +608,17 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
                 close += int((np.abs(bb.astype(np.int32) - fb.astype(np.int32)) <= 1)[v].sum())
                 valid += int(v.sum())
                 mask += float((bt == ft).mean()) / len(sessions)
+                # the full sweep's label inside the node's window (label 0
+                # is always swept)
+                lo, hi = s.band()
+                f = fb.astype(np.int64)
+                covered += int(((f == 0) | ((f >= lo) & (f <= hi)))[v].sum())
+                width += int((hi.astype(np.int64) - lo + 1)[v].sum())
             row = {"threshold": threshold, "tile": tile, "max_half_width": max_half,
                    "half_width": half,
                    "agreement": close / max(valid, 1), "mask_agreement": mask,
+                   "window_coverage": covered / max(valid, 1),
+                   "mean_window": width / max(valid, 1),
                    "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
                    "band_ms": band_ms, "speedup": full_ms / band_ms}
             if end_to_end:

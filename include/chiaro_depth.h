@@ -405,6 +405,14 @@ int chd_proposal_set_dataset(chd_proposal *proposal, const uint16_t *inputs,
  * labelled pixel of each epoch. */
 int chd_proposal_train(chd_proposal *proposal, uint32_t epochs, uint32_t batch,
                        float learning_rate, uint64_t shuffle_seed, float *epoch_loss);
+/* Epochs first_epoch .. first_epoch + epochs - 1 of a run of total_epochs
+ * (the shuffles and the cosine of the whole run): successive calls covering
+ * 0 .. total_epochs - 1 train exactly as one chd_proposal_train call, so a
+ * caller can report progress between them. epoch_loss gets `epochs`
+ * entries. */
+int chd_proposal_train_epochs(chd_proposal *proposal, uint32_t first_epoch, uint32_t epochs,
+                              uint32_t total_epochs, uint32_t batch, float learning_rate,
+                              uint64_t shuffle_seed, float *epoch_loss);
 /* Per pixel of `count` inputs: centre label, log half-width, confidence. */
 int chd_proposal_predict(chd_proposal *proposal, const uint16_t *inputs, uint32_t count,
                          float *outputs);

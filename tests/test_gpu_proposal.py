@@ -121,6 +121,42 @@ def test_adam_step_matches_reference(gpu_context):
         net.close()
 
 
+def test_chunked_training_equals_one_run(gpu_context):
+    """Training in calls of a few epochs (chd_proposal_train_epochs, with
+    progress between them) is the same run as one call: the same shuffles
+    and step sizes. The backward pass sums weight gradients with float
+    atomics, so two runs of either kind agree to rounding only; the losses
+    and parameters are compared at that level, against a run whose schedule
+    restarts at every call (three separate chd_proposal_train calls), which
+    must stray further."""
+    rng = np.random.default_rng(6)
+    w, h, count = 24, 16, 12
+    inputs = rng.integers(0x3c00, 0x4000, (count, h, w, 16), dtype=np.uint16)
+    labels = rng.integers(1, 60, (count, h, w), dtype=np.uint16)
+    nets = [Proposal(gpu_context, w, h, max_batch=4, seed=9) for _ in range(3)]
+    try:
+        for net in nets:
+            net.set_label_count(65)
+            net.set_dataset(inputs, labels)
+        seen = []
+        one = np.array(nets[0].train(epochs=7, batch=4, lr=3e-3, seed=3))
+        chunked = np.array(nets[1].train(epochs=7, batch=4, lr=3e-3, seed=3, chunk=3,
+                                         log=lambda done, l: seen.append(done)))
+        restarted = np.concatenate([nets[2].train(epochs=n, batch=4, lr=3e-3, seed=3)
+                                    for n in (3, 3, 1)])
+        assert seen == [3, 6, 7]
+        p0, p1, p2 = (n.parameters() for n in nets)
+        chunk_gap = np.abs(p1 - p0).max()
+        restart_gap = np.abs(p2 - p0).max()
+        _say(f"chunked vs one run: losses {np.abs(chunked - one).max():.2e}, parameters "
+             f"{chunk_gap:.2e}; restarted schedule: parameters {restart_gap:.2e}")
+        np.testing.assert_allclose(chunked, one, rtol=1e-3)
+        assert chunk_gap < 1e-3 and restart_gap > 5 * chunk_gap
+    finally:
+        for net in nets:
+            net.close()
+
+
 def _cap(seed):
     return scene.config3(seed=seed, width=520, height=392, focal_px=800.0)
 
