@@ -3346,13 +3346,9 @@ int proposal_backward(chd_proposal *p, uint32_t batch) {
         const size_t n = layer_out(L, batch);
         const float *g = i == 6 ? p->gout : p->gact[i + 1];
         CHD_HIP(launch_grad_bf16(st, g, i == 6 ? nullptr : p->act[i + 1], n, i != 6, p->gbf));
-        CHD_HIP(launch_conv_dw(st, L, batch, p->act[i], p->gbf, p->params[i].gw));
-        CHD_HIP(launch_bias_grad(st, p->gbf, batch * L.hout * L.wout, L.cout_p, p->params[i].gb));
-        if (i > 0) {
-            const size_t nin = (size_t)batch * L.hin * L.win * L.cin_p;
-            CHD_HIP(launch_fill_u32(st, (uint32_t *)p->gact[i], 0u, nin));
-            CHD_HIP(launch_conv_dx(st, L, batch, p->gbf, p->params[i].wk, p->gact[i]));
-        }
+        CHD_HIP(launch_conv_dw(st, L, batch, p->act[i], p->gbf, p->params[i].gw,
+                               p->params[i].gb));
+        if (i > 0) CHD_HIP(launch_conv_dx(st, L, batch, p->gbf, p->params[i].wk, p->gact[i]));
     }
     return CHD_OK;
 }

@@ -461,7 +461,11 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
                                            uint32_t line_bytes, double dx, double dy, float &score,
                                            unsigned int *evaluations) {
     if (!zncc_inside(P, n, tp, dx, dy)) return false;
-    atomicAdd(evaluations, 1u);
+    // one LDS atomic per wavefront: the first active lane adds the count of
+    // the lanes that passed (per-lane atomics on one address serialise)
+    const uint64_t passed = __ballot(1);
+    if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)passed) - 1))
+        atomicAdd(evaluations, (unsigned int)__popcll(passed));
     return zncc_taps<F64Taps>(P, n, T, tp, rsrc, line_bytes, dx, dy, score);
 }
 

@@ -15,9 +15,11 @@
 // 16x16x16 instruction, lane l = 16 * q + r (q = l / 16, r = l % 16):
 //   A: row r, k = 4q .. 4q + 3;  B: column r, k = 4q .. 4q + 3;
 //   D: rows 4q .. 4q + 3 of column r.
-// Backward: dX_col = dY W^T on the matrix cores, scattered to dX with f32
-// atomics (col2im); dW = X_col^T dY per pixel chunk, accumulated with f32
-// atomics; Adam on f32 master weights, then BF16 copies in both layouts.
+// Backward: dX in gather form (each input pixel collects dY W^T over the
+// output pixels that read it, written once), dW = X_col^T dY from operands
+// staged transposed in LDS with the bias gradient alongside, accumulated in
+// registers per block and added with one f32 atomic per element and block;
+// Adam on f32 master weights, then BF16 copies in both layouts.
 #include <hip/hip_runtime.h>
 
 #include "proposal.hpp"
@@ -137,87 +139,172 @@ __global__ void k_grad_bf16(const float *__restrict__ dy, const uint16_t *__rest
     out[i] = f2bf(g);
 }
 
-// dX_col = dY W^T per 16 pixels x 16 k, scattered into dX (f32).
-__global__ void __launch_bounds__(256) k_conv_dx(ConvLayer L, uint32_t batch,
-                                                 const uint16_t *__restrict__ dy,
-                                                 const uint16_t *__restrict__ wk,
-                                                 float *__restrict__ dx) {
+// dX in gather form: one wavefront per 16 input pixels x 16 input channels,
+// dX[p][c] = sum over (tap, output pixel o that reads p at that tap, co) of
+// dY[o][co] W[tap][c][co], as an implicit GEMM over (tap, candidate, co).
+// A stride-1 or stride-2 layer has at most one candidate output pixel per
+// tap; an upsampling layer (nearest 2x2) up to four. Every element of dX
+// is written once: no atomics, no clearing.
+template <uint32_t CoutChunks, bool Upsample>
+__global__ void __launch_bounds__(256) k_conv_dx_gather(ConvLayer L, uint32_t batch,
+                                                        const uint16_t *__restrict__ dy,
+                                                        const uint16_t *__restrict__ wk,
+                                                        float *__restrict__ dx) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t plane = L.hout * L.wout, M = batch * plane;
-    const uint32_t ktiles = L.k / 16, mtiles = (M + 15) / 16;
-    if (wave >= mtiles * ktiles) return;
-    const uint32_t m0 = (wave / ktiles) * 16, k0 = (wave % ktiles) * 16;
+    const uint32_t plane_in = L.hin * L.win, M = batch * plane_in;
+    const uint32_t ctiles = L.cin_p / 16, mtiles = (M + 15) / 16;
+    if (wave >= mtiles * ctiles) return;
+    const uint32_t m0 = (wave / ctiles) * 16, c0 = (wave % ctiles) * 16;
     const uint32_t q = lane >> 4, r = lane & 15u;
+    constexpr uint32_t cout_p = CoutChunks * 16;
     const uint32_t m = m0 + r;
+    const bool mv = m < M;
+    const uint32_t n = mv ? m / plane_in : 0, rem = mv ? m % plane_in : 0;
+    const int iy = (int)(rem / L.win), ix = (int)(rem % L.win);
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (uint32_t c0 = 0; c0 < L.cout_p; c0 += 16) {
-        bf16x4 a = {0, 0, 0, 0};
-        if (m < M) a = *(const bf16x4 *)(dy + (size_t)m * L.cout_p + c0 + 4 * q);
-        const bf16x4 b = *(const bf16x4 *)(wk + (size_t)(k0 + r) * L.cout_p + c0 + 4 * q);
-        acc = mfma16(a, b, acc);
+#pragma unroll
+    for (uint32_t tap = 0; tap < 9; tap++) {
+        const int ky = (int)(tap / 3), kx = (int)(tap % 3);
+        constexpr uint32_t kCandidates = Upsample ? 4 : 1;
+#pragma unroll
+        for (uint32_t cand = 0; cand < kCandidates; cand++) {
+            int oy, ox;
+            bool valid = mv;
+            if (Upsample) {
+                // the upsampled grid rows 2 iy and 2 iy + 1 repeat this row
+                oy = 2 * iy + (int)(cand >> 1) - ky + 1;
+                ox = 2 * ix + (int)(cand & 1u) - kx + 1;
+            } else {
+                const int ty = iy - ky + 1, tx = ix - kx + 1;
+                const int s = (int)L.stride;
+                valid = valid && ty >= 0 && tx >= 0 && ty % s == 0 && tx % s == 0;
+                oy = ty / s;
+                ox = tx / s;
+            }
+            valid = valid && oy >= 0 && ox >= 0 && oy < (int)L.hout && ox < (int)L.wout;
+            const uint16_t *row =
+                dy + (((size_t)n * L.hout + (valid ? oy : 0)) * L.wout + (valid ? ox : 0)) * cout_p;
+            const uint16_t *wrow = wk + (size_t)(tap * L.cin_p + c0 + r) * cout_p;
+#pragma unroll
+            for (uint32_t cc = 0; cc < CoutChunks; cc++) {
+                bf16x4 a = {0, 0, 0, 0};
+                if (valid) a = *(const bf16x4 *)(row + cc * 16 + 4 * q);
+                const bf16x4 b = *(const bf16x4 *)(wrow + cc * 16 + 4 * q);
+                acc = mfma16(a, b, acc);
+            }
+        }
     }
-    const uint32_t k = k0 + r, tap = k / L.cin_p, c = k % L.cin_p;
-    if (c >= L.cin) return;
     for (uint32_t v = 0; v < 4; v++) {
         const uint32_t p = m0 + 4 * q + v;
-        if (p >= M) continue;
-        const uint32_t n = p / plane, rem = p % plane;
-        uint32_t iy, ix;
-        if (!conv_source(L, rem / L.wout, rem % L.wout, tap / 3, tap % 3, iy, ix)) continue;
-        atomicAdd(dx + (((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p + c, acc[v]);
+        if (p < M) dx[(size_t)p * L.cin_p + c0 + r] = acc[v];
     }
 }
 
-// dW = X_col^T dY over chunks of `chunk` pixels (a multiple of 16) per
-// wavefront; each wavefront owns a 16 k x 16 output-channel tile.
-__global__ void __launch_bounds__(256) k_conv_dw(ConvLayer L, uint32_t batch,
-                                                 const uint16_t *__restrict__ in,
-                                                 const uint16_t *__restrict__ dy, uint32_t chunk,
-                                                 float *__restrict__ dw) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t plane = L.hout * L.wout, M = batch * plane;
-    const uint32_t ktiles = L.k / 16, ctiles = L.cout_p / 16, chunks = (M + chunk - 1) / chunk;
-    if (wave >= ktiles * ctiles * chunks) return;
-    const uint32_t kt = wave % ktiles, ct = (wave / ktiles) % ctiles, mc = wave / (ktiles * ctiles);
+// dW = X_col^T dY and db = sum dY, staged through LDS: a block takes
+// kDwSteps runs of kDwPixels (= its thread count) output pixels; per run
+// each thread loads its pixel's dY row and, per tap, its gathered input row
+// with 16-byte loads and stores them transposed ([channel][pixel]), so that
+// each lane of the products reads four consecutive pixels of one channel
+// as one 8-byte LDS load. The (tap, c tile, co tile) output tiles are dealt
+// to the four wavefronts and accumulate in registers over the block's
+// pixels; one f32 atomic per element and block at the end.
+constexpr uint32_t kDwPixels = 256;
+constexpr uint32_t kDwSteps = 16;
+constexpr uint32_t kDwPitch = kDwPixels + 4;  // bf16 per LDS row (8-byte aligned rows)
+template <uint32_t CinTiles, uint32_t CoutTiles>
+__global__ void __launch_bounds__(256) k_conv_dw_staged(ConvLayer L, uint32_t batch,
+                                                        const uint16_t *__restrict__ in,
+                                                        const uint16_t *__restrict__ dy,
+                                                        float *__restrict__ dw,
+                                                        float *__restrict__ db) {
+    constexpr uint32_t cin_p = CinTiles * 16, cout_p = CoutTiles * 16;
+    constexpr uint32_t kTiles = 9 * CinTiles * CoutTiles, kPerWave = (kTiles + 3) / 4;
+    __shared__ __align__(16) uint16_t dyT[cout_p * kDwPitch];
+    __shared__ __align__(16) uint16_t xT[cin_p * kDwPitch];
+    __shared__ float bias_sum[cout_p];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t q = lane >> 4, r = lane & 15u;
-    const uint32_t k = kt * 16 + r, tap = k / L.cin_p, c = k % L.cin_p, co = ct * 16 + r;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t mend = min(M, (mc + 1) * chunk);
-    for (uint32_t m0 = mc * chunk; m0 < mend; m0 += 16) {
-        bf16x4 a, b;
-        for (uint32_t v = 0; v < 4; v++) {
-            const uint32_t p = m0 + 4 * q + v;
-            short av = 0, bv = 0;
-            if (p < mend) {
-                const uint32_t n = p / plane, rem = p % plane;
-                uint32_t iy, ix;
-                if (c < L.cin &&
-                    conv_source(L, rem / L.wout, rem % L.wout, tap / 3, tap % 3, iy, ix))
-                    av = (short)in[(((size_t)n * L.hin + iy) * L.win + ix) * L.cin_p + c];
-                bv = (short)dy[(size_t)p * L.cout_p + co];
+    const uint32_t plane = L.hout * L.wout, M = batch * plane;
+    f32x4 acc[kPerWave];
+#pragma unroll
+    for (uint32_t u = 0; u < kPerWave; u++) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // bias partial of channel tid % cout_p over pixel segment tid / cout_p
+    constexpr uint32_t kSegments = 256 / cout_p, kSegment = kDwPixels / kSegments;
+    float bias = 0.0f;
+    if (tid < cout_p) bias_sum[tid] = 0.0f;
+    for (uint32_t step = 0; step < kDwSteps; step++) {
+        const uint32_t p0 = (blockIdx.x * kDwSteps + step) * kDwPixels;
+        if (p0 >= M) break;
+        const uint32_t g = p0 + tid;  // this thread's pixel
+        const bool live = g < M;
+        const uint32_t nn = live ? g / plane : 0, rem = live ? g % plane : 0;
+        const uint32_t oy = rem / L.wout, ox = rem % L.wout;
+        __syncthreads();
+        {
+            const uint4 *src = (const uint4 *)(dy + (size_t)(live ? g : 0) * cout_p);
+#pragma unroll
+            for (uint32_t j = 0; j < cout_p / 8; j++) {
+                uint4 v = live ? src[j] : make_uint4(0u, 0u, 0u, 0u);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (uint32_t h = 0; h < 8; h++)
+                    dyT[(j * 8 + h) * kDwPitch + tid] = (uint16_t)(w[h >> 1] >> (16 * (h & 1u)));
             }
-            a[v] = av;
-            b[v] = bv;
         }
-        acc = mfma16(a, b, acc);
+        __syncthreads();
+        {
+            const uint32_t co = tid % cout_p, seg = tid / cout_p;
+            for (uint32_t p = seg * kSegment; p < (seg + 1) * kSegment; p++)
+                bias += bf2f(dyT[co * kDwPitch + p]);
+        }
+#pragma unroll
+        for (uint32_t tap = 0; tap < 9; tap++) {
+            if (tap > 0) __syncthreads();
+            {
+                uint32_t iy = 0, ix = 0;
+                const bool valid = live && conv_source(L, oy, ox, tap / 3, tap % 3, iy, ix);
+                const uint4 *src =
+                    (const uint4 *)(in + (((size_t)nn * L.hin + iy) * L.win + ix) * cin_p);
+#pragma unroll
+                for (uint32_t j = 0; j < cin_p / 8; j++) {
+                    uint4 v = valid ? src[j] : make_uint4(0u, 0u, 0u, 0u);
+                    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (uint32_t h = 0; h < 8; h++)
+                        xT[(j * 8 + h) * kDwPitch + tid] = (uint16_t)(w[h >> 1] >> (16 * (h & 1u)));
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t t = 0; t < CinTiles * CoutTiles; t++) {
+                const uint32_t j = tap * CinTiles * CoutTiles + t;
+                if (j % 4 != wave) continue;
+                const uint32_t ct = t / CoutTiles, ot = t % CoutTiles;
+                const uint16_t *arow = xT + (ct * 16 + r) * kDwPitch + 4 * q;
+                const uint16_t *brow = dyT + (ot * 16 + r) * kDwPitch + 4 * q;
+                f32x4 a_acc = acc[j / 4];
+#pragma unroll 4
+                for (uint32_t s = 0; s < kDwPixels; s += 16)
+                    a_acc = mfma16(*(const bf16x4 *)(arow + s), *(const bf16x4 *)(brow + s), a_acc);
+                acc[j / 4] = a_acc;
+            }
+        }
     }
-    for (uint32_t v = 0; v < 4; v++)
-        atomicAdd(dw + (size_t)(kt * 16 + 4 * q + v) * L.cout_p + co, acc[v]);
-}
-
-// db[c] += sum over pixels of the (masked, BF16) gradient dy[p][c], one
-// thread per (chunk, c).
-__global__ void k_bias_grad(const uint16_t *__restrict__ dy, uint32_t M, uint32_t cout_p,
-                            uint32_t chunk, float *__restrict__ db) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = i % cout_p, mc = i / cout_p;
-    if (mc * chunk >= M) return;
-    float s = 0.0f;
-    const uint32_t end = min(M, (mc + 1) * chunk);
-    for (uint32_t p = mc * chunk; p < end; p++) s += bf2f(dy[(size_t)p * cout_p + c]);
-    atomicAdd(db + c, s);
+#pragma unroll
+    for (uint32_t u = 0; u < kPerWave; u++) {
+        const uint32_t j = u * 4 + wave;
+        if (j >= kTiles) continue;
+        const uint32_t tap = j / (CinTiles * CoutTiles), t = j % (CinTiles * CoutTiles);
+        const uint32_t ct = t / CoutTiles, ot = t % CoutTiles;
+        for (uint32_t v = 0; v < 4; v++)
+            atomicAdd(dw + (size_t)(tap * cin_p + ct * 16 + 4 * q + v) * cout_p + ot * 16 + r,
+                      acc[u][v]);
+    }
+    __syncthreads();
+    atomicAdd(&bias_sum[tid % cout_p], bias);
+    __syncthreads();
+    if (tid < cout_p) atomicAdd(db + tid, bias_sum[tid]);
 }
 
 // ------------------------------------------------------------ loss
@@ -427,26 +514,36 @@ hipError_t launch_grad_bf16(hipStream_t s, const float *dy, const uint16_t *y, s
 
 hipError_t launch_conv_dx(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *dy,
                           const uint16_t *wk, float *dx) {
-    const uint32_t M = batch * L.hout * L.wout;
-    const uint32_t waves = ((M + 15) / 16) * (L.k / 16);
-    hipLaunchKernelGGL(k_conv_dx, dim3((waves + 3) / 4), dim3(256), 0, s, L, batch, dy, wk, dx);
+    const uint32_t M = batch * L.hin * L.win;
+    const uint32_t waves = ((M + 15) / 16) * (L.cin_p / 16);
+    const dim3 grid((waves + 3) / 4), block(256);
+    if (L.cout_p == 16 && L.upsample)
+        hipLaunchKernelGGL((k_conv_dx_gather<1, true>), grid, block, 0, s, L, batch, dy, wk, dx);
+    else if (L.cout_p == 16)
+        hipLaunchKernelGGL((k_conv_dx_gather<1, false>), grid, block, 0, s, L, batch, dy, wk, dx);
+    else if (L.cout_p == 32 && L.upsample)
+        hipLaunchKernelGGL((k_conv_dx_gather<2, true>), grid, block, 0, s, L, batch, dy, wk, dx);
+    else if (L.cout_p == 32)
+        hipLaunchKernelGGL((k_conv_dx_gather<2, false>), grid, block, 0, s, L, batch, dy, wk, dx);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 hipError_t launch_conv_dw(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *in,
-                          const uint16_t *dy, float *dw) {
-    const uint32_t M = batch * L.hout * L.wout, chunk = 1024;
-    const uint32_t waves = (L.k / 16) * (L.cout_p / 16) * ((M + chunk - 1) / chunk);
-    hipLaunchKernelGGL(k_conv_dw, dim3((waves + 3) / 4), dim3(256), 0, s, L, batch, in, dy, chunk,
-                       dw);
-    return hipGetLastError();
-}
-
-hipError_t launch_bias_grad(hipStream_t s, const uint16_t *dy, uint32_t M, uint32_t cout_p,
-                            float *db) {
-    const uint32_t chunk = 256, threads = ((M + chunk - 1) / chunk) * cout_p;
-    hipLaunchKernelGGL(k_bias_grad, dim3((threads + 255) / 256), dim3(256), 0, s, dy, M, cout_p,
-                       chunk, db);
+                          const uint16_t *dy, float *dw, float *db) {
+    const uint32_t M = batch * L.hout * L.wout, per_block = kDwPixels * kDwSteps;
+    const dim3 grid((M + per_block - 1) / per_block), block(256);
+    if (L.cin_p == 16 && L.cout_p == 16)
+        hipLaunchKernelGGL((k_conv_dw_staged<1, 1>), grid, block, 0, s, L, batch, in, dy, dw, db);
+    else if (L.cin_p == 16 && L.cout_p == 32)
+        hipLaunchKernelGGL((k_conv_dw_staged<1, 2>), grid, block, 0, s, L, batch, in, dy, dw, db);
+    else if (L.cin_p == 32 && L.cout_p == 16)
+        hipLaunchKernelGGL((k_conv_dw_staged<2, 1>), grid, block, 0, s, L, batch, in, dy, dw, db);
+    else if (L.cin_p == 32 && L.cout_p == 32)
+        hipLaunchKernelGGL((k_conv_dw_staged<2, 2>), grid, block, 0, s, L, batch, in, dy, dw, db);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -23,12 +23,12 @@ hipError_t launch_conv_fwd(hipStream_t s, const ConvLayer &L, uint32_t batch, co
                            const uint16_t *wt, const float *bias, void *out, bool out_f32);
 hipError_t launch_grad_bf16(hipStream_t s, const float *dy, const uint16_t *y, size_t n, bool relu,
                             uint16_t *out);
+// dX of one layer (f32, every element written; channels padded to 16 or 32).
 hipError_t launch_conv_dx(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *dy,
                           const uint16_t *wk, float *dx);
+// dW and db of one layer, added into dw and db (cleared beforehand).
 hipError_t launch_conv_dw(hipStream_t s, const ConvLayer &L, uint32_t batch, const uint16_t *in,
-                          const uint16_t *dy, float *dw);
-hipError_t launch_bias_grad(hipStream_t s, const uint16_t *dy, uint32_t M, uint32_t cout_p,
-                            float *db);
+                          const uint16_t *dy, float *dw, float *db);
 // Labels 1 .. label_count - 1 (0 = unlabelled); centre = (L-1)/2 + (L-1)/8 o0.
 hipError_t launch_proposal_loss(hipStream_t s, const float *out, const uint16_t *labels,
                                 uint32_t M, uint32_t cout_p, uint32_t label_count, float norm,
