@@ -39,9 +39,22 @@ constexpr uint32_t kMaxCandidates = 32;  // direct_depth_candidates bound (depth
 
 struct PatchLayout {
     uint32_t cams, pos, ray, ext, gc, gq, wr, sums, centre, spatial, node_ok, nv_ok, gc_ok,
-        ray_ok, total;
+        ray_ok, loff, lat, total;
     uint32_t ray_views;  // views per ray group: the per-view arrays hold [ray_views][nb]
 };
+
+// Shared tap lattice of a block whose nodes lie on one grid row, one grid
+// step apart, with an even step: every tap of every node is a point of one
+// lattice of 2-px spacing, (nb - 1) * step / 2 + 2r + 1 points wide and
+// 2r + 1 high, so the reference samples and the global-warp tap positions
+// are computed once per lattice point instead of once per (node, tap) (a
+// strip of 16 fine nodes at a 4 px step has 351 points against 1,296 taps).
+// Each point is computed with the same operations as the per-node form
+// (raster position p.x + 2 dx is the same exact integer either way), so the
+// contexts are identical.
+__host__ __device__ inline uint32_t lattice_width(uint32_t nb, uint32_t step, uint32_t radius) {
+    return (nb - 1) * (step / 2) + 2 * radius + 1;
+}
 
 // Views whose centre points and corrected rays are computed together.
 constexpr uint32_t kRayViews = 4;
@@ -50,8 +63,9 @@ __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15
 
 // f64_taps: the per-view tap positions are kept as f64 (16 B per tap, two
 // conversions fewer per tap in the loop) rather than f32 (8 B, half the LDS).
+// lat_points: points of the block's shared tap lattice (0: no lattice).
 __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uint32_t t,
-                                                    bool f64_taps) {
+                                                    bool f64_taps, uint32_t lat_points = 0) {
     PatchLayout l;
     const uint32_t rv = v < kRayViews ? (v > 0 ? v : 1u) : kRayViews;
     l.ray_views = rv;
@@ -61,7 +75,8 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
     l.ray = o; o = align16(o + rv * nb * 6u * 8u);
     l.ext = o; o = align16(o + nb * 16u);
     l.gc = o; o = align16(o + rv * nb * 8u);
-    l.gq = o; o = align16(o + nb * t * (f64_taps ? 16u : 8u));  // first holds the reference samples
+    // first holds the reference samples; a lattice replaces both
+    l.gq = o; o = align16(o + (lat_points ? 0u : nb * t * (f64_taps ? 16u : 8u)));
     l.wr = o; o = align16(o + nb * t * 8u);
     l.sums = o; o = align16(o + nb * 3u * 4u);
     l.centre = o; o = align16(o + nb * 4u);
@@ -70,6 +85,8 @@ __host__ __device__ inline PatchLayout patch_layout(uint32_t nb, uint32_t v, uin
     l.nv_ok = o; o = align16(o + rv * nb * 4u);
     l.gc_ok = o; o = align16(o + rv * nb * 4u);
     l.ray_ok = o; o = align16(o + rv * nb * 4u);
+    l.loff = o; o = align16(o + (lat_points ? t * 2u : 0u));
+    l.lat = o; o = align16(o + lat_points * 8u);
     l.total = o;
     return l;
 }
@@ -96,6 +113,15 @@ struct PatchLds {
     int *gc_ok;        // current view: global warp defined at the centre
     int *ray_ok;       // current view: corrected reference ray exists
     uint32_t ray_views;
+    // shared tap lattice (lat_w = 0: none): points [2r + 1][lat_w], node n's
+    // taps start at column n * lat_cols; loff[t] = (t / (2r + 1)) * lat_w +
+    // t % (2r + 1). Holds the reference samples (x) and their validity (y)
+    // in patch_nodes, then each view's global-warp positions (NaN where the
+    // warp is undefined; a defined warp point is never NaN).
+    float2 *lat;
+    uint16_t *loff;
+    uint32_t lat_w, lat_cols;
+    uint32_t tap_width;  // 2r + 1, for the row-wise tap loop (set by with_rows / with_lattice)
 };
 
 __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l, bool f64_taps) {
@@ -118,6 +144,28 @@ __device__ inline PatchLds bind_patch(unsigned char *base, const PatchLayout &l,
     p.gc_ok = (int *)(base + l.gc_ok);
     p.ray_ok = (int *)(base + l.ray_ok);
     p.ray_views = l.ray_views;
+    p.lat = (float2 *)(base + l.lat);
+    p.loff = (uint16_t *)(base + l.loff);
+    p.lat_w = 0;
+    p.lat_cols = 0;
+    p.tap_width = 0;
+    return p;
+}
+
+// A context whose ZNCCs run the row-wise tap loop (tap positions of row dy
+// at a stride: the node's own T taps, or a shared lattice).
+__device__ inline PatchLds with_rows(PatchLds p, uint32_t radius) {
+    p.tap_width = 2 * radius + 1;
+    return p;
+}
+
+// The lattice form of a patch context for nodes 0 .. nb - 1 at raster
+// columns x0 + n * step of one row (step even); bind_patch's layout must
+// hold lattice_width(nb, step, radius) * (2 radius + 1) points.
+__device__ inline PatchLds with_lattice(PatchLds p, uint32_t nb, uint32_t step, uint32_t radius) {
+    p.lat_w = lattice_width(nb, step, radius);
+    p.lat_cols = step / 2;
+    p.tap_width = 2 * radius + 1;
     return p;
 }
 
@@ -202,6 +250,12 @@ __device__ void patch_rays(const PatchSpec &S, const PatchLds &P, uint32_t nb, u
     }
 }
 
+// Reference sample of tap t of node n (patch_nodes has run or is past its
+// sampling step): from the lattice, or the per-node array.
+__device__ __forceinline__ float ref_sample(const PatchLds &P, uint32_t n, uint32_t t, uint32_t T) {
+    return P.lat_w ? P.lat[P.loff[t] + n * P.lat_cols].x : P.rv[n * T + t];
+}
+
 // Node half of the patch context. On entry P.pos[n] and P.node_ok[n] (node
 // exists) are set and visible. On exit node_ok[n] is 0 when the centre or any
 // tap of the reference patch falls outside the reference plane (every ZNCC of
@@ -232,24 +286,44 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             P.node_ok[n] = 0;
         }
     }
-#pragma unroll 3
-    for (uint32_t i = tid; i < nb * T; i += nth) {
-        const uint32_t n = i / T, t = i % T;
-        const double2 p = P.pos[n];
-        const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
-        float v;
-        if (!plane_sample(rp, to_plane(p.x + (double)dx * 2.0), to_plane(p.y + (double)dy * 2.0),
-                          v)) {
-            P.node_ok[n] = 0;
-            v = 0.0f;
+    if (P.lat_w) {
+        // the reference samples once per lattice point, then per (node, tap)
+        const uint32_t W = P.lat_w;
+        const double2 p0 = P.pos[0];
+        for (uint32_t t = tid; t < T; t += nth)
+            P.loff[t] = (uint16_t)((t / width) * W + t % width);
+        for (uint32_t i = tid; i < W * width; i += nth) {
+            const uint32_t col = i % W, row = i / W;
+            float v;
+            const bool ok = plane_sample(rp, to_plane(p0.x + (double)((int)col - r) * 2.0),
+                                         to_plane(p0.y + (double)((int)row - r) * 2.0), v);
+            P.lat[i] = make_float2(ok ? v : 0.0f, ok ? 1.0f : 0.0f);
         }
-        P.rv[i] = v;
+        __syncthreads();
+        for (uint32_t i = tid; i < nb * T; i += nth) {
+            const uint32_t n = i / T, t = i % T;
+            if (P.lat[P.loff[t] + n * P.lat_cols].y == 0.0f) P.node_ok[n] = 0;
+        }
+    } else {
+#pragma unroll 3
+        for (uint32_t i = tid; i < nb * T; i += nth) {
+            const uint32_t n = i / T, t = i % T;
+            const double2 p = P.pos[n];
+            const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+            float v;
+            if (!plane_sample(rp, to_plane(p.x + (double)dx * 2.0),
+                              to_plane(p.y + (double)dy * 2.0), v)) {
+                P.node_ok[n] = 0;
+                v = 0.0f;
+            }
+            P.rv[i] = v;
+        }
     }
     __syncthreads();
     for (uint32_t i = tid; i < nb * T; i += nth) {
         const uint32_t n = i / T, t = i % T;
         if (!P.node_ok[n]) continue;
-        const float rv = P.rv[i];
+        const float rv = ref_sample(P, n, t, T);
         const float range = libm_expf(-1.2f * fabsf(rv - P.centre[n]));
         const float w = P.spatial[t] * range;
         P.wr[i] = make_float2(w, w * rv);
@@ -262,7 +336,7 @@ __device__ void patch_nodes(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             const float2 w = P.wr[n * T + t];
             count += w.x;
             sum_r += w.y;
-            sum_rr += w.y * P.rv[n * T + t];
+            sum_rr += w.y * ref_sample(P, n, t, T);
         }
         P.sums[n * 3 + 0] = count;
         P.sums[n * 3 + 1] = sum_r;
@@ -319,6 +393,21 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
     const uint32_t m = n_end > n_begin ? n_end - n_begin : 0;
     const uint32_t G = m <= 4 ? 64u : m <= 8 ? 32u : 16u;
     const uint32_t per_pass = nth / G, lane = tid % G;
+    if (P.lat_w) {
+        // the global warp once per lattice point (NaN where undefined)
+        const uint32_t W = P.lat_w;
+        const double2 p0 = P.pos[0];
+        const float nan = __int_as_float(0x7fc00000);
+        for (uint32_t i = tid; i < W * width; i += nth) {
+            const uint32_t col = i % W, row = i / W;
+            float2 q;
+            if (!warp_map(gw, (float)(p0.x + (double)((int)col - r) * 2.0),
+                          (float)(p0.y + (double)((int)row - r) * 2.0), q))
+                q = make_float2(nan, nan);
+            P.lat[i] = q;
+        }
+        __syncthreads();
+    }
     for (uint32_t n0 = n_begin; n0 < n_end; n0 += per_pass) {
         const uint32_t n = n0 + tid / G;
         float x0 = __int_as_float(0x7f800000), y0 = x0;
@@ -327,18 +416,27 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
             const double2 p = P.pos[n];
 #pragma unroll 3
             for (uint32_t t = lane; t < T; t += G) {
-                const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
                 float2 q;
-                if (!warp_map(gw, (float)(p.x + (double)dx * 2.0), (float)(p.y + (double)dy * 2.0),
-                              q)) {
-                    Q.nv_ok[n] = 0;
-                    q = make_float2(0.0f, 0.0f);
+                if (P.lat_w) {
+                    q = P.lat[P.loff[t] + n * P.lat_cols];
+                    if (q.x != q.x) {
+                        Q.nv_ok[n] = 0;
+                        q = make_float2(0.0f, 0.0f);
+                    }
+                } else {
+                    const int dx = (int)(t % width) - r, dy = (int)(t / width) - r;
+                    if (!warp_map(gw, (float)(p.x + (double)dx * 2.0),
+                                  (float)(p.y + (double)dy * 2.0), q)) {
+                        Q.nv_ok[n] = 0;
+                        q = make_float2(0.0f, 0.0f);
+                    }
+                    // the tap loop reads a lattice in place
+                    const uint32_t i = n * T + t;
+                    if (P.f64_taps)
+                        P.gq[i] = make_double2((double)q.x, (double)q.y);
+                    else
+                        P.gqf[i] = q;
                 }
-                const uint32_t i = n * T + t;
-                if (P.f64_taps)
-                    P.gq[i] = make_double2((double)q.x, (double)q.y);
-                else
-                    P.gqf[i] = q;
                 x0 = fminf(x0, q.x);
                 x1 = fmaxf(x1, q.x);
                 y0 = fminf(y0, q.y);
@@ -402,11 +500,30 @@ __device__ __forceinline__ void tap_batch(const float2 *wr, const double2 *gq, c
 
 // The three target sums of warped_patch_zncc over the taps in the
 // reference's dy-major order, each in its own f32 accumulator.
-template <bool Transposed, bool F64Taps>
+// Rows (f32 tap positions only): the taps row by row, tap positions of
+// consecutive rows at the lattice's stride (or the node's own, 2r + 1);
+// the same taps in the same order.
+template <bool Transposed, bool F64Taps, bool Rows = false>
 __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
                                          double dx, double dy, float &st, float &stt, float &sp) {
     const float2 *wr = P.wr + (size_t)n * T;
+    if constexpr (Rows) {
+        static_assert(!F64Taps, "row-wise taps are f32");
+        const uint32_t width = P.tap_width;
+        const float2 *g = P.lat_w ? P.lat + n * P.lat_cols : P.gqf + (size_t)n * T;
+        const uint32_t gs = P.lat_w ? P.lat_w : width;
+        for (uint32_t row = 0; row < width; row++, wr += width, g += gs) {
+            uint32_t t0 = 0;
+            for (; t0 + kTapBatch <= width; t0 += kTapBatch)
+                tap_batch<Transposed, false, false>(wr, nullptr, g, t0, width, pitch, rsrc,
+                                                    line_bytes, dx, dy, st, stt, sp);
+            if (t0 < width)
+                tap_batch<Transposed, false, true>(wr, nullptr, g, t0, width, pitch, rsrc,
+                                                   line_bytes, dx, dy, st, stt, sp);
+        }
+        return;
+    }
     const double2 *gq = P.gq + (size_t)n * T;
     const float2 *gqf = P.gqf + (size_t)n * T;
     uint32_t t0 = 0;
@@ -431,16 +548,16 @@ __device__ __forceinline__ bool zncc_inside(const PatchLds &P, uint32_t n, const
 }
 
 // The sums half, for a displacement zncc_inside accepted.
-template <bool F64Taps>
+template <bool F64Taps, bool Rows = false>
 __device__ __forceinline__ bool zncc_taps(const PatchLds &P, uint32_t n, uint32_t T,
                                           const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
                                           uint32_t line_bytes, double dx, double dy,
                                           float &score) {
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
     if (tp.transposed)
-        tap_sums<true, F64Taps>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<true, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
     else
-        tap_sums<false, F64Taps>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<false, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
     const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
     const float covariance = sp - sr * st / count;
     const float reference_energy = srr - sr * sr / count;
@@ -455,7 +572,7 @@ __device__ __forceinline__ bool zncc_taps(const PatchLds &P, uint32_t n, uint32_
 // displaced by the raster delta (dx, dy); false is the reference's None.
 // Evaluations count the displacements that pass the bounds check.
 // F64Taps must match the layout's f64_taps.
-template <bool F64Taps>
+template <bool F64Taps, bool Rows = false>
 __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
                                            const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
                                            uint32_t line_bytes, double dx, double dy, float &score,
@@ -466,7 +583,7 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
     const uint64_t passed = __ballot(1);
     if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)passed) - 1))
         atomicAdd(evaluations, (unsigned int)__popcll(passed));
-    return zncc_taps<F64Taps>(P, n, T, tp, rsrc, line_bytes, dx, dy, score);
+    return zncc_taps<F64Taps, Rows>(P, n, T, tp, rsrc, line_bytes, dx, dy, score);
 }
 
 // map_at_depth (depth.rs:1738-1759) from the cached corrected ray of node n,
@@ -557,6 +674,29 @@ __host__ __device__ __forceinline__ uint32_t topk_slots(uint32_t best_view_count
 }
 
 __device__ __forceinline__ float nan_f() { return __int_as_float(0x7fc00000); }
+
+// Row-aligned node tiles: block b takes `count` consecutive nodes of one
+// grid row, from node `first` (rows * row_tiles(G, nb) blocks; a row's
+// tiles differ by at most one node and hold at most nb), so that the
+// block's taps share one lattice.
+struct NodeTile {
+    uint32_t first, count;
+};
+__host__ __device__ __forceinline__ uint32_t row_tiles(const GridSpec &G, uint32_t nb) {
+    return (G.columns + nb - 1) / nb;
+}
+__device__ __forceinline__ NodeTile row_tile(const GridSpec &G, uint32_t nb, uint32_t b) {
+    const uint32_t per_row = row_tiles(G, nb);
+    const uint32_t row = b / per_row, k = b % per_row;
+    const uint32_t c0 = (uint32_t)((uint64_t)k * G.columns / per_row);
+    const uint32_t c1 = (uint32_t)((uint64_t)(k + 1) * G.columns / per_row);
+    return NodeTile{row * G.columns + c0, c1 - c0};
+}
+// Lattice points of a row tile's context (0 for odd steps: no lattice).
+__host__ __device__ __forceinline__ uint32_t tile_lattice_points(uint32_t nb, uint32_t step,
+                                                                 uint32_t radius) {
+    return step % 2 == 0 ? lattice_width(nb, step, radius) * (2 * radius + 1) : 0u;
+}
 
 __device__ __forceinline__ double2 node_position(const GridSpec &G, uint32_t g) {
     const uint32_t row = g / G.columns, column = g % G.columns;
@@ -1140,6 +1280,7 @@ __device__ int best_depth_candidate(const double *depths, const float *score, ui
 // depth cand[node * stride + k - 1]; base[] is the prefix of 1 + ncand over
 // the block's nodes and item_node[] maps an item to its node. Results land
 // in res[(node * V + view) * (stride + 1) + slot].
+template <bool Rows>
 __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             uint32_t a, const double *cand, const uint32_t *base,
                             const uint8_t *item_node, uint32_t stride, float *res,
@@ -1158,7 +1299,9 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             if (slot > 0)
                 ok = delta_at_depth(P, n, *P.cam, cand[n * stride + slot - 1], dx, dy);
             float sc;
-            if (ok && patch_zncc<false>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc, evaluations)) s = sc;
+            if (ok && patch_zncc<false, Rows>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                                              evaluations))
+                s = sc;
         }
         res[((size_t)n * V + a) * (stride + 1) + slot] = s;
     }
@@ -1176,8 +1319,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int evaluation_count;
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T, false);
-    const PatchLds P = bind_patch(lds, lay, false);
+    const uint32_t lat_points = tile_lattice_points(nb, F.step, S.radius);
+    const PatchLayout lay = patch_layout(nb, V, T, false, lat_points);
+    PatchLds P = bind_patch(lds, lay, false);
+    P = lat_points ? with_lattice(P, nb, F.step, S.radius) : with_rows(P, S.radius);
     double *cand = (double *)(lds + lay.total);                 // [nb][32]
     float *res = (float *)(cand + nb * kMaxCandidates);         // [nb][V][33]
     float *centre_value = res + (size_t)nb * V * kDirectSlots;  // [nb]
@@ -1185,16 +1330,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     uint32_t *node_tested = ncand + nb;                         // [nb]
     uint32_t *base = node_tested + nb;                          // [nb + 1]
     uint8_t *item_node = (uint8_t *)(base + nb + 1);            // [nb * 33]
-    const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
+    const NodeTile tile = row_tile(F, nb, blockIdx.x);
+    const uint32_t node0 = tile.first, count = tile.count, tid = threadIdx.x;
 
     for (uint32_t n = tid; n < nb; n += blockDim.x) {
-        const uint32_t g = node0 + n;
-        const double2 p = node_position(F, g);
+        // slots past the tile continue the row (the lattice spans them;
+        // they are never scored)
+        const double2 p0 = node_position(F, node0);
+        const double2 p = make_double2(p0.x + (double)(n * F.step), p0.y);
         P.pos[n] = p;
-        P.node_ok[n] = g < nodes ? 1 : 0;
+        P.node_ok[n] = n < count ? 1 : 0;
         node_tested[n] = 0;
         bool seeded;
-        ncand[n] = g < nodes ? direct_candidates(Cg, coarse, p.x, p.y, inverse_step, o,
+        ncand[n] = n < count ? direct_candidates(Cg, coarse, p.x, p.y, inverse_step, o,
                                                  cand + n * kMaxCandidates, seeded)
                              : 0;
     }
@@ -1204,27 +1352,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         uint32_t b = 0;
         for (uint32_t n = 0; n < nb; n++) {
             base[n] = b;
-            b += node0 + n < nodes ? 1 + ncand[n] : 0;
+            b += n < count ? 1 + ncand[n] : 0;
         }
         base[nb] = b;
     }
     patch_nodes(S, P, nb, T, centre_value);
     for (uint32_t n = tid; n < nb; n += blockDim.x)
-        for (uint32_t k = 0; node0 + n < nodes && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
+        for (uint32_t k = 0; n < count && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
     __syncthreads();
     for (uint32_t a = 0; a < V; a++) {
         // a view no node of the tile sees builds no tap context: its items
         // fail the node / warp test in score_slots and record None
         bool live;
-        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, ~0u, &live);
-        score_slots(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
+        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, count, &live);
+        score_slots<true>(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
                     &evaluation_count);
         __syncthreads();
     }
     for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x) {
         const uint32_t n = i / kDirectSlots, slot = i % kDirectSlots;
         const uint32_t g = node0 + n;
-        if (g >= nodes || slot > ncand[n]) continue;
+        if (n >= count || slot > ncand[n]) continue;
         float a;
         if (!aggregate_scores(res + (size_t)n * V * kDirectSlots + slot, kDirectSlots, V,
                               o.minimum_support, o.best_view_count, a))
@@ -1234,9 +1382,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         agg[(size_t)g * kDirectSlots + slot] = a;
     }
     __syncthreads();
-    for (uint32_t n = tid; n < nb; n += blockDim.x) {
+    for (uint32_t n = tid; n < count; n += blockDim.x) {
         const uint32_t g = node0 + n;
-        if (g >= nodes) continue;
         guidance[g] = centre_value[n];
         tested[g] = (uint8_t)node_tested[n];
     }
@@ -1281,8 +1428,8 @@ __global__ void __launch_bounds__(256) k_direct_select(GridSpec F, GridSpec Cg,
     field[g] = selected;
 }
 
-size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
-    const PatchLayout l = patch_layout(nb, V, T, false);
+size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uint32_t radius) {
+    const PatchLayout l = patch_layout(nb, V, T, false, tile_lattice_points(nb, step, radius));
     return (size_t)l.total + (size_t)nb * kMaxCandidates * 8 +
            (size_t)nb * V * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4 +
            (size_t)nb * kDirectSlots;
@@ -1296,16 +1443,17 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
                          unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    // 16 nodes give 224 (seeded) to 528 (unseeded) items per view for 256
+    // 18 nodes give 252 (seeded) to 594 (unseeded) items per view for 256
     // threads; 40 KB of LDS keeps 4 blocks per CU
-    uint32_t nb = 16;
-    while (nb > 1 && direct_lds_bytes(nb, V, T) > kQuarterLds) nb--;
-    const size_t lds = direct_lds_bytes(nb, V, T);
+    const uint32_t r = patch.radius, step = fine.step;
+    uint32_t nb = 18;
+    while (nb > 1 && direct_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
+    const size_t lds = direct_lds_bytes(nb, V, T, step, r);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
-    hipLaunchKernelGGL(k_direct_score, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
-                       fine, coarse, coarse_field, inverse_step, direct_options, T, nb, agg,
-                       guidance, tested, work);
+    hipLaunchKernelGGL(k_direct_score, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds,
+                       stream, patch, fine, coarse, coarse_field, inverse_step, direct_options, T,
+                       nb, agg, guidance, tested, work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_direct_select, dim3((nodes + 255) / 256), dim3(256), 0, stream, fine,
@@ -1379,7 +1527,7 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
         __syncthreads();
         for (uint32_t a = 0; a < V; a++) {
             const PatchLds Q = patch_view(S, P, nb, T, a);
-            score_slots(S, Q, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
+            score_slots<false>(S, Q, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
                         &evaluation_count);
             __syncthreads();
         }
@@ -1437,7 +1585,7 @@ constexpr uint32_t kTargetNodes = 32;
 // the five makes the refinement None, the reference's `?`s; running all five
 // first changes nothing since they have no side effects), plus the global
 // baseline; phase B the eight +-1.5 px residuals around the best depth.
-__global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_targets(PatchSpec S, GridSpec F,
                                                  const chd_node_depth *__restrict__ field,
                                                  double inverse_step, chd_options o, uint32_t T,
                                                  uint32_t nb, chd_node_warp *decisions,
@@ -1446,8 +1594,10 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     __shared__ unsigned int list_count;
     __shared__ uint16_t list[kTargetNodes * 8];  // items of the phase that pass every gate
     const uint32_t V = S.view_count;
-    const PatchLayout lay = patch_layout(nb, V, T, false);
-    const PatchLds P = bind_patch(lds, lay, false);
+    const uint32_t lat_points = tile_lattice_points(nb, F.step, S.radius);
+    const PatchLayout lay = patch_layout(nb, V, T, false, lat_points);
+    PatchLds P = bind_patch(lds, lay, false);
+    P = lat_points ? with_lattice(P, nb, F.step, S.radius) : with_rows(P, S.radius);
     float2 *points = (float2 *)(lds + lay.total);  // [nb][5]
     float2 *best_point = points + nb * 5;          // [nb]
     float *phase = (float *)(best_point + nb);     // [nb][8]
@@ -1455,14 +1605,19 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     float *best_score = centre_value + nb;         // [nb]
     float *baseline = best_score + nb;             // [nb]
     int *state = (int *)(baseline + nb);           // [nb] 0 skip, 1 refine, -1 None
-    const uint32_t nodes = F.columns * F.rows, node0 = blockIdx.x * nb, tid = threadIdx.x;
+    const NodeTile tile = row_tile(F, nb, blockIdx.x);
+    const uint32_t nodes = F.columns * F.rows, node0 = tile.first, count = tile.count;
+    const uint32_t tid = threadIdx.x;
     const double min_inverse = 1.0 / o.far_depth, max_inverse = 1.0 / o.near_depth;
 
     int *measured = state + nb;                    // [nb] the node has a depth
     for (uint32_t n = tid; n < nb; n += blockDim.x) {
-        P.pos[n] = node_position(F, node0 + n);
-        P.node_ok[n] = node0 + n < nodes ? 1 : 0;
-        measured[n] = node0 + n < nodes && field[node0 + n].valid ? 1 : 0;
+        // slots past the tile continue the row (the lattice spans them;
+        // they are never scored)
+        const double2 p0 = node_position(F, node0);
+        P.pos[n] = make_double2(p0.x + (double)(n * F.step), p0.y);
+        P.node_ok[n] = n < count ? 1 : 0;
+        measured[n] = n < count && field[node0 + n].valid ? 1 : 0;
     }
     if (tid == 0) list_count = 0;
     unsigned int evaluations = 0;  // thread 0's tally of ZNCCs past the bounds check
@@ -1471,7 +1626,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
 
     for (uint32_t a = 0; a < V; a++) {
         bool live;
-        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, ~0u, &live, measured);
+        const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, count, &live, measured);
         chd_node_warp *out = decisions + (size_t)a * nodes;
         if (!live) {
             // no node with a depth has its reference patch and this view's
@@ -1480,7 +1635,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
             // elsewhere (depth.rs:433-440, 458-485)
             for (uint32_t n = tid; n < nb; n += blockDim.x) {
                 const uint32_t g = node0 + n;
-                if (g >= nodes) continue;
+                if (n >= count) continue;
                 chd_node_warp d;
                 d.code = Q.gc_ok[n] ? CHD_WARP_GLOBAL : CHD_WARP_UNDEFINED;
                 d.global[0] = Q.gc_ok[n] ? Q.gc[n].x : 0.0f;
@@ -1499,7 +1654,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         // Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
-            state[n] = g < nodes && Q.gc_ok[n] && field[g].valid ? 1 : 0;
+            state[n] = n < count && Q.gc_ok[n] && field[g].valid ? 1 : 0;
         }
         __syncthreads();
         // phase A: offsets k < 5 (score_one_view_at_point, depth.rs:1710-1736)
@@ -1551,7 +1706,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
                     dy = (double)(q.y - Q.gc[n].y);
                 }
                 float sc;
-                if (zncc_taps<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc)) {
+                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc)) {
                     if (k == 5)
                         baseline[n] = sc;
                     else
@@ -1615,7 +1770,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
                 const double dx = (double)(point.x - Q.gc[n].x);
                 const double dy = (double)(point.y - Q.gc[n].y);
                 float sc;
-                if (zncc_taps<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc))
+                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc))
                     phase[n * 8 + k] = sc;
             }
         }
@@ -1623,7 +1778,7 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
         if (tid == 0) list_count = 0;
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
-            if (g >= nodes) continue;
+            if (n >= count) continue;
             chd_node_warp d;
             d.code = CHD_WARP_UNDEFINED;
             d.global[0] = d.global[1] = 0.0f;
@@ -1690,8 +1845,8 @@ __global__ void __launch_bounds__(256) k_targets(PatchSpec S, GridSpec F,
     if (tid == 0 && work && evaluations) atomicAdd(work, (unsigned long long)evaluations);
 }
 
-size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T) {
-    const PatchLayout l = patch_layout(nb, V, T, false);
+size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uint32_t radius) {
+    const PatchLayout l = patch_layout(nb, V, T, false, tile_lattice_points(nb, step, radius));
     return (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 5;
 }
 
@@ -1701,14 +1856,16 @@ hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const Grid
                           unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    // 32 nodes: 192 phase-A and 256 phase-B items for 256 threads
+    // 32 nodes: 192 phase-A and 256 phase-B items for 256 threads; with the
+    // shared lattice a quarter of the CU's LDS holds them (four blocks per CU)
+    const uint32_t r = patch.radius, step = fine.step;
     uint32_t nb = kTargetNodes;
-    while (nb > 1 && targets_lds_bytes(nb, V, T) > 52 * 1024) nb--;
-    const size_t lds = targets_lds_bytes(nb, V, T);
+    // (less the kernel's static list)
+    while (nb > 1 && targets_lds_bytes(nb, V, T, step, r) > kQuarterLds - kTargetNodes * 16) nb--;
+    const size_t lds = targets_lds_bytes(nb, V, T, step, r);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const uint32_t nodes = fine.columns * fine.rows;
-    hipLaunchKernelGGL(k_targets, dim3((nodes + nb - 1) / nb), dim3(256), lds, stream, patch,
-                       fine, field, inverse_step, options, T, nb, decisions, work);
+    hipLaunchKernelGGL(k_targets, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds, stream,
+                       patch, fine, field, inverse_step, options, T, nb, decisions, work);
     return hipGetLastError();
 }
 

@@ -135,6 +135,15 @@ def test_completion_and_options_bit_exact(gpu_context):
     _full_parity(gpu_context, cap, "config0 options variant")
 
 
+def test_odd_fine_step_bit_exact(gpu_context):
+    """Grid step 10: the fine grid's step is 5, so the direct and target
+    kernels cannot share a 2-px tap lattice across a tile and build each
+    node's taps on their own; results stay bit-exact."""
+    cap = _small_config0(5)
+    cap.options = DepthOptions(grid_step=10, patch_radius=3)
+    _full_parity(gpu_context, cap, "config0 fine step 5")
+
+
 def test_sgm_stage_on_oracle_costs(gpu_context, oracle_lib):
     _say("sgm stage on random costs")
     rng = np.random.default_rng(11)

@@ -301,6 +301,23 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
         oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
                                         g.coarse_rows, g.labels, _abi.ptr(want))
         assert_bitwise("band SGM at 257 labels", sess.aggregated(), want)
+        # some tiles in fallback beside banded ones: the path sweep moves
+        # every label chunk at the fallback nodes and the window's chunks at
+        # the others; the SGM still equals the oracle's over the volume
+        mixed = None
+        for threshold in (0.5, 0.3, 0.7, 0.4, 0.6):
+            fb, tiles = net.apply(sess, threshold=threshold)
+            if 0 < fb < tiles:
+                mixed = threshold
+                break
+        assert mixed is not None, "no threshold gives a mix of banded and fallback tiles"
+        sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
+        band_scores, costs, guidance, _ = sess.cost_volume()
+        assert_band_equals_full(f"band with {fb} of {tiles} tiles in fallback", sess, band_scores,
+                                costs, want_scores, want_costs)
+        oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
+                                        g.coarse_rows, g.labels, _abi.ptr(want))
+        assert_bitwise("band SGM with fallback tiles", sess.aggregated(), want)
         clear(sess)
     finally:
         net.close()
