@@ -805,19 +805,22 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         return out
 
     if settings is None:
-        settings = [(thr, tile, mh) for mh in (16, 20, 24, 28, 32) for tile in (4, 8)
-                    for thr in (0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
+        # fallback tiles sweep every label (0) or a +-64 window
+        settings = [(thr, tile, mh, 8, wide) for wide in (0, 64) for mh in (16, 20, 24, 28, 32)
+                    for tile in (4, 8) for thr in (0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
     validation = sessions_for(VALIDATION_SEED, validation_captures)
     val_rows = batch_agreement_sessions(ctx, validation, net, settings, steps)
     for s in validation:
         s.close()
     for r in val_rows:
-        log(f"[bench] validation: max half {r['max_half_width']} tile {r['tile']} threshold "
+        log(f"[bench] validation: max half {r['max_half_width']} fallback half "
+            f"{r['fallback_half_width']} tile {r['tile']} threshold "
             f"{r['threshold']}: agreement {r['agreement']:.4f} fallback "
             f"{r['fallback_fraction']:.3f} speedup {r['speedup']:.3f}")
     ok = [r for r in val_rows if r["agreement"] >= 0.98]
     chosen = max(ok, key=lambda r: r["speedup"]) if ok else max(val_rows, key=lambda r: r["agreement"])
-    chosen_setting = (chosen["threshold"], chosen["tile"], chosen["max_half_width"])
+    chosen_setting = (chosen["threshold"], chosen["tile"], chosen["max_half_width"],
+                      chosen["half_width"], chosen["fallback_half_width"])
     spec_setting = (0.5, 8, 8)
     # the held-out captures in batches of four (the configuration's captures
     # per launch); agreements averaged over the batches, times per batch
@@ -849,7 +852,8 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
     def pick(row, keys):
         return {k: (round(row[k], 5) if isinstance(row[k], float) else row[k]) for k in keys}
 
-    keys = ("threshold", "tile", "max_half_width", "half_width", "agreement", "mask_agreement",
+    keys = ("threshold", "tile", "max_half_width", "half_width", "fallback_half_width",
+            "agreement", "mask_agreement",
             "window_coverage", "mean_window", "fallback_fraction", "full_ms", "band_ms",
             "speedup")
     e2e_keys = keys + ("consistency_mask_agreement", "decision_agreement")

@@ -385,7 +385,7 @@ DEPTH_SIGNATURES = {
     "chd_proposal_set_label_count": (C.c_int, VP, C.c_uint32),
     "chd_proposal_get_input_transform": (C.c_int, VP, P(C.c_float), P(C.c_float)),
     "chd_session_set_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
-                                 C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
+                                 C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
     "chd_session_clear_proposal": (C.c_int, VP),
     "chd_session_read_band": (C.c_int, VP, VP, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),

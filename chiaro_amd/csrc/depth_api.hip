@@ -3739,8 +3739,9 @@ int chd_session_network_input(chd_session *s, chd_proposal *p, uint16_t *out) {
 }
 
 int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
-                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
-                             uint32_t *full_tiles, uint32_t *tiles) {
+                             uint32_t half_width, uint32_t max_half_width,
+                             uint32_t fallback_half_width, uint32_t tile, uint32_t *full_tiles,
+                             uint32_t *tiles) {
     if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
         return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
     if (s->none) return CHD_NONE;
@@ -3751,7 +3752,8 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
     CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,
                                 s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
                                 threshold, half_width, std::max(half_width, max_half_width),
-                                tile, s->band_lo, s->band_hi, p->full_tiles));
+                                tile, fallback_half_width, s->band_lo, s->band_hi,
+                                p->full_tiles));
     const uint32_t window_max = 2 * std::max(half_width, max_half_width) + 2;
     if (!s->band || s->band_window_max != window_max) {
         s->band = true;

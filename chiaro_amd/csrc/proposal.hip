@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ 
                                                       uint32_t L, float centre_half,
                                                       float centre_k, float threshold,
                                                       uint32_t half, uint32_t max_half, uint32_t tile,
+                                                      uint32_t fallback_half,
                                                       uint16_t *__restrict__ lo,
                                                       uint16_t *__restrict__ hi,
                                                       uint32_t *__restrict__ full_tiles) {
@@ -471,7 +472,7 @@ __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ 
         const uint32_t c = tx * tile + i % tile, r = ty * tile + i / tile;
         if (c >= columns || r >= rows) continue;
         const size_t node = (size_t)r * columns + c;
-        if (full) {
+        if (full && fallback_half == 0) {
             lo[node] = 1;
             hi[node] = (uint16_t)(L - 1);
             continue;
@@ -479,6 +480,11 @@ __global__ void __launch_bounds__(256) k_band_windows(const float *__restrict__ 
         const float *o = out + ((size_t)min(r, h - 1) * w + min(c, w - 1)) * cout_p;
         const float centre = rintf(centre_half + centre_k * o[0]);
         const int ctr = (int)fminf(fmaxf(centre, 1.0f), (float)(L - 1));
+        if (full) {  // a wide window instead of every label
+            lo[node] = (uint16_t)max(1, ctr - (int)fallback_half);
+            hi[node] = (uint16_t)min((int)L - 1, ctr + (int)fallback_half);
+            continue;
+        }
         const float b = __expf(fminf(fmaxf(o[1], -3.0f), 5.0f));
         const int w = (int)fminf(fmaxf(ceilf(4.0f * b), (float)half), (float)max_half);
         lo[node] = (uint16_t)max(1, ctr - w);
@@ -583,12 +589,13 @@ hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
 hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
                                float threshold, uint32_t half, uint32_t max_half, uint32_t tile,
-                               uint16_t *lo, uint16_t *hi, uint32_t *full_tiles) {
+                               uint32_t fallback_half, uint16_t *lo, uint16_t *hi,
+                               uint32_t *full_tiles) {
     const float centre_half = (float)(L - 1) * 0.5f, centre_k = (float)(L - 1) * 0.125f;
     const uint32_t tiles = ((columns + tile - 1) / tile) * ((rows + tile - 1) / tile);
     hipLaunchKernelGGL(k_band_windows, dim3(tiles), dim3(256), 0, s, out, cout_p, w, h, columns,
-                       rows, L, centre_half, centre_k, threshold, half, max_half, tile, lo, hi,
-                       full_tiles);
+                       rows, L, centre_half, centre_k, threshold, half, max_half, tile,
+                       fallback_half, lo, hi, full_tiles);
     return hipGetLastError();
 }
 

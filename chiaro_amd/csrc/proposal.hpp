@@ -44,6 +44,7 @@ hipError_t launch_quarter_input(hipStream_t s, const float *const planes[3],
 hipError_t launch_band_windows(hipStream_t s, const float *out, uint32_t cout_p, uint32_t w,
                                uint32_t h, uint32_t columns, uint32_t rows, uint32_t L,
                                float threshold, uint32_t half, uint32_t max_half, uint32_t tile,
-                               uint16_t *lo, uint16_t *hi, uint32_t *full_tiles);
+                               uint32_t fallback_half, uint16_t *lo, uint16_t *hi,
+                               uint32_t *full_tiles);
 
 }  // namespace chd

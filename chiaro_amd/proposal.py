@@ -375,15 +375,19 @@ class Proposal:
         _abi.check(self.lib.chd_proposal_adam_step(self.handle, g.ctypes.data, lr), "adam step")
 
     def apply(self, session: Session, threshold: float = 0.5, half_width: int = 8,
-              tile: int = 8, report: bool = True, max_half_width: int = 0):
+              tile: int = 8, report: bool = True, max_half_width: int = 0,
+              fallback_half_width: int = 0):
         """Band windows for `session` from its planes (centre +- half_width,
         or +- ceil(4 b) clamped to [half_width, max_half_width] when
-        max_half_width is larger); returns (fallback tiles, tiles) when
-        report, else None (no synchronisation)."""
+        max_half_width is larger; tiles below `threshold` sweep every label,
+        or centre +- fallback_half_width when that is > 0); returns
+        (fallback tiles, tiles) when report, else None (no
+        synchronisation)."""
         full, tiles = C.c_uint32(), C.c_uint32()
         _abi.check(self.lib.chd_session_set_proposal(
-            session.handle, self.handle, threshold, half_width, max_half_width, tile,
-            C.byref(full) if report else None, C.byref(tiles)), "set proposal")
+            session.handle, self.handle, threshold, half_width, max_half_width,
+            fallback_half_width, tile, C.byref(full) if report else None, C.byref(tiles)),
+            "set proposal")
         return (full.value, tiles.value) if report else None
 
 
@@ -545,8 +549,9 @@ def batch_agreement(ctx: Context, caps, proposal: Proposal, settings, steps: int
 def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, settings,
                              steps: int = 3, stages: int = _abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM):
     """The sessions run as one batch (chd_batch): the full sweep against the
-    band sweep for each (threshold, tile, max_half_width[, half_width]) in
-    `settings` (half_width 8 unless given). Per setting: the fraction of
+    band sweep for each (threshold, tile, max_half_width[, half_width[,
+    fallback_half_width]]) in `settings` (half_width 8 and full-sweep
+    fallback tiles unless given). Per setting: the fraction of
     nodes tested by the full sweep whose band label is within +-1, mask
     agreement, fallback-tile fraction and both times per batch (the band
     time includes the network and the windows of every capture). With the
@@ -585,9 +590,11 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
         for setting in settings:
             threshold, tile, max_half = setting[:3]
             half = setting[3] if len(setting) > 3 else 8
+            wide = setting[4] if len(setting) > 4 else 0
             fallback = tiles = 0
             for s in sessions:
-                f, t = proposal.apply(s, threshold, half, tile, max_half_width=max_half)
+                f, t = proposal.apply(s, threshold, half, tile, max_half_width=max_half,
+                                      fallback_half_width=wide)
                 fallback += f
                 tiles += t
             batch.run(stages)
@@ -596,7 +603,7 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
             for _ in range(steps):
                 for s in sessions:
                     proposal.apply(s, threshold, half, tile, report=False,
-                                   max_half_width=max_half)
+                                   max_half_width=max_half, fallback_half_width=wide)
                 batch.run(stages)
             ctx.synchronize()
             band_ms = (time.perf_counter() - t0) * 1000 / steps
@@ -615,7 +622,7 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
                 covered += int(((f == 0) | ((f >= lo) & (f <= hi)))[v].sum())
                 width += int((hi.astype(np.int64) - lo + 1)[v].sum())
             row = {"threshold": threshold, "tile": tile, "max_half_width": max_half,
-                   "half_width": half,
+                   "half_width": half, "fallback_half_width": wide,
                    "agreement": close / max(valid, 1), "mask_agreement": mask,
                    "window_coverage": covered / max(valid, 1),
                    "mean_window": width / max(valid, 1),

@@ -429,16 +429,19 @@ int chd_proposal_adam_step(chd_proposal *proposal, const float *gradients,
  * its next cost volumes sweep label 0 and centre +- w only, w = ceil(4 b)
  * for the predicted band scale b, clamped to [half_width, max_half_width]
  * (max_half_width <= half_width: always half_width); tiles of tile x tile
- * coarse nodes whose mean confidence is below `threshold` keep the full
- * sweep. The network grid must be the reference plane at half size.
+ * coarse nodes whose mean confidence is below `threshold` (fallback tiles)
+ * keep the full sweep, or with fallback_half_width > 0 sweep centre +-
+ * fallback_half_width. The network grid must be the reference plane at half
+ * size.
  * full_tiles / tiles (optional) receive the fallback count and the tile
  * count (full_tiles synchronises). clear restores the full sweep. The
  * network's grid has one pixel per coarse node: the reference plane in cells
  * of (grid step / 2) x (grid step / 2) plane pixels (the grid step must be
  * even), and its label count must equal the session's. */
 int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float threshold,
-                             uint32_t half_width, uint32_t max_half_width, uint32_t tile,
-                             uint32_t *full_tiles, uint32_t *tiles);
+                             uint32_t half_width, uint32_t max_half_width,
+                             uint32_t fallback_half_width, uint32_t tile, uint32_t *full_tiles,
+                             uint32_t *tiles);
 int chd_session_clear_proposal(chd_session *session);
 /* The label window of every coarse node set by the last
  * chd_session_set_proposal: labels 0 and lo..hi are swept (lo = 1 and

@@ -318,6 +318,19 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
         oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
                                         g.coarse_rows, g.labels, _abi.ptr(want))
         assert_bitwise("band SGM with fallback tiles", sess.aggregated(), want)
+        # every tile in fallback with a +-20 fallback window: centre +- 20
+        fb, tiles = net.apply(sess, threshold=2.0, fallback_half_width=20)
+        assert fb == tiles
+        sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
+        band_scores, costs, guidance, _ = sess.cost_volume()
+        assert_band_equals_full("fallback windows", sess, band_scores, costs, want_scores,
+                                want_costs)
+        lo, hi = sess.band()
+        assert np.array_equal(lo, np.maximum(1, centre - 20).reshape(-1))
+        assert np.array_equal(hi, np.minimum(g.labels - 1, centre + 20).reshape(-1))
+        oracle_lib.or_semi_global_costs(_abi.ptr(costs), _abi.ptr(guidance), g.coarse_columns,
+                                        g.coarse_rows, g.labels, _abi.ptr(want))
+        assert_bitwise("band SGM with fallback windows", sess.aggregated(), want)
         clear(sess)
     finally:
         net.close()
