@@ -375,7 +375,8 @@ int run_sgm_paths(chd_session *s) {
     const chd_grid_sizes &g = s->grid;
     return timed(s, "sgm_paths", [&] {
         return launch_sgm(s->context->stream, s->costs, s->guidance, g.coarse_columns,
-                          g.coarse_rows, g.labels, s->paths, s->tuning.sgm_paths_per_block);
+                          g.coarse_rows, g.labels, s->paths, s->tuning.sgm_paths_per_block,
+                          s->band ? s->band_lo : nullptr, s->band ? s->band_hi : nullptr);
     });
 }
 
@@ -744,6 +745,19 @@ int session_create(chd_context *context, const chd_view *views, uint32_t count,
             d.columns = columns;
             any = true;
         }
+        // every target view's quad array, of the plane its tap loops read
+        for (size_t k = 0; k < s->active.size(); k++) {
+            ViewDev &d = host_views[s->active[k]];
+            const uint32_t pitch = d.columns ? d.plane_height + 1 : d.plane_pitch;
+            const uint32_t lines = d.columns ? d.plane_width + 1 : d.plane_height + 1;
+            float4 *quad = nullptr;
+            CHD_ALLOC(quad, (size_t)pitch * lines);
+            e = launch_quad_plane(st, d.columns ? d.columns : d.plane, pitch, lines, quad);
+            if (e != hipSuccess)
+                return bail(fail(CHD_ERR_HIP, std::string("quad plane: ") + hipGetErrorString(e)));
+            d.quad = quad;
+            any = true;
+        }
         if (any) CHD_UPLOAD(s->views, host_views.data(), count * sizeof(ViewDev));
     }
     s->host_views = host_views;
@@ -846,6 +860,11 @@ int chd_session_set_planes(chd_session *s, const float *const *planes, uint32_t 
         if (d.columns)
             CHD_HIP(launch_transpose_plane(st, d.plane, const_cast<float *>(d.columns),
                                            d.plane_width, d.plane_height));
+        if (d.quad)
+            CHD_HIP(launch_quad_plane(st, d.columns ? d.columns : d.plane,
+                                      d.columns ? d.plane_height + 1 : d.plane_pitch,
+                                      d.columns ? d.plane_width + 1 : d.plane_height + 1,
+                                      const_cast<float4 *>(d.quad)));
     }
     if (!device) CHD_HIP(hipStreamSynchronize(st));  // the host planes may go away
     s->stages_done = 0;
@@ -2925,6 +2944,7 @@ struct chd_batch {
     uint32_t *sgm_first = nullptr;
     uint32_t sgm_total = 0, labels = 0, sgm_paths_per_block = 4, xcd_order = 0;
     bool sgm_same_grids = true;  // every capture on the same coarse grid
+    bool sgm_windows = false;    // some capture sweeps band windows
     bool persistent = false;  // one queue launch for every cost volume
     bool one_sweep = false;   // every capture's SGM sweep in one launch, no pipelining
     std::vector<hipEvent_t> cost_done;  // per capture: its cost volume finished
@@ -3031,7 +3051,7 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
     }
     if (stages & CHD_STAGE_SGM) {
         CHD_HIP(launch_sgm_batch(st, b->sgm_jobs, b->sgm_first, n, b->sgm_total, b->labels,
-                                 b->sgm_paths_per_block, b->sgm_same_grids));
+                                 b->sgm_paths_per_block, b->sgm_same_grids, b->sgm_windows));
         for (chd_session *s : b->sessions) CHD_CHECK(run_sgm_rest(s, keep));
     }
     if (stages & CHD_STAGE_DIRECT)
@@ -3077,9 +3097,12 @@ int batch_prepare(chd_batch *b) {
     b->blocks = (uint32_t)std::max(cus, 1) * per_cu;
     std::vector<SgmJob> sgm(count);
     std::vector<uint32_t> sgm_first(count + 1, 0);
+    b->sgm_windows = false;
     for (uint32_t k = 0; k < count; k++) {
         chd_session *s = b->sessions[k];
-        sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows, s->paths};
+        sgm[k] = SgmJob{s->costs, s->guidance, s->grid.coarse_columns, s->grid.coarse_rows,
+                        s->paths, s->band ? s->band_lo : nullptr, s->band ? s->band_hi : nullptr};
+        if (s->band) b->sgm_windows = true;
         sgm_first[k + 1] = sgm_first[k] + sgm_waves(s->grid.coarse_columns, s->grid.coarse_rows);
         if (s->grid.coarse_columns != sgm[0].columns || s->grid.coarse_rows != sgm[0].rows)
             b->sgm_same_grids = false;

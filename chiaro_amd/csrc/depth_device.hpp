@@ -347,6 +347,7 @@ __device__ inline bool project(const chd_camera &c, d3 point, double &ox, double
 // in few cache lines.
 struct DevPlane {
     const float *data;
+    const float4 *quad;      // 2 x 2 neighbourhood per stored element, or null
     uint32_t width, height;  // logical size of the plane
     uint32_t pitch;          // f32 between stored rows (columns when transposed)
     uint32_t lines;          // stored rows (columns when transposed), padding included
@@ -396,42 +397,46 @@ __device__ __forceinline__ bool plane_sample(const DevPlane &p, float x, float y
     return true;
 }
 
-// Buffer resource over a padded plane (gfx9-family raw buffer, 32-bit
-// data format); reads past the plane return 0 instead of faulting. Callers
-// pass a plane that is uniform across the wavefront; reading the address
-// through readfirstlane keeps the resource in scalar registers.
+// Buffer resource over a plane's quad array (gfx9-family raw buffer, 32-bit
+// data format, 16 bytes per stored element); reads past the array return 0
+// instead of faulting. Callers pass a plane that is uniform across the
+// wavefront; reading the address through readfirstlane keeps the resource
+// in scalar registers.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const DevPlane &p) {
-    const uint64_t a = (uint64_t)p.data;
+    const uint64_t a = (uint64_t)p.quad;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    const uint32_t bytes = __builtin_amdgcn_readfirstlane(p.pitch * p.lines * 4u);
+    const uint32_t bytes = __builtin_amdgcn_readfirstlane(p.pitch * p.lines * 16u);
     return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0,
                                              (int)bytes, 0x00020000);
 }
 
 // Plane::sample without the bounds test, for points proven inside
-// [0, w-1] x [0, h-1]: two 8-byte loads fetch (x0, x0 + 1) of rows y0 and
-// y0 + 1 (the second through the scalar offset `line_bytes` = pitch * 4),
-// or, on a transposed plane, (y0, y0 + 1) of columns x0 and x0 + 1. The
+// [0, w-1] x [0, h-1], in two halves for callers that issue the loads of
+// several samples before interpolating any of them. One 16-byte load of the
+// quad array fetches (x0, x0 + 1) of rows y0 and y0 + 1, or, on a
+// transposed plane, (y0, y0 + 1) of columns x0 and x0 + 1 (the texture path
+// takes about the same time per load instruction for 8 and 16 bytes a lane,
+// so this halves its work against two 8-byte loads of the plane). The
 // interpolation is the same either way. For x >= 0, truncation is floor and
-// x - floor(x) is exact, which is what the hardware fract computes.
-// The two halves of plane_sample_inside, for callers that issue the loads of
-// several samples before interpolating any of them.
+// x - floor(x) is exact, which is what the hardware fract computes. Stored
+// lines and pitches stay below 2^24, so the line product fits the full-rate
+// 24-bit multiply.
 struct PendingSample {
-    float2 a, b;
+    float2 a, b;  // the pair on line y0 (x0) and the pair on the next line
     float tx, ty;
 };
 template <bool Transposed>
 __device__ __forceinline__ PendingSample plane_sample_issue(__amdgpu_buffer_rsrc_t rsrc,
-                                                           uint32_t pitch, uint32_t line_bytes,
-                                                           float x, float y) {
+                                                           uint32_t pitch, float x, float y) {
     const uint32_t line = Transposed ? (uint32_t)x : (uint32_t)y;
     const uint32_t along = Transposed ? (uint32_t)y : (uint32_t)x;
-    const uint32_t offset = (__umul24(line, pitch) + along) * 4u;
+    const uint32_t offset = (__umul24(line, pitch) + along) * 16u;
+    const float4 q =
+        __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, offset, 0, 0));
     PendingSample p;
-    p.a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
-    p.b = __builtin_bit_cast(float2,
-                             __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, line_bytes, 0));
+    p.a = make_float2(q.x, q.y);
+    p.b = make_float2(q.z, q.w);
     p.tx = __builtin_amdgcn_fractf(x);
     p.ty = __builtin_amdgcn_fractf(y);
     return p;
@@ -443,28 +448,6 @@ __device__ __forceinline__ float plane_sample_finish(const PendingSample &p) {
     const float top = p00 * (1.0f - p.tx) + p10 * p.tx;
     const float bottom = p01 * (1.0f - p.tx) + p11 * p.tx;
     return top * (1.0f - p.ty) + bottom * p.ty;
-}
-
-template <bool Transposed>
-__device__ __forceinline__ float plane_sample_inside(__amdgpu_buffer_rsrc_t rsrc, uint32_t pitch,
-                                                     uint32_t line_bytes, float x, float y) {
-    // stored lines and pitches stay below 2^24, so the line product fits the
-    // full-rate 24-bit multiply
-    const uint32_t line = Transposed ? (uint32_t)x : (uint32_t)y;
-    const uint32_t along = Transposed ? (uint32_t)y : (uint32_t)x;
-    const uint32_t offset = (__umul24(line, pitch) + along) * 4u;
-    const float2 a =
-        __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, 0, 0));
-    const float2 b = __builtin_bit_cast(
-        float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, offset, line_bytes, 0));
-    // a = (x0, y0) and its neighbour along the stored line; b = the same pair
-    // on the next line
-    const float p00 = a.x, p10 = Transposed ? b.x : a.y;
-    const float p01 = Transposed ? a.y : b.x, p11 = b.y;
-    const float tx = __builtin_amdgcn_fractf(x), ty = __builtin_amdgcn_fractf(y);
-    const float top = p00 * (1.0f - tx) + p10 * tx;
-    const float bottom = p01 * (1.0f - tx) + p11 * tx;
-    return top * (1.0f - ty) + bottom * ty;
 }
 
 // Warp::map (align.rs:84-111)

@@ -16,6 +16,12 @@ struct ViewDev {
     const float *columns;  // the same padded plane column by column (pitch
                            // plane_height + 1), or nullptr (see DevPlane)
     uint32_t plane_width, plane_height, plane_pitch;
+    // target views: the 2 x 2 sample neighbourhood of every stored element of
+    // the plane the ZNCC tap loops read (the column copy where there is one,
+    // else the rows): element (line, along) holds samples (line, along),
+    // (line, along + 1), (line + 1, along), (line + 1, along + 1), so that a
+    // bilinear sample is one 16-byte load; null for other views
+    const float4 *quad;
     const float2 *warp;   // global warp points (x, y), NaN = undefined
     uint32_t warp_step, warp_columns, warp_rows;
     const chd_camera *camera;  // nullptr = no calibration
@@ -96,15 +102,20 @@ hipError_t launch_cost_volume_queue(hipStream_t stream, const CostJob *jobs,
                                     int mode);
 
 // paths_per_block: wavefronts (one path each) per block, 1, 2 or 4.
+// With band windows (window_lo/hi non-null: label 0 and lo .. hi per node,
+// +inf costs elsewhere) the sweep moves only the label chunks that meet each
+// node's set; path values outside the sets are then left unwritten.
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
                       uint32_t columns, uint32_t rows, uint32_t labels, float *paths,
-                      uint32_t paths_per_block);
+                      uint32_t paths_per_block, const uint16_t *window_lo = nullptr,
+                      const uint16_t *window_hi = nullptr);
 
 // One capture's SGM inputs and path volumes for the batched sweep.
 struct SgmJob {
     const float *costs, *guidance;
     uint32_t columns, rows;
     float *paths;
+    const uint16_t *win_lo, *win_hi;  // band windows, or null
 };
 // Paths of one capture's sweep (2R + 2C + 4(C + R - 1)).
 uint32_t sgm_waves(uint32_t columns, uint32_t rows);
@@ -112,7 +123,7 @@ uint32_t sgm_waves(uint32_t columns, uint32_t rows);
 // same columns and rows (their paths are then interleaved by length).
 hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
                             uint32_t njobs, uint32_t total_paths, uint32_t labels,
-                            uint32_t paths_per_block, bool same_grids);
+                            uint32_t paths_per_block, bool same_grids, bool windows);
 
 hipError_t launch_sgm_select(hipStream_t stream, const float *paths, const float *scores,
                              const uint8_t *tested, const double *labels, uint32_t columns,
@@ -209,6 +220,9 @@ hipError_t launch_transpose_plane(hipStream_t stream, const float *rows, float *
 
 // vertical[k] = 1 when, at the reference centre, the displacement between the
 // near and far depth in view vlist[k] runs more vertically than horizontally.
+// The quad array of a padded plane (`lines` stored lines of `pitch` f32).
+hipError_t launch_quad_plane(hipStream_t stream, const float *plane, uint32_t pitch,
+                             uint32_t lines, float4 *quad);
 hipError_t launch_epipolar_axes(hipStream_t stream, const ViewDev *views, const uint32_t *vlist,
                                 uint32_t count, uint32_t reference, double near_depth,
                                 double far_depth, uint32_t *vertical);

@@ -63,9 +63,11 @@ constexpr uint32_t kSgmPrefetch = 8;
 
 // Label set of a node: every label, or with band windows (lo, hi non-null)
 // label 0 and lo[node] .. hi[node]. The costs outside the set are +inf, so
-// every path value there is +inf too (the sweep runs over the whole volume,
-// as cheap as a windowed sweep: it is bound by its serial steps, not by
-// memory); the select reads the set only and takes +inf elsewhere.
+// every path value there is +inf too. With windows the path sweep reads,
+// computes and writes only the 64-label chunks that meet a node's set (the
+// others hold +inf in registers, exactly what reading and computing them
+// would give); the select reads the set only and takes +inf elsewhere, so
+// path values outside the sets are never read.
 struct LabelSet {
     uint32_t lo, hi;  // lo = 1, hi = L - 1 without windows
     __device__ __forceinline__ bool has(uint32_t l) const { return l == 0 || (l >= lo && l <= hi); }
@@ -131,11 +133,26 @@ __device__ __forceinline__ void sgm_path_by_rank(uint32_t rank, uint32_t C, uint
         p = C + R - 1 - l;
 }
 
-template <uint32_t J>
+// Chunks of 64 labels that meet the label set of `node` (bit j = labels
+// 64j .. 64j + 63): chunk 0 (label 0) and the chunks of lo .. hi.
+// Wave-uniform.
+__device__ __forceinline__ uint32_t chunk_mask(const uint16_t *__restrict__ win_lo,
+                                               const uint16_t *__restrict__ win_hi,
+                                               int64_t node) {
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)win_lo[node]) >> 6;
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)win_hi[node]) >> 6;
+    return 1u | (((2u << b) - 1u) & ~((1u << a) - 1u));
+}
+
+// Win: band windows are given (win_lo/hi non-null; a kernel of its own, so
+// that the plain sweep is compiled without the chunk tests).
+template <uint32_t J, bool Win>
 __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
                                          const float *__restrict__ guidance, uint32_t C,
                                          uint32_t R, uint32_t L, float *__restrict__ paths,
-                                         uint32_t d, uint32_t p) {
+                                         uint32_t d, uint32_t p,
+                                         const uint16_t *__restrict__ win_lo,
+                                         const uint16_t *__restrict__ win_hi) {
     const uint32_t lane = threadIdx.x & 63u;
     int64_t start, stride;
     uint32_t len;
@@ -171,28 +188,38 @@ __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
     const float inf = __int_as_float(0x7f800000);
 
     float prev[J];
+    uint32_t prev_mask = ~0u;  // chunks of the previous node's set (Win)
+    if constexpr (Win) prev_mask = chunk_mask(win_lo, win_hi, start);
 #pragma unroll
     for (uint32_t j = 0; j < J; j++) {
         const uint32_t l = lane + 64 * j;
         prev[j] = inf;
-        if (l < L) {
+        if (l < L && (!Win || ((prev_mask >> j) & 1u))) {
             prev[j] = costs[(size_t)start * L + l];
             out[(size_t)start * L + l] = prev[j];
         }
     }
     float g_prev = guidance[start];
-    // ring of prefetched costs and guidance for steps i .. i + kSgmPrefetch - 1
+    // ring of prefetched costs, guidance (and chunk masks) for steps i .. i +
+    // kSgmPrefetch - 1
     float ahead[kSgmPrefetch][J];
     float g_ahead[kSgmPrefetch];
+    uint32_t m_ahead[kSgmPrefetch];
 #pragma unroll
     for (uint32_t k = 0; k < kSgmPrefetch; k++) {
         const uint32_t i = 1 + k;
         const int64_t node = start + (int64_t)i * stride;
         g_ahead[k] = i < len ? guidance[node] : 0.0f;
+        uint32_t mk = ~0u;
+        if constexpr (Win) {
+            mk = i < len ? chunk_mask(win_lo, win_hi, node) : 0u;
+            m_ahead[k] = mk;
+        }
 #pragma unroll
         for (uint32_t j = 0; j < J; j++) {
             const uint32_t l = lane + 64 * j;
-            ahead[k][j] = i < len && l < L ? costs[(size_t)node * L + l] : inf;
+            ahead[k][j] =
+                i < len && l < L && (!Win || ((mk >> j) & 1u)) ? costs[(size_t)node * L + l] : inf;
         }
     }
     for (uint32_t base = 1; base < len; base += kSgmPrefetch) {
@@ -205,15 +232,23 @@ __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
 #pragma unroll
             for (uint32_t j = 0; j < J; j++) cost[j] = ahead[k][j];
             const float g = g_ahead[k];
+            uint32_t mask = ~0u;
+            if constexpr (Win) mask = m_ahead[k];
             {
                 const uint32_t ip = i + kSgmPrefetch;
                 const int64_t pnode = start + (int64_t)ip * stride;
                 if (ip < len) {
                     g_ahead[k] = guidance[pnode];
+                    uint32_t pm = ~0u;
+                    if constexpr (Win) {
+                        pm = chunk_mask(win_lo, win_hi, pnode);
+                        m_ahead[k] = pm;
+                    }
 #pragma unroll
                     for (uint32_t j = 0; j < J; j++) {
                         const uint32_t l = lane + 64 * j;
-                        ahead[k][j] = l < L ? costs[(size_t)pnode * L + l] : inf;
+                        ahead[k][j] =
+                            l < L && (!Win || ((pm >> j) & 1u)) ? costs[(size_t)pnode * L + l] : inf;
                     }
                 }
             }
@@ -224,12 +259,19 @@ __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
             const float p2 = kSgmLargePenalty * rmaxf(edge_scale, 0.12f);
             float m = prev[0];
 #pragma unroll
-            for (uint32_t j = 1; j < J; j++) m = fminf(m, prev[j]);
+            for (uint32_t j = 1; j < J; j++)
+                if (!Win || ((prev_mask >> j) & 1u)) m = fminf(m, prev[j]);
             m = wave_min_uniform(m);
             const float jump = m + p2;
             float cur[J];
 #pragma unroll
             for (uint32_t j = 0; j < J; j++) {
+                if constexpr (Win) {
+                    if (!((mask >> j) & 1u)) {  // +inf costs: +inf values
+                        cur[j] = inf;
+                        continue;
+                    }
+                }
                 // neighbours across chunk edges come from lane 63 of the
                 // previous chunk and lane 0 of the next
                 const float carry_up =
@@ -244,9 +286,10 @@ __device__ __forceinline__ void sgm_path(const float *__restrict__ costs,
 #pragma unroll
             for (uint32_t j = 0; j < J; j++) {
                 const uint32_t l = lane + 64 * j;
-                if (l < L) out[(size_t)node * L + l] = cur[j];
+                if (l < L && (!Win || ((mask >> j) & 1u))) out[(size_t)node * L + l] = cur[j];
                 prev[j] = l < L ? cur[j] : inf;
             }
+            if constexpr (Win) prev_mask = mask;
         }
     }
 }
@@ -260,11 +303,13 @@ __device__ __forceinline__ uint32_t snake_rank(uint32_t round, uint32_t wave, ui
     return round * waves + (round % 2 == 0 ? wave : waves - 1 - wave);
 }
 
-template <uint32_t J>
+template <uint32_t J, bool Win>
 __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ costs,
                                                    const float *__restrict__ guidance, uint32_t C,
                                                    uint32_t R, uint32_t L,
-                                                   float *__restrict__ paths, uint32_t waves) {
+                                                   float *__restrict__ paths, uint32_t waves,
+                                                   const uint16_t *__restrict__ win_lo,
+                                                   const uint16_t *__restrict__ win_hi) {
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (wave >= waves) return;
@@ -274,7 +319,7 @@ __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ cos
         if (rank >= total) continue;
         uint32_t d, p;
         sgm_path_by_rank(rank, C, R, d, p);
-        sgm_path<J>(costs, guidance, C, R, L, paths, d, p);
+        sgm_path<J, Win>(costs, guidance, C, R, L, paths, d, p, win_lo, win_hi);
     }
 }
 
@@ -282,7 +327,7 @@ __global__ void __launch_bounds__(256) k_sgm_paths(const float *__restrict__ cos
 // of each job's path count (njobs + 1 entries). With `interleave` (every job
 // on the same grid) global rank g is rank g / njobs of job g % njobs, so the
 // longest paths of every capture come first.
-template <uint32_t J>
+template <uint32_t J, bool Win>
 __global__ void __launch_bounds__(256) k_sgm_paths_batch(const SgmJob *__restrict__ jobs,
                                                          const uint32_t *__restrict__ first,
                                                          uint32_t njobs, uint32_t L,
@@ -305,7 +350,8 @@ __global__ void __launch_bounds__(256) k_sgm_paths_batch(const SgmJob *__restric
         const SgmJob job = jobs[j];
         uint32_t d, p;
         sgm_path_by_rank(local, job.columns, job.rows, d, p);
-        sgm_path<J>(job.costs, job.guidance, job.columns, job.rows, L, job.paths, d, p);
+        sgm_path<J, Win>(job.costs, job.guidance, job.columns, job.rows, L, job.paths, d, p,
+                         job.win_lo, job.win_hi);
     }
 }
 
@@ -330,7 +376,7 @@ static uint32_t sgm_launch_waves(uint32_t paths) {
 
 hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32_t *first,
                             uint32_t njobs, uint32_t total_paths, uint32_t labels,
-                            uint32_t paths_per_block, bool same_grids) {
+                            uint32_t paths_per_block, bool same_grids, bool windows) {
     const uint32_t J = (labels + 63) / 64;
     const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
     const uint32_t waves = sgm_launch_waves(total_paths);
@@ -339,8 +385,12 @@ hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32
     switch (J) {
 #define CHD_SGM_CASE(n)                                                                        \
     case n:                                                                                    \
-        hipLaunchKernelGGL(k_sgm_paths_batch<n>, blocks, threads, 0, stream, jobs, first,      \
-                           njobs, labels, waves, interleave);                                  \
+        if (windows)                                                                           \
+            hipLaunchKernelGGL((k_sgm_paths_batch<n, true>), blocks, threads, 0, stream, jobs, \
+                               first, njobs, labels, waves, interleave);                       \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_sgm_paths_batch<n, false>), blocks, threads, 0, stream, jobs,\
+                               first, njobs, labels, waves, interleave);                       \
         break;
         CHD_SGM_CASE(1)
         CHD_SGM_CASE(2)
@@ -359,7 +409,8 @@ hipError_t launch_sgm_batch(hipStream_t stream, const SgmJob *jobs, const uint32
 
 hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guidance,
                       uint32_t columns, uint32_t rows, uint32_t labels, float *paths,
-                      uint32_t paths_per_block) {
+                      uint32_t paths_per_block, const uint16_t *window_lo,
+                      const uint16_t *window_hi) {
     const uint32_t J = (labels + 63) / 64;
     const uint32_t waves = sgm_launch_waves(sgm_path_count(columns, rows));
     const uint32_t wpb = paths_per_block == 1 || paths_per_block == 2 ? paths_per_block : 4;
@@ -367,8 +418,14 @@ hipError_t launch_sgm(hipStream_t stream, const float *costs, const float *guida
     switch (J) {
 #define CHD_SGM_CASE(n)                                                                        \
     case n:                                                                                    \
-        hipLaunchKernelGGL(k_sgm_paths<n>, blocks, threads, 0, stream, costs, guidance,        \
-                           columns, rows, labels, paths, waves);                               \
+        if (window_lo)                                                                         \
+            hipLaunchKernelGGL((k_sgm_paths<n, true>), blocks, threads, 0, stream, costs,      \
+                               guidance, columns, rows, labels, paths, waves, window_lo,       \
+                               window_hi);                                                     \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_sgm_paths<n, false>), blocks, threads, 0, stream, costs,     \
+                               guidance, columns, rows, labels, paths, waves, nullptr,         \
+                               nullptr);                                                       \
         break;
         CHD_SGM_CASE(1)
         CHD_SGM_CASE(2)
@@ -1273,6 +1330,27 @@ __global__ void __launch_bounds__(256) k_transpose_plane(const float *rows, floa
         const uint32_t y = y0 + threadIdx.x, x = x0 + k;
         if (x < w && y < h) columns[(size_t)x * h + y] = tile[threadIdx.x][k];
     }
+}
+
+// Element (line, along) of the quad array: the pair at (line, along) and
+// the pair on the next line (the last stored line and column repeat the
+// padding, which no in-bounds sample reads).
+__global__ void __launch_bounds__(256) k_quad_plane(const float *__restrict__ plane,
+                                                    uint32_t pitch, uint32_t lines,
+                                                    float4 *__restrict__ quad) {
+    const uint32_t along = blockIdx.x * blockDim.x + threadIdx.x, line = blockIdx.y;
+    if (along >= pitch) return;
+    const uint32_t a1 = along + 1 < pitch ? along + 1 : along;
+    const uint32_t l1 = line + 1 < lines ? line + 1 : line;
+    const float *r0 = plane + (size_t)line * pitch, *r1 = plane + (size_t)l1 * pitch;
+    quad[(size_t)line * pitch + along] = make_float4(r0[along], r0[a1], r1[along], r1[a1]);
+}
+
+hipError_t launch_quad_plane(hipStream_t stream, const float *plane, uint32_t pitch,
+                             uint32_t lines, float4 *quad) {
+    hipLaunchKernelGGL(k_quad_plane, dim3((pitch + 255) / 256, lines), dim3(256), 0, stream, plane,
+                       pitch, lines, quad);
+    return hipGetLastError();
 }
 
 hipError_t launch_transpose_plane(hipStream_t stream, const float *rows, float *columns,

@@ -178,6 +178,8 @@ __device__ __forceinline__ DevPlane view_plane(const ViewDev &v, bool prefer_col
     DevPlane p;
     p.width = v.plane_width;
     p.height = v.plane_height;
+    // a target view's quad array is built from the plane this picks
+    p.quad = prefer_columns ? v.quad : nullptr;
     if (prefer_columns && v.columns) {
         p.data = v.columns;
         p.pitch = v.plane_height + 1;
@@ -456,7 +458,7 @@ __device__ PatchLds patch_view(const PatchSpec &S, const PatchLds &P, uint32_t n
 }
 
 // One batch of kTapBatch taps from t0 of the three target sums of
-// warped_patch_zncc: all eighteen plane loads are issued before the first
+// warped_patch_zncc: all nine quad loads are issued before the first
 // interpolation, then the sums run in tap order. Tail batches (t0 + kTapBatch
 // > T) re-read the last tap for the missing ones and drop them; full batches
 // need no clamp, so their LDS reads take constant offsets from one address.
@@ -464,7 +466,7 @@ constexpr uint32_t kTapBatch = 9;
 template <bool Transposed, bool F64Taps, bool Tail>
 __device__ __forceinline__ void tap_batch(const float2 *wr, const double2 *gq, const float2 *gqf,
                                           uint32_t t0, uint32_t T, uint32_t pitch,
-                                          __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
+                                          __amdgpu_buffer_rsrc_t rsrc,
                                           double dx, double dy, float &st, float &stt, float &sp) {
     PendingSample ps[kTapBatch];
 #pragma unroll
@@ -480,7 +482,7 @@ __device__ __forceinline__ void tap_batch(const float2 *wr, const double2 *gq, c
             qx = (double)q.x;
             qy = (double)q.y;
         }
-        ps[k] = plane_sample_issue<Transposed>(rsrc, pitch, line_bytes, to_plane(qx + dx),
+        ps[k] = plane_sample_issue<Transposed>(rsrc, pitch, to_plane(qx + dx),
                                                to_plane(qy + dy));
     }
     // keep the batch's loads ahead of its interpolations (the scheduler
@@ -505,7 +507,7 @@ __device__ __forceinline__ void tap_batch(const float2 *wr, const double2 *gq, c
 // the same taps in the same order.
 template <bool Transposed, bool F64Taps, bool Rows = false>
 __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t T, uint32_t pitch,
-                                         __amdgpu_buffer_rsrc_t rsrc, uint32_t line_bytes,
+                                         __amdgpu_buffer_rsrc_t rsrc,
                                          double dx, double dy, float &st, float &stt, float &sp) {
     const float2 *wr = P.wr + (size_t)n * T;
     if constexpr (Rows) {
@@ -517,10 +519,10 @@ __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t
             uint32_t t0 = 0;
             for (; t0 + kTapBatch <= width; t0 += kTapBatch)
                 tap_batch<Transposed, false, false>(wr, nullptr, g, t0, width, pitch, rsrc,
-                                                    line_bytes, dx, dy, st, stt, sp);
+                                                    dx, dy, st, stt, sp);
             if (t0 < width)
                 tap_batch<Transposed, false, true>(wr, nullptr, g, t0, width, pitch, rsrc,
-                                                   line_bytes, dx, dy, st, stt, sp);
+                                                   dx, dy, st, stt, sp);
         }
         return;
     }
@@ -528,10 +530,10 @@ __device__ __forceinline__ void tap_sums(const PatchLds &P, uint32_t n, uint32_t
     const float2 *gqf = P.gqf + (size_t)n * T;
     uint32_t t0 = 0;
     for (; t0 + kTapBatch <= T; t0 += kTapBatch)
-        tap_batch<Transposed, F64Taps, false>(wr, gq, gqf, t0, T, pitch, rsrc, line_bytes, dx, dy,
+        tap_batch<Transposed, F64Taps, false>(wr, gq, gqf, t0, T, pitch, rsrc, dx, dy,
                                               st, stt, sp);
     if (t0 < T)
-        tap_batch<Transposed, F64Taps, true>(wr, gq, gqf, t0, T, pitch, rsrc, line_bytes, dx, dy,
+        tap_batch<Transposed, F64Taps, true>(wr, gq, gqf, t0, T, pitch, rsrc, dx, dy,
                                              st, stt, sp);
 }
 
@@ -551,13 +553,13 @@ __device__ __forceinline__ bool zncc_inside(const PatchLds &P, uint32_t n, const
 template <bool F64Taps, bool Rows = false>
 __device__ __forceinline__ bool zncc_taps(const PatchLds &P, uint32_t n, uint32_t T,
                                           const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
-                                          uint32_t line_bytes, double dx, double dy,
+                                          double dx, double dy,
                                           float &score) {
     float st = 0.0f, stt = 0.0f, sp = 0.0f;
     if (tp.transposed)
-        tap_sums<true, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<true, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, dx, dy, st, stt, sp);
     else
-        tap_sums<false, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, line_bytes, dx, dy, st, stt, sp);
+        tap_sums<false, F64Taps, Rows>(P, n, T, tp.pitch, rsrc, dx, dy, st, stt, sp);
     const float count = P.sums[n * 3 + 0], sr = P.sums[n * 3 + 1], srr = P.sums[n * 3 + 2];
     const float covariance = sp - sr * st / count;
     const float reference_energy = srr - sr * sr / count;
@@ -575,7 +577,7 @@ __device__ __forceinline__ bool zncc_taps(const PatchLds &P, uint32_t n, uint32_
 template <bool F64Taps, bool Rows = false>
 __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32_t T,
                                            const DevPlane &tp, __amdgpu_buffer_rsrc_t rsrc,
-                                           uint32_t line_bytes, double dx, double dy, float &score,
+                                           double dx, double dy, float &score,
                                            unsigned int *evaluations) {
     if (!zncc_inside(P, n, tp, dx, dy)) return false;
     // one LDS atomic per wavefront: the first active lane adds the count of
@@ -583,7 +585,7 @@ __device__ __forceinline__ bool patch_zncc(const PatchLds &P, uint32_t n, uint32
     const uint64_t passed = __ballot(1);
     if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)passed) - 1))
         atomicAdd(evaluations, (unsigned int)__popcll(passed));
-    return zncc_taps<F64Taps, Rows>(P, n, T, tp, rsrc, line_bytes, dx, dy, score);
+    return zncc_taps<F64Taps, Rows>(P, n, T, tp, rsrc, dx, dy, score);
 }
 
 // map_at_depth (depth.rs:1738-1759) from the cached corrected ray of node n,
@@ -755,7 +757,6 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
         }
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         for (uint32_t i = tid; i < nb * Ls; i += nth) {
             const uint32_t n = i / Ls, l = l0 + i % Ls;
             float s = nan_f();
@@ -764,7 +765,7 @@ __device__ void cost_tile(const CostJob &J, uint32_t tile, uint32_t slab, unsign
                 bool ok = true;
                 if (l > 0) ok = delta_at_depth(Q, n, *Q.cam, J.labels[l], dx, dy);
                 float sc;
-                if (ok && patch_zncc<true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                if (ok && patch_zncc<true>(Q, n, T, tp, rsrc, dx, dy, sc,
                                            evaluation_count))
                     s = sc;
             }
@@ -890,7 +891,6 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
             }
             const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
             const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-            const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
             for (uint32_t i = tid; i < cn; i += nth) {
                 const uint32_t it = c0 + i, n = item_node[it], j = it - base[n];
                 const uint32_t l = j == 0 ? 0 : lo[n] + j - 1;
@@ -900,7 +900,7 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
                     bool ok = true;
                     if (l > 0) ok = delta_at_depth(Q, n, *Q.cam, J.labels[l], dx, dy);
                     float sc;
-                    if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+                    if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, dx, dy, sc,
                                                evaluation_count))
                         s = sc;
                 }
@@ -1288,7 +1288,6 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
     const uint32_t V = S.view_count, total = base[nb];
     const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
     const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-    const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
         const uint32_t n = item_node[i];
         const uint32_t slot = i - base[n];
@@ -1299,7 +1298,7 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
             if (slot > 0)
                 ok = delta_at_depth(P, n, *P.cam, cand[n * stride + slot - 1], dx, dy);
             float sc;
-            if (ok && patch_zncc<false, Rows>(P, n, T, tp, rsrc, line_bytes, dx, dy, sc,
+            if (ok && patch_zncc<false, Rows>(P, n, T, tp, rsrc, dx, dy, sc,
                                               evaluations))
                 s = sc;
         }
@@ -1366,7 +1365,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         bool live;
         const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, count, &live);
         score_slots<true>(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
-                    &evaluation_count);
+                          &evaluation_count);
         __syncthreads();
     }
     for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x) {
@@ -1528,7 +1527,7 @@ __global__ void __launch_bounds__(256) k_direct_wide(PatchSpec S, GridSpec F, ch
         for (uint32_t a = 0; a < V; a++) {
             const PatchLds Q = patch_view(S, P, nb, T, a);
             score_slots<false>(S, Q, nb, T, a, cand, base, item_node, kWideSlots - 1, res,
-                        &evaluation_count);
+                               &evaluation_count);
             __syncthreads();
         }
         for (uint32_t i = tid; i < nb * kWideSlots; i += blockDim.x) {
@@ -1650,7 +1649,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
         const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
         const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
-        const uint32_t line_bytes = __builtin_amdgcn_readfirstlane(tp.pitch * 4u);
         // Undefined / Global / to refine (depth.rs:433-440)
         for (uint32_t n = tid; n < nb; n += blockDim.x) {
             const uint32_t g = node0 + n;
@@ -1706,7 +1704,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                     dy = (double)(q.y - Q.gc[n].y);
                 }
                 float sc;
-                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc)) {
+                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, dx, dy, sc)) {
                     if (k == 5)
                         baseline[n] = sc;
                     else
@@ -1770,7 +1768,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 const double dx = (double)(point.x - Q.gc[n].x);
                 const double dy = (double)(point.y - Q.gc[n].y);
                 float sc;
-                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, line_bytes, dx, dy, sc))
+                if (zncc_taps<false, true>(Q, n, T, tp, rsrc, dx, dy, sc))
                     phase[n * 8 + k] = sc;
             }
         }
