@@ -32,6 +32,7 @@ CHD_STAGE_SGM = 2
 CHD_STAGE_DIRECT = 4
 CHD_STAGE_TARGETS = 8
 CHD_STAGE_ALL = 15
+CHD_STAGE_PROPOSE = 16
 CHD_STAGE_KEEP_AGGREGATED = 256
 CHD_STAGE_KEEP_VIEW_SCORES = 512
 CHD_BATCH_PERSISTENT = 1
@@ -386,6 +387,8 @@ DEPTH_SIGNATURES = {
     "chd_proposal_get_input_transform": (C.c_int, VP, P(C.c_float), P(C.c_float)),
     "chd_session_set_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
                                  C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
+    "chd_session_attach_proposal": (C.c_int, VP, VP, C.c_float, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32)),
     "chd_session_clear_proposal": (C.c_int, VP),
     "chd_session_read_band": (C.c_int, VP, VP, VP),
     "chd_semi_global_costs": (C.c_int, VP, VP, VP, C.c_uint32, C.c_uint32, C.c_uint32, VP),

@@ -223,6 +223,10 @@ struct chd_session {
     uint16_t *band_lo = nullptr, *band_hi = nullptr;
     bool band = false;
     uint32_t band_window_max = 0;  // labels of the widest band window (2 max_half + 2)
+    // the proposal and band setting CHD_STAGE_PROPOSE runs (attach_proposal)
+    chd_proposal *attached = nullptr;
+    float attached_threshold = 0.0f;
+    uint32_t attached_half = 0, attached_max_half = 0, attached_fallback = 0, attached_tile = 0;
     uint32_t version = 0;  // bumped when launch parameters change (batches recapture)
     std::vector<chd_batch *> batches;  // live batches that hold this session
     chd_tuning tuning{};
@@ -362,6 +366,8 @@ int prepare_cost(chd_session *s) {
     CHD_HIP(launch_fill_u8(st, s->tested, 0, (size_t)s->grid.coarse_columns * s->grid.coarse_rows));
     return CHD_OK;  // band items write NaN / +inf outside their windows themselves
 }
+
+int propose_attached(chd_session *s);  // with the proposal entries below
 
 int run_cost(chd_session *s, bool keep_view_scores) {
     float *view_scores = nullptr;
@@ -897,6 +903,10 @@ int chd_session_run(chd_session *s, uint32_t stages) {
     CHD_HIP(hipSetDevice(s->context->device));
     const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
     const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
+    if (stages & CHD_STAGE_PROPOSE) {
+        if (!s->attached) return fail(CHD_ERR_STATE, "propose: no proposal attached");
+        CHD_CHECK(propose_attached(s));
+    }
     if (stages & CHD_STAGE_COST) {
         CHD_CHECK(run_cost(s, keep_views));
         s->stages_done = (s->stages_done & ~CHD_STAGE_KEEP_VIEW_SCORES) | CHD_STAGE_COST |
@@ -3013,6 +3023,10 @@ int batch_enqueue(chd_batch *b, uint32_t stages) {
     const bool keep = (stages & CHD_STAGE_KEEP_AGGREGATED) != 0;
     const bool keep_views = (stages & CHD_STAGE_KEEP_VIEW_SCORES) != 0;
     const uint32_t n = (uint32_t)b->sessions.size();
+    // every capture's network and band windows first (one proposal's
+    // buffers serve them in turn on this stream)
+    if (stages & CHD_STAGE_PROPOSE)
+        for (chd_session *s : b->sessions) CHD_CHECK(propose_attached(s));
     if (!b->persistent && !b->one_sweep && (stages & CHD_STAGE_COST) && (stages & CHD_STAGE_SGM)) {
         // pipelined: capture k's later stages on the side stream once its
         // cost volume is done, beside capture k + 1's cost volume
@@ -3192,6 +3206,7 @@ int chd_batch_run(chd_batch *b, uint32_t stages) {
         if ((stages & CHD_STAGE_SGM) && !(done & CHD_STAGE_COST)) return fail(CHD_ERR_STATE, "SGM before cost volume");
         if ((stages & CHD_STAGE_DIRECT) && !(done & CHD_STAGE_SGM)) return fail(CHD_ERR_STATE, "direct pass before SGM");
         if ((stages & CHD_STAGE_TARGETS) && !(done & CHD_STAGE_DIRECT)) return fail(CHD_ERR_STATE, "targets before direct pass");
+        if ((stages & CHD_STAGE_PROPOSE) && !s->attached) return fail(CHD_ERR_STATE, "propose: no proposal attached");
     }
     hipStream_t st = b->context->stream;
     // a session whose launch parameters changed (tuning, band) needs new
@@ -3482,6 +3497,11 @@ void chd_proposal_destroy(chd_proposal *p) {
     chd_context *c = p->context;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    for (chd_session *s : c->sessions)
+        if (s->attached == p) {  // batches holding s recapture without it
+            s->attached = nullptr;
+            s->version++;
+        }
     unregister(c->proposals, p);
     delete p;
     context_release_if_unused(c);
@@ -3746,6 +3766,29 @@ int proposal_input(chd_session *s, chd_proposal *p) {
     return CHD_OK;
 }
 
+// The network on the session's planes and the band windows it gives, on the
+// context's stream without host synchronisation (a batch captures it).
+int propose_windows(chd_session *s, chd_proposal *p, float threshold, uint32_t half_width,
+                    uint32_t max_half_width, uint32_t fallback_half_width, uint32_t tile) {
+    CHD_CHECK(proposal_input(s, p));
+    hipStream_t st = s->context->stream;
+    CHD_CHECK(proposal_forward(p, 1));
+    CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
+    CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,
+                                s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
+                                threshold, half_width, std::max(half_width, max_half_width),
+                                tile, fallback_half_width, s->band_lo, s->band_hi,
+                                p->full_tiles));
+    return CHD_OK;
+}
+
+// CHD_STAGE_PROPOSE: the attached proposal with its setting (the window
+// shape is the one attach_proposal set, so no launch parameter changes).
+int propose_attached(chd_session *s) {
+    return propose_windows(s, s->attached, s->attached_threshold, s->attached_half,
+                           s->attached_max_half, s->attached_fallback, s->attached_tile);
+}
+
 }  // namespace
 
 extern "C" {
@@ -3768,15 +3811,9 @@ int chd_session_set_proposal(chd_session *s, chd_proposal *p, float threshold,
     if (!s || !p || tile == 0 || tile > 16 || half_width == 0)
         return fail(CHD_ERR_ARGUMENT, "set_proposal: bad argument");
     if (s->none) return CHD_NONE;
-    CHD_CHECK(proposal_input(s, p));
     hipStream_t st = s->context->stream;
-    CHD_CHECK(proposal_forward(p, 1));
-    CHD_HIP(launch_fill_u32(st, p->full_tiles, 0u, 1));
-    CHD_HIP(launch_band_windows(st, p->out, p->layers[6].cout_p, p->width, p->height,
-                                s->grid.coarse_columns, s->grid.coarse_rows, s->grid.labels,
-                                threshold, half_width, std::max(half_width, max_half_width),
-                                tile, fallback_half_width, s->band_lo, s->band_hi,
-                                p->full_tiles));
+    CHD_CHECK(propose_windows(s, p, threshold, half_width, max_half_width, fallback_half_width,
+                              tile));
     const uint32_t window_max = 2 * std::max(half_width, max_half_width) + 2;
     if (!s->band || s->band_window_max != window_max) {
         s->band = true;
@@ -3804,10 +3841,28 @@ int chd_session_read_band(chd_session *s, uint16_t *lo, uint16_t *hi) {
     return CHD_OK;
 }
 
+int chd_session_attach_proposal(chd_session *s, chd_proposal *p, float threshold,
+                                uint32_t half_width, uint32_t max_half_width,
+                                uint32_t fallback_half_width, uint32_t tile,
+                                uint32_t *full_tiles, uint32_t *tiles) {
+    if (s && s->none) return CHD_NONE;
+    CHD_CHECK(chd_session_set_proposal(s, p, threshold, half_width, max_half_width,
+                                       fallback_half_width, tile, full_tiles, tiles));
+    s->attached = p;
+    s->attached_threshold = threshold;
+    s->attached_half = half_width;
+    s->attached_max_half = max_half_width;
+    s->attached_fallback = fallback_half_width;
+    s->attached_tile = tile;
+    s->version++;  // batch graphs recapture with this step
+    return CHD_OK;
+}
+
 int chd_session_clear_proposal(chd_session *s) {
     if (!s) return fail(CHD_ERR_ARGUMENT, "null session");
-    if (s->band) {
+    if (s->band || s->attached) {
         s->band = false;
+        s->attached = nullptr;
         s->version++;
     }
     return CHD_OK;

@@ -391,6 +391,19 @@ class Proposal:
         return (full.value, tiles.value) if report else None
 
 
+    def attach(self, session: Session, threshold: float = 0.5, half_width: int = 8,
+               tile: int = 8, max_half_width: int = 0, fallback_half_width: int = 0):
+        """apply() now, and keep this network and setting with the session:
+        runs and batches with CHD_STAGE_PROPOSE set the session's windows
+        from its current planes first (inside a batch's graph). Returns
+        (fallback tiles, tiles)."""
+        full, tiles = C.c_uint32(), C.c_uint32()
+        _abi.check(self.lib.chd_session_attach_proposal(
+            session.handle, self.handle, threshold, half_width, max_half_width,
+            fallback_half_width, tile, C.byref(full), C.byref(tiles)), "attach proposal")
+        return full.value, tiles.value
+
+
 def constant_proposal(context: Context, width: int, height: int, centre_label: float,
                       label_count: int = 129) -> Proposal:
     """The method without the network at the same hypothesis budget: zero
@@ -554,7 +567,9 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
     fallback tiles unless given). Per setting: the fraction of
     nodes tested by the full sweep whose band label is within +-1, mask
     agreement, fallback-tile fraction and both times per batch (the band
-    time includes the network and the windows of every capture). With the
+    time includes the network and the windows of every capture: the
+    proposal is attached and runs as the batch's CHD_STAGE_PROPOSE, inside
+    its graph). With the
     direct and target stages in `stages` the rows also compare the end
     products: the fraction of fine nodes whose u8 provenance (the
     consistency mask, depth.rs:548-579) and whose per-target decision codes
@@ -593,18 +608,18 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
             wide = setting[4] if len(setting) > 4 else 0
             fallback = tiles = 0
             for s in sessions:
-                f, t = proposal.apply(s, threshold, half, tile, max_half_width=max_half,
-                                      fallback_half_width=wide)
+                f, t = proposal.attach(s, threshold, half, tile, max_half_width=max_half,
+                                       fallback_half_width=wide)
                 fallback += f
                 tiles += t
-            batch.run(stages)
+            # every step runs the network and the windows of each capture
+            # inside the batch's graph, before its cost volume
+            band_stages = stages | _abi.CHD_STAGE_PROPOSE
+            batch.run(band_stages)
             ctx.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
-                for s in sessions:
-                    proposal.apply(s, threshold, half, tile, report=False,
-                                   max_half_width=max_half, fallback_half_width=wide)
-                batch.run(stages)
+                batch.run(band_stages)
             ctx.synchronize()
             band_ms = (time.perf_counter() - t0) * 1000 / steps
             close, valid, mask, covered, width = 0, 0, 0.0, 0, 0

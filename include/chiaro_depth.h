@@ -261,6 +261,10 @@ void chd_session_destroy(chd_session *session);
 #define CHD_STAGE_DIRECT 4u   /* measure_direct_depths + filters, :715-1060 */
 #define CHD_STAGE_TARGETS 8u  /* per-target decisions, :420-547 */
 #define CHD_STAGE_ALL 15u
+/* Run the proposal attached by chd_session_attach_proposal first (network on
+ * the session's planes, then its band windows); no counterpart in the
+ * reference. Not part of CHD_STAGE_ALL. */
+#define CHD_STAGE_PROPOSE 16u
 /* Keep the aggregated SGM volume (debug/parity only; costs HBM traffic). */
 #define CHD_STAGE_KEEP_AGGREGATED 256u
 /* Keep every view's own ZNCC slice of the cost volume (the pairwise scores
@@ -442,6 +446,17 @@ int chd_session_set_proposal(chd_session *session, chd_proposal *proposal, float
                              uint32_t half_width, uint32_t max_half_width,
                              uint32_t fallback_half_width, uint32_t tile, uint32_t *full_tiles,
                              uint32_t *tiles);
+/* chd_session_set_proposal now (same arguments and results), and keeps the
+ * proposal and setting with the session: every later chd_session_run or
+ * chd_batch_run with CHD_STAGE_PROPOSE runs the network on the session's
+ * current planes and sets the windows before the cost volume, inside the
+ * batch's graph, so a batch needs no host call per capture for its bands.
+ * chd_session_clear_proposal detaches; destroying the proposal detaches it
+ * from every session (a PROPOSE run then returns CHD_ERR_STATE). */
+int chd_session_attach_proposal(chd_session *session, chd_proposal *proposal, float threshold,
+                                uint32_t half_width, uint32_t max_half_width,
+                                uint32_t fallback_half_width, uint32_t tile,
+                                uint32_t *full_tiles, uint32_t *tiles);
 int chd_session_clear_proposal(chd_session *session);
 /* The label window of every coarse node set by the last
  * chd_session_set_proposal: labels 0 and lo..hi are swept (lo = 1 and

@@ -230,16 +230,17 @@ def test_config4_batch_bit_exact_with_oracle(gpu_context):
             s.close()
 
 
-def _banded_session(ctx, cap, seed):
+def _banded_session(ctx, cap, seed, transform_from=None):
     """A session with band windows from a seeded random proposal (about half
-    of the tiles banded)."""
+    of the tiles banded); the input transform from `transform_from`'s planes
+    (default: cap's)."""
     from chiaro_amd.proposal import Proposal, input_transform, layer_specs
     sess = Session(ctx, cap.modules, cap.reference_index, cap.options)
     factor = max(cap.options.grid_step, 4) // 2
     h, w = (n // factor for n in cap.modules[cap.reference_index].luminance.shape)
     net = Proposal(ctx, w, h, max_batch=1, seed=seed)
     net.set_label_count(sess.grid.labels)
-    net.set_input_transform(*input_transform([cap]))
+    net.set_input_transform(*input_transform([transform_from or cap]))
     rng = np.random.default_rng(seed)
     parts = []
     for (cin, cout, *_r) in layer_specs(w, h):
@@ -289,3 +290,61 @@ def test_band_batches_match_single_sessions(gpu_context, persistent):
             for s, net in pairs:
                 net.close()
                 s.close()
+
+
+def test_attached_proposal_runs_inside_the_batch(gpu_context):
+    """A proposal attached to the sessions runs as the batch's PROPOSE stage:
+    after new planes are set, a batched PROPOSE run sets each capture's
+    windows from its new planes (inside the graph) and every output equals a
+    fresh session of those planes given the same windows by apply(); with
+    the proposal destroyed, a PROPOSE run is refused."""
+    import copy
+    caps = []
+    for s in (42, 43):
+        c = scene.config2(seed=s, scale=0.125)
+        c.options.grid_step = 4
+        caps.append(c)
+    # the same rigs seen with other planes: each capture's modules with the
+    # luminance of the other capture (same sizes)
+    swapped = []
+    for k, cap in enumerate(caps):
+        other = caps[1 - k]
+        c = copy.copy(cap)
+        c.modules = [copy.copy(m) for m in cap.modules]
+        for m, o in zip(c.modules, other.modules):
+            assert m.luminance.shape == o.luminance.shape
+            m.luminance = o.luminance
+        swapped.append(c)
+    wants = []
+    for k, cap in enumerate(swapped):
+        # the network as attached below (input transform of the first planes)
+        sess, net = _banded_session(gpu_context, cap, 7 + k, transform_from=caps[k])
+        try:
+            sess.run(ALL_KEEP)
+            wants.append(_snapshot(sess))
+        finally:
+            net.close()
+            sess.close()
+    pairs = [_banded_session(gpu_context, cap, 7 + k) for k, cap in enumerate(caps)]
+    batch = Batch(gpu_context, [s for s, _ in pairs])
+    try:
+        for s, net in pairs:
+            net.attach(s, threshold=0.5, half_width=8, tile=4, max_half_width=24)
+        batch.run(ALL_KEEP | _abi.CHD_STAGE_PROPOSE)
+        for k, (s, _) in enumerate(pairs):
+            s.set_planes([m.luminance for m in swapped[k].modules])
+        for _ in range(2):  # the captured graph, replayed
+            batch.run(ALL_KEEP | _abi.CHD_STAGE_PROPOSE)
+            gpu_context.synchronize()
+            for k, ((s, _), want) in enumerate(zip(pairs, wants)):
+                _same(f"attached proposal capture {k}", _snapshot(s), want)
+        pairs[0][1].close()
+        with pytest.raises(RuntimeError):
+            batch.run(ALL_KEEP | _abi.CHD_STAGE_PROPOSE)
+        with pytest.raises(RuntimeError):
+            pairs[0][0].run(ALL_KEEP | _abi.CHD_STAGE_PROPOSE)
+    finally:
+        batch.close()
+        for s, net in pairs:
+            net.close()
+            s.close()
