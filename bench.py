@@ -251,7 +251,7 @@ def main() -> int:
     ap.add_argument("--eval-scenes", type=int, default=24, help="config3: held-out scenes")
     ap.add_argument("--epochs", type=int, default=10,
                     help="config3 / config4-learned: training epochs")
-    ap.add_argument("--learned-scenes", type=int, default=1600,
+    ap.add_argument("--learned-scenes", type=int, default=3200,
                     help="config1: training captures of the config 4 learned-band measurement "
                          "in the same line (0 skips it)")
     ap.add_argument("--learned-epochs", type=int, default=100)
@@ -805,9 +805,10 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         return out
 
     if settings is None:
-        # fallback tiles sweep every label (0) or a +-64 window
-        settings = [(thr, tile, mh, 8, wide) for wide in (0, 64) for mh in (16, 20, 24, 28, 32)
-                    for tile in (4, 8) for thr in (0.2, 0.25, 0.3, 0.35, 0.4, 0.5)]
+        # fallback tiles sweep every label (0) or a +-48 / +-64 window
+        settings = [(thr, tile, mh, 8, wide) for wide in (0, 48, 64)
+                    for mh in (16, 20, 22, 24, 26, 28, 32) for tile in (4, 8)
+                    for thr in (0.25, 0.3, 0.35, 0.4, 0.5)]
     validation = sessions_for(VALIDATION_SEED, validation_captures)
     val_rows = batch_agreement_sessions(ctx, validation, net, settings, steps)
     for s in validation:

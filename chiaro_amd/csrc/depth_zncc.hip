@@ -824,16 +824,19 @@ __device__ __forceinline__ void xcd_run(uint32_t x, uint32_t total, uint32_t &st
 // and its window lo .. hi only (window_lo/hi; a full window is 1 .. L - 1),
 // as proposed by the learned band proposal. Scores and costs of the labels
 // outside the window are written as NaN and +inf, so the SGM that follows
-// never selects them; inside the window every value is the full sweep's. One slab; items are (node, label) over the nodes' windows.
-// The tap positions are held as f32 (half the LDS of the full sweep's f64
-// layout), which keeps four blocks per CU at 12 nodes per item.
+// never selects them; inside the window every value is the full sweep's.
+// One slab; items are (node, label) over the nodes' windows. The tap
+// positions are held as f64, as in the full sweep (two conversions fewer
+// per tap than f32 positions: 3-5 % less time per ZNCC on config 4 bands;
+// a tile lattice shared by the nodes, as in the direct pass, was slower
+// here, where nodes are 8 px apart).
 template <bool TopK>
 __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *lds,
                                unsigned int *evaluation_count) {
     const PatchSpec &S = J.S;
     const uint32_t V = S.view_count, T = J.T, L = J.L, nb = J.nb;
-    const PatchLayout lay = patch_layout(nb, V, T, false);
-    const PatchLds P = bind_patch(lds, lay, false);
+    const PatchLayout lay = patch_layout(nb, V, T, true);
+    const PatchLds P = bind_patch(lds, lay, true);
     const uint32_t KS = TopK ? topk_slots(J.best_view_count, V) : V;
     const uint32_t C = J.slab;                        // items per pass over the views
     float *res = (float *)(lds + lay.total);          // [C][KS] running top-K, or [V][C]
@@ -900,8 +903,8 @@ __device__ void cost_tile_band(const CostJob &J, uint32_t tile, unsigned char *l
                     bool ok = true;
                     if (l > 0) ok = delta_at_depth(Q, n, *Q.cam, J.labels[l], dx, dy);
                     float sc;
-                    if (ok && patch_zncc<false>(Q, n, T, tp, rsrc, dx, dy, sc,
-                                               evaluation_count))
+                    if (ok && patch_zncc<true>(Q, n, T, tp, rsrc, dx, dy, sc,
+                                              evaluation_count))
                         s = sc;
                 }
                 if (TopK)
@@ -1063,7 +1066,7 @@ k_cost_volume_queue(const CostJob *__restrict__ jobs, const uint32_t *__restrict
 // (band items also hold base, lo and item_node for up to L labels a node)
 size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
                       uint32_t best_view_count, bool band = false, uint32_t labels = 0) {
-    const PatchLayout l = patch_layout(nb, views, taps, !band);
+    const PatchLayout l = patch_layout(nb, views, taps, true);
     const uint32_t k = topk_slots(best_view_count, views);
     const size_t slots = (size_t)(band ? slab : nb * slab) * (k < views ? k : views);
     size_t bytes = (size_t)l.total + slots * 4 + (size_t)nb * 8;
