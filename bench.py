@@ -663,7 +663,7 @@ def learned_summary(res: dict, workload: str) -> dict:
         "max_half_width": chosen["max_half_width"],
         "speedup": round(chosen["speedup"], 3),
         "agreement": round(chosen["agreement"], 5),
-        "mask_agreement": round(chosen["mask_agreement"], 5),
+        "tested_agreement": round(chosen["tested_agreement"], 5),
         "fallback_fraction": round(chosen["fallback_fraction"], 4),
         "full_ms": round(chosen["full_ms"], 4),
         "band_ms": round(chosen["band_ms"], 4),
@@ -834,7 +834,7 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         rows = []
         for j in range(2):
             r = dict(per[0][j])
-            for key in ("agreement", "mask_agreement", "window_coverage", "mean_window",
+            for key in ("agreement", "tested_agreement", "window_coverage", "mean_window",
                         "fallback_fraction", "full_ms", "band_ms",
                         "consistency_mask_agreement", "decision_agreement"):
                 if key in r:
@@ -854,7 +854,7 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
         return {k: (round(row[k], 5) if isinstance(row[k], float) else row[k]) for k in keys}
 
     keys = ("threshold", "tile", "max_half_width", "half_width", "fallback_half_width",
-            "agreement", "mask_agreement",
+            "agreement", "tested_agreement",
             "window_coverage", "mean_window", "fallback_fraction", "full_ms", "band_ms",
             "speedup")
     e2e_keys = keys + ("consistency_mask_agreement", "decision_agreement")

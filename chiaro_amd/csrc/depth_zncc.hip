@@ -1076,8 +1076,12 @@ size_t cost_lds_bytes(uint32_t nb, uint32_t views, uint32_t taps, uint32_t slab,
 
 // Dynamic LDS per block for four blocks per CU (the occupancy the tap loop's
 // registers allow): a quarter of the CU's 160 KB less room for the kernels'
-// static variables, which share the block's allocation.
-constexpr size_t kQuarterLds = 40 * 1024 - 256;
+// static variables, which share the block's allocation (272 B in the cost
+// and direct kernels, 784 B with the target kernel's item list). A block a
+// few bytes over the quarter runs three to a CU: the band sweep at 40,704 B
+// of dynamic LDS held 11.5 waves per CU against the full sweep's 15.5 and
+// took 18 % longer for its items.
+constexpr size_t kQuarterLds = 40 * 1024 - 1024;
 
 // Default shape: about 1024 (node, label) units per block, within a quarter
 // of the CU's LDS, one slab.
@@ -1861,8 +1865,7 @@ hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const Grid
     // shared lattice a quarter of the CU's LDS holds them (four blocks per CU)
     const uint32_t r = patch.radius, step = fine.step;
     uint32_t nb = kTargetNodes;
-    // (less the kernel's static list)
-    while (nb > 1 && targets_lds_bytes(nb, V, T, step, r) > kQuarterLds - kTargetNodes * 16) nb--;
+    while (nb > 1 && targets_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
     const size_t lds = targets_lds_bytes(nb, V, T, step, r);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_targets, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds, stream,

@@ -520,7 +520,7 @@ def agreement(ctx: Context, cap, proposal: Proposal, threshold: float, half_widt
     close = np.abs(band_best.astype(np.int32) - full_best.astype(np.int32)) <= 1
     stats = {
         "agreement": float(close[valid].mean()) if valid.any() else 1.0,
-        "mask_agreement": float((band_tested == full_tested).mean()),
+        "tested_agreement": float((band_tested == full_tested).mean()),
         "fallback_fraction": full_tiles / max(tiles, 1),
         "full_ms": full_ms,
         "band_ms": band_ms,
@@ -638,7 +638,7 @@ def batch_agreement_sessions(ctx: Context, sessions, proposal: Proposal, setting
                 width += int((hi.astype(np.int64) - lo + 1)[v].sum())
             row = {"threshold": threshold, "tile": tile, "max_half_width": max_half,
                    "half_width": half, "fallback_half_width": wide,
-                   "agreement": close / max(valid, 1), "mask_agreement": mask,
+                   "agreement": close / max(valid, 1), "tested_agreement": mask,
                    "window_coverage": covered / max(valid, 1),
                    "mean_window": width / max(valid, 1),
                    "fallback_fraction": fallback / max(tiles, 1), "full_ms": full_ms,
