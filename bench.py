@@ -718,6 +718,7 @@ def bench_learned(args) -> int:
 
 
 TRAIN_SEED, VALIDATION_SEED, TEST_SEED = 10_000, 200_000, 300_000
+VALIDATION_MARGIN = 0.0015
 
 
 def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: int,
@@ -818,7 +819,10 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
             f"{r['fallback_half_width']} tile {r['tile']} threshold "
             f"{r['threshold']}: agreement {r['agreement']:.4f} fallback "
             f"{r['fallback_fraction']:.3f} speedup {r['speedup']:.3f}")
-    ok = [r for r in val_rows if r["agreement"] >= 0.98]
+    # the choice keeps a margin over 98 % on the validation captures: with
+    # eight of them the same setting's held-out agreement has come out up
+    # to 0.1 points below its validation agreement
+    ok = [r for r in val_rows if r["agreement"] >= 0.98 + VALIDATION_MARGIN]
     chosen = max(ok, key=lambda r: r["speedup"]) if ok else max(val_rows, key=lambda r: r["agreement"])
     chosen_setting = (chosen["threshold"], chosen["tile"], chosen["max_half_width"],
                       chosen["half_width"], chosen["fallback_half_width"])
@@ -863,7 +867,8 @@ def learned_config4(ctx, train_captures: int, epochs: int, validation_captures: 
                      f"hypotheses, in batches of 4; network trained on {train_captures} half-scale "
                      f"config4 captures ({'five wide' if wide_only else 'all sixteen'} modules, "
                      f"4 px grid) x {epochs} epochs; setting chosen on {validation_captures} "
-                     f"other full captures"),
+                     f"other full captures (fastest with agreement >= "
+                     f"{100 * (0.98 + VALIDATION_MARGIN):.2f} % there)"),
         "met_98_percent": cost_rows[0]["agreement"] >= 0.98,
         "speedup": round(cost_rows[0]["speedup"], 3),
         "agreement": round(cost_rows[0]["agreement"], 5),

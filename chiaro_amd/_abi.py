@@ -367,6 +367,7 @@ DEPTH_SIGNATURES = {
     "chd_session_read_view_scores": (C.c_int, VP, VP),
     "chd_session_set_tuning": (C.c_int, VP, P(Tuning)),
     "chd_session_get_tuning": (C.c_int, VP, P(Tuning)),
+    "chd_session_occupancy": (C.c_int, VP, P(C.c_uint32)),
     "chd_autotune": (C.c_int, VP, C.c_char_p, P(Tuning)),
     "chd_batch_create": (C.c_int, VP, P(VP), C.c_uint32, C.c_uint32, P(VP)),
     "chd_batch_run": (C.c_int, VP, C.c_uint32),

@@ -1103,6 +1103,19 @@ int chd_session_get_tuning(const chd_session *s, chd_tuning *out) {
     return CHD_OK;
 }
 
+int chd_session_occupancy(chd_session *s, uint32_t blocks_per_cu[3]) {
+    if (!s || !blocks_per_cu) return fail(CHD_ERR_ARGUMENT, "occupancy: bad argument");
+    if (s->none) return CHD_NONE;
+    CHD_HIP(hipSetDevice(s->context->device));
+    const chd_options &o = s->options;
+    int blocks[3] = {0, 0, 0};
+    CHD_HIP(zncc_occupancy(cost_job(s, nullptr),
+                           patch_spec(s, std::max(o.patch_radius, 3u)),
+                           patch_spec(s, o.patch_radius), s->grid.fine_step, blocks));
+    for (int k = 0; k < 3; k++) blocks_per_cu[k] = (uint32_t)blocks[k];
+    return CHD_OK;
+}
+
 int chd_session_read_coarse(chd_session *s, chd_node_depth *field, uint8_t *fillable,
                             uint16_t *best_label) {
     if (!s) return fail(CHD_ERR_ARGUMENT, "null session");

@@ -1441,6 +1441,16 @@ size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uint
            (size_t)nb * kDirectSlots;
 }
 
+// Nodes per k_direct_score block and its dynamic LDS: 18 nodes give 252
+// (seeded) to 594 (unseeded) items per view for 256 threads, within the LDS
+// quarter that keeps four blocks per CU.
+uint32_t direct_block_nodes(uint32_t V, uint32_t T, uint32_t step, uint32_t r, size_t &lds) {
+    uint32_t nb = 18;
+    while (nb > 1 && direct_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
+    lds = direct_lds_bytes(nb, V, T, step, r);
+    return nb;
+}
+
 hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                          const GridSpec &coarse, const chd_node_depth *coarse_field,
                          double inverse_step, const chd_options &direct_options,
@@ -1449,12 +1459,8 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
                          unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    // 18 nodes give 252 (seeded) to 594 (unseeded) items per view for 256
-    // threads; 40 KB of LDS keeps 4 blocks per CU
-    const uint32_t r = patch.radius, step = fine.step;
-    uint32_t nb = 18;
-    while (nb > 1 && direct_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
-    const size_t lds = direct_lds_bytes(nb, V, T, step, r);
+    size_t lds = 0;
+    const uint32_t nb = direct_block_nodes(V, T, fine.step, patch.radius, lds);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
     hipLaunchKernelGGL(k_direct_score, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds,
@@ -1855,18 +1861,46 @@ size_t targets_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uin
     return (size_t)l.total + nb * 5 * 8 + nb * 8 + nb * 8 * 4 + nb * 4 * 5;
 }
 
+// Nodes per k_targets block and its dynamic LDS: 32 nodes give 192 phase-A
+// and 256 phase-B items for 256 threads; with the shared lattice a quarter
+// of the CU's LDS holds them (four blocks per CU).
+uint32_t targets_block_nodes(uint32_t V, uint32_t T, uint32_t step, uint32_t r, size_t &lds) {
+    uint32_t nb = kTargetNodes;
+    while (nb > 1 && targets_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
+    lds = targets_lds_bytes(nb, V, T, step, r);
+    return nb;
+}
+
+hipError_t zncc_occupancy(const CostJob &cost, const PatchSpec &direct, const PatchSpec &target,
+                          uint32_t fine_step, int blocks[3]) {
+    const void *cost_fn =
+        cost.window_lo ? (cost.topk ? (const void *)k_cost_volume_band<true>
+                                    : (const void *)k_cost_volume_band<false>)
+                       : (cost.topk ? (const void *)k_cost_volume<true>
+                                    : (const void *)k_cost_volume<false>);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks[0], cost_fn, 256,
+                                                                cost_job_lds(cost));
+    if (e != hipSuccess) return e;
+    size_t lds = 0;
+    const uint32_t Td = (2 * direct.radius + 1) * (2 * direct.radius + 1);
+    direct_block_nodes(direct.view_count, Td, fine_step, direct.radius, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks[1], (const void *)k_direct_score,
+                                                     256, lds);
+    if (e != hipSuccess) return e;
+    const uint32_t Tt = (2 * target.radius + 1) * (2 * target.radius + 1);
+    targets_block_nodes(target.view_count, Tt, fine_step, target.radius, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks[2], (const void *)k_targets, 256,
+                                                        lds);
+}
+
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,
                           const chd_node_depth *field, double inverse_step,
                           const chd_options &options, chd_node_warp *decisions,
                           unsigned long long *work) {
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
-    // 32 nodes: 192 phase-A and 256 phase-B items for 256 threads; with the
-    // shared lattice a quarter of the CU's LDS holds them (four blocks per CU)
-    const uint32_t r = patch.radius, step = fine.step;
-    uint32_t nb = kTargetNodes;
-    while (nb > 1 && targets_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
-    const size_t lds = targets_lds_bytes(nb, V, T, step, r);
+    size_t lds = 0;
+    const uint32_t nb = targets_block_nodes(V, T, fine.step, patch.radius, lds);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_targets, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds, stream,
                        patch, fine, field, inverse_step, options, T, nb, decisions, work);

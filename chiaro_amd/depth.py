@@ -261,6 +261,13 @@ class Session:
         _abi.check(self.lib.chd_session_get_tuning(self.handle, C.byref(t)), "get tuning")
         return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_}
 
+    def occupancy(self) -> dict:
+        """Work-groups per CU of the ZNCC kernels at this session's launch
+        shapes (the band sweep while windows are set)."""
+        out = (C.c_uint32 * 3)()
+        _abi.check(self.lib.chd_session_occupancy(self.handle, out), "occupancy")
+        return {"cost_volume": out[0], "direct": out[1], "targets": out[2]}
+
     def autotune(self, cache_path: str | None = None) -> dict:
         t = _abi.Tuning()
         path = None if cache_path is None else str(cache_path).encode()

@@ -334,6 +334,13 @@ typedef struct chd_tuning {
 } chd_tuning;
 int chd_session_set_tuning(chd_session *session, const chd_tuning *tuning);
 int chd_session_get_tuning(const chd_session *session, chd_tuning *out);
+/* Work-groups per CU that the runtime fits (registers, static and dynamic
+ * LDS) for the session's ZNCC kernels at the shapes its runs launch:
+ * [0] the cost volume (the band sweep while a proposal's windows are set),
+ * [1] the direct pass, [2] the per-target stage. The tap loops are tuned for
+ * four. No counterpart in the reference (it runs on the CPU); for tests and
+ * launch tuning. */
+int chd_session_occupancy(chd_session *session, uint32_t blocks_per_cu[3]);
 
 /* Times candidate shapes on the session's own data (it runs the cost and
  * SGM stages several times) and sets the fastest. With cache_path, results

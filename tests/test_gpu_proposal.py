@@ -274,6 +274,9 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
     g = sess.grid
     sess.run(_abi.CHD_STAGE_COST)
     want_scores, want_costs, _, _ = sess.cost_volume()
+    # four work-groups per CU for every ZNCC kernel (the tap loops' register
+    # budget assumes it; a block a few bytes over its LDS quarter runs three)
+    assert sess.occupancy() == {"cost_volume": 4, "direct": 4, "targets": 4}
     ref = cap.modules[0].luminance
     h, w = ref.shape[0] // 4, ref.shape[1] // 4
     net = Proposal(gpu_context, w, h, max_batch=1, seed=5)
@@ -285,6 +288,7 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
         net.set_parameters(params)
         fb, tiles = net.apply(sess, threshold=0.0)
         assert fb == 0 and tiles > 0
+        assert sess.occupancy()["cost_volume"] == 4
         sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
         band_scores, costs, guidance, _ = sess.cost_volume()
         assert_band_equals_full("band at 257 labels", sess, band_scores, costs, want_scores,
@@ -321,6 +325,7 @@ def test_band_on_an_8px_grid_with_257_labels(gpu_context, oracle_lib):
         # every tile in fallback with a +-20 fallback window: centre +- 20
         fb, tiles = net.apply(sess, threshold=2.0, fallback_half_width=20)
         assert fb == tiles
+        assert sess.occupancy()["cost_volume"] == 4
         sess.run(_abi.CHD_STAGE_COST | _abi.CHD_STAGE_SGM | _abi.CHD_STAGE_KEEP_AGGREGATED)
         band_scores, costs, guidance, _ = sess.cost_volume()
         assert_band_equals_full("fallback windows", sess, band_scores, costs, want_scores,
