@@ -414,12 +414,26 @@ def main() -> int:
     t_sgm = units * 96 / (PEAK_HBM_GBS * 1e9)
     t_select = units * 4 / (PEAK_HBM_GBS * 1e9)
     t_lb = (t_cost + t_sgm + t_select) * 1000.0
+    # the same bound with the cost stage at the bit-exact issue ceiling:
+    # unfused, unpacked lane operations (no FMA contraction, no packed FP32),
+    # one per cycle per SIMD lane, FP32 and FP64 alike
+    issue_tops = PEAK_FP32_TFLOPS / 4
+    t_cost_issue = max(units * 8 / (PEAK_HBM_GBS * 1e9),
+                       work["cost_zncc"] * (FP32_PER_ZNCC + FP64_PER_ZNCC) / (issue_tops * 1e12))
+    t_lb_issue = (t_cost_issue + t_sgm + t_select) * 1000.0
     composite = {
         "definition": "sum over cost, SGM paths, select of max(bytes / 8 TB/s, "
                       "FP32 / 157.3 TFLOP/s + FP64 / 78.6 TFLOP/s); frac = lower bound / step",
         "lower_bound_ms": round(t_lb, 4),
         "step_ms": round(ms_per_step, 4),
         "frac": round(t_lb / ms_per_step, 4),
+        "issue_ceiling": {
+            "definition": "the cost stage at the unfused, unpacked issue ceiling "
+                          f"({issue_tops:.2f} T lane-ops/s, FP32 and FP64 alike) instead of the "
+                          "published peaks; SGM and select at 8 TB/s",
+            "lower_bound_ms": round(t_lb_issue, 4),
+            "frac": round(t_lb_issue / ms_per_step, 4),
+        },
     }
 
     cpu = None

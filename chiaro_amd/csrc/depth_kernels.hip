@@ -1162,48 +1162,97 @@ hipError_t launch_target_outputs(hipStream_t stream, const chd_node_warp *decisi
     return hipGetLastError();
 }
 
-// Report inputs (depth.rs:394-418): counts and total_cmp sort keys of the
-// selected depths and of the measured nodes' improvements.
-// Wavefront-aggregated: one atomic per wave for each count and key list.
-__device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
-    const unsigned long long mask = __ballot(take);
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __builtin_ctzll(mask | (1ull << 63));
-    uint32_t base = 0;
-    if (mask && (int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+// Report inputs (depth.rs:394-418). Counts of the fine field (tested, measured, regularised nodes) and the keys
+// whose medians the report takes (depth of every valid node, improvement of
+// every measured node; the medians sort them, so their order is free). Each
+// block takes a run of kStatsNodes nodes in two passes: the first counts and
+// reserves the block's key ranges with one atomic per counter, the second
+// writes the keys in node order. (One atomic per wave and counter on five
+// shared addresses took 0.31 ms for 813 k nodes, serialised in L2.)
+constexpr uint32_t kStatsNodes = 2048;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (uint32_t m = 32; m > 0; m >>= 1) v += __shfl_xor(v, (int)m, 64);
+    return v;
 }
 
-__global__ void k_field_stats(const chd_node_depth *field, const uint8_t *tested, uint32_t n,
-                              unsigned long long *counts, unsigned long long *depth_keys,
-                              unsigned int *improvement_keys, uint32_t *key_counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    const bool in = i < n;
-    chd_node_depth d;
-    clear_node(d);
-    if (in) d = field[i];
-    const bool t = in && tested[i];
-    const bool measured = d.valid && !d.regularized, regularized = d.valid && d.regularized;
-    const unsigned long long mt = __ballot(t), mm = __ballot(measured), mr = __ballot(regularized);
-    if (lane == 0) {
-        if (mt) atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
-        if (mm) atomicAdd(&counts[1], (unsigned long long)__popcll(mm));
-        if (mr) atomicAdd(&counts[2], (unsigned long long)__popcll(mr));
+__global__ void __launch_bounds__(256) k_field_stats(const chd_node_depth *field,
+                                                     const uint8_t *tested, uint32_t n,
+                                                     unsigned long long *counts,
+                                                     unsigned long long *depth_keys,
+                                                     unsigned int *improvement_keys,
+                                                     uint32_t *key_counts) {
+    __shared__ uint32_t partial[4][4];  // [wave][tested, measured, regularised, valid]
+    __shared__ uint32_t base[2];        // the block's first depth / improvement key slot
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t first = blockIdx.x * kStatsNodes;
+    const uint32_t last = min(n, first + kStatsNodes);
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+        const chd_node_depth d = field[i];
+        c[0] += tested[i] ? 1u : 0u;
+        c[1] += d.valid && !d.regularized ? 1u : 0u;
+        c[2] += d.valid && d.regularized ? 1u : 0u;
+        c[3] += d.valid ? 1u : 0u;
     }
-    const uint32_t slot = wave_append(d.valid, &key_counts[0]);
-    if (d.valid) depth_keys[slot] = (unsigned long long)tkey64(d.depth) ^ 0x8000000000000000ull;
-    const uint32_t s2 = wave_append(measured, &key_counts[1]);
-    if (measured) improvement_keys[s2] = (unsigned int)tkey32(d.improvement) ^ 0x80000000u;
+    for (int k = 0; k < 4; k++) {
+        const uint32_t s = wave_sum(c[k]);
+        if (lane == 0) partial[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot[4] = {0, 0, 0, 0};
+        for (uint32_t w = 0; w < blockDim.x / 64; w++)
+            for (int k = 0; k < 4; k++) tot[k] += partial[w][k];
+        for (int k = 0; k < 3; k++)
+            if (tot[k]) atomicAdd(&counts[k], (unsigned long long)tot[k]);
+        base[0] = tot[3] ? atomicAdd(&key_counts[0], tot[3]) : 0u;
+        base[1] = tot[1] ? atomicAdd(&key_counts[1], tot[1]) : 0u;
+    }
+    __syncthreads();
+    uint32_t run_v = base[0], run_m = base[1];
+    for (uint32_t i0 = first; i0 < last; i0 += blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        chd_node_depth d;
+        clear_node(d);
+        if (i < last) d = field[i];
+        const bool valid = d.valid, measured = d.valid && !d.regularized;
+        const unsigned long long mv = __ballot(valid), mm = __ballot(measured);
+        __syncthreads();  // the previous round's partial[] reads are done
+        if (lane == 0) {
+            partial[wave][0] = (uint32_t)__popcll(mv);
+            partial[wave][1] = (uint32_t)__popcll(mm);
+        }
+        __syncthreads();
+        uint32_t off_v = 0, off_m = 0, all_v = 0, all_m = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; w++) {
+            if (w < wave) {
+                off_v += partial[w][0];
+                off_m += partial[w][1];
+            }
+            all_v += partial[w][0];
+            all_m += partial[w][1];
+        }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (valid)
+            depth_keys[run_v + off_v + (uint32_t)__popcll(mv & below)] =
+                (unsigned long long)tkey64(d.depth) ^ 0x8000000000000000ull;
+        if (measured)
+            improvement_keys[run_m + off_m + (uint32_t)__popcll(mm & below)] =
+                (unsigned int)tkey32(d.improvement) ^ 0x80000000u;
+        run_v += all_v;
+        run_m += all_m;
+    }
 }
 
 hipError_t launch_field_stats(hipStream_t stream, const chd_node_depth *field,
                               const uint8_t *tested, uint32_t nodes,
                               unsigned long long *counts, unsigned long long *depth_keys,
                               unsigned int *improvement_keys, uint32_t *key_counts) {
-    hipLaunchKernelGGL(k_field_stats, dim3((nodes + 255) / 256), dim3(256), 0, stream, field,
-                       tested, nodes, counts, depth_keys, improvement_keys, key_counts);
+    if (nodes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_stats, dim3((nodes + kStatsNodes - 1) / kStatsNodes), dim3(256), 0,
+                       stream, field, tested, nodes, counts, depth_keys, improvement_keys,
+                       key_counts);
     return hipGetLastError();
 }
 
