@@ -842,28 +842,92 @@ __device__ __forceinline__ bool cc_edge(const chd_node_depth *source, const floa
     return image_edge <= 0.6f && inverse_difference <= tol;
 }
 
-// Each node starts under its smallest-index neighbour it is related to (or
-// itself), which already links most of a surface into short chains.
-__global__ void k_cc_init(const chd_node_depth *source, const float *guidance, uint32_t C,
-                          uint32_t R, double tol, uint32_t *parent, uint32_t *sizes) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C * R) return;
-    sizes[i] = 0;
-    uint32_t p = i;
-    if (source[i].valid) {
-        const int row = (int)(i / C), column = (int)(i % C);
-        const int nx[4] = {column - 1, column, column + 1, column - 1};
-        const int ny[4] = {row - 1, row - 1, row - 1, row};
-        for (int k = 0; k < 4; k++) {  // neighbours of smaller index, ascending
-            if (nx[k] < 0 || ny[k] < 0 || nx[k] >= (int)C) continue;
-            const uint32_t j = (uint32_t)ny[k] * C + (uint32_t)nx[k];
-            if (source[j].valid && cc_edge(source, guidance, i, j, tol)) {
-                p = j;
-                break;
+// Components are first joined inside tiles of kCcTileW x kCcTileH nodes:
+// each block runs a union-find over its tile's edges in LDS (parents by
+// local index, always a smaller one), then stores every node's parent as its
+// local root's global index. Row-major order within a tile is the global
+// order, so parents still point to smaller indices of the same component,
+// and k_cc_union only has the edges that cross tile borders left to join in
+// global memory (a global union-find over every edge, each node first under
+// its smallest related neighbour, took 0.83-1.1 ms for 813 k fine nodes).
+constexpr uint32_t kCcTileW = 64, kCcTileH = 16;
+
+__device__ __forceinline__ uint32_t cc_local_find(uint32_t *lp, uint32_t x) {
+    uint32_t p = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return x;
+}
+
+__device__ __forceinline__ bool cc_related(const double *inverse, const float *guide, uint32_t a,
+                                           uint32_t b, double tol) {
+    // cc_edge with the reciprocal depths computed once per node
+    return fabsf(guide[a] - guide[b]) <= 0.6f && fabs(inverse[b] - inverse[a]) <= tol;
+}
+
+__global__ void __launch_bounds__(256) k_cc_local(const chd_node_depth *source,
+                                                  const float *guidance, uint32_t C, uint32_t R,
+                                                  double tol, uint32_t *parent, uint32_t *sizes) {
+    constexpr uint32_t N = kCcTileW * kCcTileH;
+    __shared__ uint32_t lp[N];
+    __shared__ double inverse[N];
+    __shared__ float guide[N];
+    __shared__ uint8_t valid[N];
+    const uint32_t tiles_x = (C + kCcTileW - 1) / kCcTileW;
+    const uint32_t x0 = (blockIdx.x % tiles_x) * kCcTileW, y0 = (blockIdx.x / tiles_x) * kCcTileH;
+    const uint32_t w = min(kCcTileW, C - x0), h = min(kCcTileH, R - y0);
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+        const uint32_t lx = k % kCcTileW, ly = k / kCcTileW;
+        lp[k] = k;
+        valid[k] = 0;
+        if (lx < w && ly < h) {
+            const uint32_t g = (y0 + ly) * C + x0 + lx;
+            sizes[g] = 0;
+            const chd_node_depth d = source[g];
+            valid[k] = d.valid ? 1 : 0;
+            inverse[k] = 1.0 / d.depth;
+            guide[k] = guidance[g];
+        }
+    }
+    __syncthreads();
+    // forward edges inside the tile: right, down-left, down, down-right
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+        if (!valid[k]) continue;
+        const int lx = (int)(k % kCcTileW), ly = (int)(k / kCcTileW);
+        const int nx[4] = {lx + 1, lx - 1, lx, lx + 1};
+        const int ny[4] = {ly, ly + 1, ly + 1, ly + 1};
+        for (int e = 0; e < 4; e++) {
+            if (nx[e] < 0 || nx[e] >= (int)w || ny[e] >= (int)h) continue;
+            const uint32_t j = (uint32_t)ny[e] * kCcTileW + (uint32_t)nx[e];
+            if (!valid[j] || !cc_related(inverse, guide, k, j, tol)) continue;
+            uint32_t a = k, b = j;
+            while (true) {
+                a = cc_local_find(lp, a);
+                b = cc_local_find(lp, b);
+                if (a == b) break;
+                if (a > b) {
+                    const uint32_t s = a;
+                    a = b;
+                    b = s;
+                }
+                uint32_t expected = b;
+                if (__hip_atomic_compare_exchange_strong(&lp[b], &expected, a, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP))
+                    break;
+                b = expected;
             }
         }
     }
-    parent[i] = p;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+        const uint32_t lx = k % kCcTileW, ly = k / kCcTileW;
+        if (lx >= w || ly >= h) continue;
+        const uint32_t r = cc_local_find(lp, k);
+        parent[(y0 + ly) * C + x0 + lx] = (y0 + r / kCcTileW) * C + x0 + r % kCcTileW;
+    }
 }
 
 __global__ void k_cc_union(const chd_node_depth *source, const float *guidance, uint32_t C,
@@ -875,6 +939,10 @@ __global__ void k_cc_union(const chd_node_depth *source, const float *guidance, 
     const int ny[4] = {row, row + 1, row + 1, row + 1};
     for (int k = 0; k < 4; k++) {
         if (nx[k] < 0 || nx[k] >= (int)C || ny[k] >= (int)R) continue;
+        // edges inside a tile were joined by k_cc_local
+        if ((uint32_t)nx[k] / kCcTileW == (uint32_t)column / kCcTileW &&
+            (uint32_t)ny[k] / kCcTileH == (uint32_t)row / kCcTileH)
+            continue;
         const uint32_t j = (uint32_t)ny[k] * C + (uint32_t)nx[k];
         if (!source[j].valid || !cc_edge(source, guidance, i, j, tol)) continue;
         uint32_t a = i, b = j;
@@ -935,7 +1003,8 @@ hipError_t launch_reject_components(hipStream_t stream, const chd_node_depth *so
                                       hipMemcpyDeviceToDevice, stream);
         return e;
     }
-    hipLaunchKernelGGL(k_cc_init, grid, block, 0, stream, source, guidance, columns, rows,
+    const uint32_t tiles = ((columns + kCcTileW - 1) / kCcTileW) * ((rows + kCcTileH - 1) / kCcTileH);
+    hipLaunchKernelGGL(k_cc_local, dim3(tiles), block, 0, stream, source, guidance, columns, rows,
                        inverse_tolerance, parent, sizes);
     hipLaunchKernelGGL(k_cc_union, grid, block, 0, stream, source, guidance, columns, rows,
                        inverse_tolerance, parent);
