@@ -1111,7 +1111,8 @@ int chd_session_occupancy(chd_session *s, uint32_t blocks_per_cu[3]) {
     int blocks[3] = {0, 0, 0};
     CHD_HIP(zncc_occupancy(cost_job(s, nullptr),
                            patch_spec(s, std::max(o.patch_radius, 3u)),
-                           patch_spec(s, o.patch_radius), s->grid.fine_step, blocks));
+                           patch_spec(s, o.patch_radius), s->grid.fine_step,
+                           o.best_view_count, blocks));
     for (int k = 0; k < 3; k++) blocks_per_cu[k] = (uint32_t)blocks[k];
     return CHD_OK;
 }

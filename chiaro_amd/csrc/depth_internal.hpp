@@ -172,7 +172,7 @@ hipError_t launch_fit_planes(hipStream_t stream, const chd_node_depth *source,
 // the cost-volume kernel of `cost` (full or band), k_direct_score and
 // k_targets at the launch shapes their launches use.
 hipError_t zncc_occupancy(const CostJob &cost, const PatchSpec &direct, const PatchSpec &target,
-                          uint32_t fine_step, int blocks[3]);
+                          uint32_t fine_step, uint32_t best_view_count, int blocks[3]);
 
 // Every view of patch.vlist is a target; decisions is [view slot][node].
 hipError_t launch_targets(hipStream_t stream, const PatchSpec &patch, const GridSpec &fine,

@@ -1287,11 +1287,14 @@ __device__ int best_depth_candidate(const double *depths, const float *score, ui
 // depth cand[node * stride + k - 1]; base[] is the prefix of 1 + ncand over
 // the block's nodes and item_node[] maps an item to its node. Results land
 // in res[(node * V + view) * (stride + 1) + slot].
+// KS = 0: res is [node][view][stride + 1] (every view's score kept);
+// KS > 0: res is [node][stride + 1][KS], a running top-K of the views'
+// scores per (node, slot) (topk_insert; the views come one at a time).
 template <bool Rows>
 __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, uint32_t T,
                             uint32_t a, const double *cand, const uint32_t *base,
                             const uint8_t *item_node, uint32_t stride, float *res,
-                            unsigned int *evaluations) {
+                            unsigned int *evaluations, uint32_t KS = 0) {
     const uint32_t V = S.view_count, total = base[nb];
     const DevPlane tp = view_plane(S.views[S.vlist[a]], true);
     const __amdgpu_buffer_rsrc_t rsrc = plane_rsrc(tp);
@@ -1309,7 +1312,10 @@ __device__ void score_slots(const PatchSpec &S, const PatchLds &P, uint32_t nb, 
                                               evaluations))
                 s = sc;
         }
-        res[((size_t)n * V + a) * (stride + 1) + slot] = s;
+        if (KS)
+            topk_insert(res + ((size_t)n * (stride + 1) + slot) * KS, KS - 1, s);
+        else
+            res[((size_t)n * V + a) * (stride + 1) + slot] = s;
     }
 }
 
@@ -1330,8 +1336,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     PatchLds P = bind_patch(lds, lay, false);
     P = lat_points ? with_lattice(P, nb, F.step, S.radius) : with_rows(P, S.radius);
     double *cand = (double *)(lds + lay.total);                 // [nb][32]
-    float *res = (float *)(cand + nb * kMaxCandidates);         // [nb][V][33]
-    float *centre_value = res + (size_t)nb * V * kDirectSlots;  // [nb]
+    // the views' scores per (node, slot): all V, or a running top-K when
+    // best_view_count < V (KS slots: count + the K largest)
+    const uint32_t KS = topk_slots(o.best_view_count, V) < V ? topk_slots(o.best_view_count, V) : 0;
+    float *res = (float *)(cand + nb * kMaxCandidates);  // [nb][V][33] or [nb][33][KS]
+    float *centre_value = res + (size_t)nb * (KS ? KS : V) * kDirectSlots;  // [nb]
     uint32_t *ncand = (uint32_t *)(centre_value + nb);          // [nb]
     uint32_t *node_tested = ncand + nb;                         // [nb]
     uint32_t *base = node_tested + nb;                          // [nb + 1]
@@ -1365,6 +1374,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     patch_nodes(S, P, nb, T, centre_value);
     for (uint32_t n = tid; n < nb; n += blockDim.x)
         for (uint32_t k = 0; n < count && k <= ncand[n]; k++) item_node[base[n] + k] = (uint8_t)n;
+    if (KS)  // no view scored yet (a slot is only touched by its item's thread)
+        for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x)
+            res[(size_t)i * KS] = __uint_as_float(0u);
     __syncthreads();
     for (uint32_t a = 0; a < V; a++) {
         // a view no node of the tile sees builds no tap context: its items
@@ -1372,7 +1384,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         bool live;
         const PatchLds Q = patch_view(S, P, nb, T, a, true, 0, count, &live);
         score_slots<true>(S, Q, nb, T, a, cand, base, item_node, kMaxCandidates, res,
-                          &evaluation_count);
+                          &evaluation_count, KS);
         __syncthreads();
     }
     for (uint32_t i = tid; i < nb * kDirectSlots; i += blockDim.x) {
@@ -1380,8 +1392,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const uint32_t g = node0 + n;
         if (n >= count || slot > ncand[n]) continue;
         float a;
-        if (!aggregate_scores(res + (size_t)n * V * kDirectSlots + slot, kDirectSlots, V,
-                              o.minimum_support, o.best_view_count, a))
+        const bool scored =
+            KS ? topk_aggregate(res + ((size_t)n * kDirectSlots + slot) * KS, o.minimum_support,
+                                o.best_view_count, a)
+               : aggregate_scores(res + (size_t)n * V * kDirectSlots + slot, kDirectSlots, V,
+                                  o.minimum_support, o.best_view_count, a);
+        if (!scored)
             a = nan_f();
         else
             node_tested[n] = 1;  // baseline or any candidate measured (depth.rs:794-795)
@@ -1434,20 +1450,23 @@ __global__ void __launch_bounds__(256) k_direct_select(GridSpec F, GridSpec Cg,
     field[g] = selected;
 }
 
-size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uint32_t radius) {
+size_t direct_lds_bytes(uint32_t nb, uint32_t V, uint32_t T, uint32_t step, uint32_t radius,
+                        uint32_t best_view_count) {
     const PatchLayout l = patch_layout(nb, V, T, false, tile_lattice_points(nb, step, radius));
+    const uint32_t ks = topk_slots(best_view_count, V), per = ks < V ? ks : V;
     return (size_t)l.total + (size_t)nb * kMaxCandidates * 8 +
-           (size_t)nb * V * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4 +
+           (size_t)nb * per * kDirectSlots * 4 + (size_t)nb * 4 * 3 + (nb + 1) * 4 +
            (size_t)nb * kDirectSlots;
 }
 
 // Nodes per k_direct_score block and its dynamic LDS: 18 nodes give 252
 // (seeded) to 594 (unseeded) items per view for 256 threads, within the LDS
 // quarter that keeps four blocks per CU.
-uint32_t direct_block_nodes(uint32_t V, uint32_t T, uint32_t step, uint32_t r, size_t &lds) {
+uint32_t direct_block_nodes(uint32_t V, uint32_t T, uint32_t step, uint32_t r,
+                            uint32_t best_view_count, size_t &lds) {
     uint32_t nb = 18;
-    while (nb > 1 && direct_lds_bytes(nb, V, T, step, r) > kQuarterLds) nb--;
-    lds = direct_lds_bytes(nb, V, T, step, r);
+    while (nb > 1 && direct_lds_bytes(nb, V, T, step, r, best_view_count) > kQuarterLds) nb--;
+    lds = direct_lds_bytes(nb, V, T, step, r, best_view_count);
     return nb;
 }
 
@@ -1460,7 +1479,8 @@ hipError_t launch_direct(hipStream_t stream, const PatchSpec &patch, const GridS
     const uint32_t T = (2 * patch.radius + 1) * (2 * patch.radius + 1);
     const uint32_t V = patch.view_count;
     size_t lds = 0;
-    const uint32_t nb = direct_block_nodes(V, T, fine.step, patch.radius, lds);
+    const uint32_t nb =
+        direct_block_nodes(V, T, fine.step, patch.radius, direct_options.best_view_count, lds);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const uint32_t nodes = fine.columns * fine.rows;
     hipLaunchKernelGGL(k_direct_score, dim3(fine.rows * row_tiles(fine, nb)), dim3(256), lds,
@@ -1872,7 +1892,7 @@ uint32_t targets_block_nodes(uint32_t V, uint32_t T, uint32_t step, uint32_t r, 
 }
 
 hipError_t zncc_occupancy(const CostJob &cost, const PatchSpec &direct, const PatchSpec &target,
-                          uint32_t fine_step, int blocks[3]) {
+                          uint32_t fine_step, uint32_t best_view_count, int blocks[3]) {
     const void *cost_fn =
         cost.window_lo ? (cost.topk ? (const void *)k_cost_volume_band<true>
                                     : (const void *)k_cost_volume_band<false>)
@@ -1883,7 +1903,7 @@ hipError_t zncc_occupancy(const CostJob &cost, const PatchSpec &direct, const Pa
     if (e != hipSuccess) return e;
     size_t lds = 0;
     const uint32_t Td = (2 * direct.radius + 1) * (2 * direct.radius + 1);
-    direct_block_nodes(direct.view_count, Td, fine_step, direct.radius, lds);
+    direct_block_nodes(direct.view_count, Td, fine_step, direct.radius, best_view_count, lds);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks[1], (const void *)k_direct_score,
                                                      256, lds);
     if (e != hipSuccess) return e;
